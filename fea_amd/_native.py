@@ -1,0 +1,115 @@
+"""ctypes binding of libfea_amd.so (the C ABI declared in include/fea_amd.h).
+
+The library is loaded from fea_amd/_lib/ (built in-tree by fea_amd.build).
+There is no fallback: if the library is missing or a call fails, a
+FeaNativeError is raised.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, '_lib', 'libfea_amd.so')
+
+F32, F64 = 0, 1
+C_INT, C_I64, C_DBL, C_P = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+
+
+class FeaNativeError(RuntimeError):
+    """A fea_amd C-ABI call returned an error status (or the library is absent)."""
+
+
+_SIGS = {
+    'fea_last_error': (ctypes.c_char_p, []),
+    'fea_version': (C_INT, []),
+    'fea_pattern_create': (C_INT, [C_INT, C_P, C_INT, C_P, C_P, C_P, C_P, ctypes.POINTER(C_P)]),
+    'fea_pattern_sizes': (C_INT, [C_P, C_P]),
+    'fea_pattern_export_csr': (C_INT, [C_P, C_P, C_P, C_P]),
+    'fea_pattern_export_blocks': (C_INT, [C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_pattern_export_group': (C_INT, [C_P, C_INT, C_P, C_P]),
+    'fea_pattern_destroy': (None, [C_P]),
+    'fea_elem_residual_tangent': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_elem_residual': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_elem_energy': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_elem_energy_blocks': (C_INT, [C_INT]),
+    'fea_elem_prepare': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_elem_lumped_mass': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_DBL, C_P, C_P, C_P]),
+    'fea_surface_pressure': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_DBL, C_P, C_P, C_P, C_P, C_P]),
+    'fea_surface_volume': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P, C_P]),
+    'fea_surface_traction': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P, C_DBL, C_P, C_P, C_P, C_P]),
+    'fea_surface_blocks': (C_INT, [C_INT]),
+    'fea_gather_csr': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_gather_residual': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P]),
+    'fea_reduce_sum': (C_INT, [C_INT, C_P, C_P, C_P]),
+    'fea_spmv': (C_INT, [C_INT, C_INT, C_I64, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_jacobi_inv': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P]),
+    'fea_pcg_ws_size': (C_I64, [C_INT]),
+    'fea_pcg_start': (C_INT, [C_INT, C_INT, C_I64, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_pcg_iterate': (C_INT, [C_INT, C_INT, C_I64, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P,
+                                C_DBL, C_INT, C_INT, C_P, C_P]),
+    'fea_pcg_spmv': (C_INT, [C_INT, C_INT, C_I64, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_spmv_residual_mixed': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_vec_blocks': (C_INT, [C_INT]),
+    'fea_dof_axpy': (C_INT, [C_INT, C_P, C_P, C_DBL, C_P, C_P, C_P]),
+    'fea_convert': (C_INT, [C_INT, C_INT, C_P, C_INT, C_P, C_P]),
+    'fea_dot_partial': (C_INT, [C_INT, C_P, C_P, C_P, C_P]),
+    'fea_axpy': (C_INT, [C_INT, C_DBL, C_INT, C_P, C_P, C_P]),
+    'fea_vec_stats': (C_INT, [C_INT, C_P, C_P, C_P]),
+    'fea_explicit_update': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_DBL, C_DBL, C_P, C_P, C_P, C_P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load (once) and return the ctypes library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FeaNativeError('fea_amd HIP library not found at %s: run `python -m fea_amd.build`'
+                             % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    return list(_SIGS.keys())
+
+
+def check(rc, what=''):
+    if rc != 0:
+        msg = lib().fea_last_error()
+        raise FeaNativeError('%s failed (status %d): %s' % (what, rc, msg.decode() if msg else ''))
+
+
+def call(name, *args):
+    """Call a status-returning C-ABI function and raise on error."""
+    rc = getattr(lib(), name)(*args)
+    check(rc, name)
+
+
+def ptr(t):
+    """Device/host address of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def require_gpu():
+    """The hot path runs only on a ROCm GPU; fail loudly otherwise."""
+    if not torch.cuda.is_available():
+        raise FeaNativeError('fea_amd requires a ROCm GPU (torch.cuda.is_available() is False)')
+    lib()

@@ -1,0 +1,3 @@
+from .base import BaseLoad
+from .pressure import Pressure
+from .traction import Traction

@@ -1,0 +1,310 @@
+"""Device-side assembly plan: HBM layout + orchestration of the HIP kernels.
+
+A ``DevicePlan`` owns, on one GPU:
+
+* ``X``        float64 [N,3]  reference coordinates of every node of every
+                              instance (instances concatenated in the RGC block
+                              order of assembly.py:104-128);
+* ``dof_map``  int32 [3N]     free-DOF index (GC numbering) or -1;
+* the free-DOF CSR (``rowptr``, ``col``, ``diag_slot``) and the node-pair block
+  tables + staging maps produced by ``fea_pattern_create``;
+* one record per *group*: an element group (C3D4/C3D8/C3D10 + material), a
+  pressure facet group or a traction facet group, with its int32
+  connectivity and its ``dst_k`` / ``dst_r`` staging maps;
+* staging buffers ``kstage`` (per precision) and ``rstage`` (float64).
+
+Every array is allocated once when the plan is built; assembly calls only
+launch kernels into these buffers.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from ._native import F32, F64, call, ptr
+
+_TORCH_DT = {F32: torch.float32, F64: torch.float64}
+
+
+def _i32(a, device):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
+
+
+class Group:
+    """One contiguous batch of elements or facets sharing a kernel configuration."""
+
+    def __init__(self, kind, ntype, conn, params=None, load=None, in_k=True, rho=1.0, name=''):
+        self.kind = kind            # 'elem' | 'pressure' | 'traction'
+        self.ntype = int(ntype)     # nodes per element / facet (= C-ABI type id)
+        self.conn_np = np.ascontiguousarray(conn, dtype=np.int32)
+        self.n = int(self.conn_np.shape[0])
+        self.params = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        self.load = load            # the load object (pressure magnitude / traction vector)
+        self.in_k = bool(in_k)
+        self.rho = float(rho)
+        self.name = name
+        self.conn = None
+        self.dst_k = None
+        self.dst_r = None
+
+
+class DevicePlan:
+    def __init__(self, X, free_mask, groups, device=None):
+        nat.require_gpu()
+        self.device = torch.device('cuda') if device is None else torch.device(device)
+        X = np.ascontiguousarray(np.asarray(X, dtype=np.float64).reshape(-1, 3))
+        self.n_nodes = X.shape[0]
+        free = np.asarray(free_mask, dtype=bool).reshape(-1)
+        if free.shape[0] != 3 * self.n_nodes:
+            raise ValueError('free mask has %d entries, expected %d' % (free.shape[0], 3 * self.n_nodes))
+        dof = np.full(3 * self.n_nodes, -1, dtype=np.int32)
+        dof[free] = np.arange(int(free.sum()), dtype=np.int32)
+        self.free_mask = free
+        self.dof_map_np = dof
+        self.groups = list(groups)
+        self.X = torch.as_tensor(X, device=self.device)
+        self.dof_map = _i32(dof, self.device)
+        self._build_pattern()
+        self._kstage = {}
+        self._csr_val = {}
+        self.rstage = torch.zeros(max(1, 3 * self.n_rstage), dtype=torch.float64, device=self.device)
+        self.R = torch.zeros(max(1, self.n_free), dtype=torch.float64, device=self.device)[:self.n_free]
+        # energy partial segments: one per group
+        self._epart_off = []
+        off = 0
+        for g in self.groups:
+            if g.kind == 'elem':
+                nb = nat.lib().fea_elem_energy_blocks(g.n)
+            else:
+                nb = nat.lib().fea_surface_blocks(g.n)
+            self._epart_off.append((off, nb))
+            off += nb
+        self._epart = torch.zeros(max(1, off), dtype=torch.float64, device=self.device)
+        self._eout = torch.zeros(max(1, len(self.groups)), dtype=torch.float64, device=self.device)
+
+    # ------------------------------------------------------------------ pattern
+    def _build_pattern(self):
+        L = nat.lib()
+        ng = len(self.groups)
+        conns = [g.conn_np for g in self.groups]
+        conn_ptrs = (ctypes.c_void_p * max(1, ng))(*[c.ctypes.data for c in conns])
+        n_elem = np.array([g.n for g in self.groups], dtype=np.int32)
+        nen = np.array([g.ntype for g in self.groups], dtype=np.int32)
+        in_k = np.array([1 if g.in_k else 0 for g in self.groups], dtype=np.int32)
+        handle = ctypes.c_void_p()
+        rc = L.fea_pattern_create(self.n_nodes, self.dof_map_np.ctypes.data, ng, conn_ptrs,
+                                  n_elem.ctypes.data, nen.ctypes.data, in_k.ctypes.data,
+                                  ctypes.byref(handle))
+        nat.check(rc, 'fea_pattern_create')
+        try:
+            sizes = np.zeros(5, dtype=np.int64)
+            nat.check(L.fea_pattern_sizes(handle, sizes.ctypes.data), 'fea_pattern_sizes')
+            self.n_free, self.nnz, self.n_blocks, self.n_kstage, self.n_rstage = [int(s) for s in sizes]
+            rowptr = np.zeros(self.n_free + 1, dtype=np.int32)
+            col = np.zeros(max(1, self.nnz), dtype=np.int32)
+            diag = np.zeros(max(1, self.n_free), dtype=np.int32)
+            nat.check(L.fea_pattern_export_csr(handle, rowptr.ctypes.data, col.ctypes.data, diag.ctypes.data),
+                      'fea_pattern_export_csr')
+            blk_a = np.zeros(max(1, self.n_blocks), dtype=np.int32)
+            blk_b = np.zeros_like(blk_a)
+            blk_cp = np.zeros_like(blk_a)
+            kseg = np.zeros(self.n_blocks + 1, dtype=np.int32)
+            rseg = np.zeros(self.n_nodes + 1, dtype=np.int32)
+            nat.check(L.fea_pattern_export_blocks(handle, blk_a.ctypes.data, blk_b.ctypes.data,
+                                                  blk_cp.ctypes.data, kseg.ctypes.data, rseg.ctypes.data),
+                      'fea_pattern_export_blocks')
+            dev = self.device
+            self.rowptr_np, self.col_np = rowptr, col[:self.nnz]
+            self.rowptr, self.col = _i32(rowptr, dev), _i32(col, dev)
+            self.diag_slot = _i32(diag, dev)
+            self.blk_a, self.blk_b, self.blk_colpos = _i32(blk_a, dev), _i32(blk_b, dev), _i32(blk_cp, dev)
+            self.kseg, self.rseg = _i32(kseg, dev), _i32(rseg, dev)
+            for gi, g in enumerate(self.groups):
+                dk = np.zeros(max(1, g.n * g.ntype * g.ntype), dtype=np.int32) if g.in_k else None
+                dr = np.zeros(max(1, g.n * g.ntype), dtype=np.int32)
+                nat.check(L.fea_pattern_export_group(handle, gi, None if dk is None else dk.ctypes.data,
+                                                     dr.ctypes.data), 'fea_pattern_export_group')
+                g.conn = _i32(g.conn_np if g.n else np.zeros((1, g.ntype)), dev)
+                g.dst_k = None if dk is None else _i32(dk, dev)
+                g.dst_r = _i32(dr, dev)
+        finally:
+            L.fea_pattern_destroy(handle)
+
+    def kstage(self, dtype):
+        if dtype not in self._kstage:
+            self._kstage[dtype] = torch.zeros(max(1, 9 * self.n_kstage), dtype=_TORCH_DT[dtype],
+                                              device=self.device)
+        return self._kstage[dtype]
+
+    def csr_values(self, dtype):
+        if dtype not in self._csr_val:
+            self._csr_val[dtype] = torch.zeros(max(1, self.nnz), dtype=_TORCH_DT[dtype],
+                                               device=self.device)
+        return self._csr_val[dtype]
+
+    # ------------------------------------------------------------------ kernels
+    @staticmethod
+    def _s():
+        return nat.stream_ptr()
+
+    def _check_u(self, U):
+        if U.dtype != torch.float64 or not U.is_cuda or U.numel() != 3 * self.n_nodes or not U.is_contiguous():
+            raise ValueError('U must be a contiguous float64 CUDA tensor with %d entries' % (3 * self.n_nodes))
+
+    def residual(self, U, load_scale=1.0, dtype=F64, out=None):
+        """R = f_int(U) - f_ext on free DOFs (assemble_force, assembly.py:230-281)."""
+        self.residual_stage(U, load_scale, dtype)
+        R = self.R if out is None else out
+        call('fea_gather_residual', self.n_nodes, ptr(self.rseg), ptr(self.dof_map), ptr(self.rstage), ptr(R),
+             self._s())
+        return R
+
+    def residual_stage(self, U, load_scale=1.0, dtype=F64):
+        """Launch every group's residual kernel into the staging array (no gather)."""
+        self._check_u(U)
+        s = self._s()
+        for g in self.groups:
+            if g.n == 0:
+                continue
+            if g.kind == 'elem':
+                call('fea_elem_residual', g.ntype, dtype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
+                     g.params.ctypes.data, ptr(g.dst_r), ptr(self.rstage), s)
+            elif g.kind == 'pressure':
+                call('fea_surface_pressure', g.ntype, dtype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
+                     float(g.load.pressure) * load_scale, None, None, ptr(g.dst_r), ptr(self.rstage), s)
+            elif g.kind == 'traction':
+                t = np.asarray(g.load.traction, dtype=np.float64)
+                call('fea_surface_traction', g.ntype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
+                     t.ctypes.data, float(load_scale), ptr(g.dst_r), ptr(self.rstage), None, s)
+
+    def tangent(self, U, load_scale=1.0, dtype=F64, with_residual=True):
+        """Tangent CSR values (dtype) and optionally the float64 residual.
+
+        Replaces _assemble_generalized_Matrix + _assemble_reduced_Matrix
+        (assembly.py:311-387).  With ``with_residual`` the residual comes from
+        the same element pass (dtype arithmetic); mixed precision calls
+        ``residual(..., F64)`` separately instead.
+        """
+        self._check_u(U)
+        s = self._s()
+        ks = self.kstage(dtype)
+        rs = self.rstage if with_residual else None
+        for g in self.groups:
+            if g.n == 0:
+                continue
+            if g.kind == 'elem':
+                call('fea_elem_residual_tangent', g.ntype, dtype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
+                     g.params.ctypes.data, ptr(g.dst_k), ptr(ks),
+                     ptr(g.dst_r) if with_residual else None, ptr(rs), s)
+            elif g.kind == 'pressure':
+                call('fea_surface_pressure', g.ntype, dtype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
+                     float(g.load.pressure) * load_scale, ptr(g.dst_k), ptr(ks),
+                     ptr(g.dst_r) if with_residual else None, ptr(rs), s)
+            elif g.kind == 'traction' and with_residual:
+                t = np.asarray(g.load.traction, dtype=np.float64)
+                call('fea_surface_traction', g.ntype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
+                     t.ctypes.data, float(load_scale), ptr(g.dst_r), ptr(self.rstage), None, s)
+        val = self.csr_values(dtype)
+        call('fea_gather_csr', dtype, self.n_blocks, ptr(self.blk_a), ptr(self.blk_b), ptr(self.blk_colpos),
+             ptr(self.kseg), ptr(self.dof_map), ptr(self.rowptr), ptr(ks), ptr(val), s)
+        R = None
+        if with_residual:
+            R = self.R
+            call('fea_gather_residual', self.n_nodes, ptr(self.rseg), ptr(self.dof_map), ptr(self.rstage),
+                 ptr(R), s)
+        return R, val
+
+    def energy_terms(self, U, load_scale=1.0):
+        """Per-group energy terms (device): elem -> strain energy, pressure -> V, traction -> f.u."""
+        self._check_u(U)
+        s = self._s()
+        for gi, g in enumerate(self.groups):
+            off, nb = self._epart_off[gi]
+            if g.n == 0:
+                continue
+            seg = self._epart[off:off + nb]
+            if g.kind == 'elem':
+                call('fea_elem_energy', g.ntype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
+                     g.params.ctypes.data, ptr(seg), s)
+            elif g.kind == 'pressure':
+                call('fea_surface_volume', g.ntype, g.n, ptr(g.conn), ptr(self.X), ptr(U), ptr(seg), s)
+            elif g.kind == 'traction':
+                t = np.asarray(g.load.traction, dtype=np.float64)
+                call('fea_surface_traction', g.ntype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
+                     t.ctypes.data, 1.0, None, None, ptr(seg), s)
+            call('fea_reduce_sum', nb, ptr(seg), ptr(self._eout[gi:gi + 1]), s)
+        return self._eout
+
+    def energy(self, U, load_scale=1.0):
+        """Total potential energy (_total_Potential_Energy, assembly.py:389-415), host float."""
+        terms = self.energy_terms(U, load_scale).cpu().numpy()
+        E = 0.0
+        for gi, g in enumerate(self.groups):
+            if g.n == 0:
+                continue
+            if g.kind == 'elem':
+                E += terms[gi]
+            elif g.kind == 'pressure':
+                E += float(g.load.pressure) * load_scale * terms[gi]     # - (-p V)
+            elif g.kind == 'traction':
+                E -= load_scale * terms[gi]                              # - f.u
+        return float(E)
+
+    def lumped_mass(self):
+        """Row-sum lumped mass on free DOFs (dynamic_explicit.py:147-166)."""
+        s = self._s()
+        self.rstage.zero_()
+        for g in self.groups:
+            if g.kind == 'elem' and g.n:
+                call('fea_elem_lumped_mass', g.ntype, g.n, ptr(g.conn), ptr(self.X), ptr(self.dof_map),
+                     g.rho, ptr(g.dst_r), ptr(self.rstage), s)
+        m = torch.zeros(max(1, self.n_free), dtype=torch.float64, device=self.device)[:self.n_free]
+        call('fea_gather_residual', self.n_nodes, ptr(self.rseg), ptr(self.dof_map), ptr(self.rstage), ptr(m), s)
+        return m
+
+    # ------------------------------------------------------------------ vectors
+    def dot(self, a, b):
+        """a.b of float64 vectors via the deterministic partial + final reduction (device scalar)."""
+        n = a.numel()
+        nb = nat.lib().fea_vec_blocks(n)
+        part = self._scratch('dot', nb)
+        out = self._scratch('dot_out', 1)
+        s = self._s()
+        call('fea_dot_partial', n, ptr(a), ptr(b), ptr(part), s)
+        call('fea_reduce_sum', nb, ptr(part), ptr(out), s)
+        return out
+
+    def _scratch(self, key, n):
+        if not hasattr(self, '_scr'):
+            self._scr = {}
+        t = self._scr.get(key)
+        if t is None or t.numel() < n:
+            t = torch.zeros(max(1, n), dtype=torch.float64, device=self.device)
+            self._scr[key] = t
+        return t[:n]
+
+    def dof_axpy(self, U0, alpha, d, out):
+        """out = U0 + alpha * d scattered through dof_map (GC0 + alpha dGC as nodal field)."""
+        call('fea_dof_axpy', self.n_nodes, ptr(self.dof_map), ptr(U0), float(alpha), ptr(d), ptr(out), self._s())
+        return out
+
+    def gc_to_u(self, GC, out=None):
+        U = torch.zeros(self.n_nodes * 3, dtype=torch.float64, device=self.device) if out is None else out
+        zero = torch.zeros_like(U)
+        return self.dof_axpy(zero, 1.0, GC, U)
+
+    def u_to_gc(self, U):
+        idx = torch.as_tensor(np.nonzero(self.free_mask)[0], device=self.device)
+        return U.reshape(-1)[idx].clone()
+
+    # ------------------------------------------------------------------ linear
+    def spmv(self, val, x, y, dtype):
+        call('fea_spmv', dtype, self.n_free, self.nnz, ptr(self.rowptr), ptr(self.col), ptr(val), ptr(x),
+             ptr(y), self._s())
+        return y
+
+    def csr_indices_coo(self):
+        """(row, col) int64 [2, nnz] of the CSR (COO view in CSR order)."""
+        rows = np.repeat(np.arange(self.n_free, dtype=np.int64), np.diff(self.rowptr_np))
+        return torch.as_tensor(np.stack([rows, self.col_np.astype(np.int64)]), device=self.device)

@@ -1,0 +1,235 @@
+"""Static implicit Newton solver (reference: solver/static_implicit.py:12-353).
+
+The Newton / Armijo line-search / Jacobi-CG structure of the reference is kept
+(same constants: c1 = 0.3, alpha halving to 1e-12, CG tolerance on the
+squared scaled residual, 1500 / 6000 CG iterations, warm start from the
+previous direction, the low-alpha bookkeeping).  All heavy work runs in HIP
+kernels on the device plan; the host only takes the branch decisions.
+
+Additions (keyword-only, defaults reproduce the reference):
+
+* ``precision='mixed'``: tangent assembly and the inner PCG in FP32 with an
+  FP64 iterative-refinement loop around the linear solve; the residual, the
+  energy, the convergence test and the J-terms stay FP64.
+* ``rel_tol``: converge on ||R||_2 <= rel_tol ||R_ref||_2 with R_ref the
+  residual of the full load at u = 0 (the external load norm); the reference
+  tests |R|_inf < tol_error absolutely (static_implicit.py:290).
+* ``load_steps``: ramp all loads in equal increments (the reference has no
+  increments; its harnesses loop the load by hand, test_multicase.py:64-72).
+* ``armijo_slack``: relative energy round-off allowance in the Armijo test, so
+  the line search does not reject full steps whose energy change is below
+  float64 resolution near convergence (0 reproduces the reference).
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+import torch
+
+from .. import _native as nat
+from .._native import F32, F64, call, ptr
+from ._linear_solver import PCG
+from .basesolver import BaseSolver
+
+
+class StaticImplicitSolver(BaseSolver):
+
+    def __init__(self, maximum_iteration: int = 10000, tol_error: float = 1e-5, *,
+                 precision: str = 'fp64', rel_tol: float | None = None, load_steps: int = 1,
+                 cg_tol: float = 1e-5, cg_max_iter: int = 1500, ir_steps: int = 1,
+                 armijo_slack: float = 0.0, verbose: bool = False, pcg_chunk: int = 32) -> None:
+        super().__init__()
+        if precision not in ('fp64', 'mixed'):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.maximum_iteration = maximum_iteration
+        self.tol_error = tol_error
+        self._iter_now = 0
+        self._maximum_step_length = 1e10
+        self._low_alpha_count = 0
+        self.precision = precision
+        self.rel_tol = rel_tol
+        self.load_steps = int(load_steps)
+        self.cg_tol = float(cg_tol)
+        self.cg_max_iter = int(cg_max_iter)
+        self.ir_steps = int(ir_steps)
+        self.armijo_slack = float(armijo_slack)
+        self.verbose = verbose
+        self.pcg_chunk = int(pcg_chunk)
+        self.stats = {}
+
+    def initialize(self, assembly, *args, **kwargs):
+        super().initialize(assembly)
+        self._pcg = {}
+
+    # ------------------------------------------------------------------ API
+    def solve(self, GC0=None, *args, **kwargs):
+        """static_implicit.py:44-76; returns the converged GC or False."""
+        t0 = time.time()
+        if GC0 is None:
+            GC0 = self.assembly.GC
+        result = self._solve_iteration(GC=GC0, tol_error=self.tol_error)
+        if isinstance(result, bool):
+            return result
+        self.GC = result
+        self.assembly.GC = self.GC
+        self.assembly.RGC = self.assembly.refine_RGC(self.assembly._GC2RGC(self.GC))
+        self.stats['time'] = time.time() - t0
+        if self.verbose:
+            print('total_iter:%d, total_time:%.2f' % (self._iter_now, self.stats['time']))
+        return self.GC
+
+    def get_total_energy(self, GC_now):
+        return self.assembly._total_Potential_Energy(GC=GC_now)
+
+    def get_stiffness_matrix(self, GC_now):
+        return self.assembly.assemble_Stiffness_Matrix(GC=GC_now)
+
+    # ------------------------------------------------------------------ internals
+    def _pcg_for(self, dtype):
+        if dtype not in self._pcg:
+            p = self.assembly.plan
+            self._pcg[dtype] = PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, dtype, p.device,
+                                   chunk=self.pcg_chunk)
+        return self._pcg[dtype]
+
+    def _vec_stats(self, x):
+        """(sum x^2, max|x|) of a float64 device vector, one host sync."""
+        p = self.assembly.plan
+        n = x.numel()
+        nb = nat.lib().fea_vec_blocks(n)
+        part = p._scratch('stats', 2 * nb)
+        call('fea_vec_stats', n, ptr(x), ptr(part), nat.stream_ptr())
+        h = part.view(nb, 2).cpu().numpy()
+        return float(h[:, 0].sum()), float(h[:, 1].max())
+
+    def _tangent(self, U, scale):
+        p = self.assembly.plan
+        if self.precision == 'mixed':
+            R = p.residual(U, scale, F64)
+            _, K = p.tangent(U, scale, F32, with_residual=False)
+        else:
+            R, K = p.tangent(U, scale, F64)
+        return R, K
+
+    def _solve_linear(self, K, R, dGC0, iter_now, alpha0):
+        """static_implicit.py:297-350 (CG branch; the Pardiso branch is replaced by PCG)."""
+        p = self.assembly.plan
+        if alpha0 < 1e-1:
+            self._low_alpha_count += 1
+        else:
+            self._low_alpha_count = 0
+        max_iter = 4 * self.cg_max_iter if (iter_now % 20 == 0 or self._low_alpha_count > 0) else self.cg_max_iter
+        s = nat.stream_ptr()
+        n = p.n_free
+        b = p._scratch('rhs', n)
+        b.zero_()
+        call('fea_axpy', n, -1.0, F64, ptr(R), ptr(b), s)              # b = -R
+        out = p._scratch('dgc', n)
+        if self.precision == 'fp64':
+            pcg = self._pcg_for(F64)
+            pcg.set_matrix(K)
+            x = pcg.solve(b, dGC0, tol=self.cg_tol, max_iter=max_iter)
+            out.copy_(x)
+            self.stats['cg_iters'] = self.stats.get('cg_iters', 0) + pcg.iterations
+        else:
+            pcg = self._pcg_for(F32)
+            pcg.set_matrix(K)
+            out.copy_(dGC0)
+            r = p._scratch('ir_r', n)
+            nb = nat.lib().fea_vec_blocks(n)
+            part = p._scratch('ir_part', nb)
+            for _ in range(max(1, self.ir_steps)):
+                call('fea_spmv_residual_mixed', n, ptr(p.rowptr), ptr(p.col), ptr(K), ptr(out), ptr(b),
+                     ptr(r), ptr(part), s)
+                d = pcg.solve(r, None, tol=self.cg_tol, max_iter=max_iter)
+                call('fea_axpy', n, 1.0, F32, ptr(d), ptr(out), s)
+                self.stats['cg_iters'] = self.stats.get('cg_iters', 0) + pcg.iterations
+        return out
+
+    def _line_search(self, U, dGC, R, energy0, scale, Unew):
+        """static_implicit.py:98-177: Armijo backtracking on the total energy."""
+        p = self.assembly.plan
+        alpha, c1 = 1.0, 0.3
+        dd = p.dot(dGC, R)
+        deltaE = float(dd.item())
+        sign = 1.0
+        if deltaE > 0:
+            sign, deltaE = -1.0, -deltaE
+        if not math.isfinite(deltaE):
+            raise ValueError('dGC has nan or inf')
+        slack = self.armijo_slack * abs(energy0)
+        while True:
+            p.dof_axpy(U, sign * alpha, dGC, Unew)
+            e_new = p.energy(Unew, scale)
+            self.stats['energy_evals'] = self.stats.get('energy_evals', 0) + 1
+            if (not math.isfinite(e_new)) or e_new > energy0 + c1 * deltaE * alpha + slack:
+                alpha *= 0.5
+                if alpha < 1e-12:
+                    return 0.0, energy0
+            else:
+                return alpha, e_new
+
+    def _newton(self, U, scale, tol_error):
+        p = self.assembly.plan
+        energy = p.energy(U, scale)
+        dGC = torch.zeros(p.n_free, dtype=torch.float64, device=p.device)
+        Unew = torch.empty_like(U)
+        low_alpha, alpha, it = 0, 0.0, 0
+        r0 = self._r_ref
+        while True:
+            if it > self.maximum_iteration:
+                return False
+            R, K = self._tangent(U, scale)
+            it += 1
+            self._iter_now += 1
+            rss, rmax = self._vec_stats(R)
+            self.stats['residual_history'].append(math.sqrt(rss))
+            if self.rel_tol is not None:
+                if r0 is None:
+                    r0 = math.sqrt(rss)
+                if math.sqrt(rss) <= self.rel_tol * r0:
+                    return True
+            d = self._solve_linear(K, R, dGC, it, alpha)
+            dGC.copy_(d)
+            alpha, e_new = self._line_search(U, dGC, R, energy, scale, Unew)
+            if alpha == 0:
+                if self.rel_tol is not None or rmax > tol_error:
+                    return False
+                return True
+            if alpha < 0.01:
+                low_alpha += 1
+            else:
+                low_alpha = max(0, low_alpha - 5)
+            if low_alpha > 10:
+                return False
+            U.copy_(Unew)
+            energy = e_new
+            if self.verbose:
+                print('iter %3d  alpha %.3g  energy %.10e  |R|inf %.3e' % (self._iter_now, alpha, energy, rmax))
+            if self.rel_tol is None:
+                _, dmax = self._vec_stats(dGC)
+                if (dmax < tol_error and rmax < tol_error) or rmax < 1e-6:
+                    return True
+
+    def _solve_iteration(self, GC, tol_error):
+        """static_implicit.py:179-295, once per load increment."""
+        p = self.assembly.plan
+        self._iter_now = 0
+        self.stats = {'cg_iters': 0, 'energy_evals': 0, 'residual_history': [], 'increments': 0}
+        U = p.gc_to_u(torch.as_tensor(GC, dtype=torch.float64, device=p.device).contiguous())
+        self._r_ref = None
+        if self.rel_tol is not None:
+            zero = torch.zeros_like(U)
+            rss, _ = self._vec_stats(p.residual(zero, 1.0, F64))
+            self._r_ref = math.sqrt(rss) if rss > 0 else None
+        for k in range(1, self.load_steps + 1):
+            ok = self._newton(U, k / self.load_steps, tol_error)
+            self.stats['increments'] = k
+            if not ok:
+                if self.verbose:
+                    print('load increment %d failed to converge' % k)
+                return False
+        self.stats['newton_iters'] = self._iter_now
+        return p.u_to_gc(U)

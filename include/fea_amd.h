@@ -1,0 +1,241 @@
+/*
+ * fea_amd — C ABI of the MI355X (gfx950) nonlinear solid-mechanics hot path.
+ *
+ * Conventions (every entry point):
+ *   - returns int status: FEA_OK (0) or a negative FEA_ERR_* code; the message
+ *     of the last failure on the calling thread is fea_last_error();
+ *   - arrays are DEVICE pointers unless the parameter name ends in `_host`;
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream);
+ *   - no entry point allocates device memory except fea_pattern_* (host) and
+ *     the explicit workspace helpers; kernels run asynchronously on `stream`.
+ *
+ * Layouts: nodal coordinates / displacements are AoS float64 [n_nodes][3]
+ * (row-major, the reference's RGC block layout, assembly.py:97-128);
+ * connectivity is int32 [n_elem][nen] in the reference node order
+ * (tetrahedral.py:6-22 C3D4, brick.py:13-22 C3D8, tetrahedral.py:66-86 C3D10).
+ * `dof_map[3*node+comp]` is the free-DOF index or -1 for a constrained DOF
+ * (the GC numbering of assembly.py:170-183 / _RGC2GC :494-499).
+ *
+ * The reference (sssnnn7119/FEA) is pure PyTorch; each function below names the
+ * reference Python function whose device work it replaces.
+ */
+#ifndef FEA_AMD_H
+#define FEA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FEA_OK 0
+#define FEA_ERR_ARG (-1)
+#define FEA_ERR_HIP (-2)
+#define FEA_ERR_UNSUPPORTED (-3)
+#define FEA_ERR_STATE (-4)
+
+/* precision of a kernel's arithmetic and of its floating outputs */
+#define FEA_F32 0
+#define FEA_F64 1
+
+/* element types = nodes per element */
+#define FEA_C3D4 4
+#define FEA_C3D8 8
+#define FEA_C3D10 10
+
+/* surface facet types = nodes per facet (T3, T6, Q4) */
+#define FEA_T3 3
+#define FEA_T6 6
+#define FEA_Q4 4
+
+/*
+ * Material: isochoric two-invariant hyperelasticity with quadratic volumetric part
+ *   W = w1 (I1bar - 3) + w2 (I2bar - 3) + kv/2 (J - 1)^2,  params = {w1, w2, kv}.
+ * NeoHookean(mu, kappa) of hyperelastic.py:6-84 is {mu/2, 0, kappa};
+ * Mooney-Rivlin(C10, C01, D1) (absent in the reference) is {C10, C01, 2/D1}.
+ */
+
+const char *fea_last_error(void);
+int fea_version(void);
+
+/* ---------------------------------------------------------------- pattern --
+ * Host-side integer work (replaces Element_3D.initialize's COO pattern and
+ * coalesce map, C3base.py:109-153, and the free-DOF compaction + COO->CSR
+ * coalesce of assembly.py:371-380 / _linear_solver.py:18-19).
+ *
+ * Groups are element groups or surface-facet groups; all connectivity uses
+ * global node ids.  For every group with in_k != 0 its (e,a,b) node-pair
+ * blocks enter the CSR sparsity; every group gets residual staging slots.
+ * The result is the free-DOF CSR (rows sorted, columns ascending, duplicates
+ * merged) plus the staging maps the assembly kernels write through.
+ */
+typedef struct fea_pattern fea_pattern;
+
+int fea_pattern_create(int n_nodes, const int32_t *dof_map_host, int n_groups,
+                       const int32_t *const *conn_host, const int32_t *n_elem_host,
+                       const int32_t *nen_host, const int32_t *in_k_host,
+                       fea_pattern **out);
+/* sizes[0]=n_free, [1]=nnz, [2]=n_blocks, [3]=n_kstage (blocks of 9),
+ * [4]=n_rstage (blocks of 3) */
+int fea_pattern_sizes(const fea_pattern *p, int64_t sizes[5]);
+/* host outputs; arrays sized from fea_pattern_sizes:
+ *   rowptr[n_free+1], col[nnz], diag_slot[n_free],
+ *   blk_a[n_blocks], blk_b[n_blocks], blk_colpos[n_blocks], kseg[n_blocks+1],
+ *   rseg[n_nodes+1] */
+int fea_pattern_export_csr(const fea_pattern *p, int32_t *rowptr_host, int32_t *col_host,
+                           int32_t *diag_slot_host);
+int fea_pattern_export_blocks(const fea_pattern *p, int32_t *blk_a_host, int32_t *blk_b_host,
+                              int32_t *blk_colpos_host, int32_t *kseg_host, int32_t *rseg_host);
+/* group maps: dst_k[n_elem*nen*nen] (only for in_k groups; -1 = block with no
+ * free x free entry), dst_r[n_elem*nen] (-1 = node with no free DOF) */
+int fea_pattern_export_group(const fea_pattern *p, int group, int32_t *dst_k_host,
+                             int32_t *dst_r_host);
+void fea_pattern_destroy(fea_pattern *p);
+
+/* ---------------------------------------------------------------- elements --
+ * fea_elem_residual_tangent replaces Element_3D.structural_Force
+ * (C3base.py:413-452) + components_Solid (:454-480) +
+ * NeoHookean.material_Constitutive_C3 (hyperelastic.py:35-84):
+ * per element and Gauss point it forms F = I + grad u from X and U, the
+ * stress and consistent tangent, and writes the 3x3 blocks K_ab (dtype) to
+ * kstage[dst_k[..]*9 ..] and the nodal forces R_a (float64) to
+ * rstage[dst_r[..]*3 ..].  kstage/dst_k may be NULL (residual only);
+ * rstage/dst_r may be NULL (tangent only).  Gauss-point geometry (dN/dX,
+ * w*detJ, C3base.py:155-216) is recomputed from X in registers.
+ */
+int fea_elem_residual_tangent(int elem_type, int dtype, int n_elem, const int32_t *conn,
+                              const double *X, const double *U, const double *params_host,
+                              const int32_t *dst_k, void *kstage, const int32_t *dst_r,
+                              double *rstage, void *stream);
+
+/* Residual only (force branch of structural_Force, C3base.py:428-431), float64 or
+ * float32 arithmetic, float64 output into rstage. */
+int fea_elem_residual(int elem_type, int dtype, int n_elem, const int32_t *conn,
+                      const double *X, const double *U, const double *params_host,
+                      const int32_t *dst_r, double *rstage, void *stream);
+
+/* Strain energy per element block (Element_3D.potential_Energy,
+ * C3base.py:387-411): writes one float64 partial sum per 256-element block to
+ * partial[0 .. ceil(n_elem/256)). */
+int fea_elem_energy(int elem_type, int n_elem, const int32_t *conn, const double *X,
+                    const double *U, const double *params_host, double *partial, void *stream);
+int fea_elem_energy_blocks(int n_elem);
+
+/* Gauss-point geometry export (Element_3D._pre_load_gaussian, C3base.py:155-216):
+ * dNdX [g][n_elem][3][nen], wdet [g][n_elem], N [g][n_elem][nen] (float64). */
+int fea_elem_prepare(int elem_type, int n_elem, const int32_t *conn, const double *X,
+                     double *dNdX, double *wdet, double *N, void *stream);
+
+/* Row-sum lumped mass restricted to free DOFs (Element_3D.get_mass_matrix
+ * C3base.py:348-384 -> Assembly.assemble_mass_matrix assembly.py:421-443 ->
+ * DynamicExplicitSolver.get_lumped_mass_inv dynamic_explicit.py:147-166):
+ * writes per (e,a) the 3 component masses sum_b(free) M_ab into rstage. */
+int fea_elem_lumped_mass(int elem_type, int n_elem, const int32_t *conn, const double *X,
+                         const int32_t *dof_map, double rho, const int32_t *dst_r,
+                         double *rstage, void *stream);
+
+/* ---------------------------------------------------------------- surfaces --
+ * Follower pressure on T3/T6/Q4 facets (Pressure.get_stiffness,
+ * pressure.py:80-132, with the assembly's load sign, assembly.py:337-341):
+ * writes scale*dV/dx to rstage and scale*d2V/dx2 blocks to kstage, where
+ * V = 1/3 sum_g w_g det[x, x_xi, x_eta] and scale = +pressure. */
+int fea_surface_pressure(int surf_type, int dtype, int n_facets, const int32_t *conn,
+                         const double *X, const double *U, double scale,
+                         const int32_t *dst_k, void *kstage, const int32_t *dst_r,
+                         double *rstage, void *stream);
+/* Enclosed-volume partials V per 256-facet block (pressure.py:134-159). */
+int fea_surface_volume(int surf_type, int n_facets, const int32_t *conn, const double *X,
+                       const double *U, double *partial, void *stream);
+/* Dead-load traction (new; no reference counterpart): writes -scale * f_a with
+ * f_a = sum_g N_a t w_g detJ0 into rstage (the residual sign of a load force,
+ * assembly.py:337-341); `work_partial` (optional) receives per-block sums of
+ * f . u for the load potential. */
+int fea_surface_traction(int surf_type, int n_facets, const int32_t *conn, const double *X,
+                         const double *U, const double *traction_host, double scale,
+                         const int32_t *dst_r, double *rstage, double *work_partial,
+                         void *stream);
+int fea_surface_blocks(int n_facets);
+
+/* ---------------------------------------------------------------- assembly --
+ * Deterministic, atomic-free gathers replacing the scatter_add calls of
+ * C3base.py:449 and assembly.py:347-354 plus the reduction of :371-384. */
+int fea_gather_csr(int dtype, int n_blocks, const int32_t *blk_a, const int32_t *blk_b,
+                   const int32_t *blk_colpos, const int32_t *kseg, const int32_t *dof_map,
+                   const int32_t *rowptr, const void *kstage, void *csr_val, void *stream);
+/* R[dof] = sum of the node's staged contributions, for free DOFs */
+int fea_gather_residual(int n_nodes, const int32_t *rseg, const int32_t *dof_map,
+                        const double *rstage, double *R, void *stream);
+
+/* Deterministic sum of n float64 partials into out[0] (single workgroup). */
+int fea_reduce_sum(int n, const double *partial, double *out, void *stream);
+
+/* ---------------------------------------------------------------- linear algebra --
+ * CSR SpMV y = A x (the CSR `A @ p` of _linear_solver.py:27,32). */
+int fea_spmv(int dtype, int n, int64_t nnz, const int32_t *rowptr, const int32_t *col,
+             const void *val, const void *x, void *y, void *stream);
+/* Jacobi diagonal of static_implicit.py:314-317: dinv[r] = 1/|A_rr| (1 if 0). */
+int fea_jacobi_inv(int dtype, int n, const int32_t *diag_slot, const void *val, void *dinv,
+                   void *stream);
+
+/* Jacobi-preconditioned CG, the conjugate_gradient of _linear_solver.py:4-42 on
+ * the symmetrically scaled system of static_implicit.py:314-321 (identical
+ * iterates), with all scalars on the device.
+ * ws: float64 workspace of fea_pcg_ws_size(n) doubles, zero-filled once before
+ * its first use (the kernels leave its reduction counter at zero); vectors (dtype) x, b, r,
+ * z, p, q of length n.  fea_pcg_start sets r = b - A x, z = dinv r, p = z;
+ * fea_pcg_iterate runs `iters` iterations (each a no-op once converged:
+ * r.z / b.(dinv b) < tol, or max_iter reached) and copies {iterations, done,
+ * rz, bz} to status_host (pinned or pageable host memory, synchronises). */
+int64_t fea_pcg_ws_size(int n);
+int fea_pcg_start(int dtype, int n, int64_t nnz, const int32_t *rowptr, const int32_t *col, const void *val,
+                  const void *dinv, const void *b, void *x, void *r, void *z, void *p,
+                  double *ws, void *stream);
+int fea_pcg_iterate(int dtype, int n, int64_t nnz, const int32_t *rowptr, const int32_t *col,
+                    const void *val, const void *dinv, void *x, void *r, void *z, void *p,
+                    void *q, double *ws, double tol, int max_iter, int iters,
+                    double *status_host, void *stream);
+
+/* One SpMV stage of the PCG iteration alone: q = A p and p.q -> alpha in ws
+ * (the CSR `A @ p` of _linear_solver.py:32 fused with the dot of :33); a no-op
+ * when ws marks the solve converged. */
+int fea_pcg_spmv(int dtype, int n, int64_t nnz, const int32_t *rowptr, const int32_t *col,
+                 const void *val, const void *p, void *q, double *ws, void *stream);
+
+/* FP64 residual of an FP32-stored matrix: r64 = b64 - A32 x64 (iterative
+ * refinement outer loop), plus r64 . r64 into `partial` (fea_vec_blocks(n)
+ * per-block sums). */
+int fea_spmv_residual_mixed(int n, const int32_t *rowptr, const int32_t *col, const float *val,
+                            const double *x, const double *b, double *r, double *partial,
+                            void *stream);
+int fea_vec_blocks(int n);
+
+/* Vector helpers on dof-mapped state.
+ * fea_dof_axpy: U[node][c] = U0[node][c] + alpha * d[dof_map[3 node + c]] for
+ * free DOFs, U[..] = U0[..] otherwise (_GC2RGC of assembly.py:470-492 fused
+ * with the line-search update GC0 + alpha dGC, static_implicit.py:126). */
+int fea_dof_axpy(int n_nodes, const int32_t *dof_map, const double *U0, double alpha,
+                 const double *d, double *U, void *stream);
+/* y(dtype_y) = x(dtype_x) elementwise conversion */
+int fea_convert(int n, int dtype_x, const void *x, int dtype_y, void *y, void *stream);
+/* y(float64) += alpha * x(dtype_x) */
+int fea_axpy(int n, double alpha, int dtype_x, const void *x, double *y, void *stream);
+/* per block (sum x^2, max |x|) pairs into partial[2*fea_vec_blocks(n)]: the
+ * norms of the residual used by the Newton convergence tests
+ * (static_implicit.py:231, :290). */
+int fea_vec_stats(int n, const double *x, double *partial, void *stream);
+/* partial[blk] = sum a.b (float64 vectors), fea_vec_blocks(n) partials */
+int fea_dot_partial(int n, const double *a, const double *b, double *partial, void *stream);
+
+/* ---------------------------------------------------------------- explicit --
+ * Fused force gather + central-difference update (the loop body of
+ * DynamicExplicitSolver.solve, dynamic_explicit.py:100-114): per free DOF
+ * R = sum of staged forces, a = -minv R, vh += dt_v a, U += dt_u vh (nodal).
+ * The first step uses dt_v = dt/2 (v_{1/2} = v_0 + dt/2 a_0), later ones dt_v = dt. */
+int fea_explicit_update(int n_nodes, const int32_t *rseg, const int32_t *dof_map,
+                        const double *rstage, const double *minv, double dt_v, double dt_u,
+                        double *a, double *vh, double *U, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEA_AMD_H */

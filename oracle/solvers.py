@@ -1,0 +1,182 @@
+"""Linear / Newton / explicit solvers restated from the reference (FP64, CPU).
+
+Test infrastructure only (see oracle/__init__.py).
+"""
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from . import assembly as asm
+
+
+def conjugate_gradient(A, b, x0=None, tol=1e-3, max_iter=1500):
+    """conjugate_gradient (_linear_solver.py:4-42); tol is on r.r / b.b (:37)."""
+    x = np.zeros_like(b) if x0 is None else x0.copy()
+    r_r0 = b @ b
+    r = b - A @ x
+    p = r.copy()
+    rsold = r @ r
+    it = 0
+    for it in range(max_iter):
+        Ap = A @ p
+        alpha = rsold / (p @ Ap)
+        x = x + alpha * p
+        r = r - alpha * Ap
+        rsnew = r @ r
+        if rsnew / r_r0 < tol:
+            break
+        p = r + rsnew / rsold * p
+        rsold = rsnew
+    return x, it + 1
+
+
+def jacobi_scaled_solve(K, rhs, x0, linear='cg', tol=1e-5, max_iter=1500):
+    """_solve_linear_equation (static_implicit.py:297-350).
+
+    diag = sqrt|K_ii| (0 -> 1); solve D^-1/2 K D^-1/2 y = D^-1/2 rhs; x = y / diag.
+    ``linear='direct'`` stands in for the reference's Pardiso branch
+    (scipy spsolve; pypardiso is not installed).
+    """
+    diag = np.sqrt(np.abs(K.diagonal()))
+    diag[diag == 0] = 1.0
+    Dinv = sp.diags(1.0 / diag)
+    Ks = (Dinv @ K @ Dinv).tocsr()
+    bs = rhs / diag
+    if linear == 'direct':
+        y = spla.spsolve(Ks.tocsc(), bs)
+        its = 0
+    else:
+        y, its = conjugate_gradient(Ks, bs, x0 * diag, tol=tol, max_iter=max_iter)
+    return y / diag, its
+
+
+def line_search(model, GC0, dGC, R, energy0, armijo_slack=0.0):
+    """_line_search (static_implicit.py:98-177): Armijo c1=0.3, alpha halving to 1e-12.
+
+    ``armijo_slack`` (absolute energy) is 0 in the reference.
+    """
+    alpha, c1 = 1.0, 0.3
+    deltaE = float(dGC @ R)
+    if deltaE > 0:
+        dGC, deltaE = -dGC, -deltaE
+    while True:
+        GCnew = GC0 + alpha * dGC
+        e_new = asm.total_energy(model, asm.gc_to_u(model, GCnew))
+        if (not np.isfinite(e_new)) or e_new > energy0 + c1 * deltaE * alpha + armijo_slack:
+            alpha *= 0.5
+            if alpha < 1e-12:
+                return 0.0, GC0.copy(), energy0
+        else:
+            return alpha, GCnew, e_new
+
+
+def scale_loads(model, factor):
+    """Copy of the model with pressure and traction magnitudes scaled (load increments)."""
+    m = dict(model)
+    m['pressures'] = [dict(p, p=p['p'] * factor) for p in model.get('pressures', [])]
+    m['tractions'] = []
+    for t in model.get('tractions', []):
+        asm.prepare(model)
+        m['tractions'].append(dict(t, t=tuple(np.asarray(t['t']) * factor), f=t['f'] * factor))
+    return m
+
+
+def newton(model, GC, tol_error=1e-5, rel_tol=None, max_iter=200, linear='cg',
+           cg_tol=1e-5, armijo_slack_rel=0.0, r_ref=None):
+    """_solve_iteration (static_implicit.py:179-295).
+
+    Reference convergence: (|dGC|inf < tol and |R|inf < tol) or |R|inf < 1e-6 (:290).
+    With ``rel_tol`` the test is ||R||2 <= rel_tol * r_ref instead (r_ref = the
+    load's residual norm at u = 0, default: first residual of this call).
+    Returns (GC or False, iterations).
+    """
+    energy = asm.total_energy(model, asm.gc_to_u(model, GC))
+    dGC = np.zeros_like(GC)
+    alpha, it, r0 = 0.0, 0, None
+    while True:
+        if it > max_iter:
+            return False, it
+        R, K = asm.assemble_reduced(model, asm.gc_to_u(model, GC))
+        it += 1
+        nR = np.linalg.norm(R)
+        if r0 is None:
+            r0 = nR if r_ref is None else r_ref
+        if rel_tol is not None and nR <= rel_tol * r0:
+            return GC, it
+        dGC, _ = jacobi_scaled_solve(K, -R, dGC, linear=linear, tol=cg_tol,
+                                     max_iter=6000 if it % 20 == 0 else 1500)
+        alpha, GCnew, e_new = line_search(model, GC, dGC, R, energy,
+                                          armijo_slack=armijo_slack_rel * abs(energy))
+        if alpha == 0 and np.abs(R).max() > tol_error and rel_tol is None:
+            return False, it
+        if alpha == 0:
+            if rel_tol is not None:
+                return False, it
+            break
+        GC = GCnew
+        energy = asm.total_energy(model, asm.gc_to_u(model, GC))
+        if rel_tol is None and ((np.abs(dGC).max() < tol_error and np.abs(R).max() < tol_error)
+                                or np.abs(R).max() < 1e-6):
+            break
+    return GC, it
+
+
+def static_solve(model, increments=1, **kw):
+    """Load-incremented static solve: Newton per increment from the previous state."""
+    free = asm.free_mask(model)
+    GC = np.zeros(int(free.sum()))
+    total = 0
+    kw.setdefault('r_ref', float(np.linalg.norm(asm.assemble_force(model, np.zeros_like(model['nodes'])))))
+    for k in range(1, increments + 1):
+        GC, its = newton(scale_loads(model, k / increments), GC, **kw)
+        total += its
+        if GC is False:
+            return False, total
+    return GC, total
+
+
+# --- explicit dynamics ---------------------------------------------------------
+
+def lumped_mass(model):
+    """Row-sum lumped mass on free DOFs (dynamic_explicit.py:147-166, C3base.py:348-384)."""
+    from . import elements as el
+    asm.prepare(model)
+    N = model['nodes'].shape[0]
+    M = np.zeros(3 * N)
+    for g in model['groups']:
+        vals = el.mass_values(g['N'], g['wdet'], g.get('rho', 1.0))
+        r, _ = el.coo_indices(g['conn'])
+        np.add.at(M, r, vals)
+    free = asm.free_mask(model)
+    m = M[free]
+    m[np.abs(m) < 1e-12] = 1.0
+    return m
+
+
+def explicit_steps(model, GC0, GV0, dt, nsteps, extra_force=None):
+    """DynamicExplicitSolver.solve inner loop (dynamic_explicit.py:80-134).
+
+    ``extra_force(u_nodes) -> [3N]`` adds an external force (e.g. contact).
+    Returns (GC, GV, GA) after nsteps.
+    """
+    free = asm.free_mask(model)
+    minv = 1.0 / lumped_mass(model)
+
+    def accel(GC):
+        u = asm.gc_to_u(model, GC)
+        R = asm.assemble_force(model, u)
+        if extra_force is not None:
+            R = R - extra_force(u)[free]
+        return minv * (-R)
+
+    GA = accel(GC0)
+    GVh = GV0 - 0.5 * dt * GA
+    GC = GC0.copy()
+    GV = GV0.copy()
+    for _ in range(nsteps):
+        GVh_new = GVh + dt * GA
+        GC = GC + dt * GVh_new
+        GA = accel(GC)
+        GV = 0.5 * (GVh_new + GVh)
+        GVh = GVh_new
+    return GC, GV, GA

@@ -1,0 +1,103 @@
+"""GPU solver parity: Jacobi-PCG vs a direct solve, static Newton (FP64 and
+mixed FP32/FP64-IR) vs the oracle's Newton with a direct linear solver
+(converged displacement within 1e-8 relative), explicit central difference
+vs the oracle's restatement of dynamic_explicit.py."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+import torch
+
+from fea_amd import solver, workloads
+from fea_amd._native import F64
+from oracle import assembly as oasm
+from oracle import solvers as osol
+from tests.helpers import mr, nh, oracle_model, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def cube_models(n=3, p=0.2):
+    fe, mesh = workloads.cube_pressure(n, pressure=p, jitter=0.05, seed=2)
+    om = oracle_model(mesh, nh(1.0, 100.0), 'z0', pressure=p, load_set='z1')
+    return fe, om
+
+
+def test_pcg_matches_direct_solve(gpu):
+    fe, om = cube_models()
+    fe.assembly.initialize()
+    asm = fe.assembly
+    p = asm.plan
+    U = p.gc_to_u(torch.as_tensor(0.01 * np.random.default_rng(0).standard_normal(p.n_free), device='cuda'))
+    R, K = p.tangent(U, 1.0, F64)
+    A = sp.csr_matrix((K.cpu().numpy(), p.col_np, p.rowptr_np), shape=(p.n_free, p.n_free))
+    b = -R.cpu().numpy()
+    x_ref = spla.spsolve(A.tocsc(), b)
+    pcg = solver.PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, F64, p.device)
+    pcg.set_matrix(K)
+    x = pcg.solve(-R, None, tol=1e-24, max_iter=5000).cpu().numpy()
+    assert rel_err(x, x_ref) < 1e-8
+    assert pcg.iterations < 5000
+    # same iteration count as the oracle's reference CG on the scaled system
+    D = np.sqrt(np.abs(A.diagonal()))
+    As = sp.diags(1 / D) @ A @ sp.diags(1 / D)
+    _, its = osol.conjugate_gradient(As, b / D, np.zeros_like(b), tol=1e-10, max_iter=5000)
+    pcg.solve(-R, None, tol=1e-10, max_iter=5000)
+    assert abs(pcg.iterations - its) <= 2
+
+
+def test_reference_cg_signature(gpu):
+    A = sp.random(60, 60, density=0.1, random_state=1)
+    A = (A @ A.T + 60 * sp.eye(60)).tocoo()
+    b = np.random.default_rng(1).standard_normal(60)
+    x = solver.conjugate_gradient(torch.as_tensor(np.stack([A.row, A.col])), torch.as_tensor(A.data),
+                                  torch.as_tensor(b), tol=1e-24, max_iter=500)
+    assert rel_err(x.cpu().numpy(), np.linalg.solve(A.toarray(), b)) < 1e-9
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_static_cube_matches_oracle(gpu, precision):
+    fe, om = cube_models(3, 0.2)
+    fe.solver = solver.StaticImplicitSolver(precision=precision, rel_tol=1e-10, load_steps=5, cg_tol=1e-12,
+                                            armijo_slack=1e-12)
+    GC = fe.solve()
+    assert GC is not False
+    GC_o, _ = osol.static_solve(om, increments=5, rel_tol=1e-10, linear='direct', armijo_slack_rel=1e-12)
+    assert GC_o is not False
+    assert rel_err(GC.cpu().numpy(), GC_o) < 1e-8
+
+
+def test_static_beam_mixed_matches_fp64(gpu):
+    res = {}
+    for prec in ('fp64', 'mixed'):
+        fe, mesh = workloads.beam_traction(8, 2, 2, traction=(0.0, 0.0, -0.01))
+        fe.solver = solver.StaticImplicitSolver(precision=prec, rel_tol=1e-10, load_steps=4, cg_tol=1e-12,
+                                                armijo_slack=1e-12)
+        res[prec] = fe.solve().cpu().numpy()
+    om = oracle_model(mesh, mr(0.5, 0.1, 0.01), 'x0', traction=(0.0, 0.0, -0.01), load_set='x1')
+    GC_o, _ = osol.static_solve(om, increments=4, rel_tol=1e-10, linear='direct', armijo_slack_rel=1e-12)
+    assert rel_err(res['fp64'], GC_o) < 1e-8
+    assert rel_err(res['mixed'], GC_o) < 1e-8
+
+
+def test_reference_convergence_rule(gpu):
+    # default constructor: absolute max-norm test of static_implicit.py:290
+    fe, om = cube_models(2, 0.05)
+    fe.solver = solver.StaticImplicitSolver()
+    GC = fe.solve()
+    assert GC is not False
+    R = fe.assembly.assemble_force(GC=GC)
+    assert float(R.abs().max()) < 1e-5
+
+
+def test_explicit_matches_oracle(gpu):
+    fe, om = cube_models(2, 0.05)
+    dt, n = 2e-3, 25
+    fe.solver = solver.DynamicExplicitSolver(time_end=dt * n, time_per_storage=1.0, dt=dt)
+    fe.initialize()
+    GC0 = torch.zeros_like(fe.assembly.GC)
+    fe.solver.solve(GC0=GC0)
+    GC_o, GV_o, GA_o = osol.explicit_steps(om, np.zeros(GC0.shape[0]), np.zeros(GC0.shape[0]), dt, n)
+    assert rel_err(fe.solver.GC.cpu().numpy(), GC_o) < 1e-10
+    m = fe.assembly.plan.lumped_mass().cpu().numpy()
+    assert rel_err(m, osol.lumped_mass(om)) < 1e-13
