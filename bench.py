@@ -1,7 +1,7 @@
 #!/usr/bin/env python
 """Benchmark: mixed-precision static Newton solves/s on one MI355X.
 
-Workload (default): cantilever beam 4x1x1 meshed as 128x32x32 C3D8 (131072
+Workload default: cantilever beam 4x1x1 meshed as 128x32x32 C3D8 (131072
 elements, 8 Gauss points each, 418176 free DOF), Mooney-Rivlin C10=0.5 C01=0.1
 D1=0.01, clamped at x=0, uniform dead traction (0,0,-0.01) on x=4 ramped over 10
 increments; Newton to ||R|| <= 1e-10 ||f_ext|| with FP32 element tangents and
@@ -71,12 +71,12 @@ def spmv_roofline(fe, torch, reps):
     ws = torch.zeros(int(nat.lib().fea_pcg_ws_size(p.n_free)), dtype=torch.float64, device='cuda')
     s = nat.stream_ptr()
     for _ in range(5):
-        call('fea_pcg_spmv', F32, p.n_free, p.nnz, ptr(p.rowptr), ptr(p.col), ptr(val), ptr(x), ptr(q), ptr(ws), s)
+        call('fea_pcg_spmv', F32, p.n_free, p.nnz, p.rowptr, p.col, val, x, q, ws, s)
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(reps):
-        call('fea_pcg_spmv', F32, p.n_free, p.nnz, ptr(p.rowptr), ptr(p.col), ptr(val), ptr(x), ptr(q), ptr(ws), s)
+        call('fea_pcg_spmv', F32, p.n_free, p.nnz, p.rowptr, p.col, val, x, q, ws, s)
     e1.record(stream)
     e1.synchronize()
     t = e0.elapsed_time(e1) / 1e3 / reps
@@ -139,7 +139,7 @@ def cpu_baseline(args, plan, stats):
     t_solve = n_newton * t_asm + stats['cg_iters'] * t_cg + stats['energy_evals'] * t_energy
     return {'value': 1.0 / t_solve, 'unit': 'solves/s', 'cores': threads, 'kind': 'port',
             'sample': ('FP64 numpy oracle: tangent assembly + free-DOF reduction of a %dx%dx%d C3D8 slice '
-                       '(scaled x%d), energy of the slice (scaled), 20 oracle-CG iterations on the full '
+                       '(scaled x%d), energy of the slice scaled, 20 oracle-CG iterations on the full '
                        '418k-DOF CSR structure; extrapolated with the GPU solve\'s counts (%d Newton, %d CG, '
                        '%d energy evals): %.1f s per solve'
                        % (args.nx, args.ny, nzs, args.nz // nzs, n_newton, stats['cg_iters'],
@@ -214,7 +214,7 @@ def main():
         'newton_iters_per_solve': stats.get('newton_iters'), 'cg_iters_per_solve': stats.get('cg_iters'),
         'energy_evals_per_solve': stats.get('energy_evals'),
         'elem_tangent_evals_per_s': elem_rate, 'tangent_assembly_ms': 1e3 * t_tan,
-        'roofline': {'bound': 'hbm', 'kernel': 'k_pcg_spmv<float,32> (fea_pcg_spmv)',
+        'roofline': {'bound': 'hbm', 'kernel': 'k_pcg_spmv<float,32> fea_pcg_spmv',
                      'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
                      'alg_bytes_per_launch': alg_bytes, 'avg_launch_us': t_spmv * 1e6},

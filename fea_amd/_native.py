@@ -7,6 +7,7 @@ FeaNativeError is raised.
 import ctypes
 import os
 
+import numpy as np
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -91,9 +92,22 @@ def check(rc, what=''):
         raise FeaNativeError('%s failed (status %d): %s' % (what, rc, msg.decode() if msg else ''))
 
 
+def _arg(a):
+    if isinstance(a, torch.Tensor):
+        return ctypes.c_void_p(a.data_ptr())
+    if isinstance(a, np.ndarray):
+        return ctypes.c_void_p(a.ctypes.data)
+    return a
+
+
 def call(name, *args):
-    """Call a status-returning C-ABI function and raise on error."""
-    rc = getattr(lib(), name)(*args)
+    """Call a status-returning C-ABI function and raise on error.
+
+    Tensors and numpy arrays are passed by address; ``args`` keeps every one of
+    them referenced until the call returns (and the kernels are enqueued), so a
+    temporary cannot be freed and its memory reused before the launch.
+    """
+    rc = getattr(lib(), name)(*[_arg(a) for a in args])
     check(rc, name)
 
 

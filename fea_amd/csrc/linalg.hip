@@ -98,41 +98,42 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int32_t *__restric
 }
 
 // ------------------------------------------------------------------ PCG
-// ws layout (float64): see the W_* offsets; partials live at ws + W_PART.
+// Device-resident Jacobi-PCG without grid-wide fences.  Every reduction
+// writes one partial per workgroup; the NEXT kernel (kernel boundary =>
+// visibility) sums the partials redundantly in every workgroup, in the same
+// fixed order, so all workgroups derive identical scalars (deterministic, no
+// atomics, no L2 writeback).  Scalars carried across iterations live in
+// partial rings indexed by the iteration parity.
+//
+// ws layout (float64):
+//   W_ITER  iteration counter (written only by workgroup 0 of k_pcg_spmv)
+//   W_DONE  converged flag     (written only by workgroup 0 of k_pcg_dir)
+//   W_RZ, W_BZ  last r.z and b.(dinv b) for the host status
+//   W_PQ .. W_PQ+G      p.q partials
+//   W_RZP + (i&1)*G ..  r.z partials of iteration i (ring of 2)
+//   W_BZP ..            b.(dinv b) partials
 enum : int {
-  W_RZ = 0, W_BZ = 1, W_PQ = 2, W_ALPHA = 3, W_BETA = 4, W_ITER = 5, W_DONE = 6,
-  W_COUNTER = 8,  // reinterpreted as unsigned int
-  W_PART = 16
+  PCG_G = 512,  // workgroups of every PCG kernel (partials per reduction)
+  W_ITER = 0, W_DONE = 1, W_RZ = 2, W_BZ = 3,
+  W_PQ = 16, W_RZP = W_PQ + PCG_G, W_BZP = W_RZP + 2 * PCG_G, W_END = W_BZP + PCG_G
 };
 
-// Block partial -> last-block fixed-order final sum of `nv` values per block.
-// Returns true in the block that performed the final sum (all threads).
-template <int NV>
-__device__ __forceinline__ bool last_block_sum(double (&v)[NV], double *ws, double (&out)[NV]) {
-  __shared__ bool am_last;
-  double *part = ws + W_PART;
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    const double s = block_sum(v[i]);
-    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * NV + i] = s;
-  }
-  __threadfence();
-  if (threadIdx.x == 0) {
-    unsigned int *ctr = reinterpret_cast<unsigned int *>(ws + W_COUNTER);
-    am_last = (atomicAdd(ctr, 1u) == gridDim.x - 1);
-  }
+// sum of g partials, identical in every workgroup (all threads get the result)
+__device__ __forceinline__ double sum_partials(const double *__restrict__ part, int g) {
+  __shared__ double res;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < g; b += blockDim.x) acc += part[b];
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) res = s;
   __syncthreads();
-  if (!am_last) return false;
-  __threadfence();
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-      acc += part[(int64_t)b * NV + i];
-    out[i] = block_sum(acc);
-  }
-  if (threadIdx.x == 0) *reinterpret_cast<unsigned int *>(ws + W_COUNTER) = 0u;
-  return true;
+  const double out = res;
+  __syncthreads();
+  return out;
+}
+
+__device__ __forceinline__ void write_partial(double *part, double v) {
+  const double s = block_sum(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 // r = b - A x
@@ -149,39 +150,39 @@ __global__ __launch_bounds__(kBlock) void k_spmv_resid(int n, const int32_t *__r
   if (lane == 0) r[row] = b[row] - v;
 }
 
+// z = dinv r, p = z; partials of r.z (ring slot 0) and b.(dinv b)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(int n, const T *__restrict__ dinv,
                                                      const T *__restrict__ b,
                                                      const T *__restrict__ r, T *__restrict__ z,
-                                                     T *__restrict__ p, double *ws) {
-  double v[2] = {0.0, 0.0};
+                                                     T *__restrict__ p, double *__restrict__ ws) {
+  double rz = 0.0, bz = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const T zi = dinv[i] * r[i];
     z[i] = zi;
     p[i] = zi;
-    v[0] += (double)r[i] * (double)zi;
-    v[1] += (double)b[i] * (double)dinv[i] * (double)b[i];
+    rz += (double)r[i] * (double)zi;
+    bz += (double)b[i] * (double)dinv[i] * (double)b[i];
   }
-  double out[2];
-  if (last_block_sum<2>(v, ws, out) && threadIdx.x == 0) {
-    ws[W_RZ] = out[0];
-    ws[W_BZ] = out[1];
+  write_partial(ws + W_RZP, rz);
+  write_partial(ws + W_BZP, bz);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     ws[W_ITER] = 0.0;
-    ws[W_DONE] = (out[1] == 0.0) ? 1.0 : 0.0;
+    ws[W_DONE] = 0.0;
   }
 }
 
-// q = A p, pq = p.q, alpha = rz / pq
+// iteration i = ITER+1: q = A p, partials of p.q; workgroup 0 advances ITER
 template <typename T, int SW>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int32_t *__restrict__ rowptr,
                                                      const int32_t *__restrict__ col,
                                                      const T *__restrict__ val,
                                                      const T *__restrict__ p, T *__restrict__ q,
-                                                     double *ws) {
+                                                     double *__restrict__ ws) {
   if (ws[W_DONE] != 0.0) return;
-  constexpr int RPB = kBlock / SW;  // rows per block
-  double v[1] = {0.0};
+  constexpr int RPB = kBlock / SW;  // rows per workgroup pass
+  double pq = 0.0;
   const int lane = threadIdx.x % SW;
   for (int64_t row0 = (int64_t)blockIdx.x * RPB; row0 < n; row0 += (int64_t)gridDim.x * RPB) {
     const int row = (int)(row0 + threadIdx.x / SW);
@@ -189,27 +190,26 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, const int32_t *__res
       const T a = row_dot<T, SW>(row, lane, rowptr, col, val, p);
       if (lane == 0) {
         q[row] = a;
-        v[0] += (double)a * (double)p[row];
+        pq += (double)a * (double)p[row];
       }
     }
   }
-  double out[1];
-  if (last_block_sum<1>(v, ws, out) && threadIdx.x == 0) {
-    ws[W_PQ] = out[0];
-    ws[W_ALPHA] = ws[W_RZ] / out[0];
-  }
+  write_partial(ws + W_PQ, pq);
+  if (blockIdx.x == 0 && threadIdx.x == 0) ws[W_ITER] += 1.0;
 }
 
-// x += alpha p, r -= alpha q, z = dinv r, rz_new = r.z, convergence, beta
+// alpha = rz_{i-1} / pq; x += alpha p; r -= alpha q; z = dinv r; partials of r.z (ring i&1)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const T *__restrict__ dinv,
                                                        T *__restrict__ x, T *__restrict__ r,
                                                        T *__restrict__ z, const T *__restrict__ p,
-                                                       const T *__restrict__ q, double *ws,
-                                                       double tol, int max_iter) {
+                                                       const T *__restrict__ q, double *__restrict__ ws) {
   if (ws[W_DONE] != 0.0) return;
-  const T alpha = (T)ws[W_ALPHA];
-  double v[1] = {0.0};
+  const int it = (int)ws[W_ITER];
+  const double rz_old = sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, gridDim.x);
+  const double pq = sum_partials(ws + W_PQ, gridDim.x);
+  const T alpha = (T)(rz_old / pq);
+  double rz = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     x[i] += alpha * p[i];
@@ -217,24 +217,28 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const T *__restric
     r[i] = ri;
     const T zi = dinv[i] * ri;
     z[i] = zi;
-    v[0] += (double)ri * (double)zi;
+    rz += (double)ri * (double)zi;
   }
-  double out[1];
-  if (last_block_sum<1>(v, ws, out) && threadIdx.x == 0) {
-    const double rz_new = out[0];
-    const double it = ws[W_ITER] + 1.0;
-    ws[W_ITER] = it;
-    ws[W_BETA] = rz_new / ws[W_RZ];
-    ws[W_RZ] = rz_new;
-    if (rz_new / ws[W_BZ] < tol || it >= (double)max_iter || !(rz_new == rz_new)) ws[W_DONE] = 1.0;
-  }
+  write_partial(ws + W_RZP + (it & 1) * PCG_G, rz);
 }
 
+// convergence test (rz_i / bz < tol, _linear_solver.py:37) and p = z + beta p
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, const T *__restrict__ z, T *__restrict__ p,
-                                                    const double *ws) {
+                                                    double *__restrict__ ws, double tol, int max_iter) {
   if (ws[W_DONE] != 0.0) return;
-  const T beta = (T)ws[W_BETA];
+  const int it = (int)ws[W_ITER];
+  const double rz_new = sum_partials(ws + W_RZP + (it & 1) * PCG_G, gridDim.x);
+  const double rz_old = sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, gridDim.x);
+  const double bz = sum_partials(ws + W_BZP, gridDim.x);
+  const bool done = (rz_new / bz < tol) || (it >= max_iter) || !(rz_new == rz_new);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ws[W_RZ] = rz_new;
+    ws[W_BZ] = bz;
+    if (done) ws[W_DONE] = 1.0;
+  }
+  if (done) return;
+  const T beta = (T)(rz_new / rz_old);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     p[i] = z[i] + beta * p[i];
@@ -411,15 +415,11 @@ inline int red_grid(int64_t n) {
 template <typename T>
 static void launch_pcg_spmv(int sw, int n, const int32_t *rp, const int32_t *c, const T *v,
                             const T *p, T *q, double *ws, hipStream_t st) {
-  const int rows_per_block = kBlock / sw;
-  int grid = (n + rows_per_block - 1) / rows_per_block;
-  if (grid > 2048) grid = 2048;
-  if (grid < 1) grid = 1;
   switch (sw) {
-    case 4: k_pcg_spmv<T, 4><<<grid, kBlock, 0, st>>>(n, rp, c, v, p, q, ws); break;
-    case 8: k_pcg_spmv<T, 8><<<grid, kBlock, 0, st>>>(n, rp, c, v, p, q, ws); break;
-    case 16: k_pcg_spmv<T, 16><<<grid, kBlock, 0, st>>>(n, rp, c, v, p, q, ws); break;
-    default: k_pcg_spmv<T, 32><<<grid, kBlock, 0, st>>>(n, rp, c, v, p, q, ws); break;
+    case 4: k_pcg_spmv<T, 4><<<PCG_G, kBlock, 0, st>>>(n, rp, c, v, p, q, ws); break;
+    case 8: k_pcg_spmv<T, 8><<<PCG_G, kBlock, 0, st>>>(n, rp, c, v, p, q, ws); break;
+    case 16: k_pcg_spmv<T, 16><<<PCG_G, kBlock, 0, st>>>(n, rp, c, v, p, q, ws); break;
+    default: k_pcg_spmv<T, 32><<<PCG_G, kBlock, 0, st>>>(n, rp, c, v, p, q, ws); break;
   }
 }
 
@@ -428,7 +428,7 @@ static int pcg_start_t(int n, int64_t nnz, const int32_t *rp, const int32_t *c, 
                        const T *dinv, const T *b, T *x, T *r, T *z, T *p, double *ws, hipStream_t st) {
   const int sw = pick_sw(n, nnz);
   launch_resid<T>(sw, n, rp, c, v, x, b, r, st);
-  k_pcg_init<T><<<red_grid(n), kBlock, 0, st>>>(n, dinv, b, r, z, p, ws);
+  k_pcg_init<T><<<PCG_G, kBlock, 0, st>>>(n, dinv, b, r, z, p, ws);
   FEA_CHECK_LAUNCH("fea_pcg_start");
   return FEA_OK;
 }
@@ -438,15 +438,14 @@ static int pcg_iterate_t(int n, int64_t nnz, const int32_t *rp, const int32_t *c
                          const T *dinv, T *x, T *r, T *z, T *p, T *q, double *ws, double tol,
                          int max_iter, int iters, double *status_host, hipStream_t st) {
   const int sw = pick_sw(n, nnz);
-  const int g = red_grid(n);
   for (int it = 0; it < iters; ++it) {
     launch_pcg_spmv<T>(sw, n, rp, c, v, p, q, ws, st);
-    k_pcg_update<T><<<g, kBlock, 0, st>>>(n, dinv, x, r, z, p, q, ws, tol, max_iter);
-    k_pcg_dir<T><<<g, kBlock, 0, st>>>(n, z, p, ws);
+    k_pcg_update<T><<<PCG_G, kBlock, 0, st>>>(n, dinv, x, r, z, p, q, ws);
+    k_pcg_dir<T><<<PCG_G, kBlock, 0, st>>>(n, z, p, ws, tol, max_iter);
   }
   FEA_CHECK_LAUNCH("fea_pcg_iterate");
   if (status_host != nullptr) {
-    double h[8];
+    double h[4];
     FEA_HIP(hipMemcpyAsync(h, ws, sizeof(h), hipMemcpyDeviceToHost, st), "fea_pcg_iterate");
     FEA_HIP(hipStreamSynchronize(st), "fea_pcg_iterate");
     status_host[0] = h[W_ITER];
@@ -529,8 +528,7 @@ extern "C" int fea_jacobi_inv(int dtype, int n, const int32_t *diag_slot, const 
   return FEA_OK;
 }
 
-// partials of every PCG kernel: at most 2048 blocks x 2 values (grids are capped at 2048)
-extern "C" int64_t fea_pcg_ws_size(int) { return (int64_t)W_PART + 2LL * 2048 + 16; }
+extern "C" int64_t fea_pcg_ws_size(int) { return (int64_t)W_END; }
 
 extern "C" int fea_pcg_start(int dtype, int n, int64_t nnz, const int32_t *rowptr, const int32_t *col,
                              const void *val, const void *dinv, const void *b, void *x, void *r,

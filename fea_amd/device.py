@@ -11,7 +11,7 @@ A ``DevicePlan`` owns, on one GPU:
 * one record per *group*: an element group (C3D4/C3D8/C3D10 + material), a
   pressure facet group or a traction facet group, with its int32
   connectivity and its ``dst_k`` / ``dst_r`` staging maps;
-* staging buffers ``kstage`` (per precision) and ``rstage`` (float64).
+* staging buffers ``kstage`` (per precision) and ``rstage`` float64.
 
 Every array is allocated once when the plan is built; assembly calls only
 launch kernels into these buffers.
@@ -156,7 +156,7 @@ class DevicePlan:
         """R = f_int(U) - f_ext on free DOFs (assemble_force, assembly.py:230-281)."""
         self.residual_stage(U, load_scale, dtype)
         R = self.R if out is None else out
-        call('fea_gather_residual', self.n_nodes, ptr(self.rseg), ptr(self.dof_map), ptr(self.rstage), ptr(R),
+        call('fea_gather_residual', self.n_nodes, self.rseg, self.dof_map, self.rstage, R,
              self._s())
         return R
 
@@ -168,18 +168,18 @@ class DevicePlan:
             if g.n == 0:
                 continue
             if g.kind == 'elem':
-                call('fea_elem_residual', g.ntype, dtype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
-                     g.params.ctypes.data, ptr(g.dst_r), ptr(self.rstage), s)
+                call('fea_elem_residual', g.ntype, dtype, g.n, g.conn, self.X, U,
+                     g.params, g.dst_r, self.rstage, s)
             elif g.kind == 'pressure':
-                call('fea_surface_pressure', g.ntype, dtype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
-                     float(g.load.pressure) * load_scale, None, None, ptr(g.dst_r), ptr(self.rstage), s)
+                call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, self.X, U,
+                     float(g.load.pressure) * load_scale, None, None, g.dst_r, self.rstage, s)
             elif g.kind == 'traction':
                 t = np.asarray(g.load.traction, dtype=np.float64)
-                call('fea_surface_traction', g.ntype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
-                     t.ctypes.data, float(load_scale), ptr(g.dst_r), ptr(self.rstage), None, s)
+                call('fea_surface_traction', g.ntype, g.n, g.conn, self.X, U,
+                     t, float(load_scale), g.dst_r, self.rstage, None, s)
 
     def tangent(self, U, load_scale=1.0, dtype=F64, with_residual=True):
-        """Tangent CSR values (dtype) and optionally the float64 residual.
+        """Tangent CSR values dtype and optionally the float64 residual.
 
         Replaces _assemble_generalized_Matrix + _assemble_reduced_Matrix
         (assembly.py:311-387).  With ``with_residual`` the residual comes from
@@ -194,29 +194,29 @@ class DevicePlan:
             if g.n == 0:
                 continue
             if g.kind == 'elem':
-                call('fea_elem_residual_tangent', g.ntype, dtype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
-                     g.params.ctypes.data, ptr(g.dst_k), ptr(ks),
-                     ptr(g.dst_r) if with_residual else None, ptr(rs), s)
+                call('fea_elem_residual_tangent', g.ntype, dtype, g.n, g.conn, self.X, U,
+                     g.params, g.dst_k, ks,
+                     g.dst_r if with_residual else None, rs, s)
             elif g.kind == 'pressure':
-                call('fea_surface_pressure', g.ntype, dtype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
-                     float(g.load.pressure) * load_scale, ptr(g.dst_k), ptr(ks),
-                     ptr(g.dst_r) if with_residual else None, ptr(rs), s)
+                call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, self.X, U,
+                     float(g.load.pressure) * load_scale, g.dst_k, ks,
+                     g.dst_r if with_residual else None, rs, s)
             elif g.kind == 'traction' and with_residual:
                 t = np.asarray(g.load.traction, dtype=np.float64)
-                call('fea_surface_traction', g.ntype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
-                     t.ctypes.data, float(load_scale), ptr(g.dst_r), ptr(self.rstage), None, s)
+                call('fea_surface_traction', g.ntype, g.n, g.conn, self.X, U,
+                     t, float(load_scale), g.dst_r, self.rstage, None, s)
         val = self.csr_values(dtype)
-        call('fea_gather_csr', dtype, self.n_blocks, ptr(self.blk_a), ptr(self.blk_b), ptr(self.blk_colpos),
-             ptr(self.kseg), ptr(self.dof_map), ptr(self.rowptr), ptr(ks), ptr(val), s)
+        call('fea_gather_csr', dtype, self.n_blocks, self.blk_a, self.blk_b, self.blk_colpos,
+             self.kseg, self.dof_map, self.rowptr, ks, val, s)
         R = None
         if with_residual:
             R = self.R
-            call('fea_gather_residual', self.n_nodes, ptr(self.rseg), ptr(self.dof_map), ptr(self.rstage),
-                 ptr(R), s)
+            call('fea_gather_residual', self.n_nodes, self.rseg, self.dof_map, self.rstage,
+                 R, s)
         return R, val
 
     def energy_terms(self, U, load_scale=1.0):
-        """Per-group energy terms (device): elem -> strain energy, pressure -> V, traction -> f.u."""
+        """Per-group energy terms device: elem -> strain energy, pressure -> V, traction -> f.u."""
         self._check_u(U)
         s = self._s()
         for gi, g in enumerate(self.groups):
@@ -225,15 +225,15 @@ class DevicePlan:
                 continue
             seg = self._epart[off:off + nb]
             if g.kind == 'elem':
-                call('fea_elem_energy', g.ntype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
-                     g.params.ctypes.data, ptr(seg), s)
+                call('fea_elem_energy', g.ntype, g.n, g.conn, self.X, U,
+                     g.params, seg, s)
             elif g.kind == 'pressure':
-                call('fea_surface_volume', g.ntype, g.n, ptr(g.conn), ptr(self.X), ptr(U), ptr(seg), s)
+                call('fea_surface_volume', g.ntype, g.n, g.conn, self.X, U, seg, s)
             elif g.kind == 'traction':
                 t = np.asarray(g.load.traction, dtype=np.float64)
-                call('fea_surface_traction', g.ntype, g.n, ptr(g.conn), ptr(self.X), ptr(U),
-                     t.ctypes.data, 1.0, None, None, ptr(seg), s)
-            call('fea_reduce_sum', nb, ptr(seg), ptr(self._eout[gi:gi + 1]), s)
+                call('fea_surface_traction', g.ntype, g.n, g.conn, self.X, U,
+                     t, 1.0, None, None, seg, s)
+            call('fea_reduce_sum', nb, seg, self._eout[gi:gi + 1], s)
         return self._eout
 
     def energy(self, U, load_scale=1.0):
@@ -257,10 +257,10 @@ class DevicePlan:
         self.rstage.zero_()
         for g in self.groups:
             if g.kind == 'elem' and g.n:
-                call('fea_elem_lumped_mass', g.ntype, g.n, ptr(g.conn), ptr(self.X), ptr(self.dof_map),
-                     g.rho, ptr(g.dst_r), ptr(self.rstage), s)
+                call('fea_elem_lumped_mass', g.ntype, g.n, g.conn, self.X, self.dof_map,
+                     g.rho, g.dst_r, self.rstage, s)
         m = torch.zeros(max(1, self.n_free), dtype=torch.float64, device=self.device)[:self.n_free]
-        call('fea_gather_residual', self.n_nodes, ptr(self.rseg), ptr(self.dof_map), ptr(self.rstage), ptr(m), s)
+        call('fea_gather_residual', self.n_nodes, self.rseg, self.dof_map, self.rstage, m, s)
         return m
 
     # ------------------------------------------------------------------ vectors
@@ -271,8 +271,8 @@ class DevicePlan:
         part = self._scratch('dot', nb)
         out = self._scratch('dot_out', 1)
         s = self._s()
-        call('fea_dot_partial', n, ptr(a), ptr(b), ptr(part), s)
-        call('fea_reduce_sum', nb, ptr(part), ptr(out), s)
+        call('fea_dot_partial', n, a, b, part, s)
+        call('fea_reduce_sum', nb, part, out, s)
         return out
 
     def _scratch(self, key, n):
@@ -286,7 +286,7 @@ class DevicePlan:
 
     def dof_axpy(self, U0, alpha, d, out):
         """out = U0 + alpha * d scattered through dof_map (GC0 + alpha dGC as nodal field)."""
-        call('fea_dof_axpy', self.n_nodes, ptr(self.dof_map), ptr(U0), float(alpha), ptr(d), ptr(out), self._s())
+        call('fea_dof_axpy', self.n_nodes, self.dof_map, U0, float(alpha), d, out, self._s())
         return out
 
     def gc_to_u(self, GC, out=None):
@@ -300,8 +300,8 @@ class DevicePlan:
 
     # ------------------------------------------------------------------ linear
     def spmv(self, val, x, y, dtype):
-        call('fea_spmv', dtype, self.n_free, self.nnz, ptr(self.rowptr), ptr(self.col), ptr(val), ptr(x),
-             ptr(y), self._s())
+        call('fea_spmv', dtype, self.n_free, self.nnz, self.rowptr, self.col, val, x,
+             y, self._s())
         return y
 
     def csr_indices_coo(self):

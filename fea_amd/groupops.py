@@ -33,7 +33,7 @@ def elem_prepare(etype, ngp, elems, X):
     dN = torch.empty((ngp, E, 3, etype), dtype=torch.float64, device='cuda')
     w = torch.empty((ngp, E), dtype=torch.float64, device='cuda')
     N = torch.empty((ngp, E, etype), dtype=torch.float64, device='cuda')
-    call('fea_elem_prepare', etype, E, ptr(_conn32(elems)), ptr(X), ptr(dN), ptr(w), ptr(N), nat.stream_ptr())
+    call('fea_elem_prepare', etype, E, _conn32(elems), X, dN, w, N, nat.stream_ptr())
     return dN, w, N
 
 
@@ -66,8 +66,8 @@ def elem_residual_per_node(elem, U, dtype=F64):
     ident = torch.arange(E * nen, dtype=torch.int32, device='cuda')
     out = torch.empty(E * nen * 3, dtype=torch.float64, device='cuda')
     X = elem._nodes().reshape(-1)
-    call('fea_elem_residual', nen, dtype, E, ptr(_conn32(elem._elems)), ptr(X), ptr(U),
-         elem.kernel_params().ctypes.data, ptr(ident), ptr(out), nat.stream_ptr())
+    call('fea_elem_residual', nen, dtype, E, _conn32(elem._elems), X, U,
+         elem.kernel_params(), ident, out, nat.stream_ptr())
     return out.view(E, nen, 3).permute(1, 2, 0).reshape(-1)
 
 
@@ -77,9 +77,9 @@ def elem_energy(elem, U):
     part = torch.zeros(max(1, nb), dtype=torch.float64, device='cuda')
     out = torch.zeros(1, dtype=torch.float64, device='cuda')
     s = nat.stream_ptr()
-    call('fea_elem_energy', elem.etype, E, ptr(_conn32(elem._elems)), ptr(elem._nodes().reshape(-1)), ptr(U),
-         elem.kernel_params().ctypes.data, ptr(part), s)
-    call('fea_reduce_sum', nb, ptr(part), ptr(out), s)
+    call('fea_elem_energy', elem.etype, E, _conn32(elem._elems), elem._nodes().reshape(-1), U,
+         elem.kernel_params(), part, s)
+    call('fea_reduce_sum', nb, part, out, s)
     return out[0]
 
 
@@ -95,8 +95,8 @@ def surface_pressure_per_facet(stype, conn, X, U, pressure, onlyforce):
     if not onlyforce:
         dk = torch.arange(E * n * n, dtype=torch.int32, device='cuda')
         ks = torch.empty(E * n * n * 9, dtype=torch.float64, device='cuda')
-    call('fea_surface_pressure', n, F64, E, ptr(c32), ptr(X), ptr(U), float(-pressure),
-         ptr(dk), ptr(ks), ptr(dr), ptr(rs), nat.stream_ptr())
+    call('fea_surface_pressure', n, F64, E, c32, X, U, float(-pressure),
+         dk, ks, dr, rs, nat.stream_ptr())
     Fv = rs.view(E, n, 3).permute(0, 2, 1).reshape(-1)
     if onlyforce:
         return Fv, None
@@ -110,8 +110,8 @@ def surface_volume(stype, conn, X, U):
     part = torch.zeros(max(1, nb), dtype=torch.float64, device='cuda')
     out = torch.zeros(1, dtype=torch.float64, device='cuda')
     s = nat.stream_ptr()
-    call('fea_surface_volume', stype, E, ptr(_conn32(conn)), ptr(X), ptr(U), ptr(part), s)
-    call('fea_reduce_sum', nb, ptr(part), ptr(out), s)
+    call('fea_surface_volume', stype, E, _conn32(conn), X, U, part, s)
+    call('fea_reduce_sum', nb, part, out, s)
     return out[0]
 
 
@@ -123,6 +123,6 @@ def surface_traction_force(stype, conn, X, traction):
     rs = torch.empty(E * n * 3, dtype=torch.float64, device='cuda')
     t = np.asarray(traction, dtype=np.float64)
     U0 = torch.zeros_like(X)
-    call('fea_surface_traction', n, E, ptr(_conn32(conn)), ptr(X), ptr(U0), t.ctypes.data, -1.0,
-         ptr(dr), ptr(rs), None, nat.stream_ptr())
+    call('fea_surface_traction', n, E, _conn32(conn), X, U0, t, -1.0,
+         dr, rs, None, nat.stream_ptr())
     return rs.view(E, n, 3).permute(0, 2, 1).reshape(-1)

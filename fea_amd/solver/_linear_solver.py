@@ -36,7 +36,7 @@ class PCG:
 
     def set_matrix(self, val):
         self.val = val
-        call('fea_jacobi_inv', self.dtype, self.n, ptr(self.diag_slot), ptr(val), ptr(self.dinv),
+        call('fea_jacobi_inv', self.dtype, self.n, self.diag_slot, val, self.dinv,
              nat.stream_ptr())
 
     def solve(self, b, x0=None, tol=1e-5, max_iter=1500):
@@ -49,15 +49,15 @@ class PCG:
             self.x.zero_()
         else:
             self.x.copy_(x0)
-        call('fea_pcg_start', self.dtype, self.n, self.nnz, ptr(self.rowptr), ptr(self.col), ptr(self.val),
-             ptr(self.dinv), ptr(b), ptr(self.x), ptr(self.r), ptr(self.z), ptr(self.p), ptr(self.ws), s)
+        call('fea_pcg_start', self.dtype, self.n, self.nnz, self.rowptr, self.col, self.val,
+             self.dinv, b, self.x, self.r, self.z, self.p, self.ws, s)
         done_iters = 0
         while True:
             step = min(self.chunk, max(1, max_iter - done_iters))
-            call('fea_pcg_iterate', self.dtype, self.n, self.nnz, ptr(self.rowptr), ptr(self.col),
-                 ptr(self.val), ptr(self.dinv), ptr(self.x), ptr(self.r), ptr(self.z), ptr(self.p),
-                 ptr(self.q), ptr(self.ws), float(tol), int(max_iter), int(step),
-                 self.status.ctypes.data, s)
+            call('fea_pcg_iterate', self.dtype, self.n, self.nnz, self.rowptr, self.col,
+                 self.val, self.dinv, self.x, self.r, self.z, self.p,
+                 self.q, self.ws, float(tol), int(max_iter), int(step),
+                 self.status, s)
             done_iters += step
             if self.status[1] != 0.0 or done_iters >= max_iter:
                 break

@@ -131,5 +131,5 @@ class DynamicExplicitSolver(BaseSolver):
         p = self.assembly.plan
         s = nat.stream_ptr()
         p.residual_stage(U, load_scale, F64)
-        call('fea_explicit_update', p.n_nodes, ptr(p.rseg), ptr(p.dof_map), ptr(p.rstage), ptr(self._minv),
-             0.5 * dt if first else dt, dt, ptr(acc), ptr(vh), ptr(U), s)
+        call('fea_explicit_update', p.n_nodes, p.rseg, p.dof_map, p.rstage, self._minv,
+             0.5 * dt if first else dt, dt, acc, vh, U, s)

@@ -19,6 +19,8 @@ Additions (keyword-only, defaults reproduce the reference):
 * ``armijo_slack``: relative energy round-off allowance in the Armijo test, so
   the line search does not reject full steps whose energy change is below
   float64 resolution near convergence (0 reproduces the reference).
+* ``predictor``: with load increments, start increment k from the linear
+  extrapolation 2 u_{k-1} - u_{k-2} of the converged states.
 """
 from __future__ import annotations
 
@@ -39,7 +41,8 @@ class StaticImplicitSolver(BaseSolver):
     def __init__(self, maximum_iteration: int = 10000, tol_error: float = 1e-5, *,
                  precision: str = 'fp64', rel_tol: float | None = None, load_steps: int = 1,
                  cg_tol: float = 1e-5, cg_max_iter: int = 1500, ir_steps: int = 1,
-                 armijo_slack: float = 0.0, verbose: bool = False, pcg_chunk: int = 32) -> None:
+                 armijo_slack: float = 0.0, predictor: bool = True, verbose: bool = False,
+                 pcg_chunk: int = 32) -> None:
         super().__init__()
         if precision not in ('fp64', 'mixed'):
             raise ValueError("precision must be 'fp64' or 'mixed'")
@@ -55,6 +58,7 @@ class StaticImplicitSolver(BaseSolver):
         self.cg_max_iter = int(cg_max_iter)
         self.ir_steps = int(ir_steps)
         self.armijo_slack = float(armijo_slack)
+        self.predictor = bool(predictor)
         self.verbose = verbose
         self.pcg_chunk = int(pcg_chunk)
         self.stats = {}
@@ -100,7 +104,7 @@ class StaticImplicitSolver(BaseSolver):
         n = x.numel()
         nb = nat.lib().fea_vec_blocks(n)
         part = p._scratch('stats', 2 * nb)
-        call('fea_vec_stats', n, ptr(x), ptr(part), nat.stream_ptr())
+        call('fea_vec_stats', n, x, part, nat.stream_ptr())
         h = part.view(nb, 2).cpu().numpy()
         return float(h[:, 0].sum()), float(h[:, 1].max())
 
@@ -125,7 +129,7 @@ class StaticImplicitSolver(BaseSolver):
         n = p.n_free
         b = p._scratch('rhs', n)
         b.zero_()
-        call('fea_axpy', n, -1.0, F64, ptr(R), ptr(b), s)              # b = -R
+        call('fea_axpy', n, -1.0, F64, R, b, s)              # b = -R
         out = p._scratch('dgc', n)
         if self.precision == 'fp64':
             pcg = self._pcg_for(F64)
@@ -141,10 +145,10 @@ class StaticImplicitSolver(BaseSolver):
             nb = nat.lib().fea_vec_blocks(n)
             part = p._scratch('ir_part', nb)
             for _ in range(max(1, self.ir_steps)):
-                call('fea_spmv_residual_mixed', n, ptr(p.rowptr), ptr(p.col), ptr(K), ptr(out), ptr(b),
-                     ptr(r), ptr(part), s)
+                call('fea_spmv_residual_mixed', n, p.rowptr, p.col, K, out, b,
+                     r, part, s)
                 d = pcg.solve(r, None, tol=self.cg_tol, max_iter=max_iter)
-                call('fea_axpy', n, 1.0, F32, ptr(d), ptr(out), s)
+                call('fea_axpy', n, 1.0, F32, d, out, s)
                 self.stats['cg_iters'] = self.stats.get('cg_iters', 0) + pcg.iterations
         return out
 
@@ -224,7 +228,16 @@ class StaticImplicitSolver(BaseSolver):
             zero = torch.zeros_like(U)
             rss, _ = self._vec_stats(p.residual(zero, 1.0, F64))
             self._r_ref = math.sqrt(rss) if rss > 0 else None
+        U_prev = U.clone() if (self.predictor and self.load_steps > 1) else None
+        step = torch.empty_like(U) if U_prev is not None else None
+        s = nat.stream_ptr()
         for k in range(1, self.load_steps + 1):
+            if U_prev is not None and k >= 2:
+                # predictor: U <- 2 U - U_prev (linear extrapolation), U_prev <- old U
+                step.copy_(U)
+                call('fea_axpy', U.numel(), -1.0, F64, U_prev, step, s)
+                U_prev.copy_(U)
+                call('fea_axpy', U.numel(), 1.0, F64, step, U, s)
             ok = self._newton(U, k / self.load_steps, tol_error)
             self.stats['increments'] = k
             if not ok:

@@ -121,13 +121,17 @@ def newton(model, GC, tol_error=1e-5, rel_tol=None, max_iter=200, linear='cg',
     return GC, it
 
 
-def static_solve(model, increments=1, **kw):
-    """Load-incremented static solve: Newton per increment from the previous state."""
+def static_solve(model, increments=1, predictor=True, **kw):
+    """Load-incremented static solve: Newton per increment, started from the linear
+    extrapolation of the two previous converged states (``predictor``)."""
     free = asm.free_mask(model)
     GC = np.zeros(int(free.sum()))
     total = 0
     kw.setdefault('r_ref', float(np.linalg.norm(asm.assemble_force(model, np.zeros_like(model['nodes'])))))
+    prev = GC.copy()
     for k in range(1, increments + 1):
+        if predictor and k >= 2:
+            GC, prev = GC + (GC - prev), GC
         GC, its = newton(scale_loads(model, k / increments), GC, **kw)
         total += its
         if GC is False:
@@ -138,45 +142,55 @@ def static_solve(model, increments=1, **kw):
 # --- explicit dynamics ---------------------------------------------------------
 
 def lumped_mass(model):
-    """Row-sum lumped mass on free DOFs (dynamic_explicit.py:147-166, C3base.py:348-384)."""
+    """Row-sum lumped mass on free DOFs (dynamic_explicit.py:147-166, C3base.py:348-384).
+
+    The rows are summed over the free-DOF columns only: assemble_mass_matrix
+    drops every entry with a constrained row or column (assembly.py:434-437)
+    before get_lumped_mass_inv sums the rows."""
     from . import elements as el
     asm.prepare(model)
     N = model['nodes'].shape[0]
     M = np.zeros(3 * N)
+    free = asm.free_mask(model)
     for g in model['groups']:
         vals = el.mass_values(g['N'], g['wdet'], g.get('rho', 1.0))
-        r, _ = el.coo_indices(g['conn'])
-        np.add.at(M, r, vals)
-    free = asm.free_mask(model)
+        r, c = el.coo_indices(g['conn'])
+        keep = free[c]                      # reduced matrix keeps free x free entries only
+        np.add.at(M, r[keep], vals[keep])
     m = M[free]
     m[np.abs(m) < 1e-12] = 1.0
     return m
 
 
+def central_difference(residual, minv, GC0, GV0, dt, nsteps):
+    """The central-difference loop of DynamicExplicitSolver.solve (dynamic_explicit.py:80-134)
+    for a generic residual R(GC) (a = -M^-1 R).  Returns (GC, GV, GA) after nsteps."""
+    GA = minv * (-residual(GC0))
+    GVh = GV0 - 0.5 * dt * GA
+    GC = np.array(GC0, dtype=float, copy=True)
+    GV = np.array(GV0, dtype=float, copy=True)
+    for _ in range(nsteps):
+        GVh_new = GVh + dt * GA
+        GC = GC + dt * GVh_new
+        GA = minv * (-residual(GC))
+        GV = 0.5 * (GVh_new + GVh)
+        GVh = GVh_new
+    return GC, GV, GA
+
+
 def explicit_steps(model, GC0, GV0, dt, nsteps, extra_force=None):
-    """DynamicExplicitSolver.solve inner loop (dynamic_explicit.py:80-134).
+    """Explicit dynamics of a model with the row-sum lumped mass.
 
     ``extra_force(u_nodes) -> [3N]`` adds an external force (e.g. contact).
-    Returns (GC, GV, GA) after nsteps.
     """
     free = asm.free_mask(model)
     minv = 1.0 / lumped_mass(model)
 
-    def accel(GC):
+    def residual(GC):
         u = asm.gc_to_u(model, GC)
         R = asm.assemble_force(model, u)
         if extra_force is not None:
             R = R - extra_force(u)[free]
-        return minv * (-R)
+        return R
 
-    GA = accel(GC0)
-    GVh = GV0 - 0.5 * dt * GA
-    GC = GC0.copy()
-    GV = GV0.copy()
-    for _ in range(nsteps):
-        GVh_new = GVh + dt * GA
-        GC = GC + dt * GVh_new
-        GA = accel(GC)
-        GV = 0.5 * (GVh_new + GVh)
-        GVh = GVh_new
-    return GC, GV, GA
+    return central_difference(residual, minv, GC0, GV0, dt, nsteps)
