@@ -35,24 +35,37 @@ def parse():
     ap.add_argument('--ny', type=int, default=32)
     ap.add_argument('--nz', type=int, default=32)
     ap.add_argument('--increments', type=int, default=10)
-    ap.add_argument('--precision', default='mixed', choices=['mixed', 'fp64'])
+    ap.add_argument('--precision', default='mixed', choices=['mixed', 'mixed-f32', 'mixed-ir', 'fp64'])
     ap.add_argument('--rel-tol', type=float, default=1e-10)
     ap.add_argument('--cg-tol', type=float, default=1e-8)
     ap.add_argument('--no-fp64-compare', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--spmv-reps', type=int, default=200)
+    ap.add_argument('--max-newton', type=int, default=400, help='Newton iterations per solve (all increments) before failing')
     return ap.parse_args()
+
+
+_T0 = [time.perf_counter()]
+
+
+def _progress(k, st):
+    print('#   t=%.2fs increment %d newton %d cg %d alpha %.3g |R| %.3e' % (
+        time.perf_counter() - _T0[0], k, st['newton_iters'], st['cg_iters'], st['alpha'], st['residual']),
+        file=sys.stderr, flush=True)
 
 
 def make_solver(precision, args):
     from fea_amd import solver
-    return solver.StaticImplicitSolver(precision=precision, rel_tol=args.rel_tol, load_steps=args.increments,
-                                       cg_tol=args.cg_tol, cg_max_iter=20000, armijo_slack=1e-12)
+    return solver.StaticImplicitSolver(maximum_iteration=args.max_newton, precision=precision,
+                                       rel_tol=args.rel_tol, load_steps=args.increments,
+                                       cg_tol=args.cg_tol, cg_max_iter=20000, armijo_slack=1e-12,
+                                       progress=_progress)
 
 
 def run_solve(fe, torch):
     GC0 = torch.zeros_like(fe.assembly.GC)
     t0 = time.perf_counter()
+    _T0[0] = t0
     GC = fe.solver.solve(GC0=GC0)
     torch.cuda.synchronize()
     if GC is False:
@@ -61,41 +74,52 @@ def run_solve(fe, torch):
 
 
 def spmv_roofline(fe, torch, reps):
-    """Average duration of the PCG SpMV stage (fea_pcg_spmv, FP32) via HIP events."""
+    """Average duration of the PCG SpMV stage (fea_pcg_spmv: FP32 matrix, FP64 vectors) via HIP events.
+
+    Compulsory bytes of one launch: FP32 values (4 B per nonzero), one int32
+    column index per entry of each node's shared column list (the rows of a
+    node reuse it), the FP64 input vector read once and the FP64 output written
+    once (8 B per row each), and the row pointers of the group heads plus the
+    group table (4 B each)."""
     from fea_amd import _native as nat
-    from fea_amd._native import F32, call, ptr
+    from fea_amd._native import F32, F64, call
     p = fe.assembly.plan
     val = p.csr_values(F32)
-    x = torch.rand(p.n_free, dtype=torch.float32, device='cuda')
+    x = torch.rand(p.n_free, dtype=torch.float64, device='cuda')
     q = torch.empty_like(x)
     ws = torch.zeros(int(nat.lib().fea_pcg_ws_size(p.n_free)), dtype=torch.float64, device='cuda')
     s = nat.stream_ptr()
     for _ in range(5):
-        call('fea_pcg_spmv', F32, p.n_free, p.nnz, p.rowptr, p.col, val, x, q, ws, s)
+        call('fea_pcg_spmv', F32, F64, p.n_free, p.nnz, p.n_groups, p.grp, p.rowptr, p.col, val, x, q, ws, s)
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(reps):
-        call('fea_pcg_spmv', F32, p.n_free, p.nnz, p.rowptr, p.col, val, x, q, ws, s)
+        call('fea_pcg_spmv', F32, F64, p.n_free, p.nnz, p.n_groups, p.grp, p.rowptr, p.col, val, x, q, ws, s)
     e1.record(stream)
     e1.synchronize()
     t = e0.elapsed_time(e1) / 1e3 / reps
     n, nnz = p.n_free, p.nnz
-    alg_bytes = nnz * (4 + 4) + (n + 1) * 4 + 2 * n * 4
+    lens = np.diff(p.rowptr_np)
+    grp = p.grp.cpu().numpy()
+    shared_cols = int(lens[grp[:-1]].sum())          # one column list per node group
+    alg_bytes = nnz * 4 + shared_cols * 4 + n * (8 + 8) + (p.n_groups + 1) * 4 * 2
     return t, alg_bytes
 
 
-def element_rate(fe, torch, reps=10):
-    """FP32 tangent assembly (element kernel + CSR gather) timed with events."""
-    from fea_amd._native import F32
+def element_rate(fe, torch, reps=10, kernel_dtype=None):
+    """Tangent assembly as the mixed solver runs it (FP64 element kernel + residual,
+    CSR gather rounded to FP32), timed with events on the solver's stream."""
+    from fea_amd._native import F32, F64
     p = fe.assembly.plan
+    kd = F64 if kernel_dtype is None else kernel_dtype
     U = torch.zeros(3 * p.n_nodes, dtype=torch.float64, device='cuda')
-    p.tangent(U, 1.0, F32, with_residual=False)
+    p.tangent(U, 1.0, kd, val_dtype=F32)
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(reps):
-        p.tangent(U, 1.0, F32, with_residual=False)
+        p.tangent(U, 1.0, kd, val_dtype=F32)
     e1.record(stream)
     e1.synchronize()
     t = e0.elapsed_time(e1) / 1e3 / reps
@@ -182,7 +206,7 @@ def main():
     value = args.steps / elapsed
 
     fp64 = {}
-    if args.precision == 'mixed' and not args.no_fp64_compare:
+    if args.precision != 'fp64' and not args.no_fp64_compare:
         fe64, _ = workloads.beam_traction(args.nx, args.ny, args.nz)
         fe64.solver = make_solver('fp64', args)
         fe64.initialize()
@@ -196,6 +220,8 @@ def main():
     t_spmv, alg_bytes = spmv_roofline(fe, torch, args.spmv_reps)
     achieved = alg_bytes / t_spmv / 1e9
     elem_rate, t_tan = element_rate(fe, torch)
+    from fea_amd._native import F32 as _F32
+    elem_rate32, t_tan32 = element_rate(fe, torch, kernel_dtype=_F32)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, fe.assembly.plan, stats)
@@ -204,7 +230,7 @@ def main():
         'metric': 'static Newton solves/s (mixed FP32/FP64-IR) at 1e-8 rel err',
         'value': value, 'unit': 'solves/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': 1e3 * elapsed / args.steps, 'higher_is_better': True, 'scaling': 'weak',
-        'vs_baseline': None, 'dtype': 'f32+f64' if args.precision == 'mixed' else 'f64',
+        'vs_baseline': None, 'dtype': 'f64' if args.precision == 'fp64' else 'f32+f64',
         'data': 'synthetic (structured mesh generated in-process)',
         'config': {'workload': 'cantilever beam 4x1x1, %dx%dx%d C3D8, Mooney-Rivlin C10=0.5 C01=0.1 D1=0.01, '
                                'clamped x=0, dead traction (0,0,-0.01) on x=4, %d increments, Newton rel tol %g'
@@ -214,7 +240,8 @@ def main():
         'newton_iters_per_solve': stats.get('newton_iters'), 'cg_iters_per_solve': stats.get('cg_iters'),
         'energy_evals_per_solve': stats.get('energy_evals'),
         'elem_tangent_evals_per_s': elem_rate, 'tangent_assembly_ms': 1e3 * t_tan,
-        'roofline': {'bound': 'hbm', 'kernel': 'k_pcg_spmv<float,32> fea_pcg_spmv',
+        'elem_tangent_evals_per_s_f32_contraction': elem_rate32,
+        'roofline': {'bound': 'hbm', 'kernel': 'k_pcg_spmv<float,double,32,true> (fea_pcg_spmv)',
                      'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
                      'alg_bytes_per_launch': alg_bytes, 'avg_launch_us': t_spmv * 1e6},

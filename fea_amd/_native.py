@@ -40,16 +40,17 @@ _SIGS = {
     'fea_surface_volume': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P, C_P]),
     'fea_surface_traction': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P, C_DBL, C_P, C_P, C_P, C_P]),
     'fea_surface_blocks': (C_INT, [C_INT]),
-    'fea_gather_csr': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_gather_csr': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_gather_residual': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P]),
     'fea_reduce_sum': (C_INT, [C_INT, C_P, C_P, C_P]),
-    'fea_spmv': (C_INT, [C_INT, C_INT, C_I64, C_P, C_P, C_P, C_P, C_P, C_P]),
-    'fea_jacobi_inv': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P]),
+    'fea_spmv': (C_INT, [C_INT, C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_jacobi_inv': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_P]),
     'fea_pcg_ws_size': (C_I64, [C_INT]),
-    'fea_pcg_start': (C_INT, [C_INT, C_INT, C_I64, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
-    'fea_pcg_iterate': (C_INT, [C_INT, C_INT, C_I64, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P,
-                                C_DBL, C_INT, C_INT, C_P, C_P]),
-    'fea_pcg_spmv': (C_INT, [C_INT, C_INT, C_I64, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_pcg_start': (C_INT, [C_INT, C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P,
+                              C_P, C_P, C_P, C_P]),
+    'fea_pcg_iterate': (C_INT, [C_INT, C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P,
+                                C_P, C_P, C_P, C_DBL, C_INT, C_INT, C_P, C_P]),
+    'fea_pcg_spmv': (C_INT, [C_INT, C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_spmv_residual_mixed': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_vec_blocks': (C_INT, [C_INT]),
     'fea_dof_axpy': (C_INT, [C_INT, C_P, C_P, C_DBL, C_P, C_P, C_P]),

@@ -18,14 +18,14 @@ __global__ __launch_bounds__(kBlock) void k_elem_rk(int n_elem, const int32_t *_
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)n_elem * NEN) return;
   const int e = (int)(t / NEN), a = (int)(t - (int64_t)e * NEN);
-  T x[NEN][3], u[NEN][3];
+  double xd[NEN][3], ud[NEN][3];
 #pragma unroll
   for (int c = 0; c < NEN; ++c) {
     const int64_t n = conn[(int64_t)e * NEN + c];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      x[c][i] = (T)X[3 * n + i];
-      u[c][i] = (T)U[3 * n + i];
+      xd[c][i] = X[3 * n + i];
+      ud[c][i] = U[3 * n + i];
     }
   }
   T K[NEN][9];
@@ -36,16 +36,35 @@ __global__ __launch_bounds__(kBlock) void k_elem_rk(int n_elem, const int32_t *_
   double Ra[3] = {0.0, 0.0, 0.0};
 
   for (int g = 0; g < NGP; ++g) {
-    T dN[NEN][3];
-    const T w = gp_geometry<T, ET>(g, x, dN);
-    T H[9];
-    disp_grad<T, NEN>(u, dN, H);
+    // Gauss-point state (geometry, kinematics, S and the material tangent) in
+    // float64: near incompressibility the deviatoric part of the tangent is a
+    // small difference of bulk-modulus-sized terms.  Only the B^T D B
+    // contraction below runs in T.
+    double dNd[NEN][3];
+    const double wd = gp_geometry<double, ET>(g, xd, dNd);
+    double Hd[9];
+    disp_grad<double, NEN>(ud, dNd, Hd);
+    GPState<double> sd;
+    kinematics(Hd, sd);
+    double Sd[6];
+    stress(sd, mp, Sd);
+    double Dd[6][6];
+    tangent(sd, mp, Dd);
+    const T w = (T)wd;
+    T dN[NEN][3], S[6], D[6][6];
     GPState<T> s;
-    kinematics(H, s);
-    T S[6];
-    stress(s, mp, S);
-    T D[6][6];
-    tangent(s, mp, D);
+#pragma unroll
+    for (int c = 0; c < NEN; ++c)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) dN[c][i] = (T)dNd[c][i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) s.F[i] = (T)sd.F[i];
+#pragma unroll
+    for (int I = 0; I < 6; ++I) {
+      S[I] = (T)Sd[I];
+#pragma unroll
+      for (int J = 0; J < 6; ++J) D[I][J] = (T)Dd[I][J];
+    }
     T Ba[6][3];
     bmat(s.F, dN[a], Ba);
 #pragma unroll

@@ -1,0 +1,399 @@
+// CSR SpMV and device-resident Jacobi-PCG (replaces the CSR `A @ p` and the CG
+// loop of _linear_solver.py:4-42 on the scaled system of static_implicit.py:314-321).
+//
+// Precision is split between the matrix (TM: the HBM-heavy values) and the
+// vectors / recurrences (TV): (double,double) plain FP64, (float,double) mixed
+// (FP32 matrix storage, FP64 vectors and accumulation), (float,float) FP32.
+//
+// Node-grouped SpMV: in the free-DOF CSR every free row of a node has the same
+// column list (columns come from the node's neighbours), and a node's free
+// rows are consecutive.  With a group table grp[g] = first row of group g
+// (rows grp[g] .. grp[g+1]-1, at most 3, all of length L), a SW-lane sub-wave
+// reads each column index and gathers each x value once for all rows of the
+// node, cutting index + gather traffic by up to 3x.  grp == NULL selects one
+// row per group (plain CSR).
+#include "common.h"
+
+namespace fea {
+
+// ws layout (float64); partial rings as described in k_pcg_update / k_pcg_dir
+enum : int {
+  G1_MAX = 4096,  // max workgroups of the SpMV stage (p.q partials)
+  PCG_G = 512,    // workgroups of the vector stages (r.z / b.z partials)
+  W_ITER = 0, W_DONE = 1, W_RZ = 2, W_BZ = 3, W_G1 = 4,
+  W_PQ = 16, W_RZP = W_PQ + G1_MAX, W_BZP = W_RZP + 2 * PCG_G, W_END = W_BZP + PCG_G
+};
+
+template <typename TM, typename TV, int SW, bool NODE>
+__device__ __forceinline__ int group_spmv(int g, int lane, const int32_t *__restrict__ grp,
+                                          const int32_t *__restrict__ rowptr,
+                                          const int32_t *__restrict__ col, const TM *__restrict__ val,
+                                          const TV *__restrict__ x, TV out[3]) {
+  const int r0 = NODE ? grp[g] : g;
+  const int nr = NODE ? grp[g + 1] - r0 : 1;
+  const int s0 = rowptr[r0];
+  const int L = rowptr[r0 + 1] - s0;
+  TV a0 = TV(0), a1 = TV(0), a2 = TV(0);
+  for (int t = lane; t < L; t += SW) {
+    const TV xv = x[col[s0 + t]];
+    a0 += (TV)val[s0 + t] * xv;
+    if (NODE) {
+      if (nr > 1) a1 += (TV)val[s0 + L + t] * xv;
+      if (nr > 2) a2 += (TV)val[s0 + 2 * L + t] * xv;
+    }
+  }
+#pragma unroll
+  for (int off = SW / 2; off > 0; off >>= 1) {
+    a0 += __shfl_xor(a0, off, SW);
+    if (NODE) {
+      a1 += __shfl_xor(a1, off, SW);
+      a2 += __shfl_xor(a2, off, SW);
+    }
+  }
+  out[0] = a0;
+  out[1] = a1;
+  out[2] = a2;
+  return r0;
+}
+
+// sum of g partials, identical in every workgroup (all threads get the result)
+__device__ __forceinline__ double sum_partials(const double *__restrict__ part, int g) {
+  __shared__ double res;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < g; b += blockDim.x) acc += part[b];
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) res = s;
+  __syncthreads();
+  const double out = res;
+  __syncthreads();
+  return out;
+}
+
+__device__ __forceinline__ void write_partial(double *part, double v) {
+  const double s = block_sum(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// y = A x   (or y = b - A x when b != nullptr); grid-stride over groups
+template <typename TM, typename TV, int SW, bool NODE>
+__global__ __launch_bounds__(kBlock) void k_spmv(int ngroups, const int32_t *__restrict__ grp,
+                                                 const int32_t *__restrict__ rowptr,
+                                                 const int32_t *__restrict__ col,
+                                                 const TM *__restrict__ val, const TV *__restrict__ x,
+                                                 const TV *__restrict__ b, TV *__restrict__ y) {
+  constexpr int GPB = kBlock / SW;
+  const int lane = threadIdx.x % SW;
+  for (int64_t g0 = (int64_t)blockIdx.x * GPB; g0 < ngroups; g0 += (int64_t)gridDim.x * GPB) {
+    const int g = (int)(g0 + threadIdx.x / SW);
+    if (g >= ngroups) continue;
+    TV o[3];
+    const int r0 = group_spmv<TM, TV, SW, NODE>(g, lane, grp, rowptr, col, val, x, o);
+    if (lane == 0) {
+      const int nr = NODE ? grp[g + 1] - r0 : 1;
+      for (int j = 0; j < nr; ++j) y[r0 + j] = (b != nullptr) ? b[r0 + j] - o[j] : o[j];
+    }
+  }
+}
+
+// z = dinv r, p = z; partials of r.z (ring slot 0) and b.(dinv b)
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_pcg_init(int n, const TV *__restrict__ dinv,
+                                                     const TV *__restrict__ b,
+                                                     const TV *__restrict__ r, TV *__restrict__ z,
+                                                     TV *__restrict__ p, double *__restrict__ ws) {
+  double rz = 0.0, bz = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const TV zi = dinv[i] * r[i];
+    z[i] = zi;
+    p[i] = zi;
+    rz += (double)r[i] * (double)zi;
+    bz += (double)b[i] * (double)dinv[i] * (double)b[i];
+  }
+  write_partial(ws + W_RZP, rz);
+  write_partial(ws + W_BZP, bz);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ws[W_ITER] = 0.0;
+    ws[W_DONE] = 0.0;
+  }
+}
+
+// iteration i = ITER+1: q = A p and p.q partials; workgroup 0 advances ITER and
+// records the SpMV grid (the number of p.q partials)
+template <typename TM, typename TV, int SW, bool NODE>
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv(int ngroups, const int32_t *__restrict__ grp,
+                                                     const int32_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ col,
+                                                     const TM *__restrict__ val,
+                                                     const TV *__restrict__ p, TV *__restrict__ q,
+                                                     double *__restrict__ ws) {
+  if (ws[W_DONE] != 0.0) return;
+  constexpr int GPB = kBlock / SW;
+  const int lane = threadIdx.x % SW;
+  double pq = 0.0;
+  for (int64_t g0 = (int64_t)blockIdx.x * GPB; g0 < ngroups; g0 += (int64_t)gridDim.x * GPB) {
+    const int g = (int)(g0 + threadIdx.x / SW);
+    if (g >= ngroups) continue;
+    TV o[3];
+    const int r0 = group_spmv<TM, TV, SW, NODE>(g, lane, grp, rowptr, col, val, p, o);
+    if (lane == 0) {
+      const int nr = NODE ? grp[g + 1] - r0 : 1;
+      for (int j = 0; j < nr; ++j) {
+        q[r0 + j] = o[j];
+        pq += (double)o[j] * (double)p[r0 + j];
+      }
+    }
+  }
+  write_partial(ws + W_PQ, pq);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ws[W_ITER] += 1.0;
+    ws[W_G1] = (double)gridDim.x;
+  }
+}
+
+// alpha = rz_{i-1} / pq; x += alpha p; r -= alpha q; z = dinv r; r.z partials (ring i&1)
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const TV *__restrict__ dinv,
+                                                       TV *__restrict__ x, TV *__restrict__ r,
+                                                       TV *__restrict__ z, const TV *__restrict__ p,
+                                                       const TV *__restrict__ q, double *__restrict__ ws) {
+  if (ws[W_DONE] != 0.0) return;
+  const int it = (int)ws[W_ITER];
+  const int g1 = (int)ws[W_G1];
+  const double rz_old = sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, gridDim.x);
+  const double pq = sum_partials(ws + W_PQ, g1);
+  const TV alpha = (TV)(rz_old / pq);
+  double rz = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const TV ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    const TV zi = dinv[i] * ri;
+    z[i] = zi;
+    rz += (double)ri * (double)zi;
+  }
+  write_partial(ws + W_RZP + (it & 1) * PCG_G, rz);
+}
+
+// convergence (rz_i / bz < tol, _linear_solver.py:37) and p = z + beta p
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, const TV *__restrict__ z, TV *__restrict__ p,
+                                                    double *__restrict__ ws, double tol, int max_iter) {
+  if (ws[W_DONE] != 0.0) return;
+  const int it = (int)ws[W_ITER];
+  const double rz_new = sum_partials(ws + W_RZP + (it & 1) * PCG_G, gridDim.x);
+  const double rz_old = sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, gridDim.x);
+  const double bz = sum_partials(ws + W_BZP, gridDim.x);
+  const bool done = (rz_new / bz < tol) || (it >= max_iter) || !(rz_new == rz_new);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ws[W_RZ] = rz_new;
+    ws[W_BZ] = bz;
+    if (done) ws[W_DONE] = 1.0;
+  }
+  if (done) return;
+  const TV beta = (TV)(rz_new / rz_old);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+
+template <typename TM, typename TV>
+__global__ __launch_bounds__(kBlock) void k_jacobi_inv(int n, const int32_t *__restrict__ diag_slot,
+                                                       const TM *__restrict__ val, TV *__restrict__ dinv) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int s = diag_slot[r];
+  const TV d = (s >= 0) ? fabs((TV)val[s]) : TV(0);
+  dinv[r] = (d == TV(0)) ? TV(1) : TV(1) / d;
+}
+
+// ------------------------------------------------------------------ host side
+struct SpmvShape {
+  int sw;
+  bool node;
+  int ngroups;
+};
+
+static SpmvShape shape_of(int n, int64_t nnz, int n_groups, const int32_t *grp) {
+  SpmvShape s;
+  s.node = (grp != nullptr);
+  s.ngroups = s.node ? n_groups : n;
+  const double len = n > 0 ? (double)nnz / n : 0.0;  // row length
+  s.sw = len > 48 ? 32 : (len > 24 ? 16 : (len > 12 ? 8 : 4));
+  return s;
+}
+
+static int spmv_grid(const SpmvShape &s) {
+  const int64_t g = ((int64_t)s.ngroups * s.sw + kBlock - 1) / kBlock;
+  return (int)(g < 1 ? 1 : (g > G1_MAX ? G1_MAX : g));
+}
+
+template <typename TM, typename TV>
+static void launch_spmv(const SpmvShape &s, const int32_t *grp, const int32_t *rp, const int32_t *c,
+                        const TM *v, const TV *x, const TV *b, TV *y, hipStream_t st) {
+  const int grid = spmv_grid(s);
+#define FEA_SPMV(SW, ND) k_spmv<TM, TV, SW, ND><<<grid, kBlock, 0, st>>>(s.ngroups, grp, rp, c, v, x, b, y)
+  if (s.node) {
+    switch (s.sw) { case 4: FEA_SPMV(4, true); break; case 8: FEA_SPMV(8, true); break;
+                    case 16: FEA_SPMV(16, true); break; default: FEA_SPMV(32, true); }
+  } else {
+    switch (s.sw) { case 4: FEA_SPMV(4, false); break; case 8: FEA_SPMV(8, false); break;
+                    case 16: FEA_SPMV(16, false); break; default: FEA_SPMV(32, false); }
+  }
+#undef FEA_SPMV
+}
+
+template <typename TM, typename TV>
+static void launch_pcg_spmv(const SpmvShape &s, const int32_t *grp, const int32_t *rp, const int32_t *c,
+                            const TM *v, const TV *p, TV *q, double *ws, hipStream_t st) {
+  const int grid = spmv_grid(s);
+#define FEA_PSPMV(SW, ND) k_pcg_spmv<TM, TV, SW, ND><<<grid, kBlock, 0, st>>>(s.ngroups, grp, rp, c, v, p, q, ws)
+  if (s.node) {
+    switch (s.sw) { case 4: FEA_PSPMV(4, true); break; case 8: FEA_PSPMV(8, true); break;
+                    case 16: FEA_PSPMV(16, true); break; default: FEA_PSPMV(32, true); }
+  } else {
+    switch (s.sw) { case 4: FEA_PSPMV(4, false); break; case 8: FEA_PSPMV(8, false); break;
+                    case 16: FEA_PSPMV(16, false); break; default: FEA_PSPMV(32, false); }
+  }
+#undef FEA_PSPMV
+}
+
+struct PcgArgs {
+  int n;
+  int64_t nnz;
+  int n_groups;
+  const int32_t *grp, *rowptr, *col;
+  const void *val, *dinv, *b;
+  void *x, *r, *z, *p, *q;
+  double *ws;
+  double tol;
+  int max_iter, iters;
+  double *status_host;
+  hipStream_t st;
+};
+
+template <typename TM, typename TV>
+static int pcg_start_t(const PcgArgs &a) {
+  const SpmvShape s = shape_of(a.n, a.nnz, a.n_groups, a.grp);
+  launch_spmv<TM, TV>(s, a.grp, a.rowptr, a.col, (const TM *)a.val, (const TV *)a.x, (const TV *)a.b,
+                      (TV *)a.r, a.st);
+  k_pcg_init<TV><<<PCG_G, kBlock, 0, a.st>>>(a.n, (const TV *)a.dinv, (const TV *)a.b, (const TV *)a.r,
+                                            (TV *)a.z, (TV *)a.p, a.ws);
+  FEA_CHECK_LAUNCH("fea_pcg_start");
+  return FEA_OK;
+}
+
+template <typename TM, typename TV>
+static int pcg_iterate_t(const PcgArgs &a) {
+  const SpmvShape s = shape_of(a.n, a.nnz, a.n_groups, a.grp);
+  for (int it = 0; it < a.iters; ++it) {
+    launch_pcg_spmv<TM, TV>(s, a.grp, a.rowptr, a.col, (const TM *)a.val, (const TV *)a.p, (TV *)a.q,
+                            a.ws, a.st);
+    k_pcg_update<TV><<<PCG_G, kBlock, 0, a.st>>>(a.n, (const TV *)a.dinv, (TV *)a.x, (TV *)a.r, (TV *)a.z,
+                                                (const TV *)a.p, (const TV *)a.q, a.ws);
+    k_pcg_dir<TV><<<PCG_G, kBlock, 0, a.st>>>(a.n, (const TV *)a.z, (TV *)a.p, a.ws, a.tol, a.max_iter);
+  }
+  FEA_CHECK_LAUNCH("fea_pcg_iterate");
+  if (a.status_host != nullptr) {
+    double h[4];
+    FEA_HIP(hipMemcpyAsync(h, a.ws, sizeof(h), hipMemcpyDeviceToHost, a.st), "fea_pcg_iterate");
+    FEA_HIP(hipStreamSynchronize(a.st), "fea_pcg_iterate");
+    for (int i = 0; i < 4; ++i) a.status_host[i] = h[i];
+  }
+  return FEA_OK;
+}
+
+static bool valid_combo(int m, int v) {
+  return (m == FEA_F64 && v == FEA_F64) || (m == FEA_F32 && v == FEA_F64) || (m == FEA_F32 && v == FEA_F32);
+}
+
+}  // namespace fea
+
+using namespace fea;
+
+#define FEA_DISPATCH(m, v, fn, arg)                              \
+  ((m) == FEA_F64 ? fn<double, double>(arg)                      \
+                  : ((v) == FEA_F64 ? fn<float, double>(arg) : fn<float, float>(arg)))
+
+extern "C" int64_t fea_pcg_ws_size(int) { return (int64_t)W_END; }
+
+extern "C" int fea_spmv(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
+                        const int32_t *rowptr, const int32_t *col, const void *val, const void *x, void *y,
+                        void *stream) {
+  FEA_CHECK_ARG(valid_combo(mat_dtype, vec_dtype), "fea_spmv: unsupported precision pair");
+  FEA_CHECK_ARG(n >= 0 && (grp == nullptr || n_groups >= 0), "fea_spmv: bad sizes");
+  if (n == 0) return FEA_OK;
+  FEA_CHECK_ARG(rowptr && col && val && x && y, "fea_spmv: null argument");
+  const SpmvShape s = shape_of(n, nnz, n_groups, grp);
+  hipStream_t st = as_stream(stream);
+  if (mat_dtype == FEA_F64)
+    launch_spmv<double, double>(s, grp, rowptr, col, (const double *)val, (const double *)x, nullptr, (double *)y, st);
+  else if (vec_dtype == FEA_F64)
+    launch_spmv<float, double>(s, grp, rowptr, col, (const float *)val, (const double *)x, nullptr, (double *)y, st);
+  else
+    launch_spmv<float, float>(s, grp, rowptr, col, (const float *)val, (const float *)x, nullptr, (float *)y, st);
+  FEA_CHECK_LAUNCH("fea_spmv");
+  return FEA_OK;
+}
+
+extern "C" int fea_jacobi_inv(int mat_dtype, int vec_dtype, int n, const int32_t *diag_slot, const void *val,
+                              void *dinv, void *stream) {
+  FEA_CHECK_ARG(valid_combo(mat_dtype, vec_dtype), "fea_jacobi_inv: unsupported precision pair");
+  FEA_CHECK_ARG(n >= 0, "fea_jacobi_inv: bad n");
+  if (n == 0) return FEA_OK;
+  FEA_CHECK_ARG(diag_slot && val && dinv, "fea_jacobi_inv: null argument");
+  hipStream_t st = as_stream(stream);
+  const int g = blocks_for(n);
+  if (mat_dtype == FEA_F64)
+    k_jacobi_inv<double, double><<<g, kBlock, 0, st>>>(n, diag_slot, (const double *)val, (double *)dinv);
+  else if (vec_dtype == FEA_F64)
+    k_jacobi_inv<float, double><<<g, kBlock, 0, st>>>(n, diag_slot, (const float *)val, (double *)dinv);
+  else
+    k_jacobi_inv<float, float><<<g, kBlock, 0, st>>>(n, diag_slot, (const float *)val, (float *)dinv);
+  FEA_CHECK_LAUNCH("fea_jacobi_inv");
+  return FEA_OK;
+}
+
+extern "C" int fea_pcg_start(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups,
+                             const int32_t *grp, const int32_t *rowptr, const int32_t *col, const void *val,
+                             const void *dinv, const void *b, void *x, void *r, void *z, void *p, double *ws,
+                             void *stream) {
+  FEA_CHECK_ARG(valid_combo(mat_dtype, vec_dtype), "fea_pcg_start: unsupported precision pair");
+  FEA_CHECK_ARG(n > 0 && rowptr && col && val && dinv && b && x && r && z && p && ws,
+                "fea_pcg_start: bad arguments");
+  PcgArgs a{n, nnz, n_groups, grp, rowptr, col, val, dinv, b, x, r, z, p, nullptr, ws, 0.0, 0, 0, nullptr,
+            as_stream(stream)};
+  return FEA_DISPATCH(mat_dtype, vec_dtype, pcg_start_t, a);
+}
+
+extern "C" int fea_pcg_iterate(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups,
+                               const int32_t *grp, const int32_t *rowptr, const int32_t *col,
+                               const void *val, const void *dinv, void *x, void *r, void *z, void *p, void *q,
+                               double *ws, double tol, int max_iter, int iters, double *status_host,
+                               void *stream) {
+  FEA_CHECK_ARG(valid_combo(mat_dtype, vec_dtype), "fea_pcg_iterate: unsupported precision pair");
+  FEA_CHECK_ARG(n > 0 && iters >= 0 && max_iter >= 1 && rowptr && col && val && dinv && x && r && z && p &&
+                    q && ws,
+                "fea_pcg_iterate: bad arguments");
+  PcgArgs a{n, nnz, n_groups, grp, rowptr, col, val, dinv, nullptr, x, r, z, p, q, ws, tol, max_iter, iters,
+            status_host, as_stream(stream)};
+  return FEA_DISPATCH(mat_dtype, vec_dtype, pcg_iterate_t, a);
+}
+
+extern "C" int fea_pcg_spmv(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
+                            const int32_t *rowptr, const int32_t *col, const void *val, const void *p, void *q,
+                            double *ws, void *stream) {
+  FEA_CHECK_ARG(valid_combo(mat_dtype, vec_dtype), "fea_pcg_spmv: unsupported precision pair");
+  FEA_CHECK_ARG(n > 0 && rowptr && col && val && p && q && ws, "fea_pcg_spmv: bad arguments");
+  const SpmvShape s = shape_of(n, nnz, n_groups, grp);
+  hipStream_t st = as_stream(stream);
+  if (mat_dtype == FEA_F64)
+    launch_pcg_spmv<double, double>(s, grp, rowptr, col, (const double *)val, (const double *)p, (double *)q, ws, st);
+  else if (vec_dtype == FEA_F64)
+    launch_pcg_spmv<float, double>(s, grp, rowptr, col, (const float *)val, (const double *)p, (double *)q, ws, st);
+  else
+    launch_pcg_spmv<float, float>(s, grp, rowptr, col, (const float *)val, (const float *)p, (float *)q, ws, st);
+  FEA_CHECK_LAUNCH("fea_pcg_spmv");
+  return FEA_OK;
+}

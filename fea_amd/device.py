@@ -116,6 +116,16 @@ class DevicePlan:
                       'fea_pattern_export_blocks')
             dev = self.device
             self.rowptr_np, self.col_np = rowptr, col[:self.nnz]
+            # node row groups: the free rows of one node are consecutive and share columns
+            nf = self.free_mask.reshape(-1, 3).sum(1)
+            grp = np.concatenate([[0], np.cumsum(nf[nf > 0])]).astype(np.int32)
+            rl = np.diff(rowptr)
+            for j in (1, 2):
+                sel = nf[nf > 0] > j
+                if not np.array_equal(rl[grp[:-1][sel] + j], rl[grp[:-1][sel]]):
+                    raise RuntimeError('CSR rows of a node differ in length')
+            self.n_groups = int(grp.shape[0] - 1)
+            self.grp = _i32(grp, dev)
             self.rowptr, self.col = _i32(rowptr, dev), _i32(col, dev)
             self.diag_slot = _i32(diag, dev)
             self.blk_a, self.blk_b, self.blk_colpos = _i32(blk_a, dev), _i32(blk_b, dev), _i32(blk_cp, dev)
@@ -178,13 +188,14 @@ class DevicePlan:
                 call('fea_surface_traction', g.ntype, g.n, g.conn, self.X, U,
                      t, float(load_scale), g.dst_r, self.rstage, None, s)
 
-    def tangent(self, U, load_scale=1.0, dtype=F64, with_residual=True):
-        """Tangent CSR values dtype and optionally the float64 residual.
+    def tangent(self, U, load_scale=1.0, dtype=F64, with_residual=True, val_dtype=None):
+        """Tangent CSR values (``val_dtype``, default ``dtype``) and optionally the float64 residual.
 
         Replaces _assemble_generalized_Matrix + _assemble_reduced_Matrix
-        (assembly.py:311-387).  With ``with_residual`` the residual comes from
-        the same element pass (dtype arithmetic); mixed precision calls
-        ``residual(..., F64)`` separately instead.
+        (assembly.py:311-387).  ``dtype`` is the element kernels' arithmetic and
+        staging precision; the staged blocks are summed in it and rounded once
+        to ``val_dtype``.  With ``with_residual`` the residual comes from the
+        same element pass (``dtype`` arithmetic, float64 output).
         """
         self._check_u(U)
         s = self._s()
@@ -205,8 +216,9 @@ class DevicePlan:
                 t = np.asarray(g.load.traction, dtype=np.float64)
                 call('fea_surface_traction', g.ntype, g.n, g.conn, self.X, U,
                      t, float(load_scale), g.dst_r, self.rstage, None, s)
-        val = self.csr_values(dtype)
-        call('fea_gather_csr', dtype, self.n_blocks, self.blk_a, self.blk_b, self.blk_colpos,
+        val_dtype = dtype if val_dtype is None else val_dtype
+        val = self.csr_values(val_dtype)
+        call('fea_gather_csr', dtype, val_dtype, self.n_blocks, self.blk_a, self.blk_b, self.blk_colpos,
              self.kseg, self.dof_map, self.rowptr, ks, val, s)
         R = None
         if with_residual:
@@ -299,9 +311,10 @@ class DevicePlan:
         return U.reshape(-1)[idx].clone()
 
     # ------------------------------------------------------------------ linear
-    def spmv(self, val, x, y, dtype):
-        call('fea_spmv', dtype, self.n_free, self.nnz, self.rowptr, self.col, val, x,
-             y, self._s())
+    def spmv(self, val, x, y, mat_dtype, vec_dtype=None):
+        vec_dtype = mat_dtype if vec_dtype is None else vec_dtype
+        call('fea_spmv', mat_dtype, vec_dtype, self.n_free, self.nnz, self.n_groups, self.grp, self.rowptr,
+             self.col, val, x, y, self._s())
         return y
 
     def csr_indices_coo(self):

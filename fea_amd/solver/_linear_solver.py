@@ -22,42 +22,52 @@ class PCG:
     the symmetrically scaled system D^-1/2 K D^-1/2 (static_implicit.py:314-321)
     with identical iterates; its stopping test r.z / b.(M^-1 b) < tol is the
     reference's rsnew / r_r0 < tol on the scaled system (_linear_solver.py:37).
+
+    ``mat_dtype`` is the precision of the matrix values, ``vec_dtype`` that of
+    the vectors and recurrences; ``grp`` (optional) the node row groups.
     """
 
-    def __init__(self, n, nnz, rowptr, col, diag_slot, dtype, device, chunk=32):
-        self.n, self.nnz, self.dtype, self.chunk = int(n), int(nnz), dtype, int(chunk)
+    def __init__(self, n, nnz, rowptr, col, diag_slot, mat_dtype, device, vec_dtype=None, grp=None,
+                 n_groups=0, chunk=32):
+        self.n, self.nnz, self.chunk = int(n), int(nnz), int(chunk)
+        self.mat_dtype = mat_dtype
+        self.vec_dtype = mat_dtype if vec_dtype is None else vec_dtype
         self.rowptr, self.col, self.diag_slot = rowptr, col, diag_slot
-        T = _DT[dtype]
+        self.grp, self.n_groups = grp, int(n_groups)
+        T = _DT[self.vec_dtype]
         mk = lambda: torch.zeros(max(1, self.n), dtype=T, device=device)[:self.n]
         self.x, self.r, self.z, self.p, self.q, self.dinv, self.b = mk(), mk(), mk(), mk(), mk(), mk(), mk()
         self.ws = torch.zeros(int(nat.lib().fea_pcg_ws_size(self.n)), dtype=torch.float64, device=device)
         self.status = np.zeros(4, dtype=np.float64)
         self.iterations = 0
 
+    @property
+    def dtype(self):
+        return self.vec_dtype
+
     def set_matrix(self, val):
         self.val = val
-        call('fea_jacobi_inv', self.dtype, self.n, self.diag_slot, val, self.dinv,
+        call('fea_jacobi_inv', self.mat_dtype, self.vec_dtype, self.n, self.diag_slot, val, self.dinv,
              nat.stream_ptr())
 
     def solve(self, b, x0=None, tol=1e-5, max_iter=1500):
-        """Solve K x = b (b, x0 in this PCG's dtype or float64); returns x (this dtype)."""
+        """Solve K x = b; returns x (vector precision)."""
         s = nat.stream_ptr()
-        if b.dtype != _DT[self.dtype]:
+        if b.dtype != _DT[self.vec_dtype]:
             self.b.copy_(b)
             b = self.b
         if x0 is None:
             self.x.zero_()
         else:
             self.x.copy_(x0)
-        call('fea_pcg_start', self.dtype, self.n, self.nnz, self.rowptr, self.col, self.val,
-             self.dinv, b, self.x, self.r, self.z, self.p, self.ws, s)
+        call('fea_pcg_start', self.mat_dtype, self.vec_dtype, self.n, self.nnz, self.n_groups, self.grp,
+             self.rowptr, self.col, self.val, self.dinv, b, self.x, self.r, self.z, self.p, self.ws, s)
         done_iters = 0
         while True:
             step = min(self.chunk, max(1, max_iter - done_iters))
-            call('fea_pcg_iterate', self.dtype, self.n, self.nnz, self.rowptr, self.col,
-                 self.val, self.dinv, self.x, self.r, self.z, self.p,
-                 self.q, self.ws, float(tol), int(max_iter), int(step),
-                 self.status, s)
+            call('fea_pcg_iterate', self.mat_dtype, self.vec_dtype, self.n, self.nnz, self.n_groups, self.grp,
+                 self.rowptr, self.col, self.val, self.dinv, self.x, self.r, self.z, self.p, self.q, self.ws,
+                 float(tol), int(max_iter), int(step), self.status, s)
             done_iters += step
             if self.status[1] != 0.0 or done_iters >= max_iter:
                 break

@@ -8,12 +8,18 @@ kernels on the device plan; the host only takes the branch decisions.
 
 Additions (keyword-only, defaults reproduce the reference):
 
-* ``precision='mixed'``: tangent assembly and the inner PCG in FP32 with an
-  FP64 iterative-refinement loop around the linear solve; the residual, the
-  energy, the convergence test and the J-terms stay FP64.
+* ``precision``: 'fp64' (everything FP64); 'mixed' (FP64 element arithmetic,
+  tangent stored once-rounded in FP32, PCG on the FP32 matrix with FP64
+  vectors and recurrences); 'mixed-f32' (as 'mixed' but the element B^T D B
+  contraction in FP32 arithmetic); 'mixed-ir' (FP32 element contraction, FP32
+  inner PCG inside an FP64 iterative-refinement loop of ``ir_steps`` passes).
+  In every mode the residual, energy, convergence test and J-terms are FP64.
 * ``rel_tol``: converge on ||R||_2 <= rel_tol ||R_ref||_2 with R_ref the
-  residual of the full load at u = 0 (the external load norm); the reference
-  tests |R|_inf < tol_error absolutely (static_implicit.py:290).
+  residual of the full load at u = 0 (the external load norm), or when a full
+  Newton step changes the displacement by at most ``step_tol`` relative to
+  max|u| (the correction has reached float64 round-off, below which the
+  residual cannot fall); the reference tests |R|_inf < tol_error absolutely
+  (static_implicit.py:290).
 * ``load_steps``: ramp all loads in equal increments (the reference has no
   increments; its harnesses loop the load by hand, test_multicase.py:64-72).
 * ``armijo_slack``: relative energy round-off allowance in the Armijo test, so
@@ -42,10 +48,10 @@ class StaticImplicitSolver(BaseSolver):
                  precision: str = 'fp64', rel_tol: float | None = None, load_steps: int = 1,
                  cg_tol: float = 1e-5, cg_max_iter: int = 1500, ir_steps: int = 1,
                  armijo_slack: float = 0.0, predictor: bool = True, verbose: bool = False,
-                 pcg_chunk: int = 32) -> None:
+                 pcg_chunk: int = 32, progress=None, step_tol: float = 1e-12) -> None:
         super().__init__()
-        if precision not in ('fp64', 'mixed'):
-            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if precision not in ('fp64', 'mixed', 'mixed-f32', 'mixed-ir'):
+            raise ValueError("precision must be 'fp64', 'mixed', 'mixed-f32' or 'mixed-ir'")
         self.maximum_iteration = maximum_iteration
         self.tol_error = tol_error
         self._iter_now = 0
@@ -53,12 +59,14 @@ class StaticImplicitSolver(BaseSolver):
         self._low_alpha_count = 0
         self.precision = precision
         self.rel_tol = rel_tol
+        self.step_tol = float(step_tol)
         self.load_steps = int(load_steps)
         self.cg_tol = float(cg_tol)
         self.cg_max_iter = int(cg_max_iter)
         self.ir_steps = int(ir_steps)
         self.armijo_slack = float(armijo_slack)
         self.predictor = bool(predictor)
+        self.progress = progress      # callable(increment, stats) after each Newton iteration
         self.verbose = verbose
         self.pcg_chunk = int(pcg_chunk)
         self.stats = {}
@@ -91,12 +99,13 @@ class StaticImplicitSolver(BaseSolver):
         return self.assembly.assemble_Stiffness_Matrix(GC=GC_now)
 
     # ------------------------------------------------------------------ internals
-    def _pcg_for(self, dtype):
-        if dtype not in self._pcg:
+    def _pcg_for(self, mat_dtype, vec_dtype):
+        key = (mat_dtype, vec_dtype)
+        if key not in self._pcg:
             p = self.assembly.plan
-            self._pcg[dtype] = PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, dtype, p.device,
-                                   chunk=self.pcg_chunk)
-        return self._pcg[dtype]
+            self._pcg[key] = PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, mat_dtype, p.device,
+                                 vec_dtype=vec_dtype, grp=p.grp, n_groups=p.n_groups, chunk=self.pcg_chunk)
+        return self._pcg[key]
 
     def _vec_stats(self, x):
         """(sum x^2, max|x|) of a float64 device vector, one host sync."""
@@ -110,11 +119,12 @@ class StaticImplicitSolver(BaseSolver):
 
     def _tangent(self, U, scale):
         p = self.assembly.plan
+        if self.precision == 'fp64':
+            return p.tangent(U, scale, F64)
         if self.precision == 'mixed':
-            R = p.residual(U, scale, F64)
-            _, K = p.tangent(U, scale, F32, with_residual=False)
-        else:
-            R, K = p.tangent(U, scale, F64)
+            return p.tangent(U, scale, F64, val_dtype=F32)
+        R = p.residual(U, scale, F64)
+        _, K = p.tangent(U, scale, F32, with_residual=False)
         return R, K
 
     def _solve_linear(self, K, R, dGC0, iter_now, alpha0):
@@ -131,14 +141,14 @@ class StaticImplicitSolver(BaseSolver):
         b.zero_()
         call('fea_axpy', n, -1.0, F64, R, b, s)              # b = -R
         out = p._scratch('dgc', n)
-        if self.precision == 'fp64':
-            pcg = self._pcg_for(F64)
+        if self.precision in ('fp64', 'mixed', 'mixed-f32'):
+            pcg = self._pcg_for(F64 if self.precision == 'fp64' else F32, F64)
             pcg.set_matrix(K)
             x = pcg.solve(b, dGC0, tol=self.cg_tol, max_iter=max_iter)
             out.copy_(x)
             self.stats['cg_iters'] = self.stats.get('cg_iters', 0) + pcg.iterations
         else:
-            pcg = self._pcg_for(F32)
+            pcg = self._pcg_for(F32, F32)
             pcg.set_matrix(K)
             out.copy_(dGC0)
             r = p._scratch('ir_r', n)
@@ -183,7 +193,7 @@ class StaticImplicitSolver(BaseSolver):
         low_alpha, alpha, it = 0, 0.0, 0
         r0 = self._r_ref
         while True:
-            if it > self.maximum_iteration:
+            if self._iter_now >= self.maximum_iteration:
                 return False
             R, K = self._tangent(U, scale)
             it += 1
@@ -210,6 +220,14 @@ class StaticImplicitSolver(BaseSolver):
                 return False
             U.copy_(Unew)
             energy = e_new
+            if self.rel_tol is not None and alpha == 1.0:
+                _, dmax = self._vec_stats(dGC)
+                _, umax = self._vec_stats(U)
+                if dmax <= self.step_tol * umax:
+                    return True
+            if self.progress is not None:
+                self.progress(self._increment, dict(self.stats, newton_iters=self._iter_now, alpha=alpha,
+                                                    residual=math.sqrt(rss)))
             if self.verbose:
                 print('iter %3d  alpha %.3g  energy %.10e  |R|inf %.3e' % (self._iter_now, alpha, energy, rmax))
             if self.rel_tol is None:
@@ -238,6 +256,7 @@ class StaticImplicitSolver(BaseSolver):
                 call('fea_axpy', U.numel(), -1.0, F64, U_prev, step, s)
                 U_prev.copy_(U)
                 call('fea_axpy', U.numel(), 1.0, F64, step, U, s)
+            self._increment = k
             ok = self._newton(U, k / self.load_steps, tol_error)
             self.stats['increments'] = k
             if not ok:

@@ -159,9 +159,13 @@ int fea_surface_blocks(int n_facets);
 /* ---------------------------------------------------------------- assembly --
  * Deterministic, atomic-free gathers replacing the scatter_add calls of
  * C3base.py:449 and assembly.py:347-354 plus the reduction of :371-384. */
-int fea_gather_csr(int dtype, int n_blocks, const int32_t *blk_a, const int32_t *blk_b,
-                   const int32_t *blk_colpos, const int32_t *kseg, const int32_t *dof_map,
-                   const int32_t *rowptr, const void *kstage, void *csr_val, void *stream);
+/* stage_dtype: precision of kstage (the element kernels' dtype); val_dtype: of
+ * the CSR values.  The block sums run in stage_dtype and are rounded once to
+ * val_dtype (pairs F64->F64, F64->F32, F32->F32). */
+int fea_gather_csr(int stage_dtype, int val_dtype, int n_blocks, const int32_t *blk_a,
+                   const int32_t *blk_b, const int32_t *blk_colpos, const int32_t *kseg,
+                   const int32_t *dof_map, const int32_t *rowptr, const void *kstage, void *csr_val,
+                   void *stream);
 /* R[dof] = sum of the node's staged contributions, for free DOFs */
 int fea_gather_residual(int n_nodes, const int32_t *rseg, const int32_t *dof_map,
                         const double *rstage, double *R, void *stream);
@@ -170,36 +174,43 @@ int fea_gather_residual(int n_nodes, const int32_t *rseg, const int32_t *dof_map
 int fea_reduce_sum(int n, const double *partial, double *out, void *stream);
 
 /* ---------------------------------------------------------------- linear algebra --
+ * Precision pairs (matrix values, vectors): (FEA_F64, FEA_F64) plain FP64,
+ * (FEA_F32, FEA_F64) FP32 matrix storage with FP64 vectors / accumulation,
+ * (FEA_F32, FEA_F32) FP32.
+ * Row groups: grp[g] (int32 [n_groups+1]) is the first row of group g; the rows
+ * of a group (at most 3, consecutive) must share one column list, as the free
+ * rows of one node do in the fea_pattern CSR.  grp == NULL: one row per group.
+ *
  * CSR SpMV y = A x (the CSR `A @ p` of _linear_solver.py:27,32). */
-int fea_spmv(int dtype, int n, int64_t nnz, const int32_t *rowptr, const int32_t *col,
-             const void *val, const void *x, void *y, void *stream);
+int fea_spmv(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
+             const int32_t *rowptr, const int32_t *col, const void *val, const void *x, void *y,
+             void *stream);
 /* Jacobi diagonal of static_implicit.py:314-317: dinv[r] = 1/|A_rr| (1 if 0). */
-int fea_jacobi_inv(int dtype, int n, const int32_t *diag_slot, const void *val, void *dinv,
-                   void *stream);
+int fea_jacobi_inv(int mat_dtype, int vec_dtype, int n, const int32_t *diag_slot, const void *val,
+                   void *dinv, void *stream);
 
 /* Jacobi-preconditioned CG, the conjugate_gradient of _linear_solver.py:4-42 on
  * the symmetrically scaled system of static_implicit.py:314-321 (identical
- * iterates), with all scalars on the device.
- * ws: float64 workspace of fea_pcg_ws_size(n) doubles, zero-filled once before
- * its first use (the kernels leave its reduction counter at zero); vectors (dtype) x, b, r,
- * z, p, q of length n.  fea_pcg_start sets r = b - A x, z = dinv r, p = z;
- * fea_pcg_iterate runs `iters` iterations (each a no-op once converged:
- * r.z / b.(dinv b) < tol, or max_iter reached) and copies {iterations, done,
- * rz, bz} to status_host (pinned or pageable host memory, synchronises). */
+ * iterates), with all scalars on the device and no host sync per iteration.
+ * ws: float64 workspace of fea_pcg_ws_size(n) doubles; vectors (vec_dtype) x, b,
+ * r, z, p, q of length n.  fea_pcg_start sets r = b - A x, z = dinv r, p = z;
+ * fea_pcg_iterate enqueues `iters` iterations (each a no-op once converged:
+ * r.z / b.(dinv b) < tol, or max_iter reached) and, if status_host != NULL,
+ * copies {iterations, done, r.z, b.(dinv b)} to it (synchronises the stream). */
 int64_t fea_pcg_ws_size(int n);
-int fea_pcg_start(int dtype, int n, int64_t nnz, const int32_t *rowptr, const int32_t *col, const void *val,
-                  const void *dinv, const void *b, void *x, void *r, void *z, void *p,
-                  double *ws, void *stream);
-int fea_pcg_iterate(int dtype, int n, int64_t nnz, const int32_t *rowptr, const int32_t *col,
-                    const void *val, const void *dinv, void *x, void *r, void *z, void *p,
-                    void *q, double *ws, double tol, int max_iter, int iters,
-                    double *status_host, void *stream);
-
-/* One SpMV stage of the PCG iteration alone: q = A p and p.q -> alpha in ws
+int fea_pcg_start(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
+                  const int32_t *rowptr, const int32_t *col, const void *val, const void *dinv,
+                  const void *b, void *x, void *r, void *z, void *p, double *ws, void *stream);
+int fea_pcg_iterate(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups,
+                    const int32_t *grp, const int32_t *rowptr, const int32_t *col, const void *val,
+                    const void *dinv, void *x, void *r, void *z, void *p, void *q, double *ws,
+                    double tol, int max_iter, int iters, double *status_host, void *stream);
+/* One SpMV stage of the PCG iteration alone: q = A p and the p.q partials
  * (the CSR `A @ p` of _linear_solver.py:32 fused with the dot of :33); a no-op
  * when ws marks the solve converged. */
-int fea_pcg_spmv(int dtype, int n, int64_t nnz, const int32_t *rowptr, const int32_t *col,
-                 const void *val, const void *p, void *q, double *ws, void *stream);
+int fea_pcg_spmv(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
+                 const int32_t *rowptr, const int32_t *col, const void *val, const void *p, void *q,
+                 double *ws, void *stream);
 
 /* FP64 residual of an FP32-stored matrix: r64 = b64 - A32 x64 (iterative
  * refinement outer loop), plus r64 . r64 into `partial` (fea_vec_blocks(n)

@@ -82,12 +82,13 @@ def scale_loads(model, factor):
 
 
 def newton(model, GC, tol_error=1e-5, rel_tol=None, max_iter=200, linear='cg',
-           cg_tol=1e-5, armijo_slack_rel=0.0, r_ref=None):
+           cg_tol=1e-5, armijo_slack_rel=0.0, r_ref=None, step_tol=1e-12):
     """_solve_iteration (static_implicit.py:179-295).
 
     Reference convergence: (|dGC|inf < tol and |R|inf < tol) or |R|inf < 1e-6 (:290).
     With ``rel_tol`` the test is ||R||2 <= rel_tol * r_ref instead (r_ref = the
-    load's residual norm at u = 0, default: first residual of this call).
+    load's residual norm at u = 0, default: first residual of this call), or a
+    full step with max|dGC| <= step_tol max|GC| (correction at round-off).
     Returns (GC or False, iterations).
     """
     energy = asm.total_energy(model, asm.gc_to_u(model, GC))
@@ -115,6 +116,8 @@ def newton(model, GC, tol_error=1e-5, rel_tol=None, max_iter=200, linear='cg',
             break
         GC = GCnew
         energy = asm.total_energy(model, asm.gc_to_u(model, GC))
+        if rel_tol is not None and alpha == 1.0 and np.abs(dGC).max() <= step_tol * np.abs(GC).max():
+            return GC, it
         if rel_tol is None and ((np.abs(dGC).max() < tol_error and np.abs(R).max() < tol_error)
                                 or np.abs(R).max() < 1e-6):
             break

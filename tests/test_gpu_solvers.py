@@ -33,7 +33,8 @@ def test_pcg_matches_direct_solve(gpu):
     A = sp.csr_matrix((K.cpu().numpy(), p.col_np, p.rowptr_np), shape=(p.n_free, p.n_free))
     b = -R.cpu().numpy()
     x_ref = spla.spsolve(A.tocsc(), b)
-    pcg = solver.PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, F64, p.device)
+    pcg = solver.PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, F64, p.device, grp=p.grp,
+                     n_groups=p.n_groups)
     pcg.set_matrix(K)
     x = pcg.solve(-R, None, tol=1e-24, max_iter=5000).cpu().numpy()
     assert rel_err(x, x_ref) < 1e-8
@@ -56,7 +57,7 @@ def test_reference_cg_signature(gpu):
     assert rel_err(x.cpu().numpy(), np.linalg.solve(A.toarray(), b)) < 1e-9
 
 
-@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+@pytest.mark.parametrize('precision', ['fp64', 'mixed', 'mixed-f32', 'mixed-ir'])
 def test_static_cube_matches_oracle(gpu, precision):
     fe, om = cube_models(3, 0.2)
     fe.solver = solver.StaticImplicitSolver(precision=precision, rel_tol=1e-10, load_steps=5, cg_tol=1e-12,
@@ -70,7 +71,7 @@ def test_static_cube_matches_oracle(gpu, precision):
 
 def test_static_beam_mixed_matches_fp64(gpu):
     res = {}
-    for prec in ('fp64', 'mixed'):
+    for prec in ('fp64', 'mixed', 'mixed-f32', 'mixed-ir'):
         fe, mesh = workloads.beam_traction(8, 2, 2, traction=(0.0, 0.0, -0.01))
         fe.solver = solver.StaticImplicitSolver(precision=prec, rel_tol=1e-10, load_steps=4, cg_tol=1e-12,
                                                 armijo_slack=1e-12)
@@ -79,6 +80,8 @@ def test_static_beam_mixed_matches_fp64(gpu):
     GC_o, _ = osol.static_solve(om, increments=4, rel_tol=1e-10, linear='direct', armijo_slack_rel=1e-12)
     assert rel_err(res['fp64'], GC_o) < 1e-8
     assert rel_err(res['mixed'], GC_o) < 1e-8
+    assert rel_err(res['mixed-ir'], GC_o) < 1e-8
+    assert rel_err(res['mixed-f32'], GC_o) < 1e-8
 
 
 def test_reference_convergence_rule(gpu):
@@ -102,3 +105,28 @@ def test_explicit_matches_oracle(gpu):
     assert rel_err(fe.solver.GC.cpu().numpy(), GC_o) < 1e-10
     m = fe.assembly.plan.lumped_mass().cpu().numpy()
     assert rel_err(m, osol.lumped_mass(om)) < 1e-13
+
+
+@pytest.mark.parametrize('pair', ['f64/f64', 'f32/f64', 'f32/f32'])
+@pytest.mark.parametrize('grouped', [True, False])
+def test_spmv_precision_pairs(gpu, pair, grouped):
+    from fea_amd._native import F32, call
+    from fea_amd import _native as nat
+    fe, om = cube_models(3, 0.2)
+    fe.assembly.initialize()
+    p = fe.assembly.plan
+    U = p.gc_to_u(torch.as_tensor(0.01 * np.random.default_rng(3).standard_normal(p.n_free), device='cuda'))
+    _, K = p.tangent(U, 1.0, F64)
+    A = sp.csr_matrix((K.cpu().numpy(), p.col_np, p.rowptr_np), shape=(p.n_free, p.n_free))
+    x = np.random.default_rng(4).standard_normal(p.n_free)
+    m, v = [F64 if s == 'f64' else F32 for s in pair.split('/')]
+    tm = torch.float64 if m == F64 else torch.float32
+    tv = torch.float64 if v == F64 else torch.float32
+    val = K.to(tm)
+    xd = torch.as_tensor(x, device='cuda', dtype=tv)
+    y = torch.zeros_like(xd)
+    call('fea_spmv', m, v, p.n_free, p.nnz, p.n_groups if grouped else 0, p.grp if grouped else None,
+         p.rowptr, p.col, val, xd, y, nat.stream_ptr())
+    ref = A @ x
+    tol = 1e-13 if pair == 'f64/f64' else (1e-6 if pair == 'f32/f64' else 1e-5)
+    assert rel_err(y.double().cpu().numpy(), ref) < tol
