@@ -39,6 +39,7 @@ def parse():
     ap.add_argument('--rel-tol', type=float, default=1e-10)
     ap.add_argument('--cg-tol', type=float, default=1e-8)
     ap.add_argument('--no-fp64-compare', action='store_true')
+    ap.add_argument('--forcing', default='ew', choices=['ew', 'fixed'])
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--spmv-reps', type=int, default=200)
     ap.add_argument('--max-newton', type=int, default=400, help='Newton iterations per solve (all increments) before failing')
@@ -59,7 +60,7 @@ def make_solver(precision, args):
     return solver.StaticImplicitSolver(maximum_iteration=args.max_newton, precision=precision,
                                        rel_tol=args.rel_tol, load_steps=args.increments,
                                        cg_tol=args.cg_tol, cg_max_iter=20000, armijo_slack=1e-12,
-                                       progress=_progress)
+                                       progress=_progress, forcing=args.forcing)
 
 
 def run_solve(fe, torch):
@@ -90,12 +91,12 @@ def spmv_roofline(fe, torch, reps):
     ws = torch.zeros(int(nat.lib().fea_pcg_ws_size(p.n_free)), dtype=torch.float64, device='cuda')
     s = nat.stream_ptr()
     for _ in range(5):
-        call('fea_pcg_spmv', F32, F64, p.n_free, p.nnz, p.n_groups, p.grp, p.rowptr, p.col, val, x, q, ws, s)
+        call('fea_pcg_spmv', F32, F64, p.n_free, p.nnz, p.spmv_groups, p.grp, p.rowptr, p.col, val, x, q, ws, s)
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(reps):
-        call('fea_pcg_spmv', F32, F64, p.n_free, p.nnz, p.n_groups, p.grp, p.rowptr, p.col, val, x, q, ws, s)
+        call('fea_pcg_spmv', F32, F64, p.n_free, p.nnz, p.spmv_groups, p.grp, p.rowptr, p.col, val, x, q, ws, s)
     e1.record(stream)
     e1.synchronize()
     t = e0.elapsed_time(e1) / 1e3 / reps

@@ -24,28 +24,45 @@ enum : int {
   W_PQ = 16, W_RZP = W_PQ + G1_MAX, W_BZP = W_RZP + 2 * PCG_G, W_END = W_BZP + PCG_G
 };
 
-template <typename TM, typename TV, int SW, bool NODE>
+// MODE 0: one row per group; MODE 1: node groups (rows share the column list);
+// MODE 2: node groups whose column list is made of whole neighbour-node triples
+// (3 consecutive columns per neighbour, 3 rows per group): lane j takes
+// neighbour j's 3 columns with one index load, one 3-wide x load and three
+// 3-wide value loads.
+template <typename TM, typename TV, int SW, int MODE>
 __device__ __forceinline__ int group_spmv(int g, int lane, const int32_t *__restrict__ grp,
                                           const int32_t *__restrict__ rowptr,
                                           const int32_t *__restrict__ col, const TM *__restrict__ val,
                                           const TV *__restrict__ x, TV out[3]) {
-  const int r0 = NODE ? grp[g] : g;
-  const int nr = NODE ? grp[g + 1] - r0 : 1;
+  const int r0 = MODE ? grp[g] : g;
+  const int nr = MODE ? grp[g + 1] - r0 : 1;
   const int s0 = rowptr[r0];
   const int L = rowptr[r0 + 1] - s0;
   TV a0 = TV(0), a1 = TV(0), a2 = TV(0);
-  for (int t = lane; t < L; t += SW) {
-    const TV xv = x[col[s0 + t]];
-    a0 += (TV)val[s0 + t] * xv;
-    if (NODE) {
-      if (nr > 1) a1 += (TV)val[s0 + L + t] * xv;
-      if (nr > 2) a2 += (TV)val[s0 + 2 * L + t] * xv;
+  if (MODE == 2) {
+    const int nb = L / 3;
+    for (int j = lane; j < nb; j += SW) {
+      const int o = s0 + 3 * j;
+      const int c = col[o];
+      const TV x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
+      a0 += (TV)val[o] * x0 + (TV)val[o + 1] * x1 + (TV)val[o + 2] * x2;
+      a1 += (TV)val[o + L] * x0 + (TV)val[o + L + 1] * x1 + (TV)val[o + L + 2] * x2;
+      a2 += (TV)val[o + 2 * L] * x0 + (TV)val[o + 2 * L + 1] * x1 + (TV)val[o + 2 * L + 2] * x2;
+    }
+  } else {
+    for (int t = lane; t < L; t += SW) {
+      const TV xv = x[col[s0 + t]];
+      a0 += (TV)val[s0 + t] * xv;
+      if (MODE == 1) {
+        if (nr > 1) a1 += (TV)val[s0 + L + t] * xv;
+        if (nr > 2) a2 += (TV)val[s0 + 2 * L + t] * xv;
+      }
     }
   }
 #pragma unroll
   for (int off = SW / 2; off > 0; off >>= 1) {
     a0 += __shfl_xor(a0, off, SW);
-    if (NODE) {
+    if (MODE) {
       a1 += __shfl_xor(a1, off, SW);
       a2 += __shfl_xor(a2, off, SW);
     }
@@ -75,7 +92,7 @@ __device__ __forceinline__ void write_partial(double *part, double v) {
 }
 
 // y = A x   (or y = b - A x when b != nullptr); grid-stride over groups
-template <typename TM, typename TV, int SW, bool NODE>
+template <typename TM, typename TV, int SW, int MODE>
 __global__ __launch_bounds__(kBlock) void k_spmv(int ngroups, const int32_t *__restrict__ grp,
                                                  const int32_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ col,
@@ -87,9 +104,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int ngroups, const int32_t *__r
     const int g = (int)(g0 + threadIdx.x / SW);
     if (g >= ngroups) continue;
     TV o[3];
-    const int r0 = group_spmv<TM, TV, SW, NODE>(g, lane, grp, rowptr, col, val, x, o);
+    const int r0 = group_spmv<TM, TV, SW, MODE>(g, lane, grp, rowptr, col, val, x, o);
     if (lane == 0) {
-      const int nr = NODE ? grp[g + 1] - r0 : 1;
+      const int nr = MODE ? grp[g + 1] - r0 : 1;
       for (int j = 0; j < nr; ++j) y[r0 + j] = (b != nullptr) ? b[r0 + j] - o[j] : o[j];
     }
   }
@@ -120,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int n, const TV *__restrict
 
 // iteration i = ITER+1: q = A p and p.q partials; workgroup 0 advances ITER and
 // records the SpMV grid (the number of p.q partials)
-template <typename TM, typename TV, int SW, bool NODE>
+template <typename TM, typename TV, int SW, int MODE>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int ngroups, const int32_t *__restrict__ grp,
                                                      const int32_t *__restrict__ rowptr,
                                                      const int32_t *__restrict__ col,
@@ -135,9 +152,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int ngroups, const int32_t 
     const int g = (int)(g0 + threadIdx.x / SW);
     if (g >= ngroups) continue;
     TV o[3];
-    const int r0 = group_spmv<TM, TV, SW, NODE>(g, lane, grp, rowptr, col, val, p, o);
+    const int r0 = group_spmv<TM, TV, SW, MODE>(g, lane, grp, rowptr, col, val, p, o);
     if (lane == 0) {
-      const int nr = NODE ? grp[g + 1] - r0 : 1;
+      const int nr = MODE ? grp[g + 1] - r0 : 1;
       for (int j = 0; j < nr; ++j) {
         q[r0 + j] = o[j];
         pq += (double)o[j] * (double)p[r0 + j];
@@ -211,16 +228,20 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_inv(int n, const int32_t *__r
 // ------------------------------------------------------------------ host side
 struct SpmvShape {
   int sw;
-  bool node;
+  int mode;
   int ngroups;
 };
 
+// n_groups < 0 marks a triple-structured group table (every group: 3 rows whose
+// column list is whole neighbour triples); |n_groups| is the group count.
 static SpmvShape shape_of(int n, int64_t nnz, int n_groups, const int32_t *grp) {
   SpmvShape s;
-  s.node = (grp != nullptr);
-  s.ngroups = s.node ? n_groups : n;
+  s.mode = (grp == nullptr) ? 0 : (n_groups < 0 ? 2 : 1);
+  s.ngroups = (grp == nullptr) ? n : (n_groups < 0 ? -n_groups : n_groups);
   const double len = n > 0 ? (double)nnz / n : 0.0;  // row length
-  s.sw = len > 48 ? 32 : (len > 24 ? 16 : (len > 12 ? 8 : 4));
+  const double work = (s.mode == 2) ? len / 3.0 : len;  // lane work items per group
+  s.sw = work > 48 ? 32 : (work > 24 ? 16 : (work > 12 ? 8 : 4));
+  if (s.mode == 2) s.sw = work > 16 ? 32 : (work > 8 ? 16 : 8);
   return s;
 }
 
@@ -229,18 +250,26 @@ static int spmv_grid(const SpmvShape &s) {
   return (int)(g < 1 ? 1 : (g > G1_MAX ? G1_MAX : g));
 }
 
+#define FEA_SW_SWITCH(MACRO)                                                         \
+  do {                                                                               \
+    if (s.mode == 2) {                                                               \
+      switch (s.sw) { case 8: MACRO(8, 2); break; case 16: MACRO(16, 2); break;      \
+                      default: MACRO(32, 2); }                                       \
+    } else if (s.mode == 1) {                                                        \
+      switch (s.sw) { case 4: MACRO(4, 1); break; case 8: MACRO(8, 1); break;        \
+                      case 16: MACRO(16, 1); break; default: MACRO(32, 1); }         \
+    } else {                                                                         \
+      switch (s.sw) { case 4: MACRO(4, 0); break; case 8: MACRO(8, 0); break;        \
+                      case 16: MACRO(16, 0); break; default: MACRO(32, 0); }         \
+    }                                                                                \
+  } while (0)
+
 template <typename TM, typename TV>
 static void launch_spmv(const SpmvShape &s, const int32_t *grp, const int32_t *rp, const int32_t *c,
                         const TM *v, const TV *x, const TV *b, TV *y, hipStream_t st) {
   const int grid = spmv_grid(s);
-#define FEA_SPMV(SW, ND) k_spmv<TM, TV, SW, ND><<<grid, kBlock, 0, st>>>(s.ngroups, grp, rp, c, v, x, b, y)
-  if (s.node) {
-    switch (s.sw) { case 4: FEA_SPMV(4, true); break; case 8: FEA_SPMV(8, true); break;
-                    case 16: FEA_SPMV(16, true); break; default: FEA_SPMV(32, true); }
-  } else {
-    switch (s.sw) { case 4: FEA_SPMV(4, false); break; case 8: FEA_SPMV(8, false); break;
-                    case 16: FEA_SPMV(16, false); break; default: FEA_SPMV(32, false); }
-  }
+#define FEA_SPMV(SW, MD) k_spmv<TM, TV, SW, MD><<<grid, kBlock, 0, st>>>(s.ngroups, grp, rp, c, v, x, b, y)
+  FEA_SW_SWITCH(FEA_SPMV);
 #undef FEA_SPMV
 }
 
@@ -248,14 +277,8 @@ template <typename TM, typename TV>
 static void launch_pcg_spmv(const SpmvShape &s, const int32_t *grp, const int32_t *rp, const int32_t *c,
                             const TM *v, const TV *p, TV *q, double *ws, hipStream_t st) {
   const int grid = spmv_grid(s);
-#define FEA_PSPMV(SW, ND) k_pcg_spmv<TM, TV, SW, ND><<<grid, kBlock, 0, st>>>(s.ngroups, grp, rp, c, v, p, q, ws)
-  if (s.node) {
-    switch (s.sw) { case 4: FEA_PSPMV(4, true); break; case 8: FEA_PSPMV(8, true); break;
-                    case 16: FEA_PSPMV(16, true); break; default: FEA_PSPMV(32, true); }
-  } else {
-    switch (s.sw) { case 4: FEA_PSPMV(4, false); break; case 8: FEA_PSPMV(8, false); break;
-                    case 16: FEA_PSPMV(16, false); break; default: FEA_PSPMV(32, false); }
-  }
+#define FEA_PSPMV(SW, MD) k_pcg_spmv<TM, TV, SW, MD><<<grid, kBlock, 0, st>>>(s.ngroups, grp, rp, c, v, p, q, ws)
+  FEA_SW_SWITCH(FEA_PSPMV);
 #undef FEA_PSPMV
 }
 
@@ -322,7 +345,7 @@ extern "C" int fea_spmv(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_
                         const int32_t *rowptr, const int32_t *col, const void *val, const void *x, void *y,
                         void *stream) {
   FEA_CHECK_ARG(valid_combo(mat_dtype, vec_dtype), "fea_spmv: unsupported precision pair");
-  FEA_CHECK_ARG(n >= 0 && (grp == nullptr || n_groups >= 0), "fea_spmv: bad sizes");
+  FEA_CHECK_ARG(n >= 0, "fea_spmv: bad sizes");
   if (n == 0) return FEA_OK;
   FEA_CHECK_ARG(rowptr && col && val && x && y, "fea_spmv: null argument");
   const SpmvShape s = shape_of(n, nnz, n_groups, grp);

@@ -126,6 +126,23 @@ class DevicePlan:
                     raise RuntimeError('CSR rows of a node differ in length')
             self.n_groups = int(grp.shape[0] - 1)
             self.grp = _i32(grp, dev)
+            # triple structure (selects the triple-blocked SpMV): every group has 3 rows and
+            # its column list is whole neighbour triples
+            heads = grp[:-1]
+            triple = bool(np.all(np.diff(grp) == 3)) and self.nnz > 0
+            if triple:
+                triple = bool(np.all(rl[heads] % 3 == 0))
+            if triple:
+                hrow = np.zeros(self.n_free, dtype=bool)
+                hrow[heads] = True
+                rows = np.repeat(np.arange(self.n_free), rl)
+                pos = np.arange(self.nnz) - rowptr[rows]
+                first = hrow[rows] & (pos % 3 == 0)
+                c0 = col[:self.nnz][first]
+                idx = np.nonzero(first)[0]
+                triple = bool(np.all(col[idx + 1] == c0 + 1) and np.all(col[idx + 2] == c0 + 2))
+            self.triple = triple
+            self.spmv_groups = -self.n_groups if triple else self.n_groups
             self.rowptr, self.col = _i32(rowptr, dev), _i32(col, dev)
             self.diag_slot = _i32(diag, dev)
             self.blk_a, self.blk_b, self.blk_colpos = _i32(blk_a, dev), _i32(blk_b, dev), _i32(blk_cp, dev)
@@ -313,7 +330,7 @@ class DevicePlan:
     # ------------------------------------------------------------------ linear
     def spmv(self, val, x, y, mat_dtype, vec_dtype=None):
         vec_dtype = mat_dtype if vec_dtype is None else vec_dtype
-        call('fea_spmv', mat_dtype, vec_dtype, self.n_free, self.nnz, self.n_groups, self.grp, self.rowptr,
+        call('fea_spmv', mat_dtype, vec_dtype, self.n_free, self.nnz, self.spmv_groups, self.grp, self.rowptr,
              self.col, val, x, y, self._s())
         return y
 

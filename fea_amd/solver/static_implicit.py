@@ -27,6 +27,10 @@ Additions (keyword-only, defaults reproduce the reference):
   float64 resolution near convergence (0 reproduces the reference).
 * ``predictor``: with load increments, start increment k from the linear
   extrapolation 2 u_{k-1} - u_{k-2} of the converged states.
+* ``forcing='ew'``: inexact-Newton forcing term of Eisenstat and Walker
+  (choice 2, gamma = 0.9, alpha = 2 with the safeguard gamma eta_{k-1}^2)
+  for the PCG relative tolerance eta_k, clamped to [sqrt(cg_tol), eta_max];
+  the default 'fixed' keeps the reference's constant cg_tol.
 """
 from __future__ import annotations
 
@@ -48,7 +52,8 @@ class StaticImplicitSolver(BaseSolver):
                  precision: str = 'fp64', rel_tol: float | None = None, load_steps: int = 1,
                  cg_tol: float = 1e-5, cg_max_iter: int = 1500, ir_steps: int = 1,
                  armijo_slack: float = 0.0, predictor: bool = True, verbose: bool = False,
-                 pcg_chunk: int = 32, progress=None, step_tol: float = 1e-12) -> None:
+                 pcg_chunk: int = 32, progress=None, step_tol: float = 1e-12, forcing: str = 'fixed',
+                 eta_max: float = 1e-2) -> None:
         super().__init__()
         if precision not in ('fp64', 'mixed', 'mixed-f32', 'mixed-ir'):
             raise ValueError("precision must be 'fp64', 'mixed', 'mixed-f32' or 'mixed-ir'")
@@ -60,6 +65,10 @@ class StaticImplicitSolver(BaseSolver):
         self.precision = precision
         self.rel_tol = rel_tol
         self.step_tol = float(step_tol)
+        if forcing not in ('fixed', 'ew'):
+            raise ValueError("forcing must be 'fixed' or 'ew'")
+        self.forcing = forcing
+        self.eta_max = float(eta_max)
         self.load_steps = int(load_steps)
         self.cg_tol = float(cg_tol)
         self.cg_max_iter = int(cg_max_iter)
@@ -104,7 +113,7 @@ class StaticImplicitSolver(BaseSolver):
         if key not in self._pcg:
             p = self.assembly.plan
             self._pcg[key] = PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, mat_dtype, p.device,
-                                 vec_dtype=vec_dtype, grp=p.grp, n_groups=p.n_groups, chunk=self.pcg_chunk)
+                                 vec_dtype=vec_dtype, grp=p.grp, n_groups=p.spmv_groups, chunk=self.pcg_chunk)
         return self._pcg[key]
 
     def _vec_stats(self, x):
@@ -127,7 +136,7 @@ class StaticImplicitSolver(BaseSolver):
         _, K = p.tangent(U, scale, F32, with_residual=False)
         return R, K
 
-    def _solve_linear(self, K, R, dGC0, iter_now, alpha0):
+    def _solve_linear(self, K, R, dGC0, iter_now, alpha0, tol=None):
         """static_implicit.py:297-350 (CG branch; the Pardiso branch is replaced by PCG)."""
         p = self.assembly.plan
         if alpha0 < 1e-1:
@@ -144,7 +153,7 @@ class StaticImplicitSolver(BaseSolver):
         if self.precision in ('fp64', 'mixed', 'mixed-f32'):
             pcg = self._pcg_for(F64 if self.precision == 'fp64' else F32, F64)
             pcg.set_matrix(K)
-            x = pcg.solve(b, dGC0, tol=self.cg_tol, max_iter=max_iter)
+            x = pcg.solve(b, dGC0, tol=self.cg_tol if tol is None else tol, max_iter=max_iter)
             out.copy_(x)
             self.stats['cg_iters'] = self.stats.get('cg_iters', 0) + pcg.iterations
         else:
@@ -157,7 +166,7 @@ class StaticImplicitSolver(BaseSolver):
             for _ in range(max(1, self.ir_steps)):
                 call('fea_spmv_residual_mixed', n, p.rowptr, p.col, K, out, b,
                      r, part, s)
-                d = pcg.solve(r, None, tol=self.cg_tol, max_iter=max_iter)
+                d = pcg.solve(r, None, tol=self.cg_tol if tol is None else tol, max_iter=max_iter)
                 call('fea_axpy', n, 1.0, F32, d, out, s)
                 self.stats['cg_iters'] = self.stats.get('cg_iters', 0) + pcg.iterations
         return out
@@ -192,6 +201,7 @@ class StaticImplicitSolver(BaseSolver):
         Unew = torch.empty_like(U)
         low_alpha, alpha, it = 0, 0.0, 0
         r0 = self._r_ref
+        r_prev, eta_prev = None, self.eta_max
         while True:
             if self._iter_now >= self.maximum_iteration:
                 return False
@@ -205,7 +215,20 @@ class StaticImplicitSolver(BaseSolver):
                     r0 = math.sqrt(rss)
                 if math.sqrt(rss) <= self.rel_tol * r0:
                     return True
-            d = self._solve_linear(K, R, dGC, it, alpha)
+            tol = None
+            if self.forcing == 'ew':
+                rn = math.sqrt(rss)
+                eta_min = math.sqrt(self.cg_tol)
+                if r_prev is None:
+                    eta = self.eta_max
+                else:
+                    eta = 0.9 * (rn / r_prev) ** 2
+                    if 0.9 * eta_prev ** 2 > 0.1:
+                        eta = max(eta, 0.9 * eta_prev ** 2)
+                eta = min(max(eta, eta_min), self.eta_max)
+                r_prev, eta_prev = rn, eta
+                tol = eta * eta
+            d = self._solve_linear(K, R, dGC, it, alpha, tol)
             dGC.copy_(d)
             alpha, e_new = self._line_search(U, dGC, R, energy, scale, Unew)
             if alpha == 0:

@@ -177,9 +177,12 @@ int fea_reduce_sum(int n, const double *partial, double *out, void *stream);
  * Precision pairs (matrix values, vectors): (FEA_F64, FEA_F64) plain FP64,
  * (FEA_F32, FEA_F64) FP32 matrix storage with FP64 vectors / accumulation,
  * (FEA_F32, FEA_F32) FP32.
- * Row groups: grp[g] (int32 [n_groups+1]) is the first row of group g; the rows
+ * Row groups: grp[g] (int32 [|n_groups|+1]) is the first row of group g; the rows
  * of a group (at most 3, consecutive) must share one column list, as the free
  * rows of one node do in the fea_pattern CSR.  grp == NULL: one row per group.
+ * n_groups < 0 additionally asserts that every group has 3 rows and a column
+ * list made of whole neighbour-node triples (3 consecutive column indices),
+ * which selects the triple-blocked SpMV.
  *
  * CSR SpMV y = A x (the CSR `A @ p` of _linear_solver.py:27,32). */
 int fea_spmv(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,

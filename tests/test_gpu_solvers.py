@@ -34,7 +34,7 @@ def test_pcg_matches_direct_solve(gpu):
     b = -R.cpu().numpy()
     x_ref = spla.spsolve(A.tocsc(), b)
     pcg = solver.PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, F64, p.device, grp=p.grp,
-                     n_groups=p.n_groups)
+                     n_groups=p.spmv_groups)
     pcg.set_matrix(K)
     x = pcg.solve(-R, None, tol=1e-24, max_iter=5000).cpu().numpy()
     assert rel_err(x, x_ref) < 1e-8
@@ -125,8 +125,36 @@ def test_spmv_precision_pairs(gpu, pair, grouped):
     val = K.to(tm)
     xd = torch.as_tensor(x, device='cuda', dtype=tv)
     y = torch.zeros_like(xd)
-    call('fea_spmv', m, v, p.n_free, p.nnz, p.n_groups if grouped else 0, p.grp if grouped else None,
+    call('fea_spmv', m, v, p.n_free, p.nnz, p.spmv_groups if grouped else 0, p.grp if grouped else None,
          p.rowptr, p.col, val, xd, y, nat.stream_ptr())
     ref = A @ x
     tol = 1e-13 if pair == 'f64/f64' else (1e-6 if pair == 'f32/f64' else 1e-5)
     assert rel_err(y.double().cpu().numpy(), ref) < tol
+
+
+def test_spmv_node_modes_agree(gpu):
+    """Generic node-grouped and triple-blocked SpMV give the row-CSR result (beam: triple)."""
+    from fea_amd._native import F32, call
+    from fea_amd import _native as nat
+    fe, mesh = workloads.beam_traction(6, 3, 2)
+    fe.assembly.initialize()
+    p = fe.assembly.plan
+    assert p.triple
+    U = p.gc_to_u(torch.as_tensor(0.01 * np.random.default_rng(5).standard_normal(p.n_free), device='cuda'))
+    _, K = p.tangent(U, 1.0, F64)
+    x = torch.as_tensor(np.random.default_rng(6).standard_normal(p.n_free), device='cuda')
+    outs = []
+    for ng, grp in ((0, None), (p.n_groups, p.grp), (-p.n_groups, p.grp)):
+        y = torch.zeros_like(x)
+        call('fea_spmv', F64, F64, p.n_free, p.nnz, ng, grp, p.rowptr, p.col, K, x, y, nat.stream_ptr())
+        outs.append(y.cpu().numpy())
+    assert rel_err(outs[1], outs[0]) < 1e-14 and rel_err(outs[2], outs[0]) < 1e-14
+
+
+def test_ew_forcing_same_solution(gpu):
+    fe, om = cube_models(3, 0.2)
+    fe.solver = solver.StaticImplicitSolver(rel_tol=1e-10, load_steps=3, cg_tol=1e-12, armijo_slack=1e-12,
+                                            forcing='ew')
+    GC = fe.solve()
+    GC_o, _ = osol.static_solve(om, increments=3, rel_tol=1e-10, linear='direct', armijo_slack_rel=1e-12)
+    assert rel_err(GC.cpu().numpy(), GC_o) < 1e-8
