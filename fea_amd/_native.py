@@ -58,7 +58,13 @@ _SIGS = {
     'fea_dot_partial': (C_INT, [C_INT, C_P, C_P, C_P, C_P]),
     'fea_axpy': (C_INT, [C_INT, C_DBL, C_INT, C_P, C_P, C_P]),
     'fea_vec_stats': (C_INT, [C_INT, C_P, C_P, C_P]),
-    'fea_explicit_update': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_DBL, C_DBL, C_P, C_P, C_P, C_P]),
+    'fea_explicit_update': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_DBL, C_DBL, C_P, C_P, C_P, C_P]),
+    'fea_contact_grid_build': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_DBL, C_INT, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_contact_search': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_DBL, C_INT, C_DBL, C_INT, C_P,
+                                   C_P, C_P, C_P]),
+    'fea_contact_blocks': (C_INT, [C_INT]),
+    'fea_contact_forces': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_INT, C_P, C_P, C_INT, C_DBL, C_P, C_DBL, C_INT, C_P,
+                                   C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_I64, C_P, C_P, C_P]),
 }
 
 _lib = None

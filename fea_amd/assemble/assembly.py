@@ -173,13 +173,26 @@ class Assembly:
         R, val = self.plan.tangent(U, 1.0, F64)
         return R.clone(), self.K_indices(), val.clone()
 
+    def _contact_loads(self):
+        return [l for l in self._loads.values() if isinstance(l, loads.NodeToSurfaceContact)]
+
     def assemble_force(self, RGC=None, GC=None):
-        """assembly.py:230-281."""
-        return self.plan.residual(self._u_from(RGC, GC), 1.0, F64).clone()
+        """assembly.py:230-281 (contact loads: force only, subtracted as external forces)."""
+        U = self._u_from(RGC, GC)
+        R = self.plan.residual(U, 1.0, F64).clone()
+        for c in self._contact_loads():
+            R -= self.plan.u_to_gc(c.state(self.plan).forces(U, search=True))
+        return R
 
     def _total_Potential_Energy(self, RGC=None, GC=None):
-        """assembly.py:389-415: internal energy minus load potentials."""
-        return torch.tensor(self.plan.energy(self._u_from(RGC, GC), 1.0), dtype=torch.float64)
+        """assembly.py:389-415: internal energy minus load potentials (plus contact energy)."""
+        U = self._u_from(RGC, GC)
+        E = self.plan.energy(U, 1.0)
+        for c in self._contact_loads():
+            st = c.state(self.plan)
+            st.forces(U, search=True)
+            E += st.energy()
+        return torch.tensor(E, dtype=torch.float64)
 
     # ------------------------------------------------------------------ registries
     @staticmethod

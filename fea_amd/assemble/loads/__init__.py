@@ -1,3 +1,4 @@
 from .base import BaseLoad
 from .pressure import Pressure
 from .traction import Traction
+from .contact import NodeToSurfaceContact

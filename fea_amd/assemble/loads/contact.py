@@ -1,0 +1,198 @@
+"""Node-to-surface penalty contact (new law, configs with two bodies).
+
+Slave nodes (the nodes of a surface set of one instance) are pushed out of
+master T3 facets (a surface set of another instance, faces oriented outward as
+the reference's element faces are, tetrahedral.py:14-19) with force -k A g n
+when they penetrate (gap g < 0); k is a stiffness per unit area and A the
+node's tributary area on its surface, so the law does not depend on the mesh
+size.  The reference's contact (contact.py:11-1217) is a smooth
+Gauss-point-pair barrier with a cKDTree broad phase; this load replaces its role
+in explicit dynamics (whose reference force-only branches are broken,
+contact.py:658-659, :1140-1142) with the uniform-grid search of the HIP library.
+"""
+import numpy as np
+import torch
+
+from ... import _native as nat
+from ..._native import call
+from .base import BaseLoad
+
+
+class NodeToSurfaceContact(BaseLoad):
+
+    def __init__(self, slave_instance: str, slave_surface: str, master_instance: str, master_surface: str,
+                 penalty: float = 1e3, radius: float = None, depth_max: float = None, kmax: int = 64,
+                 search_interval: int = 1) -> None:
+        super().__init__()
+        self.slave_instance, self.slave_surface = slave_instance, slave_surface
+        self.master_instance, self.master_surface = master_instance, master_surface
+        self.penalty = float(penalty)
+        self.radius = radius
+        self.depth_max = depth_max
+        self.kmax = int(kmax)
+        self.search_interval = int(search_interval)
+        self._state = None
+
+    def initialize(self, assembly):
+        super().initialize(assembly)
+        si = assembly.get_instance(self.slave_instance)
+        mi = assembly.get_instance(self.master_instance)
+        soff = assembly.RGC_list_indexStart[si._RGC_index] // 3
+        moff = assembly.RGC_list_indexStart[mi._RGC_index] // 3
+        self.slave_nodes, self.slave_area = _tributary_areas(si, self.slave_surface)
+        self.slave_nodes = self.slave_nodes + soff
+        facets = [s for s in mi.surfaces.get_elements(self.master_surface) if s._elems.shape[0] > 0]
+        if any(s.num_nodes_per_elem != 3 for s in facets):
+            raise NotImplementedError('node-to-surface contact supports T3 master facets')
+        if not facets:
+            raise ValueError('master surface %r has no facets' % self.master_surface)
+        self.master_conn = np.concatenate([s._elems.cpu().numpy() for s in facets]).astype(np.int64) + moff
+        X = mi.nodes.detach().cpu().numpy()
+        tri = X[self.master_conn - moff]
+        edge = float(np.mean([np.linalg.norm(tri[:, (k + 1) % 3] - tri[:, k], axis=1) for k in range(3)]))
+        # a slave within depth_max of a facet's plane whose projection lies inside the
+        # facet is within depth_max + (largest vertex-centroid distance) of its centroid
+        rho = float(np.linalg.norm(tri - tri.mean(axis=1, keepdims=True), axis=2).max())
+        if self.depth_max is None:
+            self.depth_max = edge
+        if self.radius is None:
+            self.radius = float(self.depth_max) + 1.25 * rho
+        self.origin = tri.reshape(-1, 3).min(0) - self.radius
+        self._state = None
+
+    def set_required_DoFs(self, RGC_remain_index):
+        return RGC_remain_index
+
+    def device_groups(self, assembly):
+        return []
+
+    def state(self, plan):
+        if self._state is None or self._state.plan is not plan:
+            self._state = ContactState(self, plan)
+        return self._state
+
+    def get_stiffness(self, RGC, if_onlyforce=False, *args, **kwargs):
+        if not if_onlyforce:
+            raise NotImplementedError('node-to-surface contact provides forces only (explicit dynamics)')
+        plan = self._assembly.plan
+        U = self._assembly._u_from(RGC=RGC)
+        fc = self.state(plan).forces(U, search=True)
+        idx = torch.arange(fc.numel(), device=fc.device)
+        return idx, fc.clone()
+
+    def get_potential_energy(self, RGC):
+        plan = self._assembly.plan
+        U = self._assembly._u_from(RGC=RGC)
+        st = self.state(plan)
+        st.forces(U, search=True)
+        return -st.energy()
+
+
+def _tributary_areas(instance, surface):
+    """Unique nodes of a surface set and their tributary areas (a third of each
+    adjacent facet's area, reference configuration; Q4 facets split in two T3)."""
+    X = instance.nodes.detach().cpu().numpy()
+    tris = []
+    for s in instance.surfaces.get_elements(surface):
+        e = s._elems.cpu().numpy()
+        if s.num_nodes_per_elem == 3:
+            tris.append(e)
+        elif s.num_nodes_per_elem == 4:
+            tris += [e[:, [0, 1, 2]], e[:, [0, 2, 3]]]
+        else:
+            tris.append(e[:, :3])           # T6: corner nodes only
+    if not tris:
+        raise ValueError('slave surface %r has no facets' % surface)
+    tri = np.concatenate(tris).astype(np.int64)
+    P = X[tri]
+    a = 0.5 * np.linalg.norm(np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]), axis=1)
+    nodes, inv = np.unique(tri.reshape(-1), return_inverse=True)
+    area = np.bincount(inv.reshape(-1), weights=np.repeat(a / 3.0, 3), minlength=nodes.shape[0])
+    return nodes, area
+
+
+class ContactState:
+    """Device buffers and the search / force pipeline of one contact pair set."""
+
+    def __init__(self, load, plan):
+        self.plan, self.load = plan, load
+        dev = plan.device
+        S, F = load.slave_nodes.shape[0], load.master_conn.shape[0]
+        self.S, self.F = S, F
+        self.bits = max(4, int(np.ceil(np.log2(max(2 * F, 16)))))
+        nb = 1 << self.bits
+        i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=dev)
+        self.slave = i32(load.slave_nodes)
+        self.area = torch.as_tensor(np.ascontiguousarray(load.slave_area, dtype=np.float64), device=dev)
+        self.conn = i32(load.master_conn)
+        mnodes, inv = np.unique(load.master_conn.reshape(-1), return_inverse=True)
+        inv = inv.reshape(-1, 3)
+        order = np.argsort(inv.reshape(-1), kind='stable')
+        fac = np.repeat(np.arange(F), 3)[order]
+        vert = np.tile(np.arange(3), F)[order]
+        ptr = np.concatenate([[0], np.cumsum(np.bincount(inv.reshape(-1), minlength=mnodes.shape[0]))])
+        self.mnodes, self.nf_ptr, self.nf_facet, self.nf_vert = i32(mnodes), i32(ptr), i32(fac), i32(vert)
+        z = lambda n, dt: torch.zeros(max(1, n), dtype=dt, device=dev)
+        self.cent = z(3 * F, torch.float64)
+        self.key = z(F, torch.int32)
+        self.bstart = z(nb + 1, torch.int32)
+        self.bitems = z(F, torch.int32)
+        self.work = z(max(nb, F), torch.int32)
+        self.cand = z(S * load.kmax, torch.int32)
+        self.count = z(S, torch.int32)
+        self.max_count = z(1, torch.int32)
+        self.chosen = z(S, torch.int32)
+        self.force = z(3 * S, torch.float64)
+        self.weight = z(3 * S, torch.float64)
+        self.sb_start = z(F + 1, torch.int32)
+        self.sb_items = z(S, torch.int32)
+        self.fc = z(3 * plan.n_nodes, torch.float64)[:3 * plan.n_nodes]
+        self.epart = z(nat.lib().fea_contact_blocks(S), torch.float64)
+        self.eout = z(1, torch.float64)
+        self.origin = np.ascontiguousarray(load.origin, dtype=np.float64)
+        self.steps = 0
+        self.searched = False
+
+    def search(self, U, check=True):
+        """Rebuild the facet grid at U and find every slave's candidate facets.
+        With check=False the overflow test is deferred to ``check()``."""
+        L, s = self.load, nat.stream_ptr()
+        call('fea_contact_grid_build', self.F, self.conn, self.plan.X, U, self.origin, float(L.radius),
+             self.bits, self.cent, self.key, self.bstart, self.bitems, self.work, s)
+        call('fea_contact_search', self.S, self.slave, self.plan.X, U, self.cent, self.bstart, self.bitems,
+             self.origin, float(L.radius), self.bits, float(L.radius), L.kmax, self.cand, self.count,
+             self.max_count, s)
+        self.searched = True
+        if check:
+            self.check()
+
+    def check(self):
+        """Raise if any search since the last check found more than kmax candidates."""
+        mx = int(self.max_count.item())
+        if mx > self.load.kmax:
+            self.max_count.zero_()
+            raise RuntimeError('contact search: %d candidate facets for one node exceed kmax=%d '
+                               '(reduce the search radius)' % (mx, self.load.kmax))
+
+    def pairs(self):
+        """Candidate (slave position, facet) pairs as a sorted int64 array [P, 2]."""
+        cnt = self.count[:self.S].cpu().numpy()
+        cand = self.cand[:self.S * self.load.kmax].cpu().numpy().reshape(self.S, self.load.kmax)
+        out = [(s, f) for s in range(self.S) for f in cand[s, :cnt[s]]]
+        return np.array(out, dtype=np.int64).reshape(-1, 2)
+
+    def forces(self, U, search=None, check=True):
+        L = self.load
+        if search or (search is None and (not self.searched or self.steps % L.search_interval == 0)):
+            self.search(U, check=check)
+        self.steps += 1
+        call('fea_contact_forces', self.S, self.slave, self.plan.X, U, self.conn, self.F, self.cand, self.count,
+             L.kmax, L.penalty, self.area, float(L.depth_max), int(self.mnodes.numel()), self.mnodes, self.nf_ptr,
+             self.nf_facet, self.nf_vert, self.chosen, self.force, self.weight, self.sb_start, self.sb_items,
+             self.work, 3 * self.plan.n_nodes, self.fc, self.epart, nat.stream_ptr())
+        return self.fc
+
+    def energy(self):
+        nb = nat.lib().fea_contact_blocks(self.S)
+        call('fea_reduce_sum', nb, self.epart, self.eout, nat.stream_ptr())
+        return float(self.eout.item())

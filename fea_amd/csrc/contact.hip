@@ -1,0 +1,404 @@
+// Node-to-surface penalty contact with a uniform-grid broad phase (new law;
+// the reference's contact is a Gauss-point-pair barrier with a scipy cKDTree
+// broad phase, contact.py:86-122, :385-401).
+//
+// Candidate predicate (exact, for bit-exact pair sets):
+//   c_f  = (x_a + x_b + x_c) / 3            (float64, per component)
+//   d2   = dx*dx + dy*dy + dz*dz, dx = x_s - c_f   (left to right, no FMA)
+//   pair (s, f) is a candidate iff d2 <= r2.
+// Deterministic throughout: bucket sorts order items by id inside each bucket
+// and every force is summed by one thread in a fixed order (no float atomics).
+#include "common.h"
+
+namespace fea {
+
+__device__ __forceinline__ double dadd(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ double dmul(double a, double b) { return __dmul_rn(a, b); }
+
+__device__ __forceinline__ void node_pos(const double *X, const double *U, int64_t n, double p[3]) {
+  p[0] = dadd(X[3 * n + 0], U[3 * n + 0]);
+  p[1] = dadd(X[3 * n + 1], U[3 * n + 1]);
+  p[2] = dadd(X[3 * n + 2], U[3 * n + 2]);
+}
+
+__device__ __forceinline__ unsigned int cell_key(int64_t ix, int64_t iy, int64_t iz, unsigned int mask) {
+  const uint64_t h = ((uint64_t)ix * 73856093ull) ^ ((uint64_t)iy * 19349663ull) ^ ((uint64_t)iz * 83492791ull);
+  return (unsigned int)(h & mask);
+}
+
+__device__ __forceinline__ int64_t cell_of(double x, double o, double inv_h) {
+  return (int64_t)floor((x - o) * inv_h);
+}
+
+// centroids and hashed cell keys of the master facets
+__global__ __launch_bounds__(kBlock) void k_facet_keys(int nf, const int32_t *__restrict__ conn,
+                                                       const double *__restrict__ X,
+                                                       const double *__restrict__ U, double ox, double oy,
+                                                       double oz, double inv_h, unsigned int mask,
+                                                       double *__restrict__ cent, int32_t *__restrict__ key) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  double a[3], b[3], c[3];
+  node_pos(X, U, conn[3 * (int64_t)f + 0], a);
+  node_pos(X, U, conn[3 * (int64_t)f + 1], b);
+  node_pos(X, U, conn[3 * (int64_t)f + 2], c);
+  double m[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) m[i] = dadd(dadd(a[i], b[i]), c[i]) / 3.0;
+  cent[3 * (int64_t)f + 0] = m[0];
+  cent[3 * (int64_t)f + 1] = m[1];
+  cent[3 * (int64_t)f + 2] = m[2];
+  key[f] = (int32_t)cell_key(cell_of(m[0], ox, inv_h), cell_of(m[1], oy, inv_h), cell_of(m[2], oz, inv_h), mask);
+}
+
+// ---------------------------------------------------------------- bucket sort
+__global__ __launch_bounds__(kBlock) void k_fill_int(int n, int32_t *__restrict__ a, int32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bucket_count(int n, const int32_t *__restrict__ keys,
+                                                         int32_t *__restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && keys[i] >= 0) atomicAdd(&cnt[keys[i]], 1);
+}
+
+// exclusive scan of cnt[0..nb) into start[0..nb], single workgroup
+__global__ __launch_bounds__(1024) void k_bucket_scan(int nb, const int32_t *__restrict__ cnt,
+                                                      int32_t *__restrict__ start) {
+  __shared__ int32_t part[1024];
+  const int t = threadIdx.x;
+  const int chunk = (nb + 1023) / 1024;
+  const int b0 = t * chunk, b1 = min(nb, b0 + chunk);
+  int32_t s = 0;
+  for (int b = b0; b < b1; ++b) s += cnt[b];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int32_t acc = 0;
+    for (int i = 0; i < 1024; ++i) {
+      const int32_t v = part[i];
+      part[i] = acc;
+      acc += v;
+    }
+    start[nb] = acc;
+  }
+  __syncthreads();
+  int32_t acc = part[t];
+  for (int b = b0; b < b1; ++b) {
+    start[b] = acc;
+    acc += cnt[b];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bucket_fill(int n, const int32_t *__restrict__ keys,
+                                                        const int32_t *__restrict__ start,
+                                                        int32_t *__restrict__ cursor,
+                                                        int32_t *__restrict__ items) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || keys[i] < 0) return;
+  const int k = keys[i];
+  items[start[k] + atomicAdd(&cursor[k], 1)] = i;
+}
+
+// insertion sort of each bucket by item id (buckets are small)
+__global__ __launch_bounds__(kBlock) void k_bucket_sort(int nb, const int32_t *__restrict__ start,
+                                                        int32_t *__restrict__ items) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int s0 = start[b], s1 = start[b + 1];
+  for (int i = s0 + 1; i < s1; ++i) {
+    const int32_t v = items[i];
+    int j = i - 1;
+    while (j >= s0 && items[j] > v) {
+      items[j + 1] = items[j];
+      --j;
+    }
+    items[j + 1] = v;
+  }
+}
+
+// ---------------------------------------------------------------- search
+constexpr int kMaxCand = 64;
+
+__global__ __launch_bounds__(kBlock) void k_contact_search(
+    int ns, const int32_t *__restrict__ slave, const double *__restrict__ X, const double *__restrict__ U,
+    const double *__restrict__ cent, const int32_t *__restrict__ bstart, const int32_t *__restrict__ bitems,
+    double ox, double oy, double oz, double inv_h, unsigned int mask, double r2, int kmax,
+    int32_t *__restrict__ cand, int32_t *__restrict__ count, int32_t *__restrict__ max_count) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  double p[3];
+  node_pos(X, U, slave[s], p);
+  const int64_t cx = cell_of(p[0], ox, inv_h), cy = cell_of(p[1], oy, inv_h), cz = cell_of(p[2], oz, inv_h);
+  unsigned int seen[27];
+  int nseen = 0;
+  int32_t loc[kMaxCand];
+  int nloc = 0, total = 0;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const unsigned int k = cell_key(cx + dx, cy + dy, cz + dz, mask);
+        bool dup = false;
+        for (int i = 0; i < nseen; ++i) dup = dup || (seen[i] == k);
+        if (dup) continue;
+        seen[nseen++] = k;
+        for (int it = bstart[k]; it < bstart[k + 1]; ++it) {
+          const int f = bitems[it];
+          const double ddx = p[0] - cent[3 * (int64_t)f + 0];
+          const double ddy = p[1] - cent[3 * (int64_t)f + 1];
+          const double ddz = p[2] - cent[3 * (int64_t)f + 2];
+          const double d2 = dadd(dadd(dmul(ddx, ddx), dmul(ddy, ddy)), dmul(ddz, ddz));
+          if (d2 <= r2) {
+            ++total;
+            if (nloc < kMaxCand) loc[nloc++] = f;
+          }
+        }
+      }
+  // ascending facet ids
+  for (int i = 1; i < nloc; ++i) {
+    const int32_t v = loc[i];
+    int j = i - 1;
+    while (j >= 0 && loc[j] > v) {
+      loc[j + 1] = loc[j];
+      --j;
+    }
+    loc[j + 1] = v;
+  }
+  const int nout = nloc < kmax ? nloc : kmax;
+  for (int i = 0; i < nout; ++i) cand[(int64_t)s * kmax + i] = loc[i];
+  count[s] = total;
+  if (max_count != nullptr && total > kmax) atomicMax(max_count, total);
+}
+
+// ---------------------------------------------------------------- penalty
+// Facet choice per slave, over the candidates whose plane it penetrates
+// (-depth_max < g < 0), with v = the smallest barycentric coordinate of the
+// slave's projection on the facet plane:
+//   tier 0  v >= -kInsideTol (inside; edges and vertices count, whatever the
+//           rounding, since conforming meshes put slaves right over master
+//           nodes): the largest g wins (least penetration);
+//   tier 1  -kExtendTol <= v < -kInsideTol, used only when no facet is in
+//           tier 0: the largest v wins.  This closes the dead zone under a
+//           concave crease (a valley, such as the dent a slave presses into the
+//           master surface), where a penetrating slave's projections on both
+//           adjacent planes fall just outside their facets and the slave would
+//           otherwise drop out of contact and re-enter deeper (an energy jump).
+// Ties keep the smaller facet id.  Force F = -k A_s g n on the slave (A_s its
+// tributary area, 1 when area is null).
+constexpr double kInsideTol = 1e-10;
+constexpr double kExtendTol = 0.1;
+
+__global__ __launch_bounds__(kBlock) void k_contact_penalty(
+    int ns, const int32_t *__restrict__ slave, const double *__restrict__ X, const double *__restrict__ U,
+    const int32_t *__restrict__ conn, const int32_t *__restrict__ cand, const int32_t *__restrict__ count,
+    int kmax, double k_pen, const double *__restrict__ area, double depth_max, int32_t *__restrict__ chosen,
+    double *__restrict__ force,
+    double *__restrict__ weight, double *__restrict__ energy_partial) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0;
+  if (s < ns) {
+    double p[3];
+    node_pos(X, U, slave[s], p);
+    int best = -1, btier = 2;
+    double bg = 0.0, bkey = 0.0, bn[3] = {0, 0, 0}, bw[3] = {0, 0, 0};
+    const int nc = min(count[s], kmax);
+    for (int i = 0; i < nc; ++i) {
+      const int f = cand[(int64_t)s * kmax + i];
+      double a[3], b[3], c[3];
+      node_pos(X, U, conn[3 * (int64_t)f + 0], a);
+      node_pos(X, U, conn[3 * (int64_t)f + 1], b);
+      node_pos(X, U, conn[3 * (int64_t)f + 2], c);
+      double e1[3], e2[3], ap[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        e1[j] = b[j] - a[j];
+        e2[j] = c[j] - a[j];
+        ap[j] = p[j] - a[j];
+      }
+      double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+      const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      if (!(nn > 0.0)) continue;
+      n[0] /= nn;
+      n[1] /= nn;
+      n[2] /= nn;
+      const double g = ap[0] * n[0] + ap[1] * n[1] + ap[2] * n[2];
+      if (!(g < 0.0) || !(g > -depth_max)) continue;
+      double q[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) q[j] = ap[j] - g * n[j];
+      const double d00 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+      const double d01 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+      const double d11 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+      const double d20 = q[0] * e1[0] + q[1] * e1[1] + q[2] * e1[2];
+      const double d21 = q[0] * e2[0] + q[1] * e2[1] + q[2] * e2[2];
+      const double den = d00 * d11 - d01 * d01;
+      const double xi = (d11 * d20 - d01 * d21) / den;
+      const double eta = (d00 * d21 - d01 * d20) / den;
+      const double w0 = 1.0 - xi - eta;
+      const double v = fmin(w0, fmin(xi, eta));
+      if (!(v >= -kExtendTol)) continue;
+      const int tier = (v >= -kInsideTol) ? 0 : 1;
+      const double key = (tier == 0) ? g : v;
+      if (tier < btier || (tier == btier && key > bkey)) {
+        best = f;
+        btier = tier;
+        bkey = key;
+        bg = g;
+        bn[0] = n[0];
+        bn[1] = n[1];
+        bn[2] = n[2];
+        bw[0] = w0;
+        bw[1] = xi;
+        bw[2] = eta;
+      }
+    }
+    chosen[s] = best;
+    const double ks = (area != nullptr) ? k_pen * area[s] : k_pen;
+    const double fm = (best >= 0) ? -ks * bg : 0.0;
+    force[3 * (int64_t)s + 0] = fm * bn[0];
+    force[3 * (int64_t)s + 1] = fm * bn[1];
+    force[3 * (int64_t)s + 2] = fm * bn[2];
+    weight[3 * (int64_t)s + 0] = bw[0];
+    weight[3 * (int64_t)s + 1] = bw[1];
+    weight[3 * (int64_t)s + 2] = bw[2];
+    if (best >= 0) e = 0.5 * ks * bg * bg;
+  }
+  if (energy_partial != nullptr) {
+    const double t = block_sum(e);
+    if (threadIdx.x == 0) energy_partial[blockIdx.x] = t;
+  }
+}
+
+// master-node reactions: for master node m, over its facets (static adjacency,
+// with m's local vertex in each) and the slaves that chose each facet (bucket
+// sorted by slave id): fc[m] -= F_s * w_s(local vertex).
+__global__ __launch_bounds__(kBlock) void k_contact_master(
+    int nm, const int32_t *__restrict__ mnode, const int32_t *__restrict__ nf_ptr,
+    const int32_t *__restrict__ nf_facet, const int32_t *__restrict__ nf_vert,
+    const int32_t *__restrict__ sb_start, const int32_t *__restrict__ sb_items,
+    const double *__restrict__ force, const double *__restrict__ weight, double *__restrict__ fc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int t = nf_ptr[i]; t < nf_ptr[i + 1]; ++t) {
+    const int f = nf_facet[t], v = nf_vert[t];
+    for (int it = sb_start[f]; it < sb_start[f + 1]; ++it) {
+      const int s = sb_items[it];
+      const double w = weight[3 * (int64_t)s + v];
+      acc[0] -= force[3 * (int64_t)s + 0] * w;
+      acc[1] -= force[3 * (int64_t)s + 1] * w;
+      acc[2] -= force[3 * (int64_t)s + 2] * w;
+    }
+  }
+  const int64_t m = mnode[i];
+  fc[3 * m + 0] += acc[0];
+  fc[3 * m + 1] += acc[1];
+  fc[3 * m + 2] += acc[2];
+}
+
+__global__ __launch_bounds__(kBlock) void k_contact_slave(int ns, const int32_t *__restrict__ slave,
+                                                          const double *__restrict__ force,
+                                                          double *__restrict__ fc) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const int64_t n = slave[s];
+  fc[3 * n + 0] += force[3 * (int64_t)s + 0];
+  fc[3 * n + 1] += force[3 * (int64_t)s + 1];
+  fc[3 * n + 2] += force[3 * (int64_t)s + 2];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_dbl(int64_t n, double *__restrict__ a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+
+static int bucket_sort(int n, const int32_t *keys, int nb, int32_t *start, int32_t *items, int32_t *work,
+                       hipStream_t st) {
+  k_fill_int<<<blocks_for(nb), kBlock, 0, st>>>(nb, work, 0);
+  k_bucket_count<<<blocks_for(n > 0 ? n : 1), kBlock, 0, st>>>(n, keys, work);
+  k_bucket_scan<<<1, 1024, 0, st>>>(nb, work, start);
+  k_fill_int<<<blocks_for(nb), kBlock, 0, st>>>(nb, work, 0);
+  k_bucket_fill<<<blocks_for(n > 0 ? n : 1), kBlock, 0, st>>>(n, keys, start, work, items);
+  k_bucket_sort<<<blocks_for(nb), kBlock, 0, st>>>(nb, start, items);
+  FEA_CHECK_LAUNCH("bucket sort");
+  return FEA_OK;
+}
+
+}  // namespace fea
+
+using namespace fea;
+
+extern "C" int fea_contact_grid_build(int n_facets, const int32_t *conn, const double *X, const double *U,
+                                      const double *origin_host, double cell, int table_bits,
+                                      double *centroid, int32_t *key, int32_t *bucket_start,
+                                      int32_t *bucket_items, int32_t *work, void *stream) {
+  FEA_CHECK_ARG(n_facets >= 0 && origin_host && cell > 0.0 && table_bits >= 1 && table_bits <= 28,
+                "fea_contact_grid_build: bad arguments");
+  FEA_CHECK_ARG(n_facets == 0 || (conn && X && U && centroid && key && bucket_start && bucket_items && work),
+                "fea_contact_grid_build: null argument");
+  hipStream_t st = as_stream(stream);
+  const unsigned int nb = 1u << table_bits;
+  if (n_facets > 0)
+    k_facet_keys<<<blocks_for(n_facets), kBlock, 0, st>>>(n_facets, conn, X, U, origin_host[0], origin_host[1],
+                                                         origin_host[2], 1.0 / cell, nb - 1, centroid, key);
+  FEA_CHECK_LAUNCH("fea_contact_grid_build");
+  return bucket_sort(n_facets, key, (int)nb, bucket_start, bucket_items, work, st);
+}
+
+extern "C" int fea_contact_search(int n_slave, const int32_t *slave, const double *X, const double *U,
+                                  const double *centroid, const int32_t *bucket_start,
+                                  const int32_t *bucket_items, const double *origin_host, double cell,
+                                  int table_bits, double radius, int kmax, int32_t *cand, int32_t *count,
+                                  int32_t *max_count, void *stream) {
+  FEA_CHECK_ARG(n_slave >= 0 && origin_host && cell > 0.0 && radius > 0.0 && radius <= cell &&
+                    kmax >= 1 && kmax <= kMaxCand && table_bits >= 1 && table_bits <= 28,
+                "fea_contact_search: bad arguments (radius must not exceed the cell size, kmax <= 64)");
+  if (n_slave == 0) return FEA_OK;
+  FEA_CHECK_ARG(slave && X && U && centroid && bucket_start && bucket_items && cand && count,
+                "fea_contact_search: null argument");
+  const unsigned int nb = 1u << table_bits;
+  k_contact_search<<<blocks_for(n_slave), kBlock, 0, as_stream(stream)>>>(
+      n_slave, slave, X, U, centroid, bucket_start, bucket_items, origin_host[0], origin_host[1], origin_host[2],
+      1.0 / cell, nb - 1, radius * radius, kmax, cand, count, max_count);
+  FEA_CHECK_LAUNCH("fea_contact_search");
+  return FEA_OK;
+}
+
+extern "C" int fea_contact_blocks(int n_slave) { return n_slave > 0 ? blocks_for(n_slave) : 0; }
+
+extern "C" int fea_contact_forces(int n_slave, const int32_t *slave, const double *X, const double *U,
+                                  const int32_t *conn, int n_facets, const int32_t *cand,
+                                  const int32_t *count, int kmax, double penalty, const double *slave_area,
+                                  double depth_max, int n_master, const int32_t *master_nodes,
+                                  const int32_t *nf_ptr,
+                                  const int32_t *nf_facet, const int32_t *nf_vert, int32_t *chosen,
+                                  double *force, double *weight, int32_t *sb_start, int32_t *sb_items,
+                                  int32_t *work, int64_t n_dof, double *fc, double *energy_partial,
+                                  void *stream) {
+  FEA_CHECK_ARG(n_slave >= 0 && n_facets >= 0 && n_master >= 0 && kmax >= 1 && kmax <= kMaxCand && n_dof >= 0,
+                "fea_contact_forces: bad arguments");
+  FEA_CHECK_ARG(fc != nullptr, "fea_contact_forces: null fc");
+  hipStream_t st = as_stream(stream);
+  if (n_dof > 0) k_fill_dbl<<<blocks_for(n_dof), kBlock, 0, st>>>(n_dof, fc);
+  if (n_slave == 0) {
+    FEA_CHECK_LAUNCH("fea_contact_forces");
+    return FEA_OK;
+  }
+  FEA_CHECK_ARG(slave && X && U && conn && cand && count && chosen && force && weight && sb_start && sb_items &&
+                    work && (n_master == 0 || (master_nodes && nf_ptr && nf_facet && nf_vert)),
+                "fea_contact_forces: null argument");
+  k_contact_penalty<<<blocks_for(n_slave), kBlock, 0, st>>>(n_slave, slave, X, U, conn, cand, count, kmax,
+                                                            penalty, slave_area, depth_max, chosen, force,
+                                                            weight,
+                                                            energy_partial);
+  FEA_CHECK_LAUNCH("fea_contact_forces");
+  const int rc = bucket_sort(n_slave, chosen, n_facets > 0 ? n_facets : 1, sb_start, sb_items, work, st);
+  if (rc != FEA_OK) return rc;
+  if (n_master > 0)
+    k_contact_master<<<blocks_for(n_master), kBlock, 0, st>>>(n_master, master_nodes, nf_ptr, nf_facet, nf_vert,
+                                                              sb_start, sb_items, force, weight, fc);
+  k_contact_slave<<<blocks_for(n_slave), kBlock, 0, st>>>(n_slave, slave, force, fc);
+  FEA_CHECK_LAUNCH("fea_contact_forces");
+  return FEA_OK;
+}

@@ -134,6 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_explicit_update(int n_nodes, const i
                                                             const int32_t *__restrict__ dof_map,
                                                             const double *__restrict__ rstage,
                                                             const double *__restrict__ minv,
+                                                            const double *__restrict__ fc,
                                                             double dt_v, double dt_u,
                                                             double *__restrict__ acc_out,
                                                             double *__restrict__ vh,
@@ -145,6 +146,11 @@ __global__ __launch_bounds__(kBlock) void k_explicit_update(int n_nodes, const i
     f[0] += rstage[(int64_t)s * 3 + 0];
     f[1] += rstage[(int64_t)s * 3 + 1];
     f[2] += rstage[(int64_t)s * 3 + 2];
+  }
+  if (fc != nullptr) {
+    f[0] -= fc[3 * (int64_t)A + 0];
+    f[1] -= fc[3 * (int64_t)A + 1];
+    f[2] -= fc[3 * (int64_t)A + 2];
   }
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
@@ -293,13 +299,14 @@ extern "C" int fea_dot_partial(int n, const double *a, const double *b, double *
 }
 
 extern "C" int fea_explicit_update(int n_nodes, const int32_t *rseg, const int32_t *dof_map,
-                                   const double *rstage, const double *minv, double dt_v,
-                                   double dt_u, double *a, double *vh, double *U, void *stream) {
+                                   const double *rstage, const double *minv, const double *fc,
+                                   double dt_v, double dt_u, double *a, double *vh, double *U,
+                                   void *stream) {
   FEA_CHECK_ARG(n_nodes >= 0 && rseg && dof_map && rstage && minv && a && vh && U,
                 "fea_explicit_update: bad arguments");
   if (n_nodes == 0) return FEA_OK;
   k_explicit_update<<<blocks_for(n_nodes), kBlock, 0, as_stream(stream)>>>(n_nodes, rseg, dof_map, rstage,
-                                                                            minv, dt_v, dt_u, a, vh, U);
+                                                                            minv, fc, dt_v, dt_u, a, vh, U);
   FEA_CHECK_LAUNCH("fea_explicit_update");
   return FEA_OK;
 }

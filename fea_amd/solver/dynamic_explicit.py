@@ -24,7 +24,8 @@ from .basesolver import BaseSolver
 class DynamicExplicitSolver(BaseSolver):
 
     def __init__(self, time_end: float = 1.0, time_per_storage: float = 1e-4, *, dt: float | None = None,
-                 cfl_factor: float | None = None, verbose: bool = False, store: bool = True) -> None:
+                 cfl_factor: float | None = None, verbose: bool = False, store: bool = True,
+                 energy_interval: int | None = None) -> None:
         super().__init__()
         self._time_end = time_end
         self._time_per_storage = time_per_storage
@@ -39,6 +40,9 @@ class DynamicExplicitSolver(BaseSolver):
         self.verbose = verbose
         self.store = store
         self.steps_done = 0
+        self._contacts = []
+        self.energy_interval = energy_interval
+        self.energy_history = []
 
     def initialize(self, assembly):
         super().initialize(assembly=assembly)
@@ -49,6 +53,9 @@ class DynamicExplicitSolver(BaseSolver):
         self._GC_list, self._GV_list, self._GA_list, self._time_list = [], [], [], []
         self._next_storage_time = 0.0
         p = self.assembly.plan
+        from ..assemble.loads.contact import NodeToSurfaceContact
+        self._contacts = [l for l in self.assembly._loads.values() if isinstance(l, NodeToSurfaceContact)]
+        self._fc_sum = torch.zeros(3 * p.n_nodes, dtype=torch.float64, device=p.device) if len(self._contacts) > 1 else None
         m = p.lumped_mass()
         m = torch.where(m.abs() < 1e-12, torch.ones_like(m), m)     # dynamic_explicit.py:165
         self._minv = 1.0 / m
@@ -107,9 +114,16 @@ class DynamicExplicitSolver(BaseSolver):
             self._GV_list.append(GV0)
             self._time_list.append(0.0)
         t = 0.0
+        self.energy_history = []
+        if self.energy_interval:
+            self.energy_history.append((0.0, self.energy_balance(U, vh, None, dt)))
         for step in range(nsteps):
             self.step(U, vh, acc, dt, first=(step == 0))
+            if self._contacts and (step % 256 == 255 or step == nsteps - 1):
+                self._check_contacts()          # candidate-list overflow, one sync per 256 steps
             t += dt
+            if self.energy_interval and ((step + 1) % self.energy_interval == 0 or step == nsteps - 1):
+                self.energy_history.append((t, self.energy_balance(U, vh, acc, dt)))
             if self.store and (t >= self._next_storage_time or step == nsteps - 1):
                 gv = vh - 0.5 * dt * acc
                 self._GC_list.append(p.u_to_gc(U))
@@ -126,10 +140,48 @@ class DynamicExplicitSolver(BaseSolver):
             print(f'Total steps: {nsteps}, Total time: {time.time() - t_start:.2f} s')
         return True
 
+    def energy_balance(self, U, vh, acc, dt):
+        """Float64 energies at the full step of U: kinetic 1/2 v.M.v with
+        v = v_{n-1/2} + dt/2 a(U) (v = vh itself at the start, acc None), the
+        potential energy of elements and loads, and the contact energy."""
+        p = self.assembly.plan
+        fc = torch.zeros(3 * p.n_nodes, dtype=torch.float64, device=p.device)
+        ce = 0.0
+        for c in self._contacts:
+            st = c.state(p)
+            fc += st.forces(U, search=True)
+            ce += st.energy()
+        if acc is None:
+            v = vh
+        else:
+            R = p.residual(U).clone()
+            a = -self._minv * (R - p.u_to_gc(fc))
+            v = vh + 0.5 * dt * a
+        ke = 0.5 * float(torch.sum(v * v / self._minv))
+        pe = p.energy(U)
+        return {'kinetic': ke, 'potential': pe, 'contact': ce, 'total': ke + pe + ce}
+
+    def _contact_force(self, U):
+        p = self.assembly.plan
+        if len(self._contacts) == 1:
+            return self._contacts[0].state(p).forces(U, check=False)
+        self._fc_sum.zero_()
+        for c in self._contacts:
+            call('fea_axpy', self._fc_sum.numel(), 1.0, F64, c.state(p).forces(U, check=False), self._fc_sum,
+                 nat.stream_ptr())
+        return self._fc_sum
+
+    def _check_contacts(self):
+        for c in self._contacts:
+            c.state(self.assembly.plan).check()
+
     def step(self, U, vh, acc, dt, first=False, load_scale=1.0):
         """One central-difference step at the current U (in place)."""
         p = self.assembly.plan
         s = nat.stream_ptr()
         p.residual_stage(U, load_scale, F64)
-        call('fea_explicit_update', p.n_nodes, p.rseg, p.dof_map, p.rstage, self._minv,
+        fc = None
+        if self._contacts:
+            fc = self._contact_force(U)
+        call('fea_explicit_update', p.n_nodes, p.rseg, p.dof_map, p.rstage, self._minv, fc,
              0.5 * dt if first else dt, dt, acc, vh, U, s)

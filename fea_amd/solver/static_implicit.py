@@ -82,6 +82,9 @@ class StaticImplicitSolver(BaseSolver):
 
     def initialize(self, assembly, *args, **kwargs):
         super().initialize(assembly)
+        from ..assemble.loads.contact import NodeToSurfaceContact
+        if any(isinstance(l, NodeToSurfaceContact) for l in assembly._loads.values()):
+            raise NotImplementedError('node-to-surface contact has no tangent; use DynamicExplicitSolver')
         self._pcg = {}
 
     # ------------------------------------------------------------------ API

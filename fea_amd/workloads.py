@@ -53,3 +53,50 @@ def beam_traction(nx=128, ny=32, nz=32, *, C10=0.5, C01=0.1, D1=0.01, traction=(
     mat = materials.MooneyRivlin(C10, C01, D1)
     return _controller_from_mesh(mesh, mat, fixed_set='x0',
                                  load=lambda nm: loads.Traction(nm, 'x1', traction)), mesh
+
+
+def two_blocks_contact(n=40, *, mu=1.0, kappa=50.0, rho=1.0, penalty=1e3, velocity=0.1, gap=None,
+                       jitter=0.0, seed=0, upper_width=1.0):
+    """Two blocks of n^3 Kuhn-split C3D4 cells, a unit cube and above it, by
+    ``gap`` (default a quarter cell), a centred upper_width x upper_width x 1
+    block moving down at ``velocity``; the lower block's base z=0 is fixed.  Node-to-surface penalty contact: slave nodes the
+    upper block's bottom face, master facets the lower block's top face.
+
+    Returns (controller, info) with info = {'meshes': (lower, upper),
+    'GV0': initial velocity in free-DOF order (after fe.initialize()), ...}."""
+    h = 1.0 / n
+    gap = 0.25 * h if gap is None else gap
+    lower = meshgen.kuhn_tet_block(n, n, n, jitter=jitter, seed=seed)
+    w = float(upper_width)
+    upper = meshgen.kuhn_tet_block(n, n, n, size=(w, w, 1.0), origin=(0.5 - 0.5 * w, 0.5 - 0.5 * w, 1.0 + gap),
+                                   jitter=jitter, seed=seed + 1)
+    mat = materials.NeoHookean(mu, kappa)
+    asm = Assembly()
+    for name, mesh in (('lower', lower), ('upper', upper)):
+        part = Part(torch.as_tensor(mesh['nodes'], dtype=torch.float64))
+        for k, v in mesh['node_sets'].items():
+            part.set_nodes[k] = np.asarray(v)
+        for k, v in mesh['face_sets'].items():
+            part.add_surface_set(k, v)
+        E = mesh['conn'].shape[0]
+        el = elements.initialize_element('C3D4', torch.arange(E), torch.as_tensor(mesh['conn']), part)
+        el.set_materials(mat)
+        el.set_density(rho)
+        part.add_element(el, 'element-0')
+        asm.add_part(part, name)
+        asm.add_instance(Instance(part), name)
+    asm.add_boundary(boundarys.Boundary_Condition(instance_name='lower', set_nodes_name='z0'))
+    asm.add_load(loads.NodeToSurfaceContact('upper', 'z0', 'lower', 'z1', penalty=penalty), 'contact')
+    fe = FEAController()
+    fe.assembly = asm
+    info = {'meshes': (lower, upper), 'velocity': velocity, 'gap': gap}
+    return fe, info
+
+
+def initial_velocity(fe, info):
+    """Free-DOF initial velocity of ``two_blocks_contact``: -velocity along z on the upper block."""
+    p = fe.assembly.plan
+    nl = info['meshes'][0]['nodes'].shape[0]
+    V = torch.zeros(p.n_nodes, 3, dtype=torch.float64, device=p.device)
+    V[nl:, 2] = -info['velocity']
+    return p.u_to_gc(V.reshape(-1).contiguous())

@@ -243,11 +243,55 @@ int fea_dot_partial(int n, const double *a, const double *b, double *partial, vo
 /* ---------------------------------------------------------------- explicit --
  * Fused force gather + central-difference update (the loop body of
  * DynamicExplicitSolver.solve, dynamic_explicit.py:100-114): per free DOF
- * R = sum of staged forces, a = -minv R, vh += dt_v a, U += dt_u vh (nodal).
- * The first step uses dt_v = dt/2 (v_{1/2} = v_0 + dt/2 a_0), later ones dt_v = dt. */
+ * R = sum of staged forces - fc (an optional nodal force [3N], e.g. contact),
+ * a = -minv R, vh += dt_v a, U += dt_u vh (nodal).  The first step uses
+ * dt_v = dt/2 (v_{1/2} = v_0 + dt/2 a_0), later ones dt_v = dt. */
 int fea_explicit_update(int n_nodes, const int32_t *rseg, const int32_t *dof_map,
-                        const double *rstage, const double *minv, double dt_v, double dt_u,
-                        double *a, double *vh, double *U, void *stream);
+                        const double *rstage, const double *minv, const double *fc, double dt_v,
+                        double dt_u, double *a, double *vh, double *U, void *stream);
+
+/* ---------------------------------------------------------------- contact --
+ * Node-to-surface penalty contact on T3 master facets with a uniform-grid broad
+ * phase (new law; the reference's is the Gauss-point-pair barrier with a
+ * cKDTree broad phase of contact.py:86-122).  Candidate predicate, exactly:
+ * c_f = (x_a + x_b + x_c)/3 and |x_s - c_f|^2 <= radius^2 in float64 without
+ * fused multiply-add, so pair sets are reproducible bit for bit.
+ *
+ * fea_contact_grid_build: facet centroids and hashed cell keys (cell size
+ * `cell`, 2^table_bits buckets, grid origin origin_host[3]) and a bucket sort
+ * of the facets (ids ascending in each bucket): centroid[3F], key[F],
+ * bucket_start[2^bits + 1], bucket_items[F], work[max(2^bits, F)] int32. */
+int fea_contact_grid_build(int n_facets, const int32_t *conn, const double *X, const double *U,
+                           const double *origin_host, double cell, int table_bits, double *centroid,
+                           int32_t *key, int32_t *bucket_start, int32_t *bucket_items, int32_t *work,
+                           void *stream);
+/* Candidate facets of every slave node (radius <= cell, kmax <= 64): cand[S*kmax]
+ * ascending facet ids, count[S] = number of candidates (may exceed kmax: the
+ * caller must treat that as an error).  max_count (optional, one int32 the
+ * caller zeroes) is raised to every count above kmax, so overflow can be
+ * checked once after many searches. */
+int fea_contact_search(int n_slave, const int32_t *slave, const double *X, const double *U,
+                       const double *centroid, const int32_t *bucket_start,
+                       const int32_t *bucket_items, const double *origin_host, double cell,
+                       int table_bits, double radius, int kmax, int32_t *cand, int32_t *count,
+                       int32_t *max_count, void *stream);
+int fea_contact_blocks(int n_slave);
+/* Penalty forces: per slave, the candidate facet it penetrates (gap g < 0,
+ * g > -depth_max, projection inside the triangle) with the largest g (ties:
+ * smallest facet id); F = -penalty A_s g n on the slave (A_s = slave_area[s],
+ * the node's tributary area, or 1 if slave_area is null), -F times the barycentric
+ * weights on the facet nodes.  fc[n_dof] is zeroed then filled; master
+ * reactions are gathered per master node through the static node->facet
+ * adjacency (nf_ptr/nf_facet/nf_vert) and a bucket sort of slaves by chosen
+ * facet (sb_start[F+1], sb_items[S]).  energy_partial (optional) receives
+ * per-block sums of penalty A_s g^2 / 2 (fea_contact_blocks(S) partials). */
+int fea_contact_forces(int n_slave, const int32_t *slave, const double *X, const double *U,
+                       const int32_t *conn, int n_facets, const int32_t *cand, const int32_t *count,
+                       int kmax, double penalty, const double *slave_area, double depth_max, int n_master,
+                       const int32_t *master_nodes, const int32_t *nf_ptr, const int32_t *nf_facet,
+                       const int32_t *nf_vert, int32_t *chosen, double *force, double *weight,
+                       int32_t *sb_start, int32_t *sb_items, int32_t *work, int64_t n_dof, double *fc,
+                       double *energy_partial, void *stream);
 
 #ifdef __cplusplus
 }

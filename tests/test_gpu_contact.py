@@ -1,0 +1,122 @@
+"""GPU parity of node-to-surface penalty contact: candidate pairs bit-exact
+with the oracle's brute-force search, forces / chosen facets against the
+oracle's penalty law, and explicit dynamics with contact against the oracle's
+central-difference loop."""
+import numpy as np
+import pytest
+import torch
+
+from fea_amd import solver, workloads
+from oracle import contact as oc
+from oracle import solvers as osol
+from tests.helpers import facets, nh, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_two_blocks(info, mu=1.0, kappa=50.0, rho=1.0):
+    lower, upper = info['meshes']
+    nl = lower['nodes'].shape[0]
+    nodes = np.concatenate([lower['nodes'], upper['nodes']])
+    fixed = np.zeros((nodes.shape[0], 3), dtype=bool)
+    fixed[lower['node_sets']['z0']] = True
+    mat = nh(mu, kappa)
+    model = {'nodes': nodes, 'fixed': fixed, 'pressures': [], 'tractions': [],
+             'groups': [{'etype': 'C3D4', 'conn': lower['conn'], 'material': mat, 'rho': rho},
+                        {'etype': 'C3D4', 'conn': upper['conn'] + nl, 'material': mat, 'rho': rho}]}
+    master = facets(lower, 'z1')[0]
+    stri = facets(upper, 'z0')[0] + nl
+    slave, area = oc.tributary_areas(stri, nodes)
+    return model, slave, master, area
+
+
+def pressed(n, depth, seed):
+    fe, info = workloads.two_blocks_contact(n, jitter=0.1, seed=seed)
+    fe.assembly.initialize()
+    p = fe.assembly.plan
+    nl = info['meshes'][0]['nodes'].shape[0]
+    rng = np.random.default_rng(seed)
+    u = 0.05 / n * rng.uniform(-1, 1, (p.n_nodes, 3))
+    u[nl:, 2] -= info['gap'] + depth / n
+    return fe, info, torch.as_tensor(u.reshape(-1), device='cuda').contiguous()
+
+
+@pytest.mark.parametrize('n,seed', [(4, 0), (7, 1)])
+def test_pairs_bit_exact_and_forces(gpu, n, seed):
+    fe, info, U = pressed(n, 0.3, seed)
+    load = fe.assembly.get_load('contact')
+    st = load.state(fe.assembly.plan)
+    fc = st.forces(U, search=True).cpu().numpy()
+    model, slave, master, area = oracle_two_blocks(info)
+    assert np.array_equal(slave, load.slave_nodes) and np.array_equal(master, load.master_conn)
+    assert np.allclose(area, load.slave_area, rtol=1e-14, atol=0)
+    x = oc.positions(model['nodes'], U.cpu().numpy())
+    pairs = oc.candidate_pairs(slave, master, x, load.radius)
+    got = st.pairs()
+    assert got.shape[0] > 5 * len(slave)
+    assert np.array_equal(got, pairs)
+    fo, chosen, eo = oc.penalty_forces(slave, master, x, pairs, load.penalty, load.depth_max, area)
+    assert np.array_equal(st.chosen[:len(slave)].cpu().numpy(), chosen)
+    assert (chosen >= 0).mean() > 0.3          # edge nodes pushed off the master face drop out
+    assert rel_err(fc, fo) < 1e-12
+    assert abs(st.energy() - eo) < 1e-12 * eo
+
+
+def test_no_contact_when_apart(gpu):
+    fe, info = workloads.two_blocks_contact(3)
+    fe.assembly.initialize()
+    st = fe.assembly.get_load('contact').state(fe.assembly.plan)
+    U = torch.zeros(3 * fe.assembly.plan.n_nodes, dtype=torch.float64, device='cuda')
+    fc = st.forces(U, search=True)
+    assert float(fc.abs().max()) == 0.0 and st.energy() == 0.0
+
+
+def test_candidate_overflow_is_an_error(gpu):
+    fe, info, U = pressed(4, 0.3, 0)
+    load = fe.assembly.get_load('contact')
+    load.kmax = 2
+    load._state = None
+    with pytest.raises(RuntimeError, match='kmax'):
+        load.state(fe.assembly.plan).forces(U, search=True)
+
+
+def test_explicit_with_contact_matches_oracle(gpu):
+    fe, info = workloads.two_blocks_contact(3, velocity=1.0, gap=0.01, jitter=0.1, seed=5)
+    dt, nsteps = 4e-3, 40
+    fe.solver = solver.DynamicExplicitSolver(time_end=dt * nsteps, time_per_storage=1.0, dt=dt)
+    fe.initialize()
+    GV0 = workloads.initial_velocity(fe, info)
+    GC0 = torch.zeros_like(fe.assembly.GC)
+    fe.solver.solve(GC0=GC0, GV0=GV0)
+    load = fe.assembly.get_load('contact')
+    model, slave, master, area = oracle_two_blocks(info)
+
+    active = []
+
+    def fc(u):
+        f = oc.contact_force(slave, master, model['nodes'], u, load.radius, load.penalty, load.depth_max, area)
+        active.append(np.count_nonzero(f))
+        return f
+
+    GC_o, GV_o, _ = osol.explicit_steps(model, np.zeros(GC0.shape[0]), GV0.cpu().numpy(), dt, nsteps,
+                                        extra_force=fc)
+    assert rel_err(fe.solver.GC.cpu().numpy(), GC_o) < 1e-10
+    assert sum(a > 0 for a in active) > 5                                 # contact was active
+
+
+def test_impact_energy_balance(gpu):
+    """FP64 energy check through an impact of equal-footprint blocks: kinetic +
+    strain + contact energy stays within 0.5% of the initial kinetic energy (the
+    oracle's loop gives 0.06%), and the contact is active."""
+    fe, info = workloads.two_blocks_contact(3, velocity=1.0, gap=0.01, jitter=0.1, seed=6)
+    dt, nsteps = 2e-3, 150
+    fe.solver = solver.DynamicExplicitSolver(time_end=dt * nsteps, time_per_storage=1.0, dt=dt,
+                                             energy_interval=10, store=False)
+    fe.initialize()
+    fe.solver.solve(GC0=torch.zeros_like(fe.assembly.GC), GV0=workloads.initial_velocity(fe, info))
+    hist = [h for _, h in fe.solver.energy_history]
+    ke0 = hist[0]['kinetic']
+    assert abs(ke0 - 0.5 * 1.0 * 1.0 ** 2) < 1e-12          # upper block mass 1, speed 1
+    assert max(h['contact'] for h in hist) > 1e-3 * ke0
+    drift = max(abs(h['total'] - hist[0]['total']) for h in hist)
+    assert drift < 5e-3 * ke0
