@@ -63,6 +63,10 @@ _SIGS = {
     'fea_contact_search': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_DBL, C_INT, C_DBL, C_INT, C_P,
                                    C_P, C_P, C_P]),
     'fea_contact_blocks': (C_INT, [C_INT]),
+    'fea_elem_adjoint_blocks': (C_INT, [C_INT]),
+    'fea_elem_adjoint': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_surface_pressure_hvp': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_DBL, C_P, C_P, C_P, C_P]),
+    'fea_gather_nodal': (C_INT, [C_INT, C_P, C_P, C_P, C_P]),
     'fea_contact_forces': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_INT, C_P, C_P, C_INT, C_DBL, C_P, C_DBL, C_INT, C_P,
                                    C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_I64, C_P, C_P, C_P]),
 }

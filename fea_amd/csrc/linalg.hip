@@ -130,6 +130,23 @@ __global__ __launch_bounds__(kBlock) void k_dot_partial(int n, const double *__r
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// out[3A + j] = sum of stage rows rseg[A] .. rseg[A+1] (every node, no DOF map)
+__global__ __launch_bounds__(kBlock) void k_gather_nodal(int n_nodes, const int32_t *__restrict__ seg,
+                                                         const double *__restrict__ stage,
+                                                         double *__restrict__ out) {
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= n_nodes) return;
+  double f[3] = {0.0, 0.0, 0.0};
+  for (int s = seg[A]; s < seg[A + 1]; ++s) {
+    f[0] += stage[(int64_t)s * 3 + 0];
+    f[1] += stage[(int64_t)s * 3 + 1];
+    f[2] += stage[(int64_t)s * 3 + 2];
+  }
+  out[3 * (int64_t)A + 0] = f[0];
+  out[3 * (int64_t)A + 1] = f[1];
+  out[3 * (int64_t)A + 2] = f[2];
+}
+
 __global__ __launch_bounds__(kBlock) void k_explicit_update(int n_nodes, const int32_t *__restrict__ rseg,
                                                             const int32_t *__restrict__ dof_map,
                                                             const double *__restrict__ rstage,
@@ -329,3 +346,12 @@ extern "C" int fea_vec_stats(int n, const double *x, double *partial, void *stre
   return FEA_OK;
 }
 
+
+extern "C" int fea_gather_nodal(int n_nodes, const int32_t *seg, const double *stage, double *out, void *stream) {
+  FEA_CHECK_ARG(n_nodes >= 0, "fea_gather_nodal: bad arguments");
+  if (n_nodes == 0) return FEA_OK;
+  FEA_CHECK_ARG(seg && stage && out, "fea_gather_nodal: null argument");
+  k_gather_nodal<<<blocks_for(n_nodes), kBlock, 0, as_stream(stream)>>>(n_nodes, seg, stage, out);
+  FEA_CHECK_LAUNCH("fea_gather_nodal");
+  return FEA_OK;
+}

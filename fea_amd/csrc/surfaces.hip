@@ -162,6 +162,64 @@ __global__ __launch_bounds__(kBlock) void k_pressure(int n_facets, const int32_t
   }
 }
 
+// Matrix-free pressure tangent: out_a = sum_b K_ab lam_b with K_ab the Hessian
+// block of scale * V (the k_pressure stiffness), one thread per (facet, a):
+//   K_ab lam_b summed over b = wd [al_a (sum_b be_b . lam_b) - sum_b be_b (al_a . lam_b)].
+template <int ST>
+__global__ __launch_bounds__(kBlock) void k_pressure_hvp(int n_facets, const int32_t *__restrict__ conn,
+                                                         const double *__restrict__ X,
+                                                         const double *__restrict__ U, double scale,
+                                                         const double *__restrict__ lam,
+                                                         const int32_t *__restrict__ dst,
+                                                         double *__restrict__ stage) {
+  constexpr int NEN = Surf<ST>::nen, NGP = Surf<ST>::ngp;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n_facets * NEN) return;
+  const int e = (int)(t / NEN), a = (int)(t - (int64_t)e * NEN);
+  double pos[NEN][3], l[NEN][3];
+#pragma unroll
+  for (int c = 0; c < NEN; ++c) {
+    const int64_t n = conn[(int64_t)e * NEN + c];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      pos[c][i] = X[3 * n + i] + U[3 * n + i];
+      l[c][i] = lam[3 * n + i];
+    }
+  }
+  double out[3] = {0.0, 0.0, 0.0};
+  for (int g = 0; g < NGP; ++g) {
+    double N[NEN], dN[2][NEN], ra[9];
+    ra_at<double, ST>(g, pos, ra, N, dN);
+    const double det = det3(ra);
+    double inv[9];
+    inv3(ra, det, inv);
+    const double wd = Surf<ST>::weight(g) / 3.0 * scale * det;
+    double al[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) al[i] = N[a] * inv[0 * 3 + i] + dN[0][a] * inv[1 * 3 + i] + dN[1][a] * inv[2 * 3 + i];
+    double s = 0.0, Tal[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int b = 0; b < NEN; ++b) {
+      double be[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) be[j] = N[b] * inv[0 * 3 + j] + dN[0][b] * inv[1 * 3 + j] + dN[1][b] * inv[2 * 3 + j];
+      const double bl = be[0] * l[b][0] + be[1] * l[b][1] + be[2] * l[b][2];
+      const double al_l = al[0] * l[b][0] + al[1] * l[b][1] + al[2] * l[b][2];
+      s += bl;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) Tal[i] += be[i] * al_l;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i] += wd * (al[i] * s - Tal[i]);
+  }
+  const int64_t d = dst[(int64_t)e * NEN + a];
+  if (d >= 0) {
+    stage[d * 3 + 0] = out[0];
+    stage[d * 3 + 1] = out[1];
+    stage[d * 3 + 2] = out[2];
+  }
+}
+
 template <int ST>
 __global__ __launch_bounds__(kBlock) void k_volume(int n_facets, const int32_t *__restrict__ conn,
                                                    const double *__restrict__ X,
@@ -326,5 +384,22 @@ extern "C" int fea_surface_traction(int surf_type, int n_facets, const int32_t *
     case 6: k_traction<6><<<grid, kBlock, 0, st>>>(n_facets, conn, X, U, tx, ty, tz, scale, dst_r, rstage, work_partial); break;
   }
   FEA_CHECK_LAUNCH("fea_surface_traction");
+  return FEA_OK;
+}
+
+extern "C" int fea_surface_pressure_hvp(int surf_type, int n_facets, const int32_t *conn, const double *X,
+                                        const double *U, double scale, const double *lam,
+                                        const int32_t *dst, double *stage, void *stream) {
+  FEA_CHECK_ARG(valid_surf(surf_type), "fea_surface_pressure_hvp: bad surface type %d", surf_type);
+  if (n_facets <= 0) return FEA_OK;
+  FEA_CHECK_ARG(conn && X && U && lam && dst && stage, "fea_surface_pressure_hvp: null input");
+  const int grid = blocks_for((int64_t)n_facets * surf_type);
+  hipStream_t st = as_stream(stream);
+  switch (surf_type) {
+    case 3: k_pressure_hvp<3><<<grid, kBlock, 0, st>>>(n_facets, conn, X, U, scale, lam, dst, stage); break;
+    case 4: k_pressure_hvp<4><<<grid, kBlock, 0, st>>>(n_facets, conn, X, U, scale, lam, dst, stage); break;
+    case 6: k_pressure_hvp<6><<<grid, kBlock, 0, st>>>(n_facets, conn, X, U, scale, lam, dst, stage); break;
+  }
+  FEA_CHECK_LAUNCH("fea_surface_pressure_hvp");
   return FEA_OK;
 }

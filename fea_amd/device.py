@@ -82,6 +82,7 @@ class DevicePlan:
             off += nb
         self._epart = torch.zeros(max(1, off), dtype=torch.float64, device=self.device)
         self._eout = torch.zeros(max(1, len(self.groups)), dtype=torch.float64, device=self.device)
+        self._nodal = None
 
     # ------------------------------------------------------------------ pattern
     def _build_pattern(self):
@@ -157,6 +158,33 @@ class DevicePlan:
                 g.dst_r = _i32(dr, dev)
         finally:
             L.fea_pattern_destroy(handle)
+
+    def nodal_staging(self):
+        """All-node staging for sensitivities: (seg [N+1] int32, dst per group,
+        two stage buffers).  Rows are ordered (group, element, node) inside each
+        node, like the residual staging, but constrained nodes keep their rows."""
+        if self._nodal is None:
+            flat = [g.conn_np.reshape(-1).astype(np.int64) for g in self.groups]
+            allc = np.concatenate(flat) if flat else np.zeros(0, dtype=np.int64)
+            order = np.argsort(allc, kind='stable')
+            slot = np.empty(allc.shape[0], dtype=np.int64)
+            slot[order] = np.arange(allc.shape[0])
+            seg = np.concatenate([[0], np.cumsum(np.bincount(allc, minlength=self.n_nodes))])
+            dsts, off = [], 0
+            for f in flat:
+                dsts.append(_i32(slot[off:off + f.shape[0]] if f.shape[0] else np.zeros(1), self.device))
+                off += f.shape[0]
+            stage = [torch.zeros(max(1, 3 * allc.shape[0]), dtype=torch.float64, device=self.device)
+                     for _ in range(2)]
+            self._nodal = (_i32(seg, self.device), dsts, stage)
+        return self._nodal
+
+    def set_coordinates(self, X):
+        """Move the nodes (shape changes; the sparsity pattern is unchanged)."""
+        X = torch.as_tensor(X, dtype=torch.float64, device=self.device).reshape(-1)
+        if X.numel() != 3 * self.n_nodes:
+            raise ValueError('X must have %d entries' % (3 * self.n_nodes))
+        self.X.view(-1).copy_(X)
 
     def kstage(self, dtype):
         if dtype not in self._kstage:

@@ -185,6 +185,12 @@ class StaticImplicitSolver(BaseSolver):
             sign, deltaE = -1.0, -deltaE
         if not math.isfinite(deltaE):
             raise ValueError('dGC has nan or inf')
+        if abs(deltaE) <= 1e-13 * abs(energy0):
+            # the predicted decrease is below the energy's rounding: the Armijo test
+            # cannot resolve it, so take the full Newton step (only near convergence)
+            p.dof_axpy(U, sign, dGC, Unew)
+            self.stats['energy_evals'] = self.stats.get('energy_evals', 0) + 1
+            return 1.0, p.energy(Unew, scale)
         slack = self.armijo_slack * abs(energy0)
         while True:
             p.dof_axpy(U, sign * alpha, dGC, Unew)

@@ -100,3 +100,14 @@ def initial_velocity(fe, info):
     V = torch.zeros(p.n_nodes, 3, dtype=torch.float64, device=p.device)
     V[nl:, 2] = -info['velocity']
     return p.u_to_gc(V.reshape(-1).contiguous())
+
+
+def void_block_pressure(m=10, k=5, ns=5, *, mu=1.0, kappa=100.0, pressure=0.05, amp=0.02, seed=0,
+                        quadratic=True):
+    """2x1x1 block of C3D10 (default 48000 elements) around a perturbed spherical
+    void of radius 0.3, NeoHookean, face x=0 fixed, follower pressure inside
+    the void (the shape-sensitivity workload)."""
+    mesh = meshgen.block_with_void(m, k, ns, amp=amp, seed=seed, quadratic=quadratic)
+    mat = materials.NeoHookean(mu, kappa)
+    return _controller_from_mesh(mesh, mat, fixed_set='x0',
+                                 load=lambda nm: loads.Pressure(nm, 'void', pressure)), mesh

@@ -293,6 +293,30 @@ int fea_contact_forces(int n_slave, const int32_t *slave, const double *X, const
                        int32_t *sb_start, int32_t *sb_items, int32_t *work, int64_t n_dof, double *fc,
                        double *energy_partial, void *stream);
 
+/* ---------------------------------------------------------------- adjoint --
+ * Vector-Jacobian products for sensitivities.  The reference differentiates by
+ * torch autograd through its assembly (C3base.py:387-452, pressure.py:80-132
+ * under requires_grad); these replace that path.
+ *
+ * fea_elem_adjoint: for phi = lam . f_int(X, U; params) of one element group,
+ * writes per (element, node) rows of d phi/dX (gx_stage) and d phi/dU
+ * (gu_stage, i.e. K_int lam) through dst (slot per element node, -1 = skip;
+ * rows of 3 doubles), and per-block partial sums of d phi/d{w1, w2, kv} into
+ * gp_partial[k * fea_elem_adjoint_blocks(n) + block].  Any output may be null
+ * (not all).  Float64 throughout; Gauss-point state is recomputed. */
+int fea_elem_adjoint_blocks(int n_elem);
+int fea_elem_adjoint(int elem_type, int n_elem, const int32_t *conn, const double *X, const double *U,
+                     const double *params_host, const double *lam, const int32_t *dst, double *gx_stage,
+                     double *gu_stage, double *gp_partial, void *stream);
+/* Pressure tangent times lam without the matrix: rows of scale * Hess(V)(X+U) lam
+ * per (facet, node) through dst.  V is the enclosed-volume functional of
+ * fea_surface_pressure, so this is also d(scale dV/dx . lam)/dX. */
+int fea_surface_pressure_hvp(int surf_type, int n_facets, const int32_t *conn, const double *X,
+                             const double *U, double scale, const double *lam, const int32_t *dst,
+                             double *stage, void *stream);
+/* out[3A + j] = sum of stage rows seg[A] .. seg[A+1]-1, every node (no DOF map). */
+int fea_gather_nodal(int n_nodes, const int32_t *seg, const double *stage, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -112,3 +112,30 @@ def test_group_pattern_matches_bruteforce():
     assert np.array_equal(ind[0][inv], rows) and np.array_equal(ind[1][inv], cols)
     keys = ind[0] * (rows.max() + 1) + ind[1]
     assert np.all(np.diff(keys) > 0)
+
+
+def test_void_block_mesh_is_conforming_and_valid():
+    from fea_amd import meshgen
+    m = meshgen.block_with_void(4, 2, 2, seed=3, quadratic=False)
+    T = m['nodes'][m['conn']]
+    vol = np.einsum('ei,ei->e', T[:, 1] - T[:, 0], np.cross(T[:, 2] - T[:, 0], T[:, 3] - T[:, 0])) / 6
+    assert vol.min() > 0
+    faces = np.sort(np.concatenate([m['conn'][:, f] for f in ([0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3])]), axis=1)
+    u, cnt = np.unique(faces, axis=0, return_counts=True)
+    assert cnt.max() == 2                                   # every face shared by at most two tets
+    P = m['nodes'][u[cnt == 1]].mean(1)
+    on_box = np.any((np.abs(P) < 1e-9) | (np.abs(P - [2.0, 1.0, 1.0]) < 1e-9), axis=1)
+    n_void = sum(len(i) for i, _ in m['face_sets']['void'])
+    assert (~on_box).sum() == n_void > 0                    # the rest of the boundary is the void
+    # volume: block minus the (perturbed) void, within the faceting error
+    assert abs(vol.sum() - (2.0 - 4.0 / 3.0 * np.pi * 0.3 ** 3)) < 0.02
+    v = m['node_sets']['void']
+    c, vd = m['void']['centre'], m['void']
+    d = m['nodes'][v] - c
+    r = np.linalg.norm(d, axis=1)
+    rr = meshgen._void_radius(d / r[:, None], vd['r0'], vd['amp'], vd['waves'], vd['phases'])
+    assert np.abs(r - rr).max() < 1e-12                     # void corner nodes on the perturbed sphere
+    assert np.ptp(r) > 0.01                                  # and the perturbation is there
+    m10 = meshgen.block_with_void(4, 2, 2, seed=3, quadratic=True)
+    _, wdet, _ = el.pre_load_gaussian('C3D10', m10['nodes'], m10['conn'])
+    assert wdet.min() > 0                                    # every C3D10 Gauss point valid
