@@ -67,6 +67,10 @@ _SIGS = {
     'fea_elem_adjoint': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_surface_pressure_hvp': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_DBL, C_P, C_P, C_P, C_P]),
     'fea_gather_nodal': (C_INT, [C_INT, C_P, C_P, C_P, C_P]),
+    'fea_pcg_system_size': (C_INT, []),
+    'fea_pcg_batch_layout': (C_INT, [C_INT, C_P, C_P, C_P]),
+    'fea_pcg_batch_start': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_INT, C_INT, C_P]),
+    'fea_pcg_batch_iterate': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_INT, C_INT, C_INT, C_P, C_P]),
     'fea_contact_forces': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_INT, C_P, C_P, C_INT, C_DBL, C_P, C_DBL, C_INT, C_P,
                                    C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_I64, C_P, C_P, C_P]),
 }

@@ -1,0 +1,299 @@
+// Batched Jacobi-PCG: one launch per CG stage advances every system of a batch
+// of independent SPD systems (one per static-solve request in the solve server).
+//
+// Each system keeps its own CSR, vectors and float64 workspace (the layout of
+// the single-system PCG in pcg.hip).  A launch's workgroups are partitioned
+// among the systems by prefix tables (spmv_start / vec_start: first workgroup
+// of each system); a workgroup finds its system with one wave-wide ballot over
+// the table.  The number of workgroups a system gets, and so the order of every
+// one of its partial sums, depends only on that system's own size: a system
+// produces bit-identical iterates alone and in any batch.
+//
+// Converged systems (ws[W_DONE] set) return at once, so a batch keeps running
+// until its slowest member finishes without disturbing the others.
+#include "pcg_device.h"
+
+namespace fea {
+
+constexpr int kBatchSW = 16;  // sub-wave per node group in batched SpMV (fixed: batch-independent)
+
+// index k of the system whose workgroup range [start[k], start[k+1]) holds b
+__device__ __forceinline__ int find_system(const int32_t *__restrict__ start, int n_sys, int b) {
+  int cnt = 0;
+  for (int base = 0; base < n_sys; base += 64) {
+    const int k = base + (int)(threadIdx.x & 63);
+    const bool passed = (k < n_sys) && (start[k + 1] <= b);
+    cnt += __popcll(__ballot(passed));
+  }
+  return __builtin_amdgcn_readfirstlane(cnt);
+}
+
+template <typename TM, typename TV>
+__device__ __forceinline__ int sys_group_spmv(const fea_pcg_system &S, int g, int lane, const TV *x,
+                                              TV o[3]) {
+  if (S.n_groups < 0)
+    return group_spmv<TM, TV, kBatchSW, 2>(g, lane, S.grp, S.rowptr, S.col, (const TM *)S.val, x, o);
+  return group_spmv<TM, TV, kBatchSW, 1>(g, lane, S.grp, S.rowptr, S.col, (const TM *)S.val, x, o);
+}
+
+// r = b - A x and dinv = 1/|diag A| (1 where the diagonal is 0) for every system
+template <typename TM, typename TV>
+__global__ __launch_bounds__(kBlock) void k_batch_residual(int n_sys, const fea_pcg_system *__restrict__ sys,
+                                                           const int32_t *__restrict__ spmv_start) {
+  const int k = find_system(spmv_start, n_sys, blockIdx.x);
+  const fea_pcg_system S = sys[k];
+  const int lb = blockIdx.x - spmv_start[k];
+  const int ngroups = S.n_groups < 0 ? -S.n_groups : S.n_groups;
+  constexpr int GPB = kBlock / kBatchSW;
+  const int lane = threadIdx.x % kBatchSW;
+  const TV *x = (const TV *)S.x;
+  const TV *b = (const TV *)S.b;
+  TV *r = (TV *)S.r;
+  TV *dinv = (TV *)S.dinv;
+  const TM *val = (const TM *)S.val;
+  for (int64_t g0 = (int64_t)lb * GPB; g0 < ngroups; g0 += (int64_t)S.spmv_blocks * GPB) {
+    const int g = (int)(g0 + threadIdx.x / kBatchSW);
+    if (g >= ngroups) continue;
+    TV o[3];
+    const int r0 = sys_group_spmv<TM, TV>(S, g, lane, x, o);
+    if (lane == 0) {
+      const int nr = S.grp[g + 1] - r0;
+      for (int j = 0; j < nr; ++j) {
+        r[r0 + j] = b[r0 + j] - o[j];
+        const int s = S.diag_slot[r0 + j];
+        const TV d = (s >= 0) ? fabs((TV)val[s]) : TV(0);
+        dinv[r0 + j] = (d == TV(0)) ? TV(1) : TV(1) / d;
+      }
+    }
+  }
+}
+
+// z = dinv r, p = z; r.z and b.(dinv b) partials; iteration counter reset
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_batch_init(int n_sys, const fea_pcg_system *__restrict__ sys,
+                                                       const int32_t *__restrict__ vec_start) {
+  const int k = find_system(vec_start, n_sys, blockIdx.x);
+  const fea_pcg_system S = sys[k];
+  const int lb = blockIdx.x - vec_start[k];
+  const TV *dinv = (const TV *)S.dinv;
+  const TV *b = (const TV *)S.b;
+  const TV *r = (const TV *)S.r;
+  TV *z = (TV *)S.z;
+  TV *p = (TV *)S.p;
+  double rz = 0.0, bz = 0.0;
+  for (int64_t i = (int64_t)lb * blockDim.x + threadIdx.x; i < S.n; i += (int64_t)S.vec_blocks * blockDim.x) {
+    const TV zi = dinv[i] * r[i];
+    z[i] = zi;
+    p[i] = zi;
+    rz += (double)r[i] * (double)zi;
+    bz += (double)b[i] * (double)dinv[i] * (double)b[i];
+  }
+  double *ws = S.ws;
+  const double srz = block_sum(rz);
+  const double sbz = block_sum(bz);
+  if (threadIdx.x == 0) {
+    ws[W_RZP + lb] = srz;
+    ws[W_BZP + lb] = sbz;
+  }
+  if (lb == 0 && threadIdx.x == 0) {
+    ws[W_ITER] = 0.0;
+    ws[W_DONE] = 0.0;
+    ws[W_G1] = (double)S.spmv_blocks;
+  }
+}
+
+// q = A p and p.q partials; the system's first workgroup advances its iteration count
+template <typename TM, typename TV>
+__global__ __launch_bounds__(kBlock) void k_batch_spmv(int n_sys, const fea_pcg_system *__restrict__ sys,
+                                                       const int32_t *__restrict__ spmv_start) {
+  const int k = find_system(spmv_start, n_sys, blockIdx.x);
+  const fea_pcg_system S = sys[k];
+  double *ws = S.ws;
+  if (ws[W_DONE] != 0.0) return;
+  const int lb = blockIdx.x - spmv_start[k];
+  const int ngroups = S.n_groups < 0 ? -S.n_groups : S.n_groups;
+  constexpr int GPB = kBlock / kBatchSW;
+  const int lane = threadIdx.x % kBatchSW;
+  const TV *p = (const TV *)S.p;
+  TV *q = (TV *)S.q;
+  double pq = 0.0;
+  for (int64_t g0 = (int64_t)lb * GPB; g0 < ngroups; g0 += (int64_t)S.spmv_blocks * GPB) {
+    const int g = (int)(g0 + threadIdx.x / kBatchSW);
+    if (g >= ngroups) continue;
+    TV o[3];
+    const int r0 = sys_group_spmv<TM, TV>(S, g, lane, p, o);
+    if (lane == 0) {
+      const int nr = S.grp[g + 1] - r0;
+      for (int j = 0; j < nr; ++j) {
+        q[r0 + j] = o[j];
+        pq += (double)o[j] * (double)p[r0 + j];
+      }
+    }
+  }
+  const double s = block_sum(pq);
+  if (threadIdx.x == 0) ws[W_PQ + lb] = s;
+  if (lb == 0 && threadIdx.x == 0) ws[W_ITER] += 1.0;
+}
+
+// alpha = rz_{i-1} / pq; x += alpha p; r -= alpha q; z = dinv r; r.z partials
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_batch_update(int n_sys, const fea_pcg_system *__restrict__ sys,
+                                                         const int32_t *__restrict__ vec_start) {
+  const int k = find_system(vec_start, n_sys, blockIdx.x);
+  const fea_pcg_system S = sys[k];
+  double *ws = S.ws;
+  if (ws[W_DONE] != 0.0) return;
+  const int lb = blockIdx.x - vec_start[k];
+  const int it = (int)ws[W_ITER];
+  const double rz_old = sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, S.vec_blocks);
+  const double pq = sum_partials(ws + W_PQ, S.spmv_blocks);
+  const TV alpha = (TV)(rz_old / pq);
+  const TV *dinv = (const TV *)S.dinv;
+  const TV *p = (const TV *)S.p;
+  const TV *q = (const TV *)S.q;
+  TV *x = (TV *)S.x;
+  TV *r = (TV *)S.r;
+  TV *z = (TV *)S.z;
+  double rz = 0.0;
+  for (int64_t i = (int64_t)lb * blockDim.x + threadIdx.x; i < S.n; i += (int64_t)S.vec_blocks * blockDim.x) {
+    x[i] += alpha * p[i];
+    const TV ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    const TV zi = dinv[i] * ri;
+    z[i] = zi;
+    rz += (double)ri * (double)zi;
+  }
+  const double s = block_sum(rz);
+  if (threadIdx.x == 0) ws[W_RZP + (it & 1) * PCG_G + lb] = s;
+}
+
+// convergence test (r.z / b.(dinv b) < tol or max_iter) and p = z + beta p
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_batch_dir(int n_sys, const fea_pcg_system *__restrict__ sys,
+                                                      const int32_t *__restrict__ vec_start) {
+  const int k = find_system(vec_start, n_sys, blockIdx.x);
+  const fea_pcg_system S = sys[k];
+  double *ws = S.ws;
+  if (ws[W_DONE] != 0.0) return;
+  const int lb = blockIdx.x - vec_start[k];
+  const int it = (int)ws[W_ITER];
+  const double rz_new = sum_partials(ws + W_RZP + (it & 1) * PCG_G, S.vec_blocks);
+  const double rz_old = sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, S.vec_blocks);
+  const double bz = sum_partials(ws + W_BZP, S.vec_blocks);
+  const bool done = (rz_new / bz < S.tol) || (it >= S.max_iter) || !(rz_new == rz_new);
+  // every workgroup of the system reads W_DONE at its start: only a later launch may see it set
+  if (lb == 0 && threadIdx.x == 0) {
+    ws[W_RZ] = rz_new;
+    ws[W_BZ] = bz;
+    if (done) ws[W_DONE] = 1.0;
+  }
+  if (done) return;
+  const TV beta = (TV)(rz_new / rz_old);
+  const TV *z = (const TV *)S.z;
+  TV *p = (TV *)S.p;
+  for (int64_t i = (int64_t)lb * blockDim.x + threadIdx.x; i < S.n; i += (int64_t)S.vec_blocks * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+
+// status[2k] = iterations, status[2k+1] = done flag of system k
+__global__ __launch_bounds__(kBlock) void k_batch_status(int n_sys, const fea_pcg_system *__restrict__ sys,
+                                                         double *__restrict__ status) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_sys) return;
+  const double *ws = sys[k].ws;
+  status[2 * k] = ws[W_ITER];
+  status[2 * k + 1] = ws[W_DONE];
+}
+
+static bool batch_combo(int m, int v) {
+  return (m == FEA_F64 && v == FEA_F64) || (m == FEA_F32 && v == FEA_F64) || (m == FEA_F32 && v == FEA_F32);
+}
+
+template <typename TM, typename TV>
+static void batch_start_t(int n_sys, const fea_pcg_system *sys, const int32_t *ss, const int32_t *vs, int st_tot,
+                          int vt_tot, hipStream_t st) {
+  k_batch_residual<TM, TV><<<st_tot, kBlock, 0, st>>>(n_sys, sys, ss);
+  k_batch_init<TV><<<vt_tot, kBlock, 0, st>>>(n_sys, sys, vs);
+}
+
+template <typename TM, typename TV>
+static void batch_iterate_t(int n_sys, const fea_pcg_system *sys, const int32_t *ss, const int32_t *vs,
+                            int st_tot, int vt_tot, int iters, hipStream_t st) {
+  for (int i = 0; i < iters; ++i) {
+    k_batch_spmv<TM, TV><<<st_tot, kBlock, 0, st>>>(n_sys, sys, ss);
+    k_batch_update<TV><<<vt_tot, kBlock, 0, st>>>(n_sys, sys, vs);
+    k_batch_dir<TV><<<vt_tot, kBlock, 0, st>>>(n_sys, sys, vs);
+  }
+}
+
+}  // namespace fea
+
+using namespace fea;
+
+extern "C" int fea_pcg_system_size(void) { return (int)sizeof(fea_pcg_system); }
+
+extern "C" int fea_pcg_batch_layout(int n_sys, fea_pcg_system *sys_host, int32_t *spmv_start_host,
+                                    int32_t *vec_start_host) {
+  FEA_CHECK_ARG(n_sys > 0 && n_sys <= FEA_PCG_BATCH_MAX, "fea_pcg_batch_layout: n_sys %d outside 1..%d", n_sys,
+                FEA_PCG_BATCH_MAX);
+  FEA_CHECK_ARG(sys_host && spmv_start_host && vec_start_host, "fea_pcg_batch_layout: null argument");
+  int64_t s_acc = 0, v_acc = 0;
+  spmv_start_host[0] = 0;
+  vec_start_host[0] = 0;
+  for (int k = 0; k < n_sys; ++k) {
+    fea_pcg_system &S = sys_host[k];
+    FEA_CHECK_ARG(S.n > 0 && S.n_groups != 0 && S.grp && S.rowptr && S.col && S.diag_slot && S.val && S.dinv &&
+                      S.b && S.x && S.r && S.z && S.p && S.q && S.ws && S.max_iter >= 1,
+                  "fea_pcg_batch_layout: system %d is incomplete", k);
+    const int64_t ng = S.n_groups < 0 ? -(int64_t)S.n_groups : S.n_groups;
+    int64_t sb = (ng * kBatchSW + kBlock - 1) / kBlock;
+    sb = sb < 1 ? 1 : (sb > G1_MAX ? G1_MAX : sb);
+    int64_t vb = ((int64_t)S.n + kBlock - 1) / kBlock;
+    vb = vb > PCG_G ? PCG_G : vb;
+    S.spmv_blocks = (int32_t)sb;
+    S.vec_blocks = (int32_t)vb;
+    s_acc += sb;
+    v_acc += vb;
+    FEA_CHECK_ARG(s_acc < (1LL << 30) && v_acc < (1LL << 30), "fea_pcg_batch_layout: batch too large");
+    spmv_start_host[k + 1] = (int32_t)s_acc;
+    vec_start_host[k + 1] = (int32_t)v_acc;
+  }
+  return FEA_OK;
+}
+
+extern "C" int fea_pcg_batch_start(int mat_dtype, int vec_dtype, int n_sys, const fea_pcg_system *sys,
+                                   const int32_t *spmv_start, const int32_t *vec_start, int spmv_total,
+                                   int vec_total, void *stream) {
+  FEA_CHECK_ARG(batch_combo(mat_dtype, vec_dtype), "fea_pcg_batch_start: unsupported precision pair");
+  FEA_CHECK_ARG(n_sys > 0 && n_sys <= FEA_PCG_BATCH_MAX && spmv_total > 0 && vec_total > 0,
+                "fea_pcg_batch_start: bad sizes");
+  FEA_CHECK_ARG(sys && spmv_start && vec_start, "fea_pcg_batch_start: null argument");
+  hipStream_t st = as_stream(stream);
+  if (mat_dtype == FEA_F64)
+    batch_start_t<double, double>(n_sys, sys, spmv_start, vec_start, spmv_total, vec_total, st);
+  else if (vec_dtype == FEA_F64)
+    batch_start_t<float, double>(n_sys, sys, spmv_start, vec_start, spmv_total, vec_total, st);
+  else
+    batch_start_t<float, float>(n_sys, sys, spmv_start, vec_start, spmv_total, vec_total, st);
+  FEA_CHECK_LAUNCH("fea_pcg_batch_start");
+  return FEA_OK;
+}
+
+extern "C" int fea_pcg_batch_iterate(int mat_dtype, int vec_dtype, int n_sys, const fea_pcg_system *sys,
+                                     const int32_t *spmv_start, const int32_t *vec_start, int spmv_total,
+                                     int vec_total, int iters, double *status, void *stream) {
+  FEA_CHECK_ARG(batch_combo(mat_dtype, vec_dtype), "fea_pcg_batch_iterate: unsupported precision pair");
+  FEA_CHECK_ARG(n_sys > 0 && n_sys <= FEA_PCG_BATCH_MAX && spmv_total > 0 && vec_total > 0 && iters >= 0,
+                "fea_pcg_batch_iterate: bad sizes");
+  FEA_CHECK_ARG(sys && spmv_start && vec_start, "fea_pcg_batch_iterate: null argument");
+  hipStream_t st = as_stream(stream);
+  if (mat_dtype == FEA_F64)
+    batch_iterate_t<double, double>(n_sys, sys, spmv_start, vec_start, spmv_total, vec_total, iters, st);
+  else if (vec_dtype == FEA_F64)
+    batch_iterate_t<float, double>(n_sys, sys, spmv_start, vec_start, spmv_total, vec_total, iters, st);
+  else
+    batch_iterate_t<float, float>(n_sys, sys, spmv_start, vec_start, spmv_total, vec_total, iters, st);
+  if (status != nullptr) k_batch_status<<<blocks_for(n_sys), kBlock, 0, st>>>(n_sys, sys, status);
+  FEA_CHECK_LAUNCH("fea_pcg_batch_iterate");
+  return FEA_OK;
+}

@@ -87,6 +87,34 @@ __device__ __forceinline__ void ra_at(int g, const T pos[][3], T ra[9], double N
   }
 }
 
+// Derivatives of det A, A = [x, x_xi, x_eta] = sum_c pos_c (x) s_c with
+// s_c = (N_c, dN_c/dxi, dN_c/deta) (pressure.py:95-127 forms them as
+// det A * A^-T and det A (A^-1 A^-1 - A^-1 A^-1); those need A invertible and
+// fail for a facet whose plane contains the origin).  Here, without division:
+//   d det / d pos_a,i           = (cof A . s_a)_i,  cof A row i = r_{i+1} x r_{i+2};
+//   d2 det / d pos_a,i d pos_b,j = eps_ijk (A (s_a x s_b))_k.
+// They equal the reference's expressions wherever A is invertible.
+template <typename T>
+__device__ __forceinline__ void cofactor3(const T ra[9], T cof[9]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int j = (i + 1) % 3, k = (i + 2) % 3;
+    cof[i * 3 + 0] = ra[j * 3 + 1] * ra[k * 3 + 2] - ra[j * 3 + 2] * ra[k * 3 + 1];
+    cof[i * 3 + 1] = ra[j * 3 + 2] * ra[k * 3 + 0] - ra[j * 3 + 0] * ra[k * 3 + 2];
+    cof[i * 3 + 2] = ra[j * 3 + 0] * ra[k * 3 + 1] - ra[j * 3 + 1] * ra[k * 3 + 0];
+  }
+}
+
+// v = A (s_a x s_b)
+template <typename T>
+__device__ __forceinline__ void hess_axis(const T ra[9], const T sa[3], const T sb[3], T v[3]) {
+  const T c0 = sa[1] * sb[2] - sa[2] * sb[1];
+  const T c1 = sa[2] * sb[0] - sa[0] * sb[2];
+  const T c2 = sa[0] * sb[1] - sa[1] * sb[0];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v[k] = ra[k * 3 + 0] * c0 + ra[k * 3 + 1] * c1 + ra[k * 3 + 2] * c2;
+}
+
 // One thread per (facet, node a): scale * dV/dx_a and scale * d2V/dx_a dx_b.
 template <typename T, int ST>
 __global__ __launch_bounds__(kBlock) void k_pressure(int n_facets, const int32_t *__restrict__ conn,
@@ -117,27 +145,26 @@ __global__ __launch_bounds__(kBlock) void k_pressure(int n_facets, const int32_t
     double N[NEN], dN[2][NEN];
     T ra[9];
     ra_at<T, ST>(g, pos, ra, N, dN);
-    const T det = det3(ra);
-    T inv[9];
-    inv3(ra, det, inv);  // inv[m*3+i]
-    const T wd = T(Surf<ST>::weight(g) / 3.0 * scale) * det;
+    T cof[9];
+    cofactor3(ra, cof);
+    const T ws = T(Surf<ST>::weight(g) / 3.0 * scale);
     const T sa[3] = {T(N[a]), T(dN[0][a]), T(dN[1][a])};
-    T al[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) al[i] = sa[0] * inv[0 * 3 + i] + sa[1] * inv[1 * 3 + i] + sa[2] * inv[2 * 3 + i];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) f[i] += (double)(wd * al[i]);
+    for (int i = 0; i < 3; ++i)
+      f[i] += (double)(ws * (cof[i * 3 + 0] * sa[0] + cof[i * 3 + 1] * sa[1] + cof[i * 3 + 2] * sa[2]));
     if (kstage != nullptr) {
 #pragma unroll
       for (int b = 0; b < NEN; ++b) {
         const T sb[3] = {T(N[b]), T(dN[0][b]), T(dN[1][b])};
-        T be[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) be[j] = sb[0] * inv[0 * 3 + j] + sb[1] * inv[1 * 3 + j] + sb[2] * inv[2 * 3 + j];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-          for (int j = 0; j < 3; ++j) K[b][i * 3 + j] += wd * (al[i] * be[j] - be[i] * al[j]);
+        T v[3];
+        hess_axis(ra, sa, sb, v);
+        // eps_ijk v_k
+        K[b][1] += ws * v[2];
+        K[b][2] -= ws * v[1];
+        K[b][3] -= ws * v[2];
+        K[b][5] += ws * v[0];
+        K[b][6] += ws * v[1];
+        K[b][7] -= ws * v[0];
       }
     }
   }
@@ -164,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void k_pressure(int n_facets, const int32_t
 
 // Matrix-free pressure tangent: out_a = sum_b K_ab lam_b with K_ab the Hessian
 // block of scale * V (the k_pressure stiffness), one thread per (facet, a):
-//   K_ab lam_b summed over b = wd [al_a (sum_b be_b . lam_b) - sum_b be_b (al_a . lam_b)].
+//   K_ab lam_b = scale w/3 sum_g lam_b x (A (s_a x s_b)).
 template <int ST>
 __global__ __launch_bounds__(kBlock) void k_pressure_hvp(int n_facets, const int32_t *__restrict__ conn,
                                                          const double *__restrict__ X,
@@ -190,27 +217,18 @@ __global__ __launch_bounds__(kBlock) void k_pressure_hvp(int n_facets, const int
   for (int g = 0; g < NGP; ++g) {
     double N[NEN], dN[2][NEN], ra[9];
     ra_at<double, ST>(g, pos, ra, N, dN);
-    const double det = det3(ra);
-    double inv[9];
-    inv3(ra, det, inv);
-    const double wd = Surf<ST>::weight(g) / 3.0 * scale * det;
-    double al[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) al[i] = N[a] * inv[0 * 3 + i] + dN[0][a] * inv[1 * 3 + i] + dN[1][a] * inv[2 * 3 + i];
-    double s = 0.0, Tal[3] = {0.0, 0.0, 0.0};
+    const double ws = Surf<ST>::weight(g) / 3.0 * scale;
+    const double sa[3] = {N[a], dN[0][a], dN[1][a]};
 #pragma unroll
     for (int b = 0; b < NEN; ++b) {
-      double be[3];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) be[j] = N[b] * inv[0 * 3 + j] + dN[0][b] * inv[1 * 3 + j] + dN[1][b] * inv[2 * 3 + j];
-      const double bl = be[0] * l[b][0] + be[1] * l[b][1] + be[2] * l[b][2];
-      const double al_l = al[0] * l[b][0] + al[1] * l[b][1] + al[2] * l[b][2];
-      s += bl;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) Tal[i] += be[i] * al_l;
+      const double sb[3] = {N[b], dN[0][b], dN[1][b]};
+      double v[3];
+      hess_axis(ra, sa, sb, v);
+      // (eps_ijk v_k) lam_j = (lam x v)_i
+      out[0] += ws * (l[b][1] * v[2] - l[b][2] * v[1]);
+      out[1] += ws * (l[b][2] * v[0] - l[b][0] * v[2]);
+      out[2] += ws * (l[b][0] * v[1] - l[b][1] * v[0]);
     }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) out[i] += wd * (al[i] * s - Tal[i]);
   }
   const int64_t d = dst[(int64_t)e * NEN + a];
   if (d >= 0) {

@@ -1,0 +1,556 @@
+"""Continuous-batching static solve server.
+
+Requests are independent static Newton solves (a mesh, a material, one loaded
+face and its opposite face fixed), each a full static_implicit.py:44-295 solve
+of the reference.  The server keeps every admitted request's state on the
+device (its DevicePlan: CSR, staging, displacement, Newton direction, PCG
+vectors) and advances ALL admitted requests together, one Newton iteration
+per round:
+
+1. tangent + residual of every request (its own kernels), one host read of
+   all residual norms;
+2. one batched Jacobi-PCG (``BatchedPCG``): a single launch per CG stage for
+   the whole batch, every system stopping on its own tolerance, one host read
+   of all convergence flags per chunk of iterations;
+3. one Armijo line search for the batch: the directional derivatives, the
+   current energies and the full-step trial energies of every request in one
+   host read; only requests that backtrack take further reads.
+
+Between rounds finished requests leave (their memory returns to the budget)
+and arrived requests are admitted in arrival order while the HBM budget and
+the batch-size cap allow; a request that arrives while others are on the GPU
+joins at the next round, so its work overlaps theirs on the device.
+Admission estimates a request's device footprint from its mesh sizes before
+anything is allocated.
+
+Every per-request reduction has a fixed shape that depends only on that
+request, so each result is bit-identical whether the request runs alone or in
+any mix (the test suite checks this).
+
+The Newton iteration is that of ``StaticImplicitSolver`` (Armijo c1 = 0.3 with
+halving, full step when the predicted energy decrease is below rounding,
+relative residual test, round-off step test, linear-extrapolation predictor
+over load increments); ``precision`` is 'fp64' (FP64 tangent and PCG) or
+'mixed' (FP64 element arithmetic, tangent stored in FP32, FP64 PCG vectors).
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from . import meshgen
+from ._native import F32, F64, call
+from .assemble import loads, materials
+from .solver._linear_solver import BatchedPCG, PcgVectors
+
+_OPPOSITE = {'x0': 'x1', 'x1': 'x0', 'y0': 'y1', 'y1': 'y0', 'z0': 'z1', 'z1': 'z0'}
+
+
+# ---------------------------------------------------------------------- requests
+@dataclass
+class SolveRequest:
+    """One static solve.
+
+    spec keys: ``etype`` ('C3D4' Kuhn-split tets or 'C3D8' bricks), ``cells``
+    (nx, ny, nz), ``size`` (Lx, Ly, Lz), ``material`` ('NH', mu, kappa) or
+    ('MR', C10, C01, D1), ``fixed`` face name ('x0' .. 'z1'), ``load``
+    ('pressure', p) or ('traction', (tx, ty, tz)) on the opposite face,
+    ``load_steps`` equal load increments.
+    """
+    rid: int
+    spec: dict
+    arrival: float = 0.0
+    # filled by the server
+    status: str = 'pending'
+    GC: object = None
+    t_admit: float = None
+    t_done: float = None
+    newton_iters: int = 0
+    cg_iters: int = 0
+    footprint_est: int = 0
+    footprint: int = 0
+    error: str = ''
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.arrival
+
+    def n_elements(self):
+        nx, ny, nz = self.spec['cells']
+        return nx * ny * nz * (6 if self.spec['etype'] == 'C3D4' else 1)
+
+
+def request_mesh(spec):
+    nx, ny, nz = spec['cells']
+    if spec['etype'] == 'C3D4':
+        return meshgen.kuhn_tet_block(nx, ny, nz, size=tuple(spec['size']))
+    if spec['etype'] == 'C3D8':
+        return meshgen.hex_block(nx, ny, nz, size=tuple(spec['size']))
+    raise ValueError('unsupported element type %r' % (spec['etype'],))
+
+
+def request_model(spec):
+    """FEAController of a request, built through the public API."""
+    from .workloads import _controller_from_mesh
+    mesh = request_mesh(spec)
+    m = spec['material']
+    if m[0] == 'NH':
+        mat = materials.NeoHookean(m[1], m[2])
+    elif m[0] == 'MR':
+        mat = materials.MooneyRivlin(m[1], m[2], m[3])
+    else:
+        raise ValueError('unsupported material %r' % (m[0],))
+    face = _OPPOSITE[spec['fixed']]
+    kind, value = spec['load']
+    if kind == 'pressure':
+        load = lambda nm: loads.Pressure(nm, face, float(value))
+    elif kind == 'traction':
+        load = lambda nm: loads.Traction(nm, face, tuple(float(v) for v in value))
+    else:
+        raise ValueError('unsupported load %r' % (kind,))
+    return _controller_from_mesh(mesh, mat, fixed_set=spec['fixed'], load=load), mesh
+
+
+def estimate_footprint(spec, precision='mixed'):
+    """Device bytes of a request before it is built (mesh sizes only).
+
+    Counts the staging of every element's node-pair blocks (the largest
+    term), the CSR (neighbour count per node: 27 for bricks, 15 for Kuhn tets),
+    the per-element maps and the nodal / free-DOF vectors."""
+    nx, ny, nz = spec['cells']
+    N = (nx + 1) * (ny + 1) * (nz + 1)
+    if spec['etype'] == 'C3D8':
+        E, nen, nb = nx * ny * nz, 8, 27
+    else:
+        E, nen, nb = 6 * nx * ny * nz, 4, 15
+    nnz = 9 * nb * N
+    kstage = E * nen * nen * 9 * 8
+    maps = E * nen * nen * 4 + 2 * E * nen * 4
+    csr = nnz * (4 + (8 if precision == 'fp64' else 4)) + 4 * N * 3 + 3 * 4 * nb * N
+    vectors = 3 * N * (8 * 12 + 4)
+    return int(kstage + maps + csr + vectors + 64 * 1024)
+
+
+def random_requests(n, qps, seed=0, e_range=(2e3, 2e5), aspect_range=(1.0, 4.0), load_steps=2):
+    """A Poisson stream of ``n`` static solve requests at ``qps`` requests/s.
+
+    Each request: C3D4 (Kuhn) or C3D8 block, element count log-uniform in
+    ``e_range``, side ratios Lx/Lz and Ly/Lz uniform in ``aspect_range``;
+    NeoHookean (mu, kappa) or Mooney-Rivlin (C10, C01, D1) with parameters
+    uniform within +-50% of mu = 1, kappa = 100 (C10 = mu/2 split 5:1 with
+    C01, D1 = 2/kappa); a random face fixed and the opposite face loaded by a
+    tensile follower pressure of magnitude 0.1 mu (+-50%) or a transverse dead
+    traction sized for a tip deflection of about 5% of the length.
+    """
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / qps, size=n)) if qps > 0 else np.zeros(n)
+    t = t - t[0] if n else t
+    out = []
+    lo, hi = np.log(e_range[0]), np.log(e_range[1])
+    for i in range(n):
+        etype = 'C3D4' if rng.random() < 0.5 else 'C3D8'
+        E = float(np.exp(rng.uniform(lo, hi)))
+        cells = E / (6.0 if etype == 'C3D4' else 1.0)
+        L = np.array([rng.uniform(*aspect_range), rng.uniform(*aspect_range), 1.0])
+        h = (np.prod(L) / cells) ** (1.0 / 3.0)
+        nc = tuple(int(max(1, round(l / h))) for l in L)
+        mu = rng.uniform(0.5, 1.5)
+        kappa = 100.0 * rng.uniform(0.5, 1.5)
+        if rng.random() < 0.5:
+            mat = ('NH', mu, kappa)
+        else:
+            mat = ('MR', 0.5 * mu * 5.0 / 6.0, 0.5 * mu / 6.0, 2.0 / kappa)
+        fixed = ['x0', 'x1', 'y0', 'y1', 'z0', 'z1'][int(rng.integers(6))]
+        ax = 'xyz'.index(fixed[0])
+        if rng.random() < 0.5:
+            load = ('pressure', -0.1 * mu * rng.uniform(0.5, 1.5))
+        else:
+            others = [a for a in range(3) if a != ax]
+            d_ax = min(others, key=lambda a: L[a])          # bend about the thinner side
+            E_y = 9.0 * kappa * mu / (3.0 * kappa + mu)
+            tmag = 0.05 * E_y * L[d_ax] ** 2 / (4.0 * L[ax] ** 2) * rng.uniform(0.5, 1.5)
+            tv = [0.0, 0.0, 0.0]
+            tv[d_ax] = tmag * (1.0 if rng.random() < 0.5 else -1.0)
+            load = ('traction', tuple(tv))
+        spec = {'etype': etype, 'cells': nc, 'size': tuple(float(v) for v in L), 'material': mat,
+                'fixed': fixed, 'load': load, 'load_steps': int(load_steps)}
+        out.append(SolveRequest(rid=i, spec=spec, arrival=float(t[i])))
+    return out
+
+
+# ---------------------------------------------------------------------- device state
+class _Reads:
+    """Device float64 tensors collected for one host read (one synchronisation)."""
+
+    def __init__(self):
+        self.items = []
+
+    def add(self, t):
+        self.items.append(t.reshape(-1))
+        return len(self.items) - 1
+
+    def fetch(self):
+        if not self.items:
+            return []
+        sizes = [t.numel() for t in self.items]
+        host = torch.cat(self.items).cpu().numpy()
+        out, o = [], 0
+        for s in sizes:
+            out.append(host[o:o + s])
+            o += s
+        self.items = []
+        return out
+
+
+@dataclass
+class _Slot:
+    req: SolveRequest
+    fe: object
+    plan: object
+    U: torch.Tensor
+    U_prev: torch.Tensor
+    Unew: torch.Tensor
+    dGC: torch.Tensor
+    vec: PcgVectors
+    r_ref: float
+    k: int = 1
+    steps: int = 1
+    it_total: int = 0
+    low_alpha: int = 0
+    alpha: float = 0.0
+    R: torch.Tensor = None
+    K: torch.Tensor = None
+    ls: dict = field(default_factory=dict)
+
+    @property
+    def scale(self):
+        return self.k / self.steps
+
+
+def _stats_launch(plan, x, reads):
+    """(sum x^2, max |x|) partials of a float64 device vector, queued for a read."""
+    n = x.numel()
+    nb = nat.lib().fea_vec_blocks(n)
+    part = torch.empty(2 * nb, dtype=torch.float64, device=plan.device)
+    call('fea_vec_stats', n, x, part, nat.stream_ptr())
+    return reads.add(part)
+
+
+def _stats_value(h):
+    v = h.reshape(-1, 2)
+    return float(v[:, 0].sum()), float(v[:, 1].max())
+
+
+def _energy_launch(plan, U, reads):
+    return reads.add(plan.energy_terms(U, 1.0).clone())
+
+
+def _energy_value(plan, terms, scale):
+    E = 0.0
+    for gi, g in enumerate(plan.groups):
+        if g.n == 0:
+            continue
+        if g.kind == 'elem':
+            E += terms[gi]
+        elif g.kind == 'pressure':
+            E += float(g.load.pressure) * scale * terms[gi]
+        elif g.kind == 'traction':
+            E -= scale * terms[gi]
+    return float(E)
+
+
+# ---------------------------------------------------------------------- server
+class SolveServer:
+    """Continuous-batching server of static solves on one GPU.
+
+    hbm_budget: bytes of device memory admitted requests may hold (estimated
+    before admission); max_batch: most requests in flight; the remaining
+    arguments are those of ``StaticImplicitSolver``.
+    """
+
+    def __init__(self, precision='mixed', hbm_budget=None, max_batch=64, rel_tol=1e-10, step_tol=1e-12,
+                 cg_tol=1e-8, cg_max_iter=20000, max_newton=200, chunk=32, use_graph=False,
+                 armijo_slack=1e-12, verbose=False):
+        nat.require_gpu()
+        if precision not in ('fp64', 'mixed'):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        self.mat_dtype = F64 if precision == 'fp64' else F32
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info()
+            hbm_budget = int(0.8 * free)
+        self.hbm_budget = int(hbm_budget)
+        self.max_batch = int(max_batch)
+        self.rel_tol, self.step_tol = float(rel_tol), float(step_tol)
+        self.cg_tol, self.cg_max_iter = float(cg_tol), int(cg_max_iter)
+        self.max_newton = int(max_newton)
+        self.armijo_slack = float(armijo_slack)
+        self.verbose = verbose
+        self.pcg = BatchedPCG(self.mat_dtype, F64, 'cuda', chunk=chunk, use_graph=use_graph)
+        self.active: list[_Slot] = []
+        self.used = 0
+        self.rounds = 0
+        self.batch_sizes = []
+        self.peak_used = 0
+
+    # -------------------------------------------------------------- admission
+    def _admit(self, req, now):
+        req.footprint_est = estimate_footprint(req.spec, self.precision)
+        fe, _ = request_model(req.spec)
+        fe.assembly.initialize()
+        plan = fe.assembly.plan
+        dev = plan.device
+        U = torch.zeros(3 * plan.n_nodes, dtype=torch.float64, device=dev)
+        slot = _Slot(req=req, fe=fe, plan=plan, U=U, U_prev=U.clone(), Unew=torch.empty_like(U),
+                     dGC=torch.zeros(plan.n_free, dtype=torch.float64, device=dev),
+                     vec=PcgVectors(plan.n_free, F64, dev), r_ref=0.0,
+                     steps=int(req.spec.get('load_steps', 1)))
+        reads = _Reads()
+        _stats_launch(plan, plan.residual(U, 1.0, F64), reads)
+        rss, _ = _stats_value(reads.fetch()[0])
+        slot.r_ref = math.sqrt(rss)
+        req.footprint = self._actual_footprint(slot)
+        req.t_admit = now
+        req.status = 'running'
+        self.used += req.footprint_est
+        self.peak_used = max(self.peak_used, self.used)
+        self.active.append(slot)
+
+    def _actual_footprint(self, s):
+        p = s.plan
+        ts = [p.X, p.dof_map, p.rowptr, p.col, p.diag_slot, p.grp, p.blk_a, p.blk_b, p.blk_colpos, p.kseg,
+              p.rseg, p.rstage, p.R, s.U, s.U_prev, s.Unew, s.dGC]
+        for g in p.groups:
+            ts += [t for t in (g.conn, g.dst_k, g.dst_r) if t is not None]
+        nb = sum(t.numel() * t.element_size() for t in ts)
+        nb += 9 * p.n_kstage * 8 + p.nnz * (8 if self.precision == 'fp64' else 4)   # F64 staging, CSR values
+        return int(nb + s.vec.nbytes())
+
+    def _retire(self, slot, status, now, reason=''):
+        req = slot.req
+        req.status = status
+        req.error = reason
+        req.t_done = now
+        req.newton_iters = slot.it_total
+        if status == 'done':
+            req.GC = slot.plan.u_to_gc(slot.U).cpu()
+        self.used -= req.footprint_est
+        self.active.remove(slot)
+
+    # -------------------------------------------------------------- one Newton round
+    def _tangent(self, s):
+        p = s.plan
+        if self.precision == 'fp64':
+            s.R, s.K = p.tangent(s.U, s.scale, F64)
+        else:
+            s.R, s.K = p.tangent(s.U, s.scale, F64, val_dtype=F32)
+
+    def _advance_increment(self, s, now):
+        """Increment converged: finish or move to the next load level with the predictor."""
+        if s.k >= s.steps:
+            self._retire(s, 'done', now())
+            return False
+        # U <- 2U - U_prev, U_prev <- old U (linear extrapolation of converged states)
+        step = s.U - s.U_prev
+        s.U_prev.copy_(s.U)
+        s.U.add_(step)
+        s.k += 1
+        return True
+
+    def _round(self, now):
+        slots = list(self.active)
+        # 1. tangents and residual norms; increments that are already converged advance
+        need = slots
+        solve = []
+        for _ in range(64):
+            if not need:
+                break
+            reads = _Reads()
+            ids = []
+            for s in need:
+                self._tangent(s)
+                ids.append(_stats_launch(s.plan, s.R, reads))
+            host = reads.fetch()
+            again = []
+            for s, i in zip(need, ids):
+                rss, _ = _stats_value(host[i])
+                if s.it_total >= self.max_newton or not math.isfinite(rss):
+                    self._retire(s, 'failed', now(), 'Newton limit %d reached or residual not finite '
+                                 '(increment %d, |R| %.3e)' % (self.max_newton, s.k, math.sqrt(rss)))
+                    continue
+                if self.verbose:
+                    print('req %d inc %d it %d |R|/|R0| %.3e alpha %.3g' % (s.req.rid, s.k, s.it_total,
+                          math.sqrt(rss) / s.r_ref, s.alpha), flush=True)
+                if math.sqrt(rss) <= self.rel_tol * s.r_ref:
+                    if self._advance_increment(s, now):
+                        again.append(s)
+                else:
+                    solve.append(s)
+            need = again
+        if not solve:
+            return
+        # 2. batched PCG: K dGC = -R, warm start from the previous direction
+        entries = []
+        for s in solve:
+            s.vec.b.copy_(s.R).neg_()
+            s.vec.x.copy_(s.dGC)
+            max_iter = 4 * self.cg_max_iter if s.it_total % 20 == 0 or s.low_alpha > 0 else self.cg_max_iter
+            entries.append((s.plan, s.K, s.vec, self.cg_tol, max_iter))
+        iters = self.pcg.solve(entries)
+        for s, it in zip(solve, iters):
+            s.dGC.copy_(s.vec.x)
+            s.req.cg_iters += it
+            s.it_total += 1
+        # 3. batched line search (static_implicit.py:98-177): the directional derivative,
+        #    the current energy and the full-step trial of every request in one read
+        reads = _Reads()
+        for s in solve:
+            p = s.plan
+            p.dof_axpy(s.U, 1.0, s.dGC, s.Unew)
+            s.ls = {'ids': (reads.add(p.dot(s.dGC, s.R).clone()), _energy_launch(p, s.U, reads),
+                            _energy_launch(p, s.Unew, reads), _stats_launch(p, s.dGC, reads),
+                            _stats_launch(p, s.Unew, reads))}
+        host = reads.fetch()
+        todo = []
+        for s in solve:
+            p = s.plan
+            i_dd, i_e0, i_e1, i_d, i_u = s.ls['ids']
+            deltaE = float(host[i_dd][0])
+            e0 = _energy_value(p, host[i_e0], s.scale)
+            sign = 1.0
+            if deltaE > 0:
+                sign, deltaE = -1.0, -deltaE
+            if not math.isfinite(deltaE):
+                self._retire(s, 'failed', now(), 'Newton direction not finite')
+                continue
+            s.ls = {'e0': e0, 'deltaE': deltaE, 'sign': sign, 'alpha': 1.0,
+                    'dmax': _stats_value(host[i_d])[1], 'umax': _stats_value(host[i_u])[1]}
+            if abs(deltaE) <= 1e-13 * abs(e0):
+                # predicted decrease below the energy's rounding: full Newton step
+                if sign < 0:
+                    p.dof_axpy(s.U, -1.0, s.dGC, s.Unew)
+                self._accept(s, now)
+                continue
+            if sign > 0:
+                verdict = self._armijo(s, _energy_value(p, host[i_e1], s.scale))
+                if verdict == 'accept':
+                    self._accept(s, now)
+                    continue
+                if verdict == 'fail':
+                    self._retire(s, 'failed', now(), 'line search found no decrease')
+                    continue
+            todo.append(s)
+        # backtracking: one read per halving for the requests still searching
+        while todo:
+            reads = _Reads()
+            ids = []
+            for s in todo:
+                st = s.ls
+                s.plan.dof_axpy(s.U, st['sign'] * st['alpha'], s.dGC, s.Unew)
+                ids.append(_energy_launch(s.plan, s.Unew, reads))
+            host = reads.fetch()
+            nxt = []
+            for s, i in zip(todo, ids):
+                verdict = self._armijo(s, _energy_value(s.plan, host[i], s.scale))
+                if verdict == 'accept':
+                    self._accept(s, now)
+                elif verdict == 'fail':
+                    self._retire(s, 'failed', now(), 'line search found no decrease')
+                else:
+                    nxt.append(s)
+            todo = nxt
+
+    def _armijo(self, s, e_new):
+        """Armijo test of the trial at s.ls['alpha'] (c1 = 0.3); halves alpha on rejection."""
+        st = s.ls
+        slack = self.armijo_slack * abs(st['e0'])
+        if math.isfinite(e_new) and e_new <= st['e0'] + 0.3 * st['deltaE'] * st['alpha'] + slack:
+            return 'accept'
+        st['alpha'] *= 0.5
+        return 'fail' if st['alpha'] < 1e-12 else 'retry'
+
+    def _accept(self, s, now):
+        """U <- Unew (= U + sign alpha dGC); low-alpha bookkeeping; round-off step test."""
+        st = s.ls
+        alpha = st['alpha']
+        s.U.copy_(s.Unew)
+        s.alpha = alpha
+        s.low_alpha = s.low_alpha + 1 if alpha < 0.01 else max(0, s.low_alpha - 5)
+        if s.low_alpha > 10:
+            self._retire(s, 'failed', now(), 'more than 10 steps with alpha < 0.01')
+            return
+        if alpha == 1.0 and st['sign'] > 0 and st['dmax'] <= self.step_tol * st['umax']:
+            self._advance_increment(s, now)
+
+    # -------------------------------------------------------------- driver
+    def run(self, requests, realtime=True, on_round=None):
+        """Serve ``requests`` (arrival offsets in seconds from the start).
+
+        realtime: a request is admissible once its arrival time has passed on
+        the wall clock (the server idles until the next arrival when nothing
+        is in flight); otherwise every request is available at once.  Returns
+        the requests with status, result GC (host), timings and counts."""
+        queue = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        if not realtime:
+            for r in queue:
+                r.arrival = 0.0
+        qi = 0
+        while qi < len(queue) or self.active:
+            t = now()
+            # admission in arrival order while slots and HBM budget allow
+            while qi < len(queue) and queue[qi].arrival <= t and len(self.active) < self.max_batch:
+                r = queue[qi]
+                est = estimate_footprint(r.spec, self.precision)
+                if self.active and self.used + est > self.hbm_budget:
+                    break
+                if est > self.hbm_budget:
+                    r.status, r.error, r.t_done = 'rejected', 'footprint %d B exceeds the budget' % est, now()
+                    qi += 1
+                    continue
+                qi += 1
+                self._admit(r, now())
+            if not self.active:
+                if qi < len(queue):
+                    time.sleep(max(0.0, min(queue[qi].arrival - now(), 0.05)))
+                continue
+            self.batch_sizes.append(len(self.active))
+            self._round(now)
+            self.rounds += 1
+            if on_round is not None:
+                on_round(self)
+        torch.cuda.synchronize()
+        return requests
+
+
+def solve_one_at_a_time(requests, precision='fp64', rel_tol=1e-10, step_tol=1e-12, cg_tol=1e-8,
+                        cg_max_iter=20000, max_newton=200, armijo_slack=1e-12):
+    """The plain path: each request built and solved alone by ``StaticImplicitSolver``,
+    in arrival order.  Returns (GC list, per-request seconds including the model
+    build, FIFO latencies under the requests' arrival times)."""
+    from .solver import StaticImplicitSolver
+    out, secs = [], []
+    for r in sorted(requests, key=lambda r: (r.arrival, r.rid)):
+        t0 = time.perf_counter()
+        fe, _ = request_model(r.spec)
+        fe.solver = StaticImplicitSolver(maximum_iteration=max_newton, precision=precision, rel_tol=rel_tol,
+                                         step_tol=step_tol, load_steps=int(r.spec.get('load_steps', 1)),
+                                         cg_tol=cg_tol, cg_max_iter=cg_max_iter, armijo_slack=armijo_slack)
+        fe.initialize()
+        GC = fe.solver.solve(GC0=torch.zeros_like(fe.assembly.GC))
+        torch.cuda.synchronize()
+        secs.append(time.perf_counter() - t0)
+        out.append((r.rid, None if GC is False else GC.cpu()))
+        del fe
+    order = sorted(requests, key=lambda r: (r.arrival, r.rid))
+    end, lat = 0.0, []
+    for r, t in zip(order, secs):
+        end = max(end, r.arrival) + t
+        lat.append(end - r.arrival)
+    return dict(out), secs, lat

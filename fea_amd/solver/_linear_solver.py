@@ -6,6 +6,8 @@ iterative-refinement variant.
 every CG scalar on the GPU and the host only checks a ``done`` flag once per
 chunk of iterations.
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -97,3 +99,130 @@ def conjugate_gradient(A_indices, A_values, b, x0=torch.zeros([0]), tol=1e-3, ma
     x0 = None if x0.numel() == 0 else x0.to(dev, T)
     x = pcg.solve(b.to(dev, T).contiguous(), x0, tol=tol, max_iter=max_iter)
     return x.clone()
+
+
+class PcgSystem(ctypes.Structure):
+    """Mirror of ``fea_pcg_system`` (include/fea_amd.h)."""
+    _fields_ = [('n', ctypes.c_int32), ('n_groups', ctypes.c_int32), ('spmv_blocks', ctypes.c_int32),
+                ('vec_blocks', ctypes.c_int32), ('max_iter', ctypes.c_int32), ('reserved', ctypes.c_int32),
+                ('nnz', ctypes.c_int64), ('tol', ctypes.c_double),
+                ('grp', ctypes.c_void_p), ('rowptr', ctypes.c_void_p), ('col', ctypes.c_void_p),
+                ('diag_slot', ctypes.c_void_p), ('val', ctypes.c_void_p), ('dinv', ctypes.c_void_p),
+                ('b', ctypes.c_void_p), ('x', ctypes.c_void_p), ('r', ctypes.c_void_p), ('z', ctypes.c_void_p),
+                ('p', ctypes.c_void_p), ('q', ctypes.c_void_p), ('ws', ctypes.c_void_p)]
+
+
+class PcgVectors:
+    """Device vectors and workspace of one system of a batched PCG."""
+
+    def __init__(self, n, vec_dtype, device):
+        T = _DT[vec_dtype]
+        mk = lambda: torch.zeros(max(1, n), dtype=T, device=device)[:n]
+        self.n = int(n)
+        self.x, self.r, self.z, self.p, self.q, self.dinv, self.b = mk(), mk(), mk(), mk(), mk(), mk(), mk()
+        self.ws = torch.zeros(int(nat.lib().fea_pcg_ws_size(self.n)), dtype=torch.float64, device=device)
+
+    def nbytes(self):
+        return sum(t.numel() * t.element_size() for t in (self.x, self.r, self.z, self.p, self.q, self.dinv,
+                                                         self.b, self.ws))
+
+
+class BatchedPCG:
+    """Jacobi-PCG over a batch of independent systems, one launch per CG stage.
+
+    ``solve(entries)`` takes a list of ``(plan, val, vectors, tol, max_iter)``:
+    ``plan`` a DevicePlan (its CSR, node groups and diagonal slots), ``val``
+    its matrix values (``mat_dtype``), ``vectors`` a :class:`PcgVectors` whose
+    ``b`` holds the right-hand side and ``x`` the warm start.  Every system is
+    iterated until its own test r.z / b.(M^-1 b) < tol (the reference's
+    criterion, _linear_solver.py:37) or max_iter; on return each ``x`` holds
+    the solution.  The host reads one status array per chunk of ``chunk``
+    iterations for the whole batch.  With ``use_graph`` the chunk (3 launches
+    per iteration plus the status kernel) is captured once per batch layout in
+    a HIP graph and replayed.
+    """
+
+    def __init__(self, mat_dtype, vec_dtype, device, chunk=32, use_graph=False):
+        self.mat_dtype, self.vec_dtype = mat_dtype, vec_dtype
+        self.device = torch.device(device)
+        self.chunk = int(chunk)
+        self.use_graph = bool(use_graph)
+        L = nat.lib()
+        if L.fea_pcg_system_size() != ctypes.sizeof(PcgSystem):
+            raise nat.FeaNativeError('fea_pcg_system layout mismatch (%d vs %d bytes)'
+                                     % (L.fea_pcg_system_size(), ctypes.sizeof(PcgSystem)))
+        self.max_sys = 1024
+        nb = self.max_sys * ctypes.sizeof(PcgSystem)
+        self._sys_dev = torch.zeros(nb, dtype=torch.uint8, device=self.device)
+        self._ss_dev = torch.zeros(self.max_sys + 1, dtype=torch.int32, device=self.device)
+        self._vs_dev = torch.zeros(self.max_sys + 1, dtype=torch.int32, device=self.device)
+        self._status_dev = torch.zeros(2 * self.max_sys, dtype=torch.float64, device=self.device)
+        self._graph = None
+        self._graph_key = None
+        self.iterations = []
+        self.chunks = 0
+
+    def _layout(self, entries):
+        n = len(entries)
+        if n > self.max_sys:
+            raise ValueError('batch of %d systems exceeds %d' % (n, self.max_sys))
+        arr = (PcgSystem * n)()
+        for k, (plan, val, v, tol, max_iter) in enumerate(entries):
+            if plan.grp is None or plan.n_free != v.n:
+                raise ValueError('system %d: vectors do not match the plan' % k)
+            want = _DT[self.mat_dtype]
+            if val.dtype != want or v.x.dtype != _DT[self.vec_dtype]:
+                raise ValueError('system %d: dtype mismatch' % k)
+            s = arr[k]
+            s.n, s.n_groups, s.max_iter, s.nnz, s.tol = v.n, int(plan.spmv_groups), int(max_iter), plan.nnz, float(tol)
+            s.grp, s.rowptr, s.col, s.diag_slot = (plan.grp.data_ptr(), plan.rowptr.data_ptr(), plan.col.data_ptr(),
+                                                   plan.diag_slot.data_ptr())
+            s.val, s.dinv, s.b, s.x = val.data_ptr(), v.dinv.data_ptr(), v.b.data_ptr(), v.x.data_ptr()
+            s.r, s.z, s.p, s.q, s.ws = v.r.data_ptr(), v.z.data_ptr(), v.p.data_ptr(), v.q.data_ptr(), v.ws.data_ptr()
+        ss = np.zeros(n + 1, dtype=np.int32)
+        vs = np.zeros(n + 1, dtype=np.int32)
+        nat.check(nat.lib().fea_pcg_batch_layout(n, ctypes.byref(arr), ss.ctypes.data, vs.ctypes.data),
+                  'fea_pcg_batch_layout')
+        raw = np.frombuffer(bytes(arr), dtype=np.uint8)
+        self._sys_dev[:raw.size].copy_(torch.from_numpy(raw.copy()))
+        self._ss_dev[:n + 1].copy_(torch.from_numpy(ss))
+        self._vs_dev[:n + 1].copy_(torch.from_numpy(vs))
+        return n, int(ss[-1]), int(vs[-1])
+
+    def _iterate(self, n, st, vt, stream):
+        call('fea_pcg_batch_iterate', self.mat_dtype, self.vec_dtype, n, self._sys_dev, self._ss_dev, self._vs_dev,
+             st, vt, self.chunk, self._status_dev, stream)
+
+    def solve(self, entries):
+        if not entries:
+            return []
+        n, st, vt = self._layout(entries)
+        s = nat.stream_ptr()
+        call('fea_pcg_batch_start', self.mat_dtype, self.vec_dtype, n, self._sys_dev, self._ss_dev, self._vs_dev,
+             st, vt, s)
+        max_iter = max(int(e[4]) for e in entries)
+        graph = None
+        if self.use_graph:
+            key = (n, st, vt, self.chunk)
+            if self._graph_key != key:
+                self._graph = None
+                g = torch.cuda.CUDAGraph()
+                torch.cuda.synchronize()
+                with torch.cuda.graph(g):
+                    self._iterate(n, st, vt, nat.stream_ptr())
+                self._graph, self._graph_key = g, key
+            graph = self._graph
+        done_iters = 0
+        status = None
+        while done_iters < max_iter:
+            if graph is not None:
+                graph.replay()
+            else:
+                self._iterate(n, st, vt, s)
+            self.chunks += 1
+            done_iters += self.chunk
+            status = self._status_dev[:2 * n].view(n, 2).cpu().numpy()
+            if np.all(status[:, 1] != 0.0):
+                break
+        self.iterations = [int(i) for i in status[:, 0]]
+        return self.iterations

@@ -215,6 +215,55 @@ int fea_pcg_spmv(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups,
                  const int32_t *rowptr, const int32_t *col, const void *val, const void *p, void *q,
                  double *ws, void *stream);
 
+/* ---------------------------------------------------------------- batched PCG --
+ * The Jacobi-PCG above over a batch of independent systems, one launch per CG
+ * stage for the whole batch (the solve server's many concurrent static solves;
+ * the reference solves one system at a time, _linear_solver.py:4-42).  No
+ * counterpart in the reference: it replaces running conjugate_gradient once per
+ * request.  Each system brings its own CSR (node row groups required, n_groups
+ * as for fea_spmv), diag_slot, vectors (vec_dtype) and ws (fea_pcg_ws_size
+ * doubles); dinv is written by fea_pcg_batch_start.  A system's iterates do not
+ * depend on the other members of the batch (bit-identical alone and batched).
+ *
+ * fea_pcg_batch_layout (host): fills spmv_blocks / vec_blocks of every system and
+ * the prefix tables spmv_start[n_sys+1], vec_start[n_sys+1]; the caller copies
+ * the system array and both tables to the device and passes the totals
+ * spmv_start[n_sys], vec_start[n_sys].  The same device copies serve every
+ * start / iterate call until the batch changes.
+ * fea_pcg_batch_start: dinv = 1/|diag A|, r = b - A x (x = warm start), z, p,
+ * reset of the iteration count and convergence flag.
+ * fea_pcg_batch_iterate: `iters` iterations of every unconverged system; with
+ * status != NULL (device, 2*n_sys doubles) writes {iterations, done} per system.
+ * Enqueues kernels only (no host synchronisation, no memset): capturable in a
+ * HIP graph. */
+#define FEA_PCG_BATCH_MAX 1024
+typedef struct fea_pcg_system {
+  int32_t n;           /* rows */
+  int32_t n_groups;    /* node row groups (< 0: triple-structured, see fea_spmv) */
+  int32_t spmv_blocks; /* set by fea_pcg_batch_layout */
+  int32_t vec_blocks;  /* set by fea_pcg_batch_layout */
+  int32_t max_iter;
+  int32_t reserved;
+  int64_t nnz;
+  double tol;          /* stop when r.z / b.(dinv b) < tol */
+  const int32_t *grp, *rowptr, *col, *diag_slot;
+  const void *val;     /* mat_dtype */
+  void *dinv;
+  const void *b;
+  void *x, *r, *z, *p, *q;
+  double *ws;
+} fea_pcg_system;
+/* sizeof(fea_pcg_system), for bindings that mirror the struct */
+int fea_pcg_system_size(void);
+int fea_pcg_batch_layout(int n_sys, fea_pcg_system *sys_host, int32_t *spmv_start_host,
+                         int32_t *vec_start_host);
+int fea_pcg_batch_start(int mat_dtype, int vec_dtype, int n_sys, const fea_pcg_system *sys,
+                        const int32_t *spmv_start, const int32_t *vec_start, int spmv_total,
+                        int vec_total, void *stream);
+int fea_pcg_batch_iterate(int mat_dtype, int vec_dtype, int n_sys, const fea_pcg_system *sys,
+                          const int32_t *spmv_start, const int32_t *vec_start, int spmv_total,
+                          int vec_total, int iters, double *status, void *stream);
+
 /* FP64 residual of an FP32-stored matrix: r64 = b64 - A32 x64 (iterative
  * refinement outer loop), plus r64 . r64 into `partial` (fea_vec_blocks(n)
  * per-block sums). */

@@ -61,3 +61,39 @@ def test_gc_rgc_roundtrip(gpu):
     assert int(free.sum()) == asm.GC.shape[0]
     # reference initial-guess quirk: last row of the last instance block = 1e-5 (assembly.py:113)
     assert float(asm.RGC[0][-1, 0]) == 1e-5
+
+
+@pytest.mark.parametrize('etype', ['C3D4', 'C3D8'])
+def test_pressure_on_face_through_origin(gpu, etype):
+    """A loaded face in a plane through the origin makes [x, x_xi, x_eta] singular,
+    where the reference's det * inverse form (pressure.py:112) divides by zero;
+    the kernels' cofactor form stays finite and equals the autograd derivatives
+    of the enclosed-volume functional V (an oracle with no inverse)."""
+    from fea_amd import groupops, meshgen
+    from oracle import autodiff as oad
+    from oracle import surfaces as osurf
+    from tests.helpers import facets
+    mesh = meshgen.kuhn_tet_block(3, 2, 2) if etype == 'C3D4' else meshgen.hex_block(3, 2, 2)
+    conn, stype = facets(mesh, 'y0')
+    X = mesh['nodes']
+    n = conn.shape[1]
+    p = 0.3
+    Xd = torch.as_tensor(X, device='cuda').reshape(-1).contiguous()
+    Ud = torch.zeros_like(Xd)
+    Fv, Kv = groupops.surface_pressure_per_facet(n, torch.as_tensor(conn), Xd, Ud, p, False)
+    Fv, Kv = Fv.cpu().numpy(), Kv.cpu().numpy()
+    assert np.all(np.isfinite(Fv)) and np.all(np.isfinite(Kv))
+    N = X.shape[0]
+    F = np.zeros(3 * N)
+    np.add.at(F, osurf.pressure_force_idx(conn), Fv)
+    K = np.zeros((3 * N, 3 * N))
+    idx = osurf.pressure_stiff_idx(conn)
+    np.add.at(K, (idx[0], idx[1]), Kv)
+    x = torch.as_tensor(X, dtype=torch.float64).requires_grad_(True)
+    V = oad.pressure_volume(stype, conn, x)
+    g = torch.autograd.grad(V, x)[0].numpy().reshape(-1)
+    H = torch.autograd.functional.hessian(lambda y: oad.pressure_volume(stype, conn, y.view(-1, 3)),
+                                          torch.as_tensor(X, dtype=torch.float64).reshape(-1)).numpy()
+    assert np.abs(F - (-p) * g).max() <= 1e-13 * max(1.0, np.abs(g).max())
+    assert np.abs(K - (-p) * H).max() <= 1e-13 * max(1.0, np.abs(H).max())
+    assert np.abs(g).max() > 0 and np.abs(H).max() > 0
