@@ -4,28 +4,32 @@ Requests are independent static Newton solves (a mesh, a material, one loaded
 face and its opposite face fixed), each a full static_implicit.py:44-295 solve
 of the reference.  The server keeps every admitted request's state on the
 device (its DevicePlan: CSR, staging, displacement, Newton direction, PCG
-vectors) and advances ALL admitted requests together, one Newton iteration
-per round:
+vectors) and schedules at the granularity of one PCG chunk.  Each admitted
+request is a state machine (tangent -> PCG -> line search -> tangent ...);
+one *tick* of the server
 
-1. tangent + residual of every request (its own kernels), one host read of
-   all residual norms;
-2. one batched Jacobi-PCG (``BatchedPCG``): a single launch per CG stage for
-   the whole batch, every system stopping on its own tolerance, one host read
-   of all convergence flags per chunk of iterations;
-3. one Armijo line search for the batch: the directional derivatives, the
-   current energies and the full-step trial energies of every request in one
-   host read; only requests that backtrack take further reads.
+* launches the tangent + residual of every request that needs one,
+* starts the PCG of requests whose linear system is new and runs ONE batched
+  chunk of Jacobi-PCG iterations (``BatchedPCG``: a single launch per CG stage
+  for all requests in the PCG stage, each stopping on its own tolerance),
+* launches the line-search evaluations (directional derivative, energies,
+  step norms) of requests whose direction is ready,
+* reads every result of the tick with one host synchronisation and advances
+  each request's state.
 
-Between rounds finished requests leave (their memory returns to the budget)
-and arrived requests are admitted in arrival order while the HBM budget and
-the batch-size cap allow; a request that arrives while others are on the GPU
-joins at the next round, so its work overlaps theirs on the device.
-Admission estimates a request's device footprint from its mesh sizes before
-anything is allocated.
+A request is never held back by another's PCG: a small request runs its
+Newton iterations while a large one is still iterating.  Between ticks
+finished requests leave (their memory returns to the budget) and arrived
+requests are admitted in arrival order while the HBM budget and the
+batch-size cap allow, so requests that arrive while others are on the GPU
+join at the next tick and their work overlaps on the device.  Admission
+estimates a request's device footprint from its mesh sizes before anything
+is allocated.
 
 Every per-request reduction has a fixed shape that depends only on that
-request, so each result is bit-identical whether the request runs alone or in
-any mix (the test suite checks this).
+request, and every decision is the request's own, so each result is
+bit-identical whether the request runs alone or in any mix (the test suite
+checks this).
 
 The Newton iteration is that of ``StaticImplicitSolver`` (Armijo c1 = 0.3 with
 halving, full step when the predicted energy decrease is below rounding,
@@ -144,8 +148,11 @@ def random_requests(n, qps, seed=0, e_range=(2e3, 2e5), aspect_range=(1.0, 4.0),
     NeoHookean (mu, kappa) or Mooney-Rivlin (C10, C01, D1) with parameters
     uniform within +-50% of mu = 1, kappa = 100 (C10 = mu/2 split 5:1 with
     C01, D1 = 2/kappa); a random face fixed and the opposite face loaded by a
-    tensile follower pressure of magnitude 0.1 mu (+-50%) or a transverse dead
-    traction sized for a tip deflection of about 5% of the length.
+    compressive follower pressure 0.05 mu (+-50%; the open-surface pressure
+    functional of pressure.py:134-159 gives a net normal force p A / 3, below
+    the Euler load of the most slender block) or a transverse dead traction
+    sized for a tip deflection of about 5% of the length, at most 0.03 mu
+    (+-50%).
     """
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / qps, size=n)) if qps > 0 else np.zeros(n)
@@ -168,12 +175,12 @@ def random_requests(n, qps, seed=0, e_range=(2e3, 2e5), aspect_range=(1.0, 4.0),
         fixed = ['x0', 'x1', 'y0', 'y1', 'z0', 'z1'][int(rng.integers(6))]
         ax = 'xyz'.index(fixed[0])
         if rng.random() < 0.5:
-            load = ('pressure', -0.1 * mu * rng.uniform(0.5, 1.5))
+            load = ('pressure', 0.05 * mu * rng.uniform(0.5, 1.5))
         else:
             others = [a for a in range(3) if a != ax]
             d_ax = min(others, key=lambda a: L[a])          # bend about the thinner side
             E_y = 9.0 * kappa * mu / (3.0 * kappa + mu)
-            tmag = 0.05 * E_y * L[d_ax] ** 2 / (4.0 * L[ax] ** 2) * rng.uniform(0.5, 1.5)
+            tmag = min(0.05 * E_y * L[d_ax] ** 2 / (4.0 * L[ax] ** 2), 0.03 * mu) * rng.uniform(0.5, 1.5)
             tv = [0.0, 0.0, 0.0]
             tv[d_ax] = tmag * (1.0 if rng.random() < 0.5 else -1.0)
             load = ('traction', tuple(tv))
@@ -226,6 +233,9 @@ class _Slot:
     R: torch.Tensor = None
     K: torch.Tensor = None
     ls: dict = field(default_factory=dict)
+    state: str = 'tangent'
+    max_iter: int = 0
+    rd: object = None
 
     @property
     def scale(self):
@@ -342,7 +352,14 @@ class SolveServer:
         self.used -= req.footprint_est
         self.active.remove(slot)
 
-    # -------------------------------------------------------------- one Newton round
+    # -------------------------------------------------------------- scheduling
+    # Every admitted request is a small state machine advanced one stage per tick:
+    #   'tangent'  -> tangent + residual norm          (read) -> 'tangent' | 'pcg_new' | done
+    #   'pcg_new'  -> joins the batched PCG (start)    -> 'pcg'
+    #   'pcg'      -> one batched chunk of iterations  (read) -> 'pcg' | 'ls'
+    #   'ls'       -> direction.R, energies, step norms (read) -> 'tangent' | 'trial'
+    #   'trial'    -> backtracking trial energy        (read) -> 'tangent' | 'trial'
+    # A tick launches the work of every request, then reads all results at once.
     def _tangent(self, s):
         p = s.plan
         if self.precision == 'fp64':
@@ -351,75 +368,75 @@ class SolveServer:
             s.R, s.K = p.tangent(s.U, s.scale, F64, val_dtype=F32)
 
     def _advance_increment(self, s, now):
-        """Increment converged: finish or move to the next load level with the predictor."""
+        """Increment converged: finish, or move to the next load level with the predictor."""
         if s.k >= s.steps:
             self._retire(s, 'done', now())
-            return False
+            return
         # U <- 2U - U_prev, U_prev <- old U (linear extrapolation of converged states)
         step = s.U - s.U_prev
         s.U_prev.copy_(s.U)
         s.U.add_(step)
         s.k += 1
-        return True
+        s.state = 'tangent'
 
-    def _round(self, now):
-        slots = list(self.active)
-        # 1. tangents and residual norms; increments that are already converged advance
-        need = slots
-        solve = []
-        for _ in range(64):
-            if not need:
-                break
-            reads = _Reads()
-            ids = []
-            for s in need:
-                self._tangent(s)
-                ids.append(_stats_launch(s.plan, s.R, reads))
-            host = reads.fetch()
-            again = []
-            for s, i in zip(need, ids):
-                rss, _ = _stats_value(host[i])
-                if s.it_total >= self.max_newton or not math.isfinite(rss):
-                    self._retire(s, 'failed', now(), 'Newton limit %d reached or residual not finite '
-                                 '(increment %d, |R| %.3e)' % (self.max_newton, s.k, math.sqrt(rss)))
-                    continue
-                if self.verbose:
-                    print('req %d inc %d it %d |R|/|R0| %.3e alpha %.3g' % (s.req.rid, s.k, s.it_total,
-                          math.sqrt(rss) / s.r_ref, s.alpha), flush=True)
-                if math.sqrt(rss) <= self.rel_tol * s.r_ref:
-                    if self._advance_increment(s, now):
-                        again.append(s)
-                else:
-                    solve.append(s)
-            need = again
-        if not solve:
-            return
-        # 2. batched PCG: K dGC = -R, warm start from the previous direction
-        entries = []
-        for s in solve:
-            s.vec.b.copy_(s.R).neg_()
-            s.vec.x.copy_(s.dGC)
-            max_iter = 4 * self.cg_max_iter if s.it_total % 20 == 0 or s.low_alpha > 0 else self.cg_max_iter
-            entries.append((s.plan, s.K, s.vec, self.cg_tol, max_iter))
-        iters = self.pcg.solve(entries)
-        for s, it in zip(solve, iters):
-            s.dGC.copy_(s.vec.x)
-            s.req.cg_iters += it
-            s.it_total += 1
-        # 3. batched line search (static_implicit.py:98-177): the directional derivative,
-        #    the current energy and the full-step trial of every request in one read
+    def _pcg_entry(self, s):
+        return (s.plan, s.K, s.vec, self.cg_tol, s.max_iter)
+
+    def _tick(self, now):
         reads = _Reads()
-        for s in solve:
+        by = {}
+        for s in self.active:
+            by.setdefault(s.state, []).append(s)
+        for s in by.get('tangent', []):
+            self._tangent(s)
+            s.rd = _stats_launch(s.plan, s.R, reads)
+        for s in by.get('ls', []):
             p = s.plan
             p.dof_axpy(s.U, 1.0, s.dGC, s.Unew)
-            s.ls = {'ids': (reads.add(p.dot(s.dGC, s.R).clone()), _energy_launch(p, s.U, reads),
-                            _energy_launch(p, s.Unew, reads), _stats_launch(p, s.dGC, reads),
-                            _stats_launch(p, s.Unew, reads))}
+            s.rd = (reads.add(p.dot(s.dGC, s.R).clone()), _energy_launch(p, s.U, reads),
+                    _energy_launch(p, s.Unew, reads), _stats_launch(p, s.dGC, reads),
+                    _stats_launch(p, s.Unew, reads))
+        for s in by.get('trial', []):
+            st = s.ls
+            s.plan.dof_axpy(s.U, st['sign'] * st['alpha'], s.dGC, s.Unew)
+            s.rd = _energy_launch(s.plan, s.Unew, reads)
+        new = by.get('pcg_new', [])
+        self.pcg.start([self._pcg_entry(s) for s in new])
+        in_pcg = by.get('pcg', []) + new
+        pcg_rd = None
+        if in_pcg:
+            pcg_rd = reads.add(self.pcg.iterate([self._pcg_entry(s) for s in in_pcg]))
         host = reads.fetch()
-        todo = []
-        for s in solve:
+        # ---- decisions
+        for s in by.get('tangent', []):
+            rss, _ = _stats_value(host[s.rd])
+            if s.it_total >= self.max_newton or not math.isfinite(rss):
+                self._retire(s, 'failed', now(), 'Newton limit %d reached or residual not finite '
+                             '(increment %d, |R| %.3e)' % (self.max_newton, s.k, math.sqrt(rss)))
+                continue
+            if self.verbose:
+                print('req %d inc %d it %d |R|/|R0| %.3e alpha %.3g' % (s.req.rid, s.k, s.it_total,
+                      math.sqrt(rss) / s.r_ref, s.alpha), flush=True)
+            if math.sqrt(rss) <= self.rel_tol * s.r_ref:
+                self._advance_increment(s, now)
+                continue
+            # K dGC = -R, warm start from the previous direction (static_implicit.py:297-350)
+            s.vec.b.copy_(s.R).neg_()
+            s.vec.x.copy_(s.dGC)
+            s.max_iter = 4 * self.cg_max_iter if s.it_total % 20 == 0 or s.low_alpha > 0 else self.cg_max_iter
+            s.state = 'pcg_new'
+        if in_pcg:
+            status = host[pcg_rd].reshape(-1, 2)
+            for s, (its, done) in zip(in_pcg, status):
+                s.state = 'pcg'
+                if done != 0.0:
+                    s.dGC.copy_(s.vec.x)
+                    s.req.cg_iters += int(its)
+                    s.it_total += 1
+                    s.state = 'ls'
+        for s in by.get('ls', []):
             p = s.plan
-            i_dd, i_e0, i_e1, i_d, i_u = s.ls['ids']
+            i_dd, i_e0, i_e1, i_d, i_u = s.rd
             deltaE = float(host[i_dd][0])
             e0 = _energy_value(p, host[i_e0], s.scale)
             sign = 1.0
@@ -435,35 +452,20 @@ class SolveServer:
                 if sign < 0:
                     p.dof_axpy(s.U, -1.0, s.dGC, s.Unew)
                 self._accept(s, now)
-                continue
-            if sign > 0:
-                verdict = self._armijo(s, _energy_value(p, host[i_e1], s.scale))
-                if verdict == 'accept':
-                    self._accept(s, now)
-                    continue
-                if verdict == 'fail':
-                    self._retire(s, 'failed', now(), 'line search found no decrease')
-                    continue
-            todo.append(s)
-        # backtracking: one read per halving for the requests still searching
-        while todo:
-            reads = _Reads()
-            ids = []
-            for s in todo:
-                st = s.ls
-                s.plan.dof_axpy(s.U, st['sign'] * st['alpha'], s.dGC, s.Unew)
-                ids.append(_energy_launch(s.plan, s.Unew, reads))
-            host = reads.fetch()
-            nxt = []
-            for s, i in zip(todo, ids):
-                verdict = self._armijo(s, _energy_value(s.plan, host[i], s.scale))
-                if verdict == 'accept':
-                    self._accept(s, now)
-                elif verdict == 'fail':
-                    self._retire(s, 'failed', now(), 'line search found no decrease')
-                else:
-                    nxt.append(s)
-            todo = nxt
+            elif sign > 0:
+                self._verdict(s, self._armijo(s, _energy_value(p, host[i_e1], s.scale)), now)
+            else:
+                s.state = 'trial'
+        for s in by.get('trial', []):
+            self._verdict(s, self._armijo(s, _energy_value(s.plan, host[s.rd], s.scale)), now)
+
+    def _verdict(self, s, verdict, now):
+        if verdict == 'accept':
+            self._accept(s, now)
+        elif verdict == 'fail':
+            self._retire(s, 'failed', now(), 'line search found no decrease')
+        else:
+            s.state = 'trial'
 
     def _armijo(self, s, e_new):
         """Armijo test of the trial at s.ls['alpha'] (c1 = 0.3); halves alpha on rejection."""
@@ -481,6 +483,7 @@ class SolveServer:
         s.U.copy_(s.Unew)
         s.alpha = alpha
         s.low_alpha = s.low_alpha + 1 if alpha < 0.01 else max(0, s.low_alpha - 5)
+        s.state = 'tangent'
         if s.low_alpha > 10:
             self._retire(s, 'failed', now(), 'more than 10 steps with alpha < 0.01')
             return
@@ -521,7 +524,7 @@ class SolveServer:
                     time.sleep(max(0.0, min(queue[qi].arrival - now(), 0.05)))
                 continue
             self.batch_sizes.append(len(self.active))
-            self._round(now)
+            self._tick(now)
             self.rounds += 1
             if on_round is not None:
                 on_round(self)

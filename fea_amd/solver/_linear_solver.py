@@ -193,7 +193,24 @@ class BatchedPCG:
         call('fea_pcg_batch_iterate', self.mat_dtype, self.vec_dtype, n, self._sys_dev, self._ss_dev, self._vs_dev,
              st, vt, self.chunk, self._status_dev, stream)
 
+    def start(self, entries):
+        """Initialise the listed systems (r = b - A x, Jacobi inverse, z, p, counters)."""
+        if not entries:
+            return
+        n, st, vt = self._layout(entries)
+        call('fea_pcg_batch_start', self.mat_dtype, self.vec_dtype, n, self._sys_dev, self._ss_dev, self._vs_dev,
+             st, vt, nat.stream_ptr())
+
+    def iterate(self, entries):
+        """Enqueue one chunk of iterations of the listed (started) systems; returns the
+        device status tensor [n, 2] = (iterations, done) written after the chunk."""
+        n, st, vt = self._layout(entries)
+        self._iterate(n, st, vt, nat.stream_ptr())
+        self.chunks += 1
+        return self._status_dev[:2 * n].view(n, 2)
+
     def solve(self, entries):
+        """Start and iterate the listed systems to convergence (one host read per chunk)."""
         if not entries:
             return []
         n, st, vt = self._layout(entries)

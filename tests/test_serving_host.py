@@ -34,7 +34,7 @@ def test_stream_is_reproducible_and_in_range():
             assert 0.5 - 1e-12 <= mu <= 1.5 + 1e-12 and 50 <= 2 / m[3] <= 150
         kind, val = r.spec['load']
         if kind == 'pressure':
-            assert val < 0          # tensile follower pressure
+            assert val > 0          # compressive follower pressure
         else:
             assert np.count_nonzero(val) == 1
 
