@@ -1,16 +1,30 @@
 #!/usr/bin/env python
-"""Benchmark: mixed-precision static Newton solves/s on one MI355X.
+"""Benchmark of the solid-mechanics hot path on one MI355X.
 
-Workload default: cantilever beam 4x1x1 meshed as 128x32x32 C3D8 (131072
-elements, 8 Gauss points each, 418176 free DOF), Mooney-Rivlin C10=0.5 C01=0.1
-D1=0.01, clamped at x=0, uniform dead traction (0,0,-0.01) on x=4 ramped over 10
-increments; Newton to ||R|| <= 1e-10 ||f_ext|| with FP32 element tangents and
-FP32 Jacobi-PCG inside an FP64 refinement loop (FP64 residual / energy / J terms).
+Default workload (``--workload beam``): cantilever beam 4x1x1 meshed as
+128x32x32 C3D8 (131072 elements, 8 Gauss points each, 418176 free DOF),
+Mooney-Rivlin C10=0.5 C01=0.1 D1=0.01, clamped at x=0, uniform dead traction
+(0,0,-0.01) on x=4 ramped over 10 increments; Newton to ||R|| <= 1e-10
+||f_ext|| with the tangent stored in FP32 and Jacobi-PCG with FP64 vectors
+(FP64 element arithmetic, residual, energy and convergence test).  One "step"
+= one full static solve (10 increments) from u = 0 with the model resident in
+HBM.  The line also carries a bounded sample of the continuous-batching solve
+server against the one-at-a-time FP64 path.
 
-One "step" = one full static solve (10 increments) from u = 0 with the model
-resident in HBM.  The line carries the roofline of the dominant kernel (the
-PCG SpMV stage, timed live with HIP events on the solver's stream) and the CPU
-baseline (the FP64 numpy oracle timed on a bounded sample of the same workload).
+Other workloads (one JSON line each):
+  cube      6000 C3D4 unit cube, NeoHookean mu=1 kappa=100, follower pressure 0.2,
+            5 increments, FP64 (solves/s, p50/p99 over the timed solves);
+  explicit  two 40^3-cell Kuhn-tet blocks (768000 C3D4), penalty contact, dt = 0.9 CFL,
+            FP32 element arithmetic with an FP64 energy balance (steps/s);
+  adjoint   48000 C3D10 block with a perturbed spherical void under pressure:
+            compliance and its gradient w.r.t. every nodal coordinate
+            (gradient evaluations/s) with a central-difference check;
+  serving   a Poisson stream of static solve requests (C3D4/C3D8, 2e3..2e5
+            elements) through the continuous-batching server vs one at a time.
+
+Every line carries the roofline of the workload's dominant kernel (timed live
+with HIP events on the stream the kernel is launched on) and the CPU baseline
+(the FP64 oracle timed on a bounded sample of the same workload).
 """
 import argparse
 import json
@@ -24,13 +38,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=2)
-    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--workload', default='beam',
+                    choices=['beam', 'cube', 'explicit', 'adjoint', 'serving', 'kernels'])
+    ap.add_argument('--steps', type=int, default=None)
+    ap.add_argument('--warmup', type=int, default=None)
     ap.add_argument('--nx', type=int, default=128)
     ap.add_argument('--ny', type=int, default=32)
     ap.add_argument('--nz', type=int, default=32)
@@ -43,9 +60,57 @@ def parse():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--spmv-reps', type=int, default=200)
     ap.add_argument('--max-newton', type=int, default=400, help='Newton iterations per solve (all increments) before failing')
+    ap.add_argument('--serving-requests', type=int, default=None)
+    ap.add_argument('--serving-qps', type=float, default=2.0)
+    ap.add_argument('--serving-emax', type=float, default=None)
+    ap.add_argument('--cells', type=int, default=40, help='explicit: cells per block edge')
     return ap.parse_args()
 
 
+def log(*a):
+    print('#', *a, file=sys.stderr, flush=True)
+
+
+def event_time(torch, fn, reps, warm=3):
+    """Average seconds per call of fn() on torch's current stream (where the
+    library launches), bracketed by HIP events."""
+    for _ in range(warm):
+        fn()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def spmv_bytes(plan, val_bytes):
+    """Compulsory bytes of one node-grouped SpMV: values, one int32 column index per
+    entry of each node's shared column list, the input vector read and the output
+    written once (float64), the group heads' row pointers and the group table."""
+    lens = np.diff(plan.rowptr_np)
+    grp = plan.grp.cpu().numpy()
+    shared_cols = int(lens[grp[:-1]].sum())
+    return plan.nnz * val_bytes + shared_cols * 4 + plan.n_free * 16 + (plan.n_groups + 1) * 8
+
+
+def roofline(kernel, achieved, peak, unit, bound, **extra):
+    d = {'bound': bound, 'kernel': kernel, 'achieved': achieved, 'peak': peak, 'unit': unit,
+         'frac': achieved / peak, 'traffic': None}
+    d.update(extra)
+    return d
+
+
+def base_line(args, metric, value, unit, steps, warmup, elapsed, dtype, config, higher=True):
+    return {'metric': metric, 'value': value, 'unit': unit, 'n_gpus': 1, 'steps': steps, 'warmup': warmup,
+            'ms_per_step': 1e3 * elapsed / max(1, steps), 'higher_is_better': higher, 'scaling': 'weak',
+            'vs_baseline': None, 'dtype': dtype, 'data': 'synthetic (structured mesh generated in-process)',
+            'config': config}
+
+
+# ============================================================================ beam (C2)
 _T0 = [time.perf_counter()]
 
 
@@ -55,12 +120,12 @@ def _progress(k, st):
         file=sys.stderr, flush=True)
 
 
-def make_solver(precision, args):
+def make_solver(precision, args, load_steps=None, progress=True):
     from fea_amd import solver
     return solver.StaticImplicitSolver(maximum_iteration=args.max_newton, precision=precision,
-                                       rel_tol=args.rel_tol, load_steps=args.increments,
+                                       rel_tol=args.rel_tol, load_steps=load_steps or args.increments,
                                        cg_tol=args.cg_tol, cg_max_iter=20000, armijo_slack=1e-12,
-                                       progress=_progress, forcing=args.forcing)
+                                       progress=_progress if progress else None, forcing=args.forcing)
 
 
 def run_solve(fe, torch):
@@ -74,38 +139,18 @@ def run_solve(fe, torch):
     return GC, time.perf_counter() - t0
 
 
-def spmv_roofline(fe, torch, reps):
-    """Average duration of the PCG SpMV stage (fea_pcg_spmv: FP32 matrix, FP64 vectors) via HIP events.
-
-    Compulsory bytes of one launch: FP32 values (4 B per nonzero), one int32
-    column index per entry of each node's shared column list (the rows of a
-    node reuse it), the FP64 input vector read once and the FP64 output written
-    once (8 B per row each), and the row pointers of the group heads plus the
-    group table (4 B each)."""
+def spmv_roofline(fe, torch, reps, mat_dtype):
     from fea_amd import _native as nat
     from fea_amd._native import F32, F64, call
     p = fe.assembly.plan
-    val = p.csr_values(F32)
+    val = p.csr_values(mat_dtype)
     x = torch.rand(p.n_free, dtype=torch.float64, device='cuda')
     q = torch.empty_like(x)
     ws = torch.zeros(int(nat.lib().fea_pcg_ws_size(p.n_free)), dtype=torch.float64, device='cuda')
     s = nat.stream_ptr()
-    for _ in range(5):
-        call('fea_pcg_spmv', F32, F64, p.n_free, p.nnz, p.spmv_groups, p.grp, p.rowptr, p.col, val, x, q, ws, s)
-    stream = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        call('fea_pcg_spmv', F32, F64, p.n_free, p.nnz, p.spmv_groups, p.grp, p.rowptr, p.col, val, x, q, ws, s)
-    e1.record(stream)
-    e1.synchronize()
-    t = e0.elapsed_time(e1) / 1e3 / reps
-    n, nnz = p.n_free, p.nnz
-    lens = np.diff(p.rowptr_np)
-    grp = p.grp.cpu().numpy()
-    shared_cols = int(lens[grp[:-1]].sum())          # one column list per node group
-    alg_bytes = nnz * 4 + shared_cols * 4 + n * (8 + 8) + (p.n_groups + 1) * 4 * 2
-    return t, alg_bytes
+    t = event_time(torch, lambda: call('fea_pcg_spmv', mat_dtype, F64, p.n_free, p.nnz, p.spmv_groups, p.grp,
+                                       p.rowptr, p.col, val, x, q, ws, s), reps)
+    return t, spmv_bytes(p, 4 if mat_dtype == F32 else 8)
 
 
 def element_rate(fe, torch, reps=10, kernel_dtype=None):
@@ -115,20 +160,12 @@ def element_rate(fe, torch, reps=10, kernel_dtype=None):
     p = fe.assembly.plan
     kd = F64 if kernel_dtype is None else kernel_dtype
     U = torch.zeros(3 * p.n_nodes, dtype=torch.float64, device='cuda')
-    p.tangent(U, 1.0, kd, val_dtype=F32)
-    stream = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        p.tangent(U, 1.0, kd, val_dtype=F32)
-    e1.record(stream)
-    e1.synchronize()
-    t = e0.elapsed_time(e1) / 1e3 / reps
+    t = event_time(torch, lambda: p.tangent(U, 1.0, kd, val_dtype=F32), reps, warm=1)
     n_elem = sum(g.n for g in p.groups if g.kind == 'elem')
     return n_elem / t, t
 
 
-def cpu_baseline(args, plan, stats):
+def cpu_baseline_beam(args, plan, stats):
     """FP64 numpy oracle on a bounded sample, extrapolated to one full solve."""
     import scipy.sparse as sp
     from fea_amd import meshgen
@@ -136,7 +173,6 @@ def cpu_baseline(args, plan, stats):
     from oracle import materials as omat
     from oracle import solvers as osol
     threads = len(os.sched_getaffinity(0))
-    # (1) tangent assembly of a 1/16 slice (same element size / material)
     nzs = max(1, args.nz // 16)
     mesh = meshgen.hex_block(args.nx, args.ny, nzs, size=(4.0, 1.0, nzs / args.nz))
     N = mesh['nodes'].shape[0]
@@ -145,14 +181,13 @@ def cpu_baseline(args, plan, stats):
     om = {'nodes': mesh['nodes'], 'fixed': fixed, 'pressures': [], 'tractions': [],
           'groups': [{'etype': 'C3D8', 'conn': mesh['conn'], 'material': omat.MooneyRivlin(0.5, 0.1, 0.01)}]}
     U = 1e-3 * np.random.default_rng(0).standard_normal(mesh['nodes'].shape)
-    oasm.assemble_reduced(om, U)                           # warm (Gauss precompute)
+    oasm.assemble_reduced(om, U)
     t0 = time.perf_counter()
     oasm.assemble_reduced(om, U)
     t_asm = (time.perf_counter() - t0) * (args.nz / nzs)
     t0 = time.perf_counter()
     oasm.total_energy(om, U)
     t_energy = (time.perf_counter() - t0) * (args.nz / nzs)
-    # (2) 20 CG iterations of the oracle CG on a matrix with the full CSR structure
     n = plan.n_free
     A = sp.csr_matrix((np.ones(plan.nnz), plan.col_np, plan.rowptr_np), shape=(n, n))
     A = A + sp.diags(np.diff(plan.rowptr_np).astype(float))
@@ -172,39 +207,33 @@ def cpu_baseline(args, plan, stats):
             't_assembly_s': t_asm, 't_cg_iter_s': t_cg, 't_energy_s': t_energy}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print('bench.py runs on exactly one GPU: this build has no multi-GPU path '
-              '(requested --gpus %d)' % args.gpus, file=sys.stderr)
-        sys.exit(2)
-    import torch
+def bench_beam(args, torch):
     from fea_amd import workloads
-    from fea_amd._native import F64
-
+    from fea_amd._native import F32, F64
+    steps = 2 if args.steps is None else args.steps
+    warmup = 1 if args.warmup is None else args.warmup
     fe, mesh = workloads.beam_traction(args.nx, args.ny, args.nz)
     fe.solver = make_solver(args.precision, args)
     fe.initialize()
-    print('# model: %d elements, %d nodes, %d free DOF, nnz %d' % (
-        mesh['conn'].shape[0], mesh['nodes'].shape[0], fe.assembly.plan.n_free, fe.assembly.plan.nnz),
-        file=sys.stderr, flush=True)
-    for i in range(args.warmup):
+    p = fe.assembly.plan
+    log('model: %d elements, %d nodes, %d free DOF, nnz %d' % (mesh['conn'].shape[0], mesh['nodes'].shape[0],
+                                                              p.n_free, p.nnz))
+    for i in range(warmup):
         _, t = run_solve(fe, torch)
-        print('# warmup solve %d: %.2f s, stats %s' % (i, t, {k: v for k, v in fe.solver.stats.items()
-                                                            if k != 'residual_history'}),
-              file=sys.stderr, flush=True)
+        log('warmup solve %d: %.2f s, stats %s' % (i, t, {k: v for k, v in fe.solver.stats.items()
+                                                         if k != 'residual_history'}))
     torch.cuda.synchronize()
     lat = []
     t_start = time.perf_counter()
     GC = None
-    for i in range(args.steps):
+    for i in range(steps):
         GC, t = run_solve(fe, torch)
         lat.append(t)
-        print('# timed solve %d: %.3f s' % (i, t), file=sys.stderr, flush=True)
+        log('timed solve %d: %.3f s' % (i, t))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     stats = dict(fe.solver.stats)
-    value = args.steps / elapsed
+    value = steps / elapsed
 
     fp64 = {}
     if args.precision != 'fp64' and not args.no_fp64_compare:
@@ -214,41 +243,430 @@ def main():
         GC64, t64 = run_solve(fe64, torch)
         err = float((GC - GC64).abs().max() / GC64.abs().max())
         fp64 = {'fp64_solves_per_s': 1.0 / t64, 'fp64_solve_s': t64, 'rel_err_vs_fp64': err,
-                'fp64_cg_iters': fe64.solver.stats['cg_iters'], 'fp64_newton_iters': fe64.solver.stats['newton_iters']}
-        print('# fp64 solve %.3f s, rel err %.2e' % (t64, err), file=sys.stderr, flush=True)
+                'fp64_cg_iters': fe64.solver.stats['cg_iters'], 'fp64_newton_iters': fe64.solver.stats['newton_iters'],
+                'speedup_vs_fp64': t64 / (elapsed / steps)}
+        log('fp64 solve %.3f s, rel err %.2e' % (t64, err))
         del fe64
 
-    t_spmv, alg_bytes = spmv_roofline(fe, torch, args.spmv_reps)
+    mat = F64 if args.precision == 'fp64' else F32
+    t_spmv, alg_bytes = spmv_roofline(fe, torch, args.spmv_reps, mat)
     achieved = alg_bytes / t_spmv / 1e9
     elem_rate, t_tan = element_rate(fe, torch)
-    from fea_amd._native import F32 as _F32
-    elem_rate32, t_tan32 = element_rate(fe, torch, kernel_dtype=_F32)
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(args, fe.assembly.plan, stats)
+    elem_rate32, t_tan32 = element_rate(fe, torch, kernel_dtype=F32)
+    serving = serving_sample(args, torch)
+    cpu = None if args.no_cpu else cpu_baseline_beam(args, p, stats)
 
-    line = {
-        'metric': 'static Newton solves/s (mixed FP32/FP64-IR) at 1e-8 rel err',
-        'value': value, 'unit': 'solves/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
-        'ms_per_step': 1e3 * elapsed / args.steps, 'higher_is_better': True, 'scaling': 'weak',
-        'vs_baseline': None, 'dtype': 'f64' if args.precision == 'fp64' else 'f32+f64',
-        'data': 'synthetic (structured mesh generated in-process)',
-        'config': {'workload': 'cantilever beam 4x1x1, %dx%dx%d C3D8, Mooney-Rivlin C10=0.5 C01=0.1 D1=0.01, '
-                               'clamped x=0, dead traction (0,0,-0.01) on x=4, %d increments, Newton rel tol %g'
-                               % (args.nx, args.ny, args.nz, args.increments, args.rel_tol),
-                   'precision': args.precision, 'free_dof': fe.assembly.plan.n_free, 'nnz': fe.assembly.plan.nnz},
+    line = base_line(args, 'static Newton solves/s (mixed FP32/FP64) at 1e-8 rel err', value, 'solves/s', steps,
+                     warmup, elapsed, 'f64' if args.precision == 'fp64' else 'f32+f64',
+                     {'workload': 'cantilever beam 4x1x1, %dx%dx%d C3D8, Mooney-Rivlin C10=0.5 C01=0.1 D1=0.01, '
+                                  'clamped x=0, dead traction (0,0,-0.01) on x=4, %d increments, Newton rel tol %g'
+                                  % (args.nx, args.ny, args.nz, args.increments, args.rel_tol),
+                      'precision': args.precision, 'free_dof': p.n_free, 'nnz': p.nnz})
+    line.update({
         'latency_p50_s': float(np.percentile(lat, 50)), 'latency_p99_s': float(np.percentile(lat, 99)),
         'newton_iters_per_solve': stats.get('newton_iters'), 'cg_iters_per_solve': stats.get('cg_iters'),
         'energy_evals_per_solve': stats.get('energy_evals'),
         'elem_tangent_evals_per_s': elem_rate, 'tangent_assembly_ms': 1e3 * t_tan,
         'elem_tangent_evals_per_s_f32_contraction': elem_rate32,
-        'roofline': {'bound': 'hbm', 'kernel': 'k_pcg_spmv<float,double,32,true> (fea_pcg_spmv)',
-                     'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
-                     'alg_bytes_per_launch': alg_bytes, 'avg_launch_us': t_spmv * 1e6},
+        'roofline': roofline('k_pcg_spmv<%s,double,32,2> (fea_pcg_spmv)' % ('float' if mat == F32 else 'double'),
+                             achieved, HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=alg_bytes,
+                             avg_launch_us=t_spmv * 1e6),
         'cpu_baseline': cpu,
-    }
+        'serving_sample': serving,
+    })
     line.update(fp64)
+    return line
+
+
+# ============================================================================ serving (C5)
+def _serving_run(torch, reqs):
+    """The stream through the server twice -- saturated (every request queued at
+    t = 0: batching throughput) and at its Poisson arrival times (latency) --
+    and the plain path: each request solved alone by the FP64 solver, timed, with
+    a FIFO queue of those times under the same arrivals for its latencies."""
+    import copy
+    from fea_amd import serving
+    warm = serving.random_requests(1, 0.0, seed=99, e_range=(500, 600))
+    serving.SolveServer(precision='mixed').run(warm, realtime=False)
+
+    def serve(rs, realtime):
+        t0 = time.perf_counter()
+
+        def on_tick(s):
+            if s.rounds % 500 == 0:
+                log('serving (%s) tick %d t=%.2f in flight %d' % ('poisson' if realtime else 'saturated', s.rounds,
+                                                                  time.perf_counter() - t0, len(s.active)))
+        srv = serving.SolveServer(precision='mixed', max_batch=64)
+        srv.run(rs, realtime=realtime, on_round=on_tick)
+        return srv, time.perf_counter() - t0
+
+    sat = [serving.SolveRequest(rid=r.rid, spec=r.spec, arrival=0.0) for r in reqs]
+    srv_sat, wall_sat = serve(sat, False)
+    srv, wall = serve(reqs, True)
+    done = [r for r in reqs if r.status == 'done']
+    lat = [r.latency for r in done]
+    secs, errs = [], []
+    for r in sorted(reqs, key=lambda r: r.arrival):
+        ref, sec, _ = serving.solve_one_at_a_time([r], precision='fp64')
+        secs.append(sec[0])
+        if r.status == 'done' and ref[r.rid] is not None:
+            errs.append(float((r.GC - ref[r.rid]).abs().max() / ref[r.rid].abs().max()))
+        log('one-at-a-time fp64 request %d (%d elements): %.2f s' % (r.rid, r.n_elements(), sec[0]))
+    end, lat1 = 0.0, []
+    for r, t in zip(sorted(reqs, key=lambda r: r.arrival), secs):
+        end = max(end, r.arrival) + t
+        lat1.append(end - r.arrival)
+    n_sat = sum(1 for r in sat if r.status == 'done')
+    return {
+        'requests': len(reqs), 'completed': len(done), 'qps_offered': None,
+        'elements_range': [int(min(r.n_elements() for r in reqs)), int(max(r.n_elements() for r in reqs))],
+        'saturated_solves_per_s': n_sat / wall_sat,
+        'one_at_a_time_fp64_solves_per_s': len(reqs) / sum(secs),
+        'saturated_throughput_ratio_vs_fp64_one_at_a_time': (n_sat / wall_sat) / (len(reqs) / sum(secs)),
+        'saturated_max_in_flight': int(max(srv_sat.batch_sizes)) if srv_sat.batch_sizes else 0,
+        'poisson_solves_per_s': len(done) / wall, 'poisson_wall_s': wall,
+        'latency_p50_s': float(np.percentile(lat, 50)) if lat else None,
+        'latency_p99_s': float(np.percentile(lat, 99)) if lat else None,
+        'one_at_a_time_fp64_latency_p50_s': float(np.percentile(lat1, 50)),
+        'one_at_a_time_fp64_latency_p99_s': float(np.percentile(lat1, 99)),
+        'one_at_a_time_fp64_makespan_s': end,
+        'max_in_flight': int(max(srv.batch_sizes)) if srv.batch_sizes else 0, 'ticks': srv.rounds,
+        'peak_admitted_gb': srv.peak_used / 1e9,
+        'max_rel_err_vs_fp64': max(errs) if errs else None,
+    }, srv
+
+
+def serving_sample(args, torch):
+    """Bounded sample of the solve server inside the default line."""
+    from fea_amd import serving
+    n = 8 if args.serving_requests is None else args.serving_requests
+    if n <= 0:
+        return None
+    emax = 3e4 if args.serving_emax is None else args.serving_emax
+    reqs = serving.random_requests(n, args.serving_qps, seed=1, e_range=(2e3, emax))
+    out, _ = _serving_run(torch, reqs)
+    out['qps_offered'] = args.serving_qps
+    out['sample'] = '%d requests, elements log-uniform in [2e3, %.0e], Poisson arrivals at %.1f/s' % (
+        n, emax, args.serving_qps)
+    return out
+
+
+def bench_serving(args, torch):
+    from fea_amd import serving
+    from fea_amd._native import F32, F64, call
+    from fea_amd import _native as nat
+    from fea_amd.solver._linear_solver import BatchedPCG, PcgVectors
+    n = 24 if args.serving_requests is None else args.serving_requests
+    emax = 2e5 if args.serving_emax is None else args.serving_emax
+    reqs = serving.random_requests(n, args.serving_qps, seed=0, e_range=(2e3, emax))
+    out, srv = _serving_run(torch, reqs)
+    out['qps_offered'] = args.serving_qps
+    # roofline: the batched SpMV stage over the stream's systems at once (tangents at u = 0)
+    sub = sorted(reqs, key=lambda r: r.n_elements())
+    entries, keep, total_bytes = [], [], 0
+    for r in sub:
+        fe, _ = serving.request_model(r.spec)
+        fe.assembly.initialize()
+        p = fe.assembly.plan
+        U = torch.zeros(3 * p.n_nodes, dtype=torch.float64, device='cuda')
+        _, K = p.tangent(U, 1.0, F64, val_dtype=F32)
+        v = PcgVectors(p.n_free, F64, 'cuda')
+        v.b.copy_(torch.rand_like(v.b))
+        entries.append((p, K.clone(), v, 1e-30, 10))
+        keep.append(fe)
+        total_bytes += spmv_bytes(p, 4)
+    bp = BatchedPCG(F32, F64, 'cuda')
+    bp.start(entries)
+    nsys, st, vt = bp._layout(entries)
+    s = nat.stream_ptr()
+    t = event_time(torch, lambda: call('fea_pcg_batch_spmv', F32, F64, nsys, bp._sys_dev, bp._ss_dev, st, s), 100)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_requests(sub[:3])
+    line = base_line(args, 'static solves/s through the continuous-batching server (mixed FP32/FP64), saturated',
+                     out['saturated_solves_per_s'], 'solves/s', n, 1, n / out['saturated_solves_per_s'], 'f32+f64',
+                     {'workload': 'Poisson stream of %d static solve requests at %.1f/s: C3D4/C3D8 blocks, '
+                                  'elements log-uniform in [2e3, %.0e], aspect ratios U[1,4], NH/MR +-50%%, '
+                                  'pressure or traction on one face, opposite face fixed, 2 increments'
+                                  % (n, args.serving_qps, emax)})
+    line.update(out)
+    line['roofline'] = roofline('k_batch_spmv<float,double> (fea_pcg_batch_spmv), all %d systems in one launch' % nsys,
+                                total_bytes / t / 1e9, HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=total_bytes,
+                                avg_launch_us=t * 1e6)
+    line['cpu_baseline'] = cpu
+    return line
+
+
+def cpu_baseline_requests(reqs):
+    """Oracle direct-solve Newton (FP64, CPU) of the given requests, one at a time."""
+    from fea_amd import serving
+    from oracle import solvers as osol
+    from tests.helpers import mr, nh, oracle_model
+    t0 = time.perf_counter()
+    for r in reqs:
+        mesh = serving.request_mesh(r.spec)
+        m = r.spec['material']
+        mat = nh(m[1], m[2]) if m[0] == 'NH' else mr(m[1], m[2], m[3])
+        face = serving._OPPOSITE[r.spec['fixed']]
+        kind, val = r.spec['load']
+        om = oracle_model(mesh, mat, r.spec['fixed'], pressure=val if kind == 'pressure' else None,
+                          traction=val if kind == 'traction' else None, load_set=face)
+        osol.static_solve(om, increments=r.spec['load_steps'], rel_tol=1e-10, linear='direct',
+                          armijo_slack_rel=1e-12)
+    t = time.perf_counter() - t0
+    return {'value': len(reqs) / t, 'unit': 'solves/s', 'cores': len(os.sched_getaffinity(0)), 'kind': 'port',
+            'sample': 'FP64 numpy oracle Newton with direct solves on the %d smallest requests (%s elements), '
+                      'one at a time: %.1f s' % (len(reqs), ', '.join(str(r.n_elements()) for r in reqs), t)}
+
+
+# ============================================================================ cube (C1)
+def bench_cube(args, torch):
+    from fea_amd import workloads, solver
+    from fea_amd._native import F64
+    from oracle import solvers as osol
+    from tests.helpers import nh, oracle_model
+    steps = 20 if args.steps is None else args.steps
+    warmup = 2 if args.warmup is None else args.warmup
+    fe, mesh = workloads.cube_pressure(10, pressure=0.2, jitter=0.05, seed=0)
+    fe.solver = solver.StaticImplicitSolver(precision='fp64', rel_tol=1e-10, load_steps=5, cg_tol=1e-12,
+                                            cg_max_iter=20000, armijo_slack=1e-12)
+    fe.initialize()
+    p = fe.assembly.plan
+    for _ in range(warmup):
+        run_solve(fe, torch)
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        GC, t = run_solve(fe, torch)
+        lat.append(t)
+    elapsed = time.perf_counter() - t_start
+    stats = dict(fe.solver.stats)
+    t_spmv, alg = spmv_roofline(fe, torch, 500, F64)
+    cpu = None
+    err = None
+    if not args.no_cpu:
+        om = oracle_model(mesh, nh(1.0, 100.0), 'z0', pressure=0.2, load_set='z1')
+        t0 = time.perf_counter()
+        GC_o, _ = osol.static_solve(om, increments=5, rel_tol=1e-10, linear='direct', armijo_slack_rel=1e-12)
+        t_cpu = time.perf_counter() - t0
+        err = float(np.abs(GC.cpu().numpy() - GC_o).max() / np.abs(GC_o).max())
+        cpu = {'value': 1.0 / t_cpu, 'unit': 'solves/s', 'cores': len(os.sched_getaffinity(0)), 'kind': 'port',
+               'sample': 'one full FP64 oracle solve of the same model (Newton with direct linear solves): %.2f s'
+                         % t_cpu}
+    line = base_line(args, 'static Newton solves/s (FP64) at 1e-8 rel err', steps / elapsed, 'solves/s', steps,
+                     warmup, elapsed, 'f64',
+                     {'workload': 'unit cube 10x10x10 Kuhn-split C3D4 (6000 elements, 3630 free DOF), NeoHookean '
+                                  'mu=1 kappa=100, z=0 fixed, follower pressure 0.2 on z=1, 5 increments, Newton rel '
+                                  'tol 1e-10, 5% interior jitter', 'precision': 'fp64'})
+    line.update({'latency_p50_s': float(np.percentile(lat, 50)), 'latency_p99_s': float(np.percentile(lat, 99)),
+                 'newton_iters_per_solve': stats['newton_iters'], 'cg_iters_per_solve': stats['cg_iters'],
+                 'rel_err_vs_oracle': err,
+                 'roofline': roofline('k_pcg_spmv<double,double,16,2> (fea_pcg_spmv)', alg / t_spmv / 1e9,
+                                      HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=alg,
+                                      avg_launch_us=t_spmv * 1e6,
+                                      note='1.9 MB working set: launch-latency bound, not bandwidth bound'),
+                 'cpu_baseline': cpu})
+    return line
+
+
+# ============================================================================ explicit (C3)
+def bench_explicit(args, torch):
+    from fea_amd import workloads, solver
+    from fea_amd._native import F32, F64
+    from fea_amd import _native as nat
+    steps = 20000 if args.steps is None else args.steps
+    warmup = 200 if args.warmup is None else args.warmup
+    fe, info = workloads.two_blocks_contact(args.cells)
+    fe.solver = solver.DynamicExplicitSolver(time_end=1.0, time_per_storage=1e30, cfl_factor=0.9, store=False,
+                                             precision='fp32')
+    fe.initialize()
+    sol = fe.solver
+    p = fe.assembly.plan
+    dt = sol._deltaT
+    GV0 = workloads.initial_velocity(fe, info)
+    U = p.gc_to_u(torch.zeros_like(fe.assembly.GC))
+    vh = GV0.clone()
+    acc = torch.zeros_like(vh)
+    e0 = sol.energy_balance(U, vh, None, dt)
+    for k in range(warmup):
+        sol.step(U, vh, acc, dt, first=(k == 0))
+    torch.cuda.synchronize()
+    hist = [e0]
+    t_start = time.perf_counter()
+    chunk = max(1, steps // 10)
+    done = 0
+    while done < steps:
+        m = min(chunk, steps - done)
+        for _ in range(m):
+            sol.step(U, vh, acc, dt)
+        done += m
+        torch.cuda.synchronize()
+        log('explicit step %d t=%.2fs' % (warmup + done, time.perf_counter() - t_start))
+    elapsed = time.perf_counter() - t_start
+    sol._check_contacts()
+    hist.append(sol.energy_balance(U, vh, acc, dt))
+    drift = abs(hist[-1]['total'] - hist[0]['total']) / hist[0]['kinetic']
+    # roofline: the FP32 residual kernel of one block pair (one step's element pass)
+    g = [g for g in p.groups if g.kind == 'elem']
+    E = sum(x.n for x in g)
+
+    def residual():
+        p.residual_stage(U, 1.0, F32)
+    t_res = event_time(torch, residual, 200)
+    alg = E * 4 * 4 + p.n_nodes * 3 * 8 * 3                 # conn + X + U read, nodal force written
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_explicit(fe, info, dt)
+    line = base_line(args, 'explicit central-difference steps/s with penalty contact', steps / elapsed, 'steps/s',
+                     steps, warmup, elapsed, 'f32+f64',
+                     {'workload': 'two %d^3-cell Kuhn-tet blocks (%d C3D4), NeoHookean mu=1 kappa=50 rho=1, upper '
+                                  'block at 0.1 onto the lower (base fixed), node-to-surface penalty 1e3, dt = 0.9 CFL '
+                                  '(%.3e)' % (args.cells, E, dt), 'precision': 'fp32 element forces, fp64 state'})
+    line.update({'dt': dt, 'simulated_time': dt * (steps + warmup),
+                 'energy': {'initial': hist[0], 'final': hist[-1], 'drift_rel_to_initial_kinetic': drift},
+                 'roofline': roofline('k_elem_r<float,4> (fea_elem_residual, one step\'s element pass)',
+                                      alg / t_res / 1e9, HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=alg,
+                                      avg_launch_us=t_res * 1e6),
+                 'cpu_baseline': cpu})
+    return line
+
+
+def cpu_baseline_explicit(fe, info, dt):
+    """One oracle residual evaluation of the full two-block model = one step's element work."""
+    from oracle import assembly as oasm
+    from oracle import materials as omat
+    lower, upper = info['meshes']
+    nl = lower['nodes'].shape[0]
+    nodes = np.concatenate([lower['nodes'], upper['nodes']])
+    fixed = np.zeros((nodes.shape[0], 3), dtype=bool)
+    fixed[lower['node_sets']['z0']] = True
+    om = {'nodes': nodes, 'fixed': fixed, 'pressures': [], 'tractions': [],
+          'groups': [{'etype': 'C3D4', 'conn': lower['conn'], 'material': omat.NeoHookean(1.0, 50.0), 'rho': 1.0},
+                     {'etype': 'C3D4', 'conn': upper['conn'] + nl, 'material': omat.NeoHookean(1.0, 50.0),
+                      'rho': 1.0}]}
+    U = 1e-4 * np.random.default_rng(0).standard_normal(nodes.shape)
+    oasm.assemble_force(om, U)
+    t0 = time.perf_counter()
+    oasm.assemble_force(om, U)
+    t = time.perf_counter() - t0
+    return {'value': 1.0 / t, 'unit': 'steps/s', 'cores': len(os.sched_getaffinity(0)), 'kind': 'port',
+            'sample': 'one FP64 numpy-oracle residual evaluation of the full two-block mesh (the element work of '
+                      'one central-difference step; update and contact omitted): %.2f s' % t}
+
+
+# ============================================================================ adjoint (C4)
+def bench_adjoint(args, torch):
+    from fea_amd import adjoint, workloads, solver
+    from fea_amd._native import F64
+    from fea_amd import _native as nat
+    steps = 2 if args.steps is None else args.steps
+    warmup = 1 if args.warmup is None else args.warmup
+    fe, mesh = workloads.void_block_pressure()
+    fe.solver = solver.StaticImplicitSolver(precision='fp64', rel_tol=1e-11, load_steps=2, cg_tol=1e-22,
+                                            cg_max_iter=20000, armijo_slack=1e-12)
+    fe.initialize()
+    p = fe.assembly.plan
+    E = sum(g.n for g in p.groups if g.kind == 'elem')
+    for _ in range(warmup):
+        adjoint.compliance_gradient(fe)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(steps):
+        J0, gX, gth = adjoint.compliance_gradient(fe)
+        torch.cuda.synchronize()
+        log('gradient %d: %.2f s' % (i, time.perf_counter() - t_start))
+    elapsed = time.perf_counter() - t_start
+    # central-difference check on seeded random directions (each costs two static solves)
+    X0 = p.X.reshape(-1).clone()
+    rng = np.random.default_rng(13)
+    h = 1e-6
+    errs = []
+    for _ in range(4):
+        d = torch.as_tensor(rng.uniform(-1, 1, X0.numel()), device='cuda')
+        Jp = adjoint.compliance(fe, X=X0 + h * d)[0].item()
+        Jm = adjoint.compliance(fe, X=X0 - h * d)[0].item()
+        fd = (Jp - Jm) / (2 * h)
+        an = float(torch.dot(gX, d))
+        errs.append(abs(fd - an) / max(abs(an), 1e-300))
+    p.set_coordinates(X0)
+    # roofline: the adjoint element kernel (d(lam.f_int)/dX, /dU, /dtheta of the C3D10 group)
+    U = torch.rand(3 * p.n_nodes, dtype=torch.float64, device='cuda') * 1e-3
+    lam = torch.rand_like(U)
+    t_vjp = event_time(torch, lambda: adjoint.elem_vjp(p, U, lam, want_x=True, want_u=True, want_theta=True), 20)
+    alg = E * 10 * 4 + p.n_nodes * 3 * 8 * (3 + 2)          # conn + X, U, lam read; gX, gU written
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_adjoint(mesh, p, E)
+    line = base_line(args, 'compliance shape-gradient evaluations/s (static solve + adjoint)', steps / elapsed,
+                     'gradients/s', steps, warmup, elapsed, 'f64',
+                     {'workload': '2x1x1 block with a perturbed spherical void (r=0.3, Fourier amplitude 0.02), '
+                                  '%d C3D10, NeoHookean mu=1 kappa=100, pressure 0.05 in the void, x=0 fixed, '
+                                  'gradient w.r.t. all %d nodal coordinates' % (E, 3 * p.n_nodes)})
+    line.update({'compliance': J0, 'fd_check_rel_err_max': max(errs), 'fd_directions': len(errs), 'fd_h': h,
+                 'roofline': roofline('k_elem_adjoint<10> (fea_elem_adjoint + nodal gather)', alg / t_vjp / 1e9,
+                                      HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=alg,
+                                      avg_launch_us=t_vjp * 1e6),
+                 'cpu_baseline': cpu})
+    return line
+
+
+def cpu_baseline_adjoint(mesh, p, E, frac=8):
+    import torch
+    from oracle import autodiff as ad
+    conn = mesh['conn'][: E // frac]
+    X = mesh['nodes']
+    U = 1e-3 * np.random.default_rng(0).standard_normal(X.shape)
+    lam = np.random.default_rng(1).standard_normal(X.shape)
+    t0 = time.perf_counter()
+    ad.internal_force_vjp('C3D10', conn, X, U, (0.5, 0.0, 100.0), lam)
+    t = (time.perf_counter() - t0) * frac
+    return {'value': 1.0 / t, 'unit': 'element-VJP passes/s', 'cores': len(os.sched_getaffinity(0)),
+            'kind': 'port', 'sample': 'torch-autograd oracle VJP of the internal force (CPU, FP64) over 1/%d of '
+                                      'the C3D10 elements, scaled: %.2f s per full pass' % (frac, t)}
+
+
+# ============================================================================ kernels (counter runs)
+def bench_kernels(args, torch):
+    """A short fixed sequence for hardware-counter runs: the beam's tangent
+    assembly (FP64 element kernel + CSR gather to FP32), 20 launches of the PCG
+    SpMV stage on the FP32 tangent, and 20 plain device copies of 256 MiB
+    (the known-size calibration for FETCH_SIZE / WRITE_SIZE)."""
+    from fea_amd import workloads
+    from fea_amd._native import F32, F64, call
+    from fea_amd import _native as nat
+    fe, mesh = workloads.beam_traction(args.nx, args.ny, args.nz)
+    fe.assembly.initialize()
+    p = fe.assembly.plan
+    U = torch.zeros(3 * p.n_nodes, dtype=torch.float64, device='cuda')
+    for _ in range(3):
+        p.tangent(U, 1.0, F64, val_dtype=F32)
+    val = p.csr_values(F32)
+    x = torch.rand(p.n_free, dtype=torch.float64, device='cuda')
+    q = torch.empty_like(x)
+    ws = torch.zeros(int(nat.lib().fea_pcg_ws_size(p.n_free)), dtype=torch.float64, device='cuda')
+    s = nat.stream_ptr()
+    for _ in range(20):
+        call('fea_pcg_spmv', F32, F64, p.n_free, p.nnz, p.spmv_groups, p.grp, p.rowptr, p.col, val, x, q, ws, s)
+    a = torch.empty(256 << 20, dtype=torch.uint8, device='cuda').fill_(1)
+    b = torch.empty_like(a)
+    for _ in range(20):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {'workload': 'kernels', 'spmv_alg_bytes': spmv_bytes(p, 4), 'copy_bytes': a.numel(),
+            'n_elem': int(mesh['conn'].shape[0]), 'nnz': p.nnz, 'n_free': p.n_free, 'n_kstage': p.n_kstage}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py runs on exactly one GPU: this build has no multi-GPU path '
+              '(requested --gpus %d)' % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    torch.set_num_threads(len(os.sched_getaffinity(0)))
+    fn = {'beam': bench_beam, 'cube': bench_cube, 'explicit': bench_explicit, 'adjoint': bench_adjoint,
+          'serving': bench_serving, 'kernels': bench_kernels}[args.workload]
+    line = fn(args, torch)
     print(json.dumps(line), flush=True)
 
 

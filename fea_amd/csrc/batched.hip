@@ -279,6 +279,22 @@ extern "C" int fea_pcg_batch_start(int mat_dtype, int vec_dtype, int n_sys, cons
   return FEA_OK;
 }
 
+extern "C" int fea_pcg_batch_spmv(int mat_dtype, int vec_dtype, int n_sys, const fea_pcg_system *sys,
+                                  const int32_t *spmv_start, int spmv_total, void *stream) {
+  FEA_CHECK_ARG(batch_combo(mat_dtype, vec_dtype), "fea_pcg_batch_spmv: unsupported precision pair");
+  FEA_CHECK_ARG(n_sys > 0 && n_sys <= FEA_PCG_BATCH_MAX && spmv_total > 0, "fea_pcg_batch_spmv: bad sizes");
+  FEA_CHECK_ARG(sys && spmv_start, "fea_pcg_batch_spmv: null argument");
+  hipStream_t st = as_stream(stream);
+  if (mat_dtype == FEA_F64)
+    k_batch_spmv<double, double><<<spmv_total, kBlock, 0, st>>>(n_sys, sys, spmv_start);
+  else if (vec_dtype == FEA_F64)
+    k_batch_spmv<float, double><<<spmv_total, kBlock, 0, st>>>(n_sys, sys, spmv_start);
+  else
+    k_batch_spmv<float, float><<<spmv_total, kBlock, 0, st>>>(n_sys, sys, spmv_start);
+  FEA_CHECK_LAUNCH("fea_pcg_batch_spmv");
+  return FEA_OK;
+}
+
 extern "C" int fea_pcg_batch_iterate(int mat_dtype, int vec_dtype, int n_sys, const fea_pcg_system *sys,
                                      const int32_t *spmv_start, const int32_t *vec_start, int spmv_total,
                                      int vec_total, int iters, double *status, void *stream) {

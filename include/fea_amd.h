@@ -263,6 +263,10 @@ int fea_pcg_batch_start(int mat_dtype, int vec_dtype, int n_sys, const fea_pcg_s
 int fea_pcg_batch_iterate(int mat_dtype, int vec_dtype, int n_sys, const fea_pcg_system *sys,
                           const int32_t *spmv_start, const int32_t *vec_start, int spmv_total,
                           int vec_total, int iters, double *status, void *stream);
+/* The SpMV stage of one batched iteration alone (q = A p and the p.q partials of
+ * every unconverged system), for timing. */
+int fea_pcg_batch_spmv(int mat_dtype, int vec_dtype, int n_sys, const fea_pcg_system *sys,
+                       const int32_t *spmv_start, int spmv_total, void *stream);
 
 /* FP64 residual of an FP32-stored matrix: r64 = b64 - A32 x64 (iterative
  * refinement outer loop), plus r64 . r64 into `partial` (fea_vec_blocks(n)
