@@ -4,121 +4,164 @@
 
 namespace fea {
 
-// One thread per (element, node a): forms the row block K_a. (3 x 3*nen) and R_a,
-// recomputing the element's Gauss-point state in registers.
+// Residual + tangent.  A workgroup takes EPW elements, one thread per (element,
+// node a).  The element's nodal coordinates / displacements and, per Gauss
+// point, dN/dX, F, w S and the packed upper triangle of w D are staged in LDS:
+// thread (e, g) computes Gauss point g's state once (float64), instead of every
+// node thread recomputing all of them.  Thread (e, a) then forms its row block
+// K_a. (3 x 3 nen) and R_a from the staged state, in passes over BH columns b
+// at a time to bound the live accumulators.  The B^T D B contraction runs in T.
+template <int ET>
+struct RkCfg;
+template <>
+struct RkCfg<4> {
+  static constexpr int EPW = 16, BH = 4;
+};
+template <>
+struct RkCfg<8> {
+  static constexpr int EPW = 8, BH = 4;
+};
+template <>
+struct RkCfg<10> {
+  static constexpr int EPW = 6, BH = 5;
+};
+
 template <typename T, int ET>
-__global__ __launch_bounds__(kBlock) void k_elem_rk(int n_elem, const int32_t *__restrict__ conn,
-                                                    const double *__restrict__ X,
-                                                    const double *__restrict__ U, MatParams mp,
-                                                    const int32_t *__restrict__ dst_k,
-                                                    T *__restrict__ kstage,
-                                                    const int32_t *__restrict__ dst_r,
-                                                    double *__restrict__ rstage) {
+__global__ __launch_bounds__(RkCfg<ET>::EPW * Elem<ET>::nen) void k_elem_rk(
+    int n_elem, const int32_t *__restrict__ conn, const double *__restrict__ X, const double *__restrict__ U,
+    MatParams mp, const int32_t *__restrict__ dst_k, T *__restrict__ kstage, const int32_t *__restrict__ dst_r,
+    double *__restrict__ rstage) {
   constexpr int NEN = Elem<ET>::nen, NGP = Elem<ET>::ngp;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)n_elem * NEN) return;
-  const int e = (int)(t / NEN), a = (int)(t - (int64_t)e * NEN);
-  double xd[NEN][3], ud[NEN][3];
-#pragma unroll
-  for (int c = 0; c < NEN; ++c) {
-    const int64_t n = conn[(int64_t)e * NEN + c];
+  constexpr int EPW = RkCfg<ET>::EPW, BH = RkCfg<ET>::BH;
+  constexpr int O_F = 3 * NEN, O_S = O_F + 9, O_D = O_S + 6, GS = O_D + 21;  // per-GP layout
+  __shared__ double sx[EPW][NEN][3], su[EPW][NEN][3];
+  __shared__ T sg[EPW][NGP][GS];
+  const int le = threadIdx.x / NEN, a = threadIdx.x - le * NEN;
+  const int64_t e = (int64_t)blockIdx.x * EPW + le;
+  const bool valid = e < n_elem;
+  if (valid) {
+    const int64_t n = conn[e * NEN + a];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      xd[c][i] = X[3 * n + i];
-      ud[c][i] = U[3 * n + i];
+      sx[le][a][i] = X[3 * n + i];
+      su[le][a][i] = U[3 * n + i];
     }
   }
-  T K[NEN][9];
-#pragma unroll
-  for (int b = 0; b < NEN; ++b)
-#pragma unroll
-    for (int k = 0; k < 9; ++k) K[b][k] = T(0);
-  double Ra[3] = {0.0, 0.0, 0.0};
-
-  for (int g = 0; g < NGP; ++g) {
-    // Gauss-point state (geometry, kinematics, S and the material tangent) in
-    // float64: near incompressibility the deviatoric part of the tangent is a
-    // small difference of bulk-modulus-sized terms.  Only the B^T D B
-    // contraction below runs in T.
+  __syncthreads();
+  if (valid && a < NGP) {
+    // Gauss point g = a of element e: state in float64 (near incompressibility
+    // the deviatoric part of the tangent is a small difference of bulk-sized terms)
     double dNd[NEN][3];
-    const double wd = gp_geometry<double, ET>(g, xd, dNd);
+    const double wd = gp_geometry<double, ET>(a, sx[le], dNd);
     double Hd[9];
-    disp_grad<double, NEN>(ud, dNd, Hd);
+    disp_grad<double, NEN>(su[le], dNd, Hd);
     GPState<double> sd;
     kinematics(Hd, sd);
     double Sd[6];
     stress(sd, mp, Sd);
     double Dd[6][6];
     tangent(sd, mp, Dd);
-    const T w = (T)wd;
-    T dN[NEN][3], S[6], D[6][6];
-    GPState<T> s;
+    T *o = sg[le][a];
 #pragma unroll
     for (int c = 0; c < NEN; ++c)
 #pragma unroll
-      for (int i = 0; i < 3; ++i) dN[c][i] = (T)dNd[c][i];
+      for (int i = 0; i < 3; ++i) o[3 * c + i] = (T)dNd[c][i];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) s.F[i] = (T)sd.F[i];
+    for (int k = 0; k < 9; ++k) o[O_F + k] = (T)sd.F[k];
 #pragma unroll
-    for (int I = 0; I < 6; ++I) {
-      S[I] = (T)Sd[I];
-#pragma unroll
-      for (int J = 0; J < 6; ++J) D[I][J] = (T)Dd[I][J];
-    }
-    T Ba[6][3];
-    bmat(s.F, dN[a], Ba);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      T acc = T(0);
-#pragma unroll
-      for (int I = 0; I < 6; ++I) acc += S[I] * Ba[I][j];
-      Ra[j] += (double)(w * acc);
-    }
-    T DB[6][3];
+    for (int I = 0; I < 6; ++I) o[O_S + I] = (T)(wd * Sd[I]);
+    int q = 0;
 #pragma unroll
     for (int I = 0; I < 6; ++I)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        T acc = T(0);
-#pragma unroll
-        for (int L = 0; L < 6; ++L) acc += D[I][L] * Ba[L][j];
-        DB[I][j] = w * acc;
-      }
-    // S dN_a (full symmetric S)
-    T SdNa[3];
-#pragma unroll
-    for (int A = 0; A < 3; ++A)
-      SdNa[A] = w * (S[vidx(A, 0)] * dN[a][0] + S[vidx(A, 1)] * dN[a][1] + S[vidx(A, 2)] * dN[a][2]);
-#pragma unroll
-    for (int b = 0; b < NEN; ++b) {
-      T Bb[6][3];
-      bmat(s.F, dN[b], Bb);
-      const T geo = SdNa[0] * dN[b][0] + SdNa[1] * dN[b][1] + SdNa[2] * dN[b][2];
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          T acc = (j == k) ? geo : T(0);
-#pragma unroll
-          for (int I = 0; I < 6; ++I) acc += DB[I][j] * Bb[I][k];
-          K[b][j * 3 + k] += acc;
-        }
-    }
+      for (int J = I; J < 6; ++J) o[O_D + q++] = (T)(wd * Dd[I][J]);
   }
-  if (kstage != nullptr) {
-    const int64_t base = ((int64_t)e * NEN + a) * NEN;
+  __syncthreads();
+  if (!valid) return;
+  double Ra[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int b0 = 0; b0 < NEN; b0 += BH) {
+    T K[BH][9];
 #pragma unroll
-    for (int b = 0; b < NEN; ++b) {
-      const int d = dst_k[base + b];
-      if (d >= 0) {
-        T *o = kstage + (int64_t)d * 9;
+    for (int bb = 0; bb < BH; ++bb)
 #pragma unroll
-        for (int k = 0; k < 9; ++k) o[k] = K[b][k];
+      for (int k = 0; k < 9; ++k) K[bb][k] = T(0);
+    for (int g = 0; g < NGP; ++g) {
+      const T *st = sg[le][g];
+      T F[9], dNa[3];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) F[k] = st[O_F + k];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) dNa[i] = st[3 * a + i];
+      T Ba[6][3];
+      bmat(F, dNa, Ba);
+      if (b0 == 0) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          T acc = T(0);
+#pragma unroll
+          for (int I = 0; I < 6; ++I) acc += st[O_S + I] * Ba[I][j];
+          Ra[j] += (double)acc;
+        }
+      }
+      // DB = (w D) Ba, D read from the packed upper triangle
+      T DB[6][3];
+#pragma unroll
+      for (int I = 0; I < 6; ++I)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) DB[I][j] = T(0);
+#pragma unroll
+      for (int I = 0; I < 6; ++I)
+#pragma unroll
+        for (int L = 0; L < 6; ++L) {
+          const int r = I < L ? I : L, c = I < L ? L : I;
+          const T d = st[O_D + r * 6 - (r * (r - 1)) / 2 + (c - r)];
+#pragma unroll
+          for (int j = 0; j < 3; ++j) DB[I][j] += d * Ba[L][j];
+        }
+      // (w S) dN_a (full symmetric S)
+      T SdNa[3];
+#pragma unroll
+      for (int A = 0; A < 3; ++A)
+        SdNa[A] = st[O_S + vidx(A, 0)] * dNa[0] + st[O_S + vidx(A, 1)] * dNa[1] + st[O_S + vidx(A, 2)] * dNa[2];
+#pragma unroll
+      for (int bb = 0; bb < BH; ++bb) {
+        const int b = b0 + bb;
+        if (b < NEN) {
+          const T dNb[3] = {st[3 * b + 0], st[3 * b + 1], st[3 * b + 2]};
+          T Bb[6][3];
+          bmat(F, dNb, Bb);
+          const T geo = SdNa[0] * dNb[0] + SdNa[1] * dNb[1] + SdNa[2] * dNb[2];
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              T acc = (j == k) ? geo : T(0);
+#pragma unroll
+              for (int I = 0; I < 6; ++I) acc += DB[I][j] * Bb[I][k];
+              K[bb][j * 3 + k] += acc;
+            }
+        }
+      }
+    }
+    if (kstage != nullptr) {
+      const int64_t base = (e * NEN + a) * NEN;
+#pragma unroll
+      for (int bb = 0; bb < BH; ++bb) {
+        const int b = b0 + bb;
+        if (b < NEN) {
+          const int d = dst_k[base + b];
+          if (d >= 0) {
+            T *o = kstage + (int64_t)d * 9;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) o[k] = K[bb][k];
+          }
+        }
       }
     }
   }
   if (rstage != nullptr) {
-    const int d = dst_r[(int64_t)e * NEN + a];
+    const int d = dst_r[e * NEN + a];
     if (d >= 0) {
       rstage[(int64_t)d * 3 + 0] = Ra[0];
       rstage[(int64_t)d * 3 + 1] = Ra[1];
@@ -296,16 +339,22 @@ static bool valid_elem(int et) { return et == 4 || et == 8 || et == 10; }
 
 static MatParams mat_from(const double *p) { return MatParams{p[0], p[1], p[2]}; }
 
+template <typename T, int ET>
+static void launch_rk_t(int n, const int32_t *conn, const double *X, const double *U, MatParams mp,
+                        const int32_t *dk, T *ks, const int32_t *dr, double *rs, hipStream_t st) {
+  constexpr int EPW = RkCfg<ET>::EPW;
+  const int grid = (int)(((int64_t)n + EPW - 1) / EPW);
+  k_elem_rk<T, ET><<<grid, EPW * Elem<ET>::nen, 0, st>>>(n, conn, X, U, mp, dk, ks, dr, rs);
+}
+
 template <typename T>
 static int launch_rk(int et, int n, const int32_t *conn, const double *X, const double *U,
                      MatParams mp, const int32_t *dk, T *ks, const int32_t *dr, double *rs,
                      hipStream_t st) {
-  const int64_t threads = (int64_t)n * et;
-  const int grid = blocks_for(threads);
   switch (et) {
-    case 4: k_elem_rk<T, 4><<<grid, kBlock, 0, st>>>(n, conn, X, U, mp, dk, ks, dr, rs); break;
-    case 8: k_elem_rk<T, 8><<<grid, kBlock, 0, st>>>(n, conn, X, U, mp, dk, ks, dr, rs); break;
-    case 10: k_elem_rk<T, 10><<<grid, kBlock, 0, st>>>(n, conn, X, U, mp, dk, ks, dr, rs); break;
+    case 4: launch_rk_t<T, 4>(n, conn, X, U, mp, dk, ks, dr, rs, st); break;
+    case 8: launch_rk_t<T, 8>(n, conn, X, U, mp, dk, ks, dr, rs, st); break;
+    case 10: launch_rk_t<T, 10>(n, conn, X, U, mp, dk, ks, dr, rs, st); break;
   }
   FEA_CHECK_LAUNCH("fea_elem_residual_tangent");
   return FEA_OK;

