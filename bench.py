@@ -139,6 +139,12 @@ def run_solve(fe, torch):
     return GC, time.perf_counter() - t0
 
 
+def _triple_sw(plan):
+    """Sub-wave width the library picks for a triple-blocked CSR (neighbour triples per node)."""
+    work = plan.nnz / max(1, plan.n_free) / 3.0
+    return 32 if work > 32 else (16 if work > 8 else 8)
+
+
 def spmv_roofline(fe, torch, reps, mat_dtype):
     from fea_amd import _native as nat
     from fea_amd._native import F32, F64, call
@@ -268,7 +274,8 @@ def bench_beam(args, torch):
         'energy_evals_per_solve': stats.get('energy_evals'),
         'elem_tangent_evals_per_s': elem_rate, 'tangent_assembly_ms': 1e3 * t_tan,
         'elem_tangent_evals_per_s_f32_contraction': elem_rate32,
-        'roofline': roofline('k_pcg_spmv<%s,double,32,2> (fea_pcg_spmv)' % ('float' if mat == F32 else 'double'),
+        'roofline': roofline('k_pcg_spmv<%s,double,%d,2> (fea_pcg_spmv)' % ('float' if mat == F32 else 'double',
+                                                                            _triple_sw(p)),
                              achieved, HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=alg_bytes,
                              avg_launch_us=t_spmv * 1e6),
         'cpu_baseline': cpu,
@@ -333,6 +340,11 @@ def _serving_run(torch, reqs):
         'max_in_flight': int(max(srv.batch_sizes)) if srv.batch_sizes else 0, 'ticks': srv.rounds,
         'peak_admitted_gb': srv.peak_used / 1e9,
         'max_rel_err_vs_fp64': max(errs) if errs else None,
+        'not_completed': [{'rid': r.rid, 'elements': r.n_elements(), 'status': r.status, 'error': r.error,
+                           'newton_iters': r.newton_iters, 'cg_iters': r.cg_iters}
+                          for r in reqs if r.status != 'done'],
+        'saturated_not_completed': [{'rid': r.rid, 'status': r.status, 'error': r.error} for r in sat
+                                    if r.status != 'done'],
     }, srv
 
 
@@ -461,7 +473,8 @@ def bench_cube(args, torch):
     line.update({'latency_p50_s': float(np.percentile(lat, 50)), 'latency_p99_s': float(np.percentile(lat, 99)),
                  'newton_iters_per_solve': stats['newton_iters'], 'cg_iters_per_solve': stats['cg_iters'],
                  'rel_err_vs_oracle': err,
-                 'roofline': roofline('k_pcg_spmv<double,double,16,2> (fea_pcg_spmv)', alg / t_spmv / 1e9,
+                 'roofline': roofline('k_pcg_spmv<double,double,%d,2> (fea_pcg_spmv)' % _triple_sw(p),
+                                      alg / t_spmv / 1e9,
                                       HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=alg,
                                       avg_launch_us=t_spmv * 1e6,
                                       note='1.9 MB working set: launch-latency bound, not bandwidth bound'),
@@ -593,7 +606,7 @@ def bench_adjoint(args, torch):
     U = torch.rand(3 * p.n_nodes, dtype=torch.float64, device='cuda') * 1e-3
     lam = torch.rand_like(U)
     t_vjp = event_time(torch, lambda: adjoint.elem_vjp(p, U, lam, want_x=True, want_u=True, want_theta=True), 20)
-    alg = E * 10 * 4 + p.n_nodes * 3 * 8 * (3 + 2)          # conn + X, U, lam read; gX, gU written
+    flops = E * 4 * 2 * (99 * 10 + 27 * 10 * 10)            # the C3D10 tangent contraction count per pass
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_adjoint(mesh, p, E)
@@ -603,9 +616,11 @@ def bench_adjoint(args, torch):
                                   '%d C3D10, NeoHookean mu=1 kappa=100, pressure 0.05 in the void, x=0 fixed, '
                                   'gradient w.r.t. all %d nodal coordinates' % (E, 3 * p.n_nodes)})
     line.update({'compliance': J0, 'fd_check_rel_err_max': max(errs), 'fd_directions': len(errs), 'fd_h': h,
-                 'roofline': roofline('k_elem_adjoint<10> (fea_elem_adjoint + nodal gather)', alg / t_vjp / 1e9,
-                                      HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=alg,
-                                      avg_launch_us=t_vjp * 1e6),
+                 'roofline': roofline('k_elem_adjoint<10> (fea_elem_adjoint + nodal gather)', flops / t_vjp / 1e12,
+                                      FP64_PEAK_TFLOPS, 'TFLOP/s', 'mfma', alg_flops_per_launch=flops,
+                                      avg_launch_us=t_vjp * 1e6,
+                                      note='FP64 vector arithmetic; peak = MI355X FP64 78.6 TFLOP/s; algorithmic '
+                                           'FLOPs = the C3D10 tangent contraction 2 g (99 nen + 27 nen^2) per element'),
                  'cpu_baseline': cpu})
     return line
 

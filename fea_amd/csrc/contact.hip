@@ -121,54 +121,77 @@ __global__ __launch_bounds__(kBlock) void k_bucket_sort(int nb, const int32_t *_
 // ---------------------------------------------------------------- search
 constexpr int kMaxCand = 64;
 
+// One wave64 per slave node.  Lane c < 27 takes neighbour cell c of the slave's
+// cell and scans its bucket unless a lower lane hashed to the same bucket (so
+// every bucket is scanned once); a wave prefix sum of the lanes' match counts
+// places the candidates in LDS (at most kMaxCand kept, all counted), and a
+// 64-lane bitonic sort orders them by facet id.  Same pair set and output as a
+// serial scan of the 27 cells followed by a sort.
+constexpr int kSearchWaves = kBlock / 64;
+
 __global__ __launch_bounds__(kBlock) void k_contact_search(
     int ns, const int32_t *__restrict__ slave, const double *__restrict__ X, const double *__restrict__ U,
     const double *__restrict__ cent, const int32_t *__restrict__ bstart, const int32_t *__restrict__ bitems,
     double ox, double oy, double oz, double inv_h, unsigned int mask, double r2, int kmax,
     int32_t *__restrict__ cand, int32_t *__restrict__ count, int32_t *__restrict__ max_count) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int32_t buf[kSearchWaves][kMaxCand];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = blockIdx.x * kSearchWaves + wave;  // uniform across the wave
   if (s >= ns) return;
   double p[3];
   node_pos(X, U, slave[s], p);
   const int64_t cx = cell_of(p[0], ox, inv_h), cy = cell_of(p[1], oy, inv_h), cz = cell_of(p[2], oz, inv_h);
-  unsigned int seen[27];
-  int nseen = 0;
-  int32_t loc[kMaxCand];
-  int nloc = 0, total = 0;
-  for (int dz = -1; dz <= 1; ++dz)
-    for (int dy = -1; dy <= 1; ++dy)
-      for (int dx = -1; dx <= 1; ++dx) {
-        const unsigned int k = cell_key(cx + dx, cy + dy, cz + dz, mask);
-        bool dup = false;
-        for (int i = 0; i < nseen; ++i) dup = dup || (seen[i] == k);
-        if (dup) continue;
-        seen[nseen++] = k;
-        for (int it = bstart[k]; it < bstart[k + 1]; ++it) {
-          const int f = bitems[it];
-          const double ddx = p[0] - cent[3 * (int64_t)f + 0];
-          const double ddy = p[1] - cent[3 * (int64_t)f + 1];
-          const double ddz = p[2] - cent[3 * (int64_t)f + 2];
-          const double d2 = dadd(dadd(dmul(ddx, ddx), dmul(ddy, ddy)), dmul(ddz, ddz));
-          if (d2 <= r2) {
-            ++total;
-            if (nloc < kMaxCand) loc[nloc++] = f;
-          }
-        }
-      }
-  // ascending facet ids
-  for (int i = 1; i < nloc; ++i) {
-    const int32_t v = loc[i];
-    int j = i - 1;
-    while (j >= 0 && loc[j] > v) {
-      loc[j + 1] = loc[j];
-      --j;
-    }
-    loc[j + 1] = v;
+  const bool active = lane < 27;
+  unsigned int k = 0xffffffffu;
+  if (active) k = cell_key(cx + (lane % 3) - 1, cy + (lane / 3) % 3 - 1, cz + lane / 9 - 1, mask);
+  bool dup = false;
+  for (int j = 0; j < 27; ++j) {
+    const unsigned int kj = __shfl(k, j);
+    dup = dup || (j < lane && kj == k);
   }
+  const bool scan = active && !dup;
+  int b0 = 0, b1 = 0;
+  if (scan) {
+    b0 = bstart[k];
+    b1 = bstart[k + 1];
+  }
+  auto hit = [&](int f) {
+    const double ddx = p[0] - cent[3 * (int64_t)f + 0];
+    const double ddy = p[1] - cent[3 * (int64_t)f + 1];
+    const double ddz = p[2] - cent[3 * (int64_t)f + 2];
+    return dadd(dadd(dmul(ddx, ddx), dmul(ddy, ddy)), dmul(ddz, ddz)) <= r2;
+  };
+  int cnt = 0;
+  for (int it = b0; it < b1; ++it) cnt += hit(bitems[it]) ? 1 : 0;
+  int incl = cnt;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(incl, off);
+    if (lane >= off) incl += y;
+  }
+  const int total = __shfl(incl, 63);
+  int pos = incl - cnt;
+  for (int it = b0; it < b1; ++it) {
+    const int f = bitems[it];
+    if (hit(f)) {
+      if (pos < kMaxCand) buf[wave][pos] = f;
+      ++pos;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  const int nloc = total < kMaxCand ? total : kMaxCand;
+  int32_t v = lane < nloc ? buf[wave][lane] : INT32_MAX;
+  for (int size = 2; size <= 64; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int32_t o = __shfl_xor(v, stride);
+      const bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
+      v = keep_min ? (o < v ? o : v) : (o > v ? o : v);
+    }
   const int nout = nloc < kmax ? nloc : kmax;
-  for (int i = 0; i < nout; ++i) cand[(int64_t)s * kmax + i] = loc[i];
-  count[s] = total;
-  if (max_count != nullptr && total > kmax) atomicMax(max_count, total);
+  if (lane < nout) cand[(int64_t)s * kmax + lane] = v;
+  if (lane == 0) {
+    count[s] = total;
+    if (max_count != nullptr && total > kmax) atomicMax(max_count, total);
+  }
 }
 
 // ---------------------------------------------------------------- penalty
@@ -358,7 +381,7 @@ extern "C" int fea_contact_search(int n_slave, const int32_t *slave, const doubl
   FEA_CHECK_ARG(slave && X && U && centroid && bucket_start && bucket_items && cand && count,
                 "fea_contact_search: null argument");
   const unsigned int nb = 1u << table_bits;
-  k_contact_search<<<blocks_for(n_slave), kBlock, 0, as_stream(stream)>>>(
+  k_contact_search<<<(n_slave + kSearchWaves - 1) / kSearchWaves, kBlock, 0, as_stream(stream)>>>(
       n_slave, slave, X, U, centroid, bucket_start, bucket_items, origin_host[0], origin_host[1], origin_host[2],
       1.0 / cell, nb - 1, radius * radius, kmax, cand, count, max_count);
   FEA_CHECK_LAUNCH("fea_contact_search");

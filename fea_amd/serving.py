@@ -90,12 +90,28 @@ class SolveRequest:
 
 
 def request_mesh(spec):
+    """The request's block.  For a pressure load the coordinate origin is put one
+    unit inside the block behind the centre of the loaded face (the face then
+    lies in the plane x.n = 1): the follower-pressure functional of an open
+    surface, V = 1/3 sum w det[x, x_xi, x_eta] (pressure.py:134-159), depends on
+    the origin, and its stiffness carries terms proportional to the in-plane
+    distance of the face points from it -- large enough on a big face with the
+    origin at a corner to make a mild compression unstable (the FP64 solver
+    diverges there).  A face plane through the origin is avoided because the
+    reference's form of the derivative (det A A^-T) is undefined there."""
     nx, ny, nz = spec['cells']
     if spec['etype'] == 'C3D4':
-        return meshgen.kuhn_tet_block(nx, ny, nz, size=tuple(spec['size']))
-    if spec['etype'] == 'C3D8':
-        return meshgen.hex_block(nx, ny, nz, size=tuple(spec['size']))
-    raise ValueError('unsupported element type %r' % (spec['etype'],))
+        mesh = meshgen.kuhn_tet_block(nx, ny, nz, size=tuple(spec['size']))
+    elif spec['etype'] == 'C3D8':
+        mesh = meshgen.hex_block(nx, ny, nz, size=tuple(spec['size']))
+    else:
+        raise ValueError('unsupported element type %r' % (spec['etype'],))
+    if spec['load'][0] == 'pressure':
+        face = _OPPOSITE[spec['fixed']]
+        n_out = np.zeros(3)
+        n_out['xyz'.index(face[0])] = 1.0 if face[1] == '1' else -1.0
+        mesh['nodes'] = mesh['nodes'] - (mesh['nodes'][mesh['node_sets'][face]].mean(axis=0) - n_out)
+    return mesh
 
 
 def request_model(spec):
