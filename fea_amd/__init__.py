@@ -1,7 +1,8 @@
 """fea_amd — MI355X-native (gfx950) nonlinear solid-mechanics hot path with the
 Python API of sssnnn7119/FEA (FEA/__init__.py): Part / Instance / Assembly,
 elements (C3D4, C3D8, C3D10), materials (NeoHookean, MooneyRivlin), loads
-(Pressure, Traction), boundary conditions and the static / explicit solvers.
+(Pressure, Traction), boundary conditions, the static / explicit solvers and INP
+ingestion (FEA.inp / FEA.from_inp).
 
 All numerical work runs in the HIP library fea_amd/_lib/libfea_amd.so (C ABI in
 include/fea_amd.h); importing works without a GPU, calling the hot path does not.
@@ -10,6 +11,9 @@ from .controller import FEAController
 from .assemble import Part, Instance, Assembly
 from .assemble import materials, elements, loads, boundarys, surfaces
 from . import solver, meshgen
+from .inp import FEA_INP, from_inp
+
+inp = FEA_INP   # the reference's tests call FEA.inp() (FEA/__init__.py:3, tests/element_test/test_element.py:19)
 
 __all__ = ['FEAController', 'Part', 'Instance', 'Assembly', 'materials', 'elements', 'loads',
-           'boundarys', 'surfaces', 'solver', 'meshgen']
+           'boundarys', 'surfaces', 'solver', 'meshgen', 'FEA_INP', 'from_inp', 'inp']
