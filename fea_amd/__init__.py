@@ -13,7 +13,5 @@ from .assemble import materials, elements, loads, boundarys, surfaces
 from . import solver, meshgen
 from .inp import FEA_INP, from_inp
 
-inp = FEA_INP   # the reference's tests call FEA.inp() (FEA/__init__.py:3, tests/element_test/test_element.py:19)
-
 __all__ = ['FEAController', 'Part', 'Instance', 'Assembly', 'materials', 'elements', 'loads',
-           'boundarys', 'surfaces', 'solver', 'meshgen', 'FEA_INP', 'from_inp', 'inp']
+           'boundarys', 'surfaces', 'solver', 'meshgen', 'FEA_INP', 'from_inp']

@@ -35,6 +35,7 @@ _SIGS = {
     'fea_elem_energy': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_elem_energy_blocks': (C_INT, [C_INT]),
     'fea_elem_prepare': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_elem_mass': (C_INT, [C_INT, C_INT, C_P, C_P, C_DBL, C_P, C_P, C_P]),
     'fea_elem_lumped_mass': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_DBL, C_P, C_P, C_P]),
     'fea_surface_pressure': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_DBL, C_P, C_P, C_P, C_P, C_P]),
     'fea_surface_volume': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P, C_P]),

@@ -335,6 +335,47 @@ __global__ __launch_bounds__(kBlock) void k_elem_lumped_mass(int n_elem, const i
   rstage[(int64_t)d * 3 + 2] = m[2];
 }
 
+// Consistent mass blocks (Element_3D.get_mass_matrix, C3base.py:348-384):
+// M_ab = rho sum_g N_a N_b w detJ times the 3x3 identity; one thread per
+// (element, node a) writes its row of blocks through the tangent's staging map.
+template <int ET>
+__global__ __launch_bounds__(kBlock) void k_elem_mass(int n_elem, const int32_t *__restrict__ conn,
+                                                      const double *__restrict__ X, double rho,
+                                                      const int32_t *__restrict__ dst_k,
+                                                      double *__restrict__ kstage) {
+  constexpr int NEN = Elem<ET>::nen, NGP = Elem<ET>::ngp;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n_elem * NEN) return;
+  const int e = (int)(t / NEN), a = (int)(t - (int64_t)e * NEN);
+  double x[NEN][3];
+#pragma unroll
+  for (int c = 0; c < NEN; ++c) {
+    const int64_t n = conn[(int64_t)e * NEN + c];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) x[c][i] = X[3 * n + i];
+  }
+  double m[NEN];
+#pragma unroll
+  for (int b = 0; b < NEN; ++b) m[b] = 0.0;
+  for (int g = 0; g < NGP; ++g) {
+    double dN[NEN][3], N[NEN];
+    const double w = gp_geometry<double, ET>(g, x, dN);
+    Elem<ET>::shape(g, N);
+#pragma unroll
+    for (int b = 0; b < NEN; ++b) m[b] += rho * N[a] * N[b] * w;
+  }
+  const int64_t base = ((int64_t)e * NEN + a) * NEN;
+#pragma unroll
+  for (int b = 0; b < NEN; ++b) {
+    const int d = dst_k[base + b];
+    if (d >= 0) {
+      double *o = kstage + (int64_t)d * 9;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) o[k] = (k == 0 || k == 4 || k == 8) ? m[b] : 0.0;
+    }
+  }
+}
+
 static bool valid_elem(int et) { return et == 4 || et == 8 || et == 10; }
 
 static MatParams mat_from(const double *p) { return MatParams{p[0], p[1], p[2]}; }
@@ -464,5 +505,22 @@ extern "C" int fea_elem_lumped_mass(int elem_type, int n_elem, const int32_t *co
     case 10: k_elem_lumped_mass<10><<<grid, kBlock, 0, st>>>(n_elem, conn, X, dof_map, rho, dst_r, rstage); break;
   }
   FEA_CHECK_LAUNCH("fea_elem_lumped_mass");
+  return FEA_OK;
+}
+
+extern "C" int fea_elem_mass(int elem_type, int n_elem, const int32_t *conn, const double *X, double rho,
+                             const int32_t *dst_k, double *kstage, void *stream) {
+  FEA_CHECK_ARG(valid_elem(elem_type), "fea_elem_mass: bad element type %d", elem_type);
+  FEA_CHECK_ARG(n_elem >= 0, "fea_elem_mass: bad arguments");
+  if (n_elem == 0) return FEA_OK;
+  FEA_CHECK_ARG(conn && X && dst_k && kstage, "fea_elem_mass: null input");
+  const int grid = blocks_for((int64_t)n_elem * elem_type);
+  hipStream_t st = as_stream(stream);
+  switch (elem_type) {
+    case 4: k_elem_mass<4><<<grid, kBlock, 0, st>>>(n_elem, conn, X, rho, dst_k, kstage); break;
+    case 8: k_elem_mass<8><<<grid, kBlock, 0, st>>>(n_elem, conn, X, rho, dst_k, kstage); break;
+    case 10: k_elem_mass<10><<<grid, kBlock, 0, st>>>(n_elem, conn, X, rho, dst_k, kstage); break;
+  }
+  FEA_CHECK_LAUNCH("fea_elem_mass");
   return FEA_OK;
 }

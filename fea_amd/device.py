@@ -185,6 +185,7 @@ class DevicePlan:
         if X.numel() != 3 * self.n_nodes:
             raise ValueError('X must have %d entries' % (3 * self.n_nodes))
         self.X.view(-1).copy_(X)
+        self._mass = None
 
     def kstage(self, dtype):
         if dtype not in self._kstage:
@@ -308,6 +309,22 @@ class DevicePlan:
                 E -= load_scale * terms[gi]                              # - f.u
         return float(E)
 
+    def mass_matrix(self):
+        """Consistent mass in the tangent's free-DOF CSR pattern, float64 values
+        (Assembly.assemble_mass_matrix, assembly.py:421-443); computed once."""
+        if getattr(self, '_mass', None) is None:
+            s = self._s()
+            ks = self.kstage(F64)
+            ks.zero_()
+            for g in self.groups:
+                if g.kind == 'elem' and g.n:
+                    call('fea_elem_mass', g.ntype, g.n, g.conn, self.X, g.rho, g.dst_k, ks, s)
+            val = torch.zeros(max(1, self.nnz), dtype=torch.float64, device=self.device)
+            call('fea_gather_csr', F64, F64, self.n_blocks, self.blk_a, self.blk_b, self.blk_colpos,
+                 self.kseg, self.dof_map, self.rowptr, ks, val, s)
+            self._mass = val
+        return self._mass
+
     def lumped_mass(self):
         """Row-sum lumped mass on free DOFs (dynamic_explicit.py:147-166)."""
         s = self._s()
@@ -352,8 +369,9 @@ class DevicePlan:
         return self.dof_axpy(zero, 1.0, GC, U)
 
     def u_to_gc(self, U):
-        idx = torch.as_tensor(np.nonzero(self.free_mask)[0], device=self.device)
-        return U.reshape(-1)[idx].clone()
+        if getattr(self, '_free_idx', None) is None:
+            self._free_idx = torch.as_tensor(np.nonzero(self.free_mask)[0], device=self.device)
+        return U.reshape(-1)[self._free_idx].clone()
 
     # ------------------------------------------------------------------ linear
     def spmv(self, val, x, y, mat_dtype, vec_dtype=None):

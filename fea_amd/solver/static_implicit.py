@@ -190,11 +190,11 @@ class StaticImplicitSolver(BaseSolver):
             # cannot resolve it, so take the full Newton step (only near convergence)
             p.dof_axpy(U, sign, dGC, Unew)
             self.stats['energy_evals'] = self.stats.get('energy_evals', 0) + 1
-            return 1.0, p.energy(Unew, scale)
+            return 1.0, self._energy(Unew, scale)
         slack = self.armijo_slack * abs(energy0)
         while True:
             p.dof_axpy(U, sign * alpha, dGC, Unew)
-            e_new = p.energy(Unew, scale)
+            e_new = self._energy(Unew, scale)
             self.stats['energy_evals'] = self.stats.get('energy_evals', 0) + 1
             if (not math.isfinite(e_new)) or e_new > energy0 + c1 * deltaE * alpha + slack:
                 alpha *= 0.5
@@ -203,9 +203,17 @@ class StaticImplicitSolver(BaseSolver):
             else:
                 return alpha, e_new
 
+    def _energy(self, U, scale):
+        """The functional the Newton iteration minimises (total potential energy)."""
+        return self.assembly.plan.energy(U, scale)
+
+    def _converged_abs(self, dmax, rmax, tol_error):
+        """Absolute convergence rule without ``rel_tol`` (static_implicit.py:290)."""
+        return (dmax < tol_error and rmax < tol_error) or rmax < 1e-6
+
     def _newton(self, U, scale, tol_error):
         p = self.assembly.plan
-        energy = p.energy(U, scale)
+        energy = self._energy(U, scale)
         dGC = torch.zeros(p.n_free, dtype=torch.float64, device=p.device)
         Unew = torch.empty_like(U)
         low_alpha, alpha, it = 0, 0.0, 0
@@ -264,7 +272,7 @@ class StaticImplicitSolver(BaseSolver):
                 print('iter %3d  alpha %.3g  energy %.10e  |R|inf %.3e' % (self._iter_now, alpha, energy, rmax))
             if self.rel_tol is None:
                 _, dmax = self._vec_stats(dGC)
-                if (dmax < tol_error and rmax < tol_error) or rmax < 1e-6:
+                if self._converged_abs(dmax, rmax, tol_error):
                     return True
 
     def _solve_iteration(self, GC, tol_error):

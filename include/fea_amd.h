@@ -134,6 +134,14 @@ int fea_elem_lumped_mass(int elem_type, int n_elem, const int32_t *conn, const d
                          const int32_t *dof_map, double rho, const int32_t *dst_r,
                          double *rstage, void *stream);
 
+/* Consistent mass blocks (Element_3D.get_mass_matrix, C3base.py:348-384, assembled
+ * by Assembly.assemble_mass_matrix, assembly.py:421-443): M_ab = rho sum_g N_a N_b
+ * w detJ on the diagonal of each 3x3 block, float64, written to kstage through
+ * the tangent's dst_k map (fea_gather_csr then forms the CSR in K's pattern;
+ * surface groups' slots must hold zeros). */
+int fea_elem_mass(int elem_type, int n_elem, const int32_t *conn, const double *X, double rho,
+                  const int32_t *dst_k, double *kstage, void *stream);
+
 /* ---------------------------------------------------------------- surfaces --
  * Follower pressure on T3/T6/Q4 facets (Pressure.get_stiffness,
  * pressure.py:80-132, with the assembly's load sign, assembly.py:337-341):

@@ -58,3 +58,58 @@ def test_central_difference_oscillator():
     x, v, a = osol.central_difference(lambda g: w * w * g, np.ones(1), np.ones(1), np.zeros(1), dt, n)
     theta = np.arccos(1 - (w * dt) ** 2 / 2)
     assert abs(x[0] - np.cos(n * theta)) < 1e-12
+
+
+def test_newmark_linear_oscillator_closed_form():
+    """Newmark beta = 1/4, gamma = 1/2 (the trapezoidal rule) on m x'' + k x = 0:
+    amplitude kept exactly and the discrete frequency 2/dt atan(w dt / 2)."""
+    import scipy.sparse as sp
+    m, k, dt, n = 2.0, 3.0, 0.3, 40
+    out = osol.newmark_steps(lambda x: (k * x, sp.csr_matrix([[k]])), lambda x: 0.5 * k * float(x @ x),
+                             sp.csr_matrix([[m]]), np.array([1.0]), np.array([0.0]), dt, n, tol_error=1e-14)
+    w = np.sqrt(k / m)
+    wd = 2.0 / dt * np.arctan(w * dt / 2.0)
+    x = np.array([o[0][0] for o in out])
+    assert np.abs(x - np.cos(wd * dt * np.arange(n + 1))).max() < 1e-12
+    # the trapezoidal rule keeps the quadratic invariant m v^2 + k x^2 exactly
+    v = np.array([o[1][0] for o in out])
+    energy = 0.5 * m * v ** 2 + 0.5 * k * x ** 2
+    assert np.abs(energy - energy[0]).max() < 1e-12 * energy[0]
+
+
+def test_consistent_mass_closed_form():
+    """Consistent mass of the reference's Gauss rules: C3D4 with its single point
+    (tetrahedral.py:30-40) gives rho V / 16 for every node pair (rank one; rows sum
+    to rho V / 4); C3D8's 2x2x2 rule integrates N_a N_b exactly, so on undistorted
+    cells of size h the entry is rho h^3 prod over axes of 1/3 (same side) or 1/6."""
+    from fea_amd import meshgen
+    from tests.helpers import nh, oracle_model
+    mesh = meshgen.kuhn_tet_block(2, 2, 2, jitter=0.1, seed=4)
+    om = oracle_model(mesh, nh(1.0, 10.0), 'z0', rho=2.5)
+    om['fixed'][:] = False
+    M = osol.consistent_mass(om).toarray()
+    X, conn = mesh['nodes'], mesh['conn']
+    P = X[conn]
+    V = np.abs(np.einsum('ei,ei->e', P[:, 1] - P[:, 0], np.cross(P[:, 2] - P[:, 0], P[:, 3] - P[:, 0]))) / 6.0
+    ref = np.zeros_like(M)
+    for e in range(conn.shape[0]):
+        for a in range(4):
+            for b in range(4):
+                for d in range(3):
+                    ref[3 * conn[e, a] + d, 3 * conn[e, b] + d] += 2.5 * V[e] / 16.0
+    assert np.abs(M - ref).max() < 1e-14 * np.abs(ref).max()
+    assert abs(M.sum() - 3 * 2.5 * V.sum()) < 1e-12
+    # C3D8, one undistorted cell of size h
+    h = 0.5
+    hm = meshgen.hex_block(1, 1, 1, size=(h, h, h))
+    om8 = oracle_model(hm, nh(1.0, 10.0), 'z0', rho=2.0)
+    om8['fixed'][:] = False
+    M8 = osol.consistent_mass(om8).toarray()
+    Xn = hm['nodes']
+    ref8 = np.zeros_like(M8)
+    for a in range(8):
+        for b in range(8):
+            w = np.prod([1.0 / 3.0 if abs(Xn[a, i] - Xn[b, i]) < 1e-12 else 1.0 / 6.0 for i in range(3)])
+            for d in range(3):
+                ref8[3 * a + d, 3 * b + d] = 2.0 * h ** 3 * w
+    assert np.abs(M8 - ref8).max() < 1e-14 * np.abs(ref8).max()
