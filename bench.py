@@ -96,9 +96,19 @@ def spmv_bytes(plan, val_bytes):
     return plan.nnz * val_bytes + shared_cols * 4 + plan.n_free * 16 + (plan.n_groups + 1) * 8
 
 
-def roofline(kernel, achieved, peak, unit, bound, **extra):
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (one counter
+# per run) on `bench.py --workload kernels`, each scaled by the same counters' reading for a
+# 256 MiB device copy (read bytes = 2 x FETCH_SIZE KiB x 1024, written = 1 x WRITE_SIZE KiB x 1024);
+# the summary and the commands are in profiles/traffic_beam128x32x32_kernels.md.  A kernel
+# whose workload or template differs from the measured one reports null.
+MEASURED_TRAFFIC = {
+    ('beam 128x32x32 C3D8', 'k_pcg_spmv<float,double,16,2>'): 201.4e6 + 3.72e6,
+}
+
+
+def roofline(kernel, achieved, peak, unit, bound, traffic_key=None, **extra):
     d = {'bound': bound, 'kernel': kernel, 'achieved': achieved, 'peak': peak, 'unit': unit,
-         'frac': achieved / peak, 'traffic': None}
+         'frac': achieved / peak, 'traffic': MEASURED_TRAFFIC.get(traffic_key)}
     d.update(extra)
     return d
 
@@ -255,6 +265,7 @@ def bench_beam(args, torch):
         del fe64
 
     mat = F64 if args.precision == 'fp64' else F32
+    kname_t = 'float' if mat == F32 else 'double'
     t_spmv, alg_bytes = spmv_roofline(fe, torch, args.spmv_reps, mat)
     achieved = alg_bytes / t_spmv / 1e9
     elem_rate, t_tan = element_rate(fe, torch)
@@ -274,10 +285,11 @@ def bench_beam(args, torch):
         'energy_evals_per_solve': stats.get('energy_evals'),
         'elem_tangent_evals_per_s': elem_rate, 'tangent_assembly_ms': 1e3 * t_tan,
         'elem_tangent_evals_per_s_f32_contraction': elem_rate32,
-        'roofline': roofline('k_pcg_spmv<%s,double,%d,2> (fea_pcg_spmv)' % ('float' if mat == F32 else 'double',
-                                                                            _triple_sw(p)),
+        'roofline': roofline('k_pcg_spmv<%s,double,%d,2> (fea_pcg_spmv)' % (kname_t, _triple_sw(p)),
                              achieved, HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=alg_bytes,
-                             avg_launch_us=t_spmv * 1e6),
+                             avg_launch_us=t_spmv * 1e6,
+                             traffic_key=('beam %dx%dx%d C3D8' % (args.nx, args.ny, args.nz),
+                                          'k_pcg_spmv<%s,double,%d,2>' % (kname_t, _triple_sw(p)))),
         'cpu_baseline': cpu,
         'serving_sample': serving,
     })

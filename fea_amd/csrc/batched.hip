@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_update(int n_sys, const fea_pc
   const int it = (int)ws[W_ITER];
   const double rz_old = sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, S.vec_blocks);
   const double pq = sum_partials(ws + W_PQ, S.spmv_blocks);
-  const TV alpha = (TV)(rz_old / pq);
+  const TV alpha = (TV)(pq != 0.0 ? rz_old / pq : 0.0);  // b = A x exactly: no step
   const TV *dinv = (const TV *)S.dinv;
   const TV *p = (const TV *)S.p;
   const TV *q = (const TV *)S.q;
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_dir(int n_sys, const fea_pcg_s
   const double rz_new = sum_partials(ws + W_RZP + (it & 1) * PCG_G, S.vec_blocks);
   const double rz_old = sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, S.vec_blocks);
   const double bz = sum_partials(ws + W_BZP, S.vec_blocks);
-  const bool done = (rz_new / bz < S.tol) || (it >= S.max_iter) || !(rz_new == rz_new);
+  const bool done = (rz_new == 0.0) || (rz_new / bz < S.tol) || (it >= S.max_iter) || !(rz_new == rz_new);
   // every workgroup of the system reads W_DONE at its start: only a later launch may see it set
   if (lb == 0 && threadIdx.x == 0) {
     ws[W_RZ] = rz_new;

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const TV *__restri
   const int g1 = (int)ws[W_G1];
   const double rz_old = sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, gridDim.x);
   const double pq = sum_partials(ws + W_PQ, g1);
-  const TV alpha = (TV)(rz_old / pq);
+  const TV alpha = (TV)(pq != 0.0 ? rz_old / pq : 0.0);  // b = A x exactly: no step
   double rz = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -118,7 +118,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const TV *__restri
   write_partial(ws + W_RZP + (it & 1) * PCG_G, rz);
 }
 
-// convergence (rz_i / bz < tol, _linear_solver.py:37) and p = z + beta p
+// convergence (rz_i / bz < tol, _linear_solver.py:37; r.z = 0 is an exact solution, which
+// also ends a solve with b = 0 where the ratio is 0/0) and p = z + beta p
 template <typename TV>
 __global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, const TV *__restrict__ z, TV *__restrict__ p,
                                                     double *__restrict__ ws, double tol, int max_iter) {
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, const TV *__restrict_
   const double rz_new = sum_partials(ws + W_RZP + (it & 1) * PCG_G, gridDim.x);
   const double rz_old = sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, gridDim.x);
   const double bz = sum_partials(ws + W_BZP, gridDim.x);
-  const bool done = (rz_new / bz < tol) || (it >= max_iter) || !(rz_new == rz_new);
+  const bool done = (rz_new == 0.0) || (rz_new / bz < tol) || (it >= max_iter) || !(rz_new == rz_new);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ws[W_RZ] = rz_new;
     ws[W_BZ] = bz;
