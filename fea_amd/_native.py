@@ -69,6 +69,9 @@ _SIGS = {
     'fea_surface_pressure_hvp': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_DBL, C_P, C_P, C_P, C_P]),
     'fea_gather_nodal': (C_INT, [C_INT, C_P, C_P, C_P, C_P]),
     'fea_pcg_system_size': (C_INT, []),
+    'fea_couple_map_size': (C_INT, []),
+    'fea_couple_kinematics': (C_INT, [C_P, C_P, C_INT, C_P]),
+    'fea_couple_condense': (C_INT, [C_P, C_INT, C_P, C_P, C_DBL, C_P, C_P, C_P]),
     'fea_pcg_batch_layout': (C_INT, [C_INT, C_P, C_P, C_P]),
     'fea_pcg_batch_start': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_INT, C_INT, C_P]),
     'fea_pcg_batch_spmv': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_INT, C_P]),

@@ -5,7 +5,9 @@
 registries, RGC blocks, required-DOF masks, assembly.py:76-185) and then builds
 one ``DevicePlan`` for the whole model: every element group of every instance
 and every surface load becomes a kernel group writing into one precomputed
-free-DOF CSR.  ``assemble_Stiffness_Matrix`` / ``assemble_force`` /
+free-DOF CSR.  A model with reference points and Couple constraints gets a
+``CoupledPlan`` (fea_amd/couple.py) with the same interface over the state
+[nodal displacements | 6 per reference point].  ``assemble_Stiffness_Matrix`` / ``assemble_force`` /
 ``_total_Potential_Energy`` run entirely in HIP kernels.
 """
 from __future__ import annotations
@@ -15,8 +17,9 @@ import torch
 
 from ..device import DevicePlan, Group
 from .._native import F64
-from . import boundarys, loads
+from . import boundarys, constraints, loads
 from .part import Instance, Part
+from .reference_points import ReferencePoint
 
 
 class Assembly:
@@ -40,7 +43,7 @@ class Assembly:
     def _allocate_RGC(self, size):
         """assembly.py:448-468."""
         index_now = len(self.RGC)
-        size = tuple(int(s) for s in size)
+        size = (int(size),) if np.ndim(size) == 0 else tuple(int(s) for s in size)
         self.RGC.append(torch.zeros(size, dtype=torch.float64, device=self.device))
         self.RGC_remain_index.append(np.zeros(size, dtype=bool))
         self._RGC_size.append(size)
@@ -55,20 +58,21 @@ class Assembly:
         self._constraints = dict(sorted(self._constraints.items()))
         self._reference_points = dict(sorted(self._reference_points.items()))
         self._boundarys = dict(sorted(self._boundarys.items()))
-        if self._reference_points or self._constraints:
-            raise NotImplementedError('reference points and constraints (Couple) are outside the '
-                                      'accelerated hot path')
-
         self.RGC, self.RGC_remain_index, self._RGC_size = [], [], []
         self.RGC_list_indexStart = [0]
         RGC_index = None
         for ins in self._instances.values():
             RGC_index = self._allocate_RGC(ins._RGC_requirements)
             ins.set_RGC_index(RGC_index)
+        for rp in self._reference_points.values():
+            RGC_index = self._allocate_RGC(rp._RGC_requirements)
+            rp.set_RGC_index(RGC_index)
         if RGC_index is not None:
             self.RGC[RGC_index][-1] = 1e-5            # initial guess quirk, assembly.py:113
         for f in self._loads.values():
             f.set_RGC_index(self._allocate_RGC(f._RGC_requirements))
+        for c in self._constraints.values():
+            c.set_RGC_index(self._allocate_RGC(c._RGC_requirements))
         for b in self._boundarys.values():
             b.set_RGC_index(self._allocate_RGC(b._RGC_requirements))
 
@@ -78,6 +82,8 @@ class Assembly:
             ins.initialize(self)
         for l in self._loads.values():
             l.initialize(self)
+        for c in self._constraints.values():
+            c.initialize(self)
         for b in self._boundarys.values():
             b.initialize(self)
 
@@ -85,6 +91,8 @@ class Assembly:
             self.RGC_remain_index = ins.set_required_DoFs(self.RGC_remain_index)
         for f in self._loads.values():
             self.RGC_remain_index = f.set_required_DoFs(self.RGC_remain_index)
+        for c in self._constraints.values():
+            self.RGC_remain_index = c.set_required_DoFs(self.RGC_remain_index)
         for b in self._boundarys.values():
             self.RGC_remain_index = b.set_required_DoFs(self.RGC_remain_index)
         self.RGC_remain_index_flatten = torch.tensor(
@@ -95,7 +103,7 @@ class Assembly:
         self._GC_list_indexStart = np.cumsum([0] + [int(m.sum()) for m in self.RGC_remain_index]).tolist()
 
     def _build_plan(self):
-        X, masks, groups = [], [], []
+        X, masks, groups, offsets = [], [], [], {}
         node_off = 0
         for name, ins in self._instances.items():
             if self.RGC_list_indexStart[ins._RGC_index] != 3 * node_off:
@@ -103,6 +111,7 @@ class Assembly:
             Xi = ins.nodes.detach().cpu().numpy().astype(np.float64)
             X.append(Xi)
             masks.append(self.RGC_remain_index[ins._RGC_index].reshape(-1))
+            offsets[name] = node_off
             for ename, e in ins.part.elems.items():
                 groups.append(Group('elem', e.etype, e.conn_np() + node_off, params=e.kernel_params(),
                                     rho=float(torch.as_tensor(e.density).reshape(-1)[0]),
@@ -113,7 +122,31 @@ class Assembly:
             groups += f.device_groups(self)
         X = np.concatenate(X, axis=0) if X else np.zeros((0, 3))
         free = np.concatenate(masks) if masks else np.zeros(0, dtype=bool)
-        self.plan = DevicePlan(X, free, groups, device=self.device)
+        self._coupled = bool(self._reference_points or self._constraints)
+        if self._coupled:
+            from ..couple import CoupledPlan
+            rp_names = list(self._reference_points.keys())
+            if not rp_names:
+                raise ValueError('a Couple needs a reference point')
+            if 3 * node_off != self.RGC_list_indexStart[self._reference_points[rp_names[0]]._RGC_index]:
+                raise RuntimeError('reference point RGC blocks must follow the instances')
+            rps = []
+            for rn in rp_names:
+                rp = self._reference_points[rn]
+                load = np.zeros(6)
+                for l in self._loads.values():
+                    if isinstance(l, (loads.Concentrate_Force, loads.Moment)) and l.rp_name == rn:
+                        load += l.generalized_load()
+                rps.append({'node': np.asarray(rp.node.detach().cpu().numpy(), dtype=np.float64),
+                            'free': self.RGC_remain_index[rp._RGC_index].copy(), 'load': load})
+            couples = []
+            for c in self._constraints.values():
+                if not isinstance(c, constraints.Couple):
+                    raise NotImplementedError('constraint %s' % type(c).__name__)
+                couples.append((c._indexNodes + offsets[c.instance_name], rp_names.index(c.rp_name)))
+            self.plan = CoupledPlan(X, free, groups, couples, rps, device=self.device)
+        else:
+            self.plan = DevicePlan(X, free, groups, device=self.device)
         self._K_indices = None
 
     def initialize_dynamic(self):
@@ -128,24 +161,34 @@ class Assembly:
 
     # ------------------------------------------------------------------ GC <-> RGC
     def _u_from(self, RGC=None, GC=None):
-        """Nodal float64 field [3N] of the instances from RGC or GC."""
+        """Solver state from RGC or GC: the nodal float64 field [3N] of the instances, followed
+        by the 6 coordinates of every reference point when the model has them."""
         if RGC is None:
             if GC is None:
                 raise ValueError("Either RGC or GC must be provided.")
             return self.plan.gc_to_u(torch.as_tensor(GC, dtype=torch.float64, device=self.device).contiguous())
         blocks = [torch.as_tensor(RGC[ins._RGC_index]).to(self.device, torch.float64).reshape(-1)
                   for ins in self._instances.values()]
-        return torch.cat(blocks).contiguous()
+        blocks += [torch.as_tensor(RGC[rp._RGC_index]).to(self.device, torch.float64).reshape(-1)
+                   for rp in self._reference_points.values()]
+        U = torch.cat(blocks).contiguous()
+        if self._coupled:
+            self.plan.sync(U)                         # Couple.modify_RGC (couple.py:36-43)
+        return U
 
     def _GC2RGC(self, GC):
-        """assembly.py:470-492: scatter GC into RGC blocks (constrained entries zero)."""
-        U = self._u_from(GC=GC).view(-1, 3)
+        """assembly.py:470-492: scatter GC into RGC blocks (constrained entries zero, coupled
+        nodes from their reference point)."""
+        U = self._u_from(GC=GC)
         RGC = []
         for i, size in enumerate(self._RGC_size):
             RGC.append(torch.zeros(size, dtype=torch.float64, device=self.device))
+        Un = U[:3 * self._n_nodes].view(-1, 3)
         for ins in self._instances.values():
             s = self.RGC_list_indexStart[ins._RGC_index] // 3
-            RGC[ins._RGC_index] = U[s:s + ins._RGC_requirements[0]].clone()
+            RGC[ins._RGC_index] = Un[s:s + ins._RGC_requirements[0]].clone()
+        for k, rp in enumerate(self._reference_points.values()):
+            RGC[rp._RGC_index] = U[3 * self._n_nodes + 6 * k:3 * self._n_nodes + 6 * k + 6].clone()
         for b in self._boundarys.values():
             RGC = b.modify_RGC(RGC)
         return RGC
@@ -236,8 +279,25 @@ class Assembly:
             raise ValueError(f"Instance with name {name} does not exist in the assembly.")
         del self._instances[name]
 
-    def add_reference_point(self, rp, name: str = None):
-        raise NotImplementedError('reference points are outside the accelerated hot path')
+    def add_reference_point(self, rp: ReferencePoint, name: str = None):
+        """assembly.py:557-576: default names rp-0, rp-1, ..."""
+        if name is None:
+            number = len(self._reference_points)
+            while ('rp-%d' % number) in self._reference_points:
+                number += 1
+            name = 'rp-%d' % number
+        self._reference_points[name] = rp
+        return name
+
+    def get_reference_point(self, name: str) -> ReferencePoint:
+        if name in self._reference_points:
+            return self._reference_points[name]
+        raise ValueError(f"Reference point '{name}' not found in the model.")
+
+    def delete_reference_point(self, name: str):
+        if name not in self._reference_points:
+            raise ValueError(f"Reference point '{name}' not found in the model.")
+        del self._reference_points[name]
 
     def add_load(self, load: loads.BaseLoad, name: str = None):
         if name is None:
@@ -262,8 +322,21 @@ class Assembly:
     def delete_all_loads(self):
         self._loads.clear()
 
-    def add_constraint(self, constraint, name: str = None):
-        raise NotImplementedError('constraints (Couple) are outside the accelerated hot path')
+    def add_constraint(self, constraint: constraints.BaseConstraint, name: str = None):
+        if name is None:
+            name = self._auto_name(self._constraints, constraint.__class__.__name__)
+        self._constraints[name] = constraint
+        return name
+
+    def get_constraint(self, name: str):
+        if name in self._constraints:
+            return self._constraints[name]
+        raise ValueError(f"Constraint '{name}' not found in the model.")
+
+    def delete_constraint(self, name: str):
+        if name not in self._constraints:
+            raise ValueError(f"Constraint '{name}' not found in the model.")
+        del self._constraints[name]
 
     def add_boundary(self, boundary: boundarys.BaseBoundary, name: str = None):
         if name is None:

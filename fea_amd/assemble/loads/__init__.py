@@ -2,3 +2,4 @@ from .base import BaseLoad
 from .pressure import Pressure
 from .traction import Traction
 from .contact import NodeToSurfaceContact
+from .point_loads import Concentrate_Force, Moment

@@ -31,6 +31,38 @@ def _i32(a, device):
     return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
 
 
+def row_group_tables(rowptr, col, sizes):
+    """Group table grp (group g = rows grp[g] .. grp[g+1]-1, ``sizes`` rows each, at most 3)
+    of a CSR whose rows in a group share one column list, and whether the pattern is
+    triple-structured (every group 3 rows, its columns whole aligned triples: the
+    triple-blocked SpMV).  Returns (None, False) when the rows of a group differ in length."""
+    sizes = np.asarray(sizes, dtype=np.int64)
+    n = rowptr.shape[0] - 1
+    if sizes.sum() != n or np.any(sizes < 1) or np.any(sizes > 3):
+        return None, False
+    grp = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    rl = np.diff(rowptr)
+    for j in (1, 2):
+        sel = sizes > j
+        if not np.array_equal(rl[grp[:-1][sel] + j], rl[grp[:-1][sel]]):
+            return None, False
+    heads = grp[:-1]
+    nnz = int(rowptr[-1])
+    triple = bool(np.all(sizes == 3)) and nnz > 0
+    if triple:
+        triple = bool(np.all(rl[heads] % 3 == 0))
+    if triple:
+        hrow = np.zeros(n, dtype=bool)
+        hrow[heads] = True
+        rows = np.repeat(np.arange(n), rl)
+        pos = np.arange(nnz) - rowptr[rows]
+        first = hrow[rows] & (pos % 3 == 0)
+        c0 = col[:nnz][first]
+        idx = np.nonzero(first)[0]
+        triple = bool(np.all(col[idx + 1] == c0 + 1) and np.all(col[idx + 2] == c0 + 2))
+    return grp, triple
+
+
 class Group:
     """One contiguous batch of elements or facets sharing a kernel configuration."""
 
@@ -119,29 +151,11 @@ class DevicePlan:
             self.rowptr_np, self.col_np = rowptr, col[:self.nnz]
             # node row groups: the free rows of one node are consecutive and share columns
             nf = self.free_mask.reshape(-1, 3).sum(1)
-            grp = np.concatenate([[0], np.cumsum(nf[nf > 0])]).astype(np.int32)
-            rl = np.diff(rowptr)
-            for j in (1, 2):
-                sel = nf[nf > 0] > j
-                if not np.array_equal(rl[grp[:-1][sel] + j], rl[grp[:-1][sel]]):
-                    raise RuntimeError('CSR rows of a node differ in length')
+            grp, triple = row_group_tables(rowptr, col[:self.nnz], nf[nf > 0])
+            if grp is None:
+                raise RuntimeError('CSR rows of a node differ in length')
             self.n_groups = int(grp.shape[0] - 1)
             self.grp = _i32(grp, dev)
-            # triple structure (selects the triple-blocked SpMV): every group has 3 rows and
-            # its column list is whole neighbour triples
-            heads = grp[:-1]
-            triple = bool(np.all(np.diff(grp) == 3)) and self.nnz > 0
-            if triple:
-                triple = bool(np.all(rl[heads] % 3 == 0))
-            if triple:
-                hrow = np.zeros(self.n_free, dtype=bool)
-                hrow[heads] = True
-                rows = np.repeat(np.arange(self.n_free), rl)
-                pos = np.arange(self.nnz) - rowptr[rows]
-                first = hrow[rows] & (pos % 3 == 0)
-                c0 = col[:self.nnz][first]
-                idx = np.nonzero(first)[0]
-                triple = bool(np.all(col[idx + 1] == c0 + 1) and np.all(col[idx + 2] == c0 + 2))
             self.triple = triple
             self.spmv_groups = -self.n_groups if triple else self.n_groups
             self.rowptr, self.col = _i32(rowptr, dev), _i32(col, dev)

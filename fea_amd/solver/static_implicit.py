@@ -296,6 +296,8 @@ class StaticImplicitSolver(BaseSolver):
                 call('fea_axpy', U.numel(), -1.0, F64, U_prev, step, s)
                 U_prev.copy_(U)
                 call('fea_axpy', U.numel(), 1.0, F64, step, U, s)
+                if hasattr(p, 'sync'):
+                    p.sync(U)                      # coupled nodes back onto their reference points
             self._increment = k
             ok = self._newton(U, k / self.load_steps, tol_error)
             self.stats['increments'] = k

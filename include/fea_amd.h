@@ -378,6 +378,68 @@ int fea_surface_pressure_hvp(int surf_type, int n_facets, const int32_t *conn, c
 /* out[3A + j] = sum of stage rows seg[A] .. seg[A+1]-1, every node (no DOF map). */
 int fea_gather_nodal(int n_nodes, const int32_t *seg, const double *stage, double *out, void *stream);
 
+/* ---------------------------------------------------------------- couple --
+ * Kinematic coupling of node sets to reference points (constraints/couple.py:7-292,
+ * reference_points.py:5-24).  A reference point owns 6 generalized coordinates
+ * (v, z): translation and rotation vector; a coupled node b follows
+ * u_b = v + R(z) y_b - y_b, y_b = X_b - X_rp (Couple.modify_RGC, couple.py:36-43).
+ *
+ * The state vector is [3 n_state_nodes nodal displacements | 6 per reference point].
+ * The element kernels assemble an *extended* system in which the coupled node DOFs
+ * are kept as rows and columns (CSR "ext"); fea_couple_condense forms the reduced
+ * system of the generalized coordinates (free node DOFs, then free reference-point
+ * DOFs, the reference's GC order) by the chain rule:
+ *   R_red = T^T R_ext - load_scale * (F, M),  K_red = T^T K_ext T + sum_b R_b d2u_b/dz2,
+ * T = du/dGC (Couple.modify_R_K, couple.py:66-184, and the reduction of
+ * _assemble_reduced_Matrix, assembly.py:357-387).  The (reference point, row)
+ * entries are written as the transposes of the (row, reference point) entries,
+ * as the reference does (K_ext symmetric).  Every output entry has one writer and
+ * every sum a fixed order: results are deterministic.
+ *
+ * All arrays are device arrays built once per model by the host (fea_amd.couple). */
+typedef struct fea_couple_map {
+  int32_t n_couples;      /* couples, one reference point each */
+  int32_t n_cnodes;       /* coupled nodes, couples contiguous: couple k owns [cnode_ptr[k], cnode_ptr[k+1]) */
+  int32_t n_pairs;        /* (reduced node row, couple) pairs whose ext row has a coupled column */
+  int32_t n_red;          /* reduced rows */
+  int32_t n_red_nodes;    /* reduced rows that are node DOFs (the first n_red_nodes) */
+  int32_t n_state_nodes;  /* nodes of the nodal part of the state vector */
+  int64_t nnz_red;        /* entries of the reduced CSR */
+  const int32_t *cnode_ptr; /* [n_couples + 1] */
+  const int32_t *rp_state;  /* [n_couples] offset of the couple's (v, z) in the state vector */
+  const int32_t *cnode;     /* [n_cnodes] node id */
+  const double *y;          /* [n_cnodes * 3] X_b - X_rp */
+  const int32_t *crow_ext;  /* [3 n_cnodes] ext row of coupled DOF 3b + c */
+  const int32_t *cr_ptr;    /* [3 n_cnodes + 1] coupled-column entries of each coupled row */
+  const int32_t *ce_slot;   /* ext CSR slot of each such entry */
+  const int32_t *ce_dof;    /* its column as a coupled DOF index 3b' + c' */
+  const int32_t *pair_ptr;  /* [n_pairs + 1] coupled-column entries of each pair */
+  const int32_t *pe_slot, *pe_dof;
+  const int32_t *pair_row_slot; /* [n_pairs * 6] reduced slot (row, point DOF d), -1 if d is fixed */
+  const int32_t *pair_col_slot; /* [n_pairs * 6] reduced slot (point DOF d, row) */
+  const int32_t *rp_slot;   /* [n_couples * 36] reduced slot of (point DOF d, point DOF e) or -1 */
+  const int32_t *rp_row;    /* [n_couples * 6] reduced row of point DOF d or -1 */
+  const double *rp_load;    /* [n_couples * 6] force (3), moment (3) on the point */
+  const int32_t *ff_src;    /* [nnz_red] ext slot of a node-node entry, -1 for the others */
+  const int32_t *rf_src;    /* [n_red_nodes] ext row of a reduced node row */
+  double *ydot;             /* workspace [n_cnodes * 9]: d u_bc / d z_p at [(3b + c) * 3 + p] */
+  double *ydot2;            /* workspace [n_cnodes * 27]: d2 u_bc / d z_p d z_q */
+  double *partial;          /* workspace [3 n_cnodes * 42] */
+} fea_couple_map;
+int fea_couple_map_size(void);
+/* Coupled node displacements from the reference points' (v, z) in `state`
+ * (Couple.modify_RGC, couple.py:36-43 with _rotation3d, :272-292) and, with
+ * with_derivs, their first and second derivatives in z into m->ydot / m->ydot2
+ * (Couple._calculate_Ydotz, couple.py:186-270).  Rodrigues in the form
+ * R(z) y - y = A(s) z x y + B(s) z x (z x y), s = |z|^2, A = sin t / t,
+ * B = (1 - cos t) / t^2, with series of A, B and their s-derivatives below
+ * s = 2.5, so every quantity is exact to rounding at z = 0 too. */
+int fea_couple_kinematics(const fea_couple_map *m, double *state, int with_derivs, void *stream);
+/* Reduced residual (and tangent when K_ext != NULL; val_red in val_prec) from the
+ * extended ones.  Needs the derivatives of the last fea_couple_kinematics call. */
+int fea_couple_condense(const fea_couple_map *m, int val_prec, const double *K_ext, const double *R_ext,
+                        double load_scale, void *val_red, double *R_red, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

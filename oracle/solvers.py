@@ -50,18 +50,31 @@ def jacobi_scaled_solve(K, rhs, x0, linear='cg', tol=1e-5, max_iter=1500):
     return y / diag, its
 
 
+def _system(model):
+    """(energy(GC), residual_tangent(GC), residual(GC), n_gc) of a model: the plain free-DOF
+    system, or the Couple-reduced one when the model has reference points."""
+    if model.get('couples') or model.get('rps'):
+        from . import couple as cp
+        return (lambda GC: cp.total_energy(model, GC), lambda GC: cp.assemble_reduced(model, GC),
+                lambda GC: cp.assemble_force(model, GC), cp.n_gc(model))
+    return (lambda GC: asm.total_energy(model, asm.gc_to_u(model, GC)),
+            lambda GC: asm.assemble_reduced(model, asm.gc_to_u(model, GC)),
+            lambda GC: asm.assemble_force(model, asm.gc_to_u(model, GC)), int(asm.free_mask(model).sum()))
+
+
 def line_search(model, GC0, dGC, R, energy0, armijo_slack=0.0):
     """_line_search (static_implicit.py:98-177): Armijo c1=0.3, alpha halving to 1e-12.
 
     ``armijo_slack`` (absolute energy) is 0 in the reference.
     """
+    energy_fn = _system(model)[0]
     alpha, c1 = 1.0, 0.3
     deltaE = float(dGC @ R)
     if deltaE > 0:
         dGC, deltaE = -dGC, -deltaE
     while True:
         GCnew = GC0 + alpha * dGC
-        e_new = asm.total_energy(model, asm.gc_to_u(model, GCnew))
+        e_new = energy_fn(GCnew)
         if (not np.isfinite(e_new)) or e_new > energy0 + c1 * deltaE * alpha + armijo_slack:
             alpha *= 0.5
             if alpha < 1e-12:
@@ -71,13 +84,16 @@ def line_search(model, GC0, dGC, R, energy0, armijo_slack=0.0):
 
 
 def scale_loads(model, factor):
-    """Copy of the model with pressure and traction magnitudes scaled (load increments)."""
+    """Copy of the model with pressure, traction and reference-point loads scaled (load increments)."""
     m = dict(model)
     m['pressures'] = [dict(p, p=p['p'] * factor) for p in model.get('pressures', [])]
     m['tractions'] = []
     for t in model.get('tractions', []):
         asm.prepare(model)
         m['tractions'].append(dict(t, t=tuple(np.asarray(t['t']) * factor), f=t['f'] * factor))
+    if model.get('rps'):
+        m['rps'] = [dict(rp, **{k: (None if rp.get(k) is None else np.asarray(rp[k], dtype=np.float64) * factor)
+                                for k in ('force', 'moment')}) for rp in model['rps']]
     return m
 
 
@@ -91,13 +107,14 @@ def newton(model, GC, tol_error=1e-5, rel_tol=None, max_iter=200, linear='cg',
     full step with max|dGC| <= step_tol max|GC| (correction at round-off).
     Returns (GC or False, iterations).
     """
-    energy = asm.total_energy(model, asm.gc_to_u(model, GC))
+    energy_fn, rk_fn = _system(model)[:2]
+    energy = energy_fn(GC)
     dGC = np.zeros_like(GC)
     alpha, it, r0 = 0.0, 0, None
     while True:
         if it > max_iter:
             return False, it
-        R, K = asm.assemble_reduced(model, asm.gc_to_u(model, GC))
+        R, K = rk_fn(GC)
         it += 1
         nR = np.linalg.norm(R)
         if r0 is None:
@@ -115,7 +132,7 @@ def newton(model, GC, tol_error=1e-5, rel_tol=None, max_iter=200, linear='cg',
                 return False, it
             break
         GC = GCnew
-        energy = asm.total_energy(model, asm.gc_to_u(model, GC))
+        energy = energy_fn(GC)
         if rel_tol is not None and alpha == 1.0 and np.abs(dGC).max() <= step_tol * np.abs(GC).max():
             return GC, it
         if rel_tol is None and ((np.abs(dGC).max() < tol_error and np.abs(R).max() < tol_error)
@@ -124,13 +141,13 @@ def newton(model, GC, tol_error=1e-5, rel_tol=None, max_iter=200, linear='cg',
     return GC, it
 
 
-def static_solve(model, increments=1, predictor=True, **kw):
+def static_solve(model, increments=1, predictor=True, GC0=None, **kw):
     """Load-incremented static solve: Newton per increment, started from the linear
     extrapolation of the two previous converged states (``predictor``)."""
-    free = asm.free_mask(model)
-    GC = np.zeros(int(free.sum()))
+    _, _, force_fn, n = _system(model)
+    GC = np.zeros(n) if GC0 is None else np.asarray(GC0, dtype=np.float64).copy()
     total = 0
-    kw.setdefault('r_ref', float(np.linalg.norm(asm.assemble_force(model, np.zeros_like(model['nodes'])))))
+    kw.setdefault('r_ref', float(np.linalg.norm(force_fn(np.zeros(n)))))
     prev = GC.copy()
     for k in range(1, increments + 1):
         if predictor and k >= 2:
