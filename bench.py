@@ -45,7 +45,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--workload', default='beam',
-                    choices=['beam', 'cube', 'explicit', 'adjoint', 'serving', 'kernels'])
+                    choices=['beam', 'cube', 'explicit', 'adjoint', 'serving', 'kernels', 'couple'])
     ap.add_argument('--steps', type=int, default=None)
     ap.add_argument('--warmup', type=int, default=None)
     ap.add_argument('--nx', type=int, default=128)
@@ -64,6 +64,8 @@ def parse():
     ap.add_argument('--serving-qps', type=float, default=2.0)
     ap.add_argument('--serving-emax', type=float, default=None)
     ap.add_argument('--cells', type=int, default=40, help='explicit: cells per block edge')
+    ap.add_argument('--couple-pressure', type=float, default=0.06, help='couple: pressure on surface_1_All')
+    ap.add_argument('--couple-increments', type=int, default=4, help='couple: load increments')
     return ap.parse_args()
 
 
@@ -130,12 +132,13 @@ def _progress(k, st):
         file=sys.stderr, flush=True)
 
 
-def make_solver(precision, args, load_steps=None, progress=True):
+def make_solver(precision, args, load_steps=None, progress=True, predictor=True):
     from fea_amd import solver
     return solver.StaticImplicitSolver(maximum_iteration=args.max_newton, precision=precision,
                                        rel_tol=args.rel_tol, load_steps=load_steps or args.increments,
                                        cg_tol=args.cg_tol, cg_max_iter=20000, armijo_slack=1e-12,
-                                       progress=_progress if progress else None, forcing=args.forcing)
+                                       progress=_progress if progress else None, forcing=args.forcing,
+                                       predictor=predictor)
 
 
 def run_solve(fe, torch):
@@ -494,6 +497,89 @@ def bench_cube(args, torch):
     return line
 
 
+# ============================================================================ Couple model
+def bench_couple(args, torch):
+    """The reference's Couple benchmark model (tests/pressure_test/C3D4Less.inp: 33 612 C3D4
+    of a soft actuator, NeoHookean from its INP, follower pressure on surface_1_All, the base
+    fixed, the head coupled to a reference point at (0, 0, 80)), solved from rest."""
+    from fea_amd import workloads
+    from fea_amd._native import F32, F64
+    steps = 1 if args.steps is None else args.steps
+    warmup = 0 if args.warmup is None else args.warmup
+    path = os.path.join(ROOT, 'tests', 'golden', 'inp', 'C3D4Less.inp.gz')
+    fe, inp = workloads.inp_couple(path, pressure=args.couple_pressure)
+    # each increment starts from the previous equilibrium: a linear extrapolation of the
+    # actuator's large rotation starts far from the next one
+    fe.solver = make_solver(args.precision, args, load_steps=args.couple_increments, predictor=False)
+    fe.initialize()
+    p = fe.assembly.plan
+    for _ in range(warmup):
+        run_solve(fe, torch)
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        GC, t = run_solve(fe, torch)
+        lat.append(t)
+    elapsed = time.perf_counter() - t_start
+    stats = dict(fe.solver.stats)
+    rp = GC[-6:].cpu().numpy()
+    mat = F64 if args.precision == 'fp64' else F32
+    t_spmv, alg = spmv_roofline(fe, torch, 500, mat)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_couple(inp, GC.cpu().numpy(), args.couple_pressure, stats['newton_iters'])
+    line = base_line(args, 'static Newton solves/s of the reference Couple model', steps / elapsed, 'solves/s',
+                     steps, warmup, elapsed, 'f64' if args.precision == 'fp64' else 'f32+f64',
+                     {'workload': 'C3D4Less.inp (reference pressure test): %d C3D4, NeoHookean mu=%.4g kappa=%.4g, '
+                                  'pressure %g on surface_1_All, surface_0_Bottom fixed, surface_0_Head coupled to a '
+                                  'reference point at (0,0,80), %d increments, Newton rel tol %g'
+                                  % (inp.part['final_model'].n_elements(), *inp.material['Rubber'].mat_para,
+                                     args.couple_pressure, args.couple_increments, args.rel_tol),
+                      'precision': args.precision, 'free_dof': p.n_free, 'nnz': p.nnz})
+    line['data'] = 'the reference\'s INP mesh (tests/pressure_test/C3D4Less.inp, committed gzipped as a fixture)'
+    line.update({'latency_p50_s': float(np.percentile(lat, 50)),
+                 'newton_iters_per_solve': stats['newton_iters'], 'cg_iters_per_solve': stats['cg_iters'],
+                 'reference_point_v_z': rp.tolist(),
+                 'roofline': roofline('k_pcg_spmv (fea_pcg_spmv) on the reduced CSR', alg / t_spmv / 1e9,
+                                      HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=alg,
+                                      avg_launch_us=t_spmv * 1e6,
+                                      note='%.1f MB working set: launch-latency bound' % (alg / 1e6)),
+                 'cpu_baseline': cpu})
+    return line
+
+
+def cpu_baseline_couple(inp, GC, pressure, newton_iters):
+    """One FP64 oracle Newton iteration of the same model (Couple-reduced assembly + direct
+    solve of the Jacobi-scaled system), timed at the GPU's solution; solves/s extrapolated
+    with the GPU run's Newton count."""
+    from oracle import couple as cp
+    from oracle import solvers as osol
+    from tests.helpers import FACE_NODES, nh
+    part = inp.part['final_model']
+    X = part.nodes[:, 1:].numpy().astype(np.float64)
+    conn = part.elems['C3D4'][:, 1:].astype(np.int64)
+    fixed = np.zeros((X.shape[0], 3), dtype=bool)
+    fixed[sorted(part.sets_nodes['surface_0_Bottom'])] = True
+    pos = {eid: i for i, eid in enumerate(part.elems['C3D4'][:, 0])}
+    facets = np.concatenate([conn[[pos[e] for e in ids]][:, FACE_NODES['C3D4'][f]]
+                             for ids, f in part.surfaces['surface_1_All']])
+    om = {'nodes': X, 'fixed': fixed,
+          'groups': [{'etype': 'C3D4', 'conn': conn, 'material': nh(*inp.material['Rubber'].mat_para)}],
+          'pressures': [{'stype': 'T3', 'conn': facets, 'p': pressure}], 'tractions': [],
+          'rps': [{'node': (0.0, 0.0, 80.0)}],
+          'couples': [{'nodes': np.array(sorted(part.sets_nodes['surface_0_Head'])), 'rp': 0}]}
+    cp.assemble_reduced(om, GC)                              # warm-up (Gauss precompute)
+    t0 = time.perf_counter()
+    R, K = cp.assemble_reduced(om, GC)
+    osol.jacobi_scaled_solve(K, -R, np.zeros_like(R), linear='direct')
+    t_it = time.perf_counter() - t0
+    return {'value': 1.0 / (t_it * newton_iters), 'unit': 'solves/s', 'cores': len(os.sched_getaffinity(0)),
+            'kind': 'port', 'residual_at_gpu_solution_rel': float(np.linalg.norm(R) / np.linalg.norm(
+                cp.assemble_force(om, np.zeros_like(GC)))),
+            'sample': 'one FP64 oracle Newton iteration (reduced assembly + direct solve): %.2f s, x %d Newton '
+                      'iterations of the GPU solve (extrapolated)' % (t_it, newton_iters)}
+
+
 # ============================================================================ explicit (C3)
 def bench_explicit(args, torch):
     from fea_amd import workloads, solver
@@ -692,7 +778,7 @@ def main():
     import torch
     torch.set_num_threads(len(os.sched_getaffinity(0)))
     fn = {'beam': bench_beam, 'cube': bench_cube, 'explicit': bench_explicit, 'adjoint': bench_adjoint,
-          'serving': bench_serving, 'kernels': bench_kernels}[args.workload]
+          'serving': bench_serving, 'kernels': bench_kernels, 'couple': bench_couple}[args.workload]
     line = fn(args, torch)
     print(json.dumps(line), flush=True)
 

@@ -7,6 +7,9 @@
 * ``beam_traction``   4x1x1 cantilever of C3D8 bricks, Mooney-Rivlin
                       C10=0.5 C01=0.1 D1=0.01, clamped at x=0, uniform dead
                       traction on x=4 (the 128x32x32 mixed-precision workload).
+* ``inp_couple``      an INP model with the reference's pressure-test set-up:
+                      pressure on surface_1_All, surface_0_Bottom fixed,
+                      surface_0_Head coupled to a reference point.
 """
 import numpy as np
 import torch
@@ -111,3 +114,20 @@ def void_block_pressure(m=10, k=5, ns=5, *, mu=1.0, kappa=100.0, pressure=0.05, 
     mat = materials.NeoHookean(mu, kappa)
     return _controller_from_mesh(mesh, mat, fixed_set='x0',
                                  load=lambda nm: loads.Pressure(nm, 'void', pressure)), mesh
+
+
+def inp_couple(path, *, pressure=0.06, rp=(0.0, 0.0, 80.0), instance='final_model'):
+    """The reference's Couple benchmark set-up (tests/pressure_test/test_benchmark.py:24-41)
+    on the INP mesh at ``path``: follower pressure on surface_1_All, surface_0_Bottom fixed,
+    the surface_0_Head node set coupled to a reference point at ``rp``."""
+    from .assemble import ReferencePoint, constraints
+    from .inp import FEA_INP, from_inp
+    inp = FEA_INP().read_inp(path)
+    fe = from_inp(inp)
+    a = fe.assembly
+    a.add_load(loads.Pressure(instance_name=instance, surface_set='surface_1_All', pressure=pressure),
+               name='pressure-1')
+    a.add_boundary(boundarys.Boundary_Condition(instance_name=instance, set_nodes_name='surface_0_Bottom'))
+    name = a.add_reference_point(ReferencePoint(list(rp)))
+    a.add_constraint(constraints.Couple(instance_name=instance, set_nodes_name='surface_0_Head', rp_name=name))
+    return fe, inp

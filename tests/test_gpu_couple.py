@@ -134,15 +134,8 @@ def test_static_solve_matches_oracle(gpu, prec):
     assert np.linalg.norm(cp.assemble_force(om, GC.cpu().numpy())) < 1e-9
 
 
-def _c3d4less():
-    inp = FEA_INP().read_inp(os.path.join(GOLDEN, 'C3D4Less.inp.gz'))
-    fe = from_inp(inp)
-    a = fe.assembly
-    a.add_load(loads.Pressure(instance_name='final_model', surface_set='surface_1_All', pressure=0.06),
-               name='pressure-1')
-    a.add_boundary(boundarys.Boundary_Condition(instance_name='final_model', set_nodes_name='surface_0_Bottom'))
-    rp = a.add_reference_point(ReferencePoint([0, 0, 80]))
-    a.add_constraint(constraints.Couple(instance_name='final_model', set_nodes_name='surface_0_Head', rp_name=rp))
+def _c3d4less(pressure):
+    fe, inp = workloads.inp_couple(os.path.join(GOLDEN, 'C3D4Less.inp.gz'), pressure=pressure)
     part = inp.part['final_model']
     X = part.nodes[:, 1:].numpy().astype(np.float64)
     conn = part.elems['C3D4'][:, 1:].astype(np.int64)
@@ -153,7 +146,7 @@ def _c3d4less():
     facets = np.concatenate([conn[[pos[e] for e in ids]][:, FACE_NODES['C3D4'][f]]
                              for ids, f in part.surfaces['surface_1_All']])
     om = {'nodes': X, 'fixed': fixed, 'groups': [{'etype': 'C3D4', 'conn': conn, 'material': nh(mu, kappa)}],
-          'pressures': [{'stype': 'T3', 'conn': facets, 'p': 0.06}], 'tractions': [],
+          'pressures': [{'stype': 'T3', 'conn': facets, 'p': pressure}], 'tractions': [],
           'rps': [{'node': (0.0, 0.0, 80.0)}],
           'couples': [{'nodes': np.array(sorted(part.sets_nodes['surface_0_Head'])), 'rp': 0}]}
     return fe, om, np.array(sorted(part.sets_nodes['surface_0_Head']))
@@ -161,10 +154,11 @@ def _c3d4less():
 
 @pytest.mark.parametrize('prec', ['mixed'])
 def test_reference_couple_model_c3d4less(gpu, prec):
-    """tests/pressure_test/test_benchmark.py's model (C3D4Less.inp), solved on the GPU."""
-    fe, om, head = _c3d4less()
-    fe.solver = solver.StaticImplicitSolver(precision=prec, rel_tol=1e-9, load_steps=4, cg_tol=1e-16,
-                                            cg_max_iter=20000, armijo_slack=1e-12)
+    """tests/pressure_test/test_benchmark.py's model (C3D4Less.inp) at half its pressure (0.03;
+    the full 0.06 curls the actuator through 2.4 rad and is the benchmark's workload)."""
+    fe, om, head = _c3d4less(0.03)
+    fe.solver = solver.StaticImplicitSolver(precision=prec, rel_tol=1e-9, load_steps=2, cg_tol=1e-16,
+                                            cg_max_iter=20000, armijo_slack=1e-12, forcing='ew', predictor=False)
     fe.initialize()
     a = fe.assembly
     assert a.GC.numel() == cp.n_gc(om)
