@@ -45,7 +45,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--workload', default='beam',
-                    choices=['beam', 'cube', 'explicit', 'adjoint', 'serving', 'kernels', 'couple'])
+                    choices=['beam', 'cube', 'explicit', 'adjoint', 'serving', 'kernels', 'couple', 'contact'])
     ap.add_argument('--steps', type=int, default=None)
     ap.add_argument('--warmup', type=int, default=None)
     ap.add_argument('--nx', type=int, default=128)
@@ -580,6 +580,134 @@ def cpu_baseline_couple(inp, GC, pressure, newton_iters):
                       'iterations of the GPU solve (extrapolated)' % (t_it, newton_iters)}
 
 
+# ============================================================================ contact barrier
+# FP64 operations of one candidate pair's gradient + Hessian in fea_barrier_assemble: 324
+# Hessian entries x 9 Gauss pairs x 34 (six outer-product terms and the d2D, d2L terms),
+# plus the d2n and projection terms (~9 000)
+BARRIER_FLOP_PER_PAIR = 324 * 9 * 34 + 9000
+
+
+def bench_contact(args, torch):
+    """The reference's self-contact model (tests/gradient_test/C3D4.inp with test_grad.py's
+    loads: pressure, four ContactSelf surfaces, Couple), solved from rest; the line also
+    carries the distance to the reference's golden vector C3D4_results.npy."""
+    from fea_amd import workloads
+    from fea_amd._native import F32, F64
+    steps = 1 if args.steps is None else args.steps
+    warmup = 0 if args.warmup is None else args.warmup
+    path = os.path.join(ROOT, 'tests', 'golden', 'inp', 'gradient_C3D4.inp.gz')
+    surfaces = ['surface_%d_All' % i for i in range(4)]
+    fe, inp = workloads.inp_couple(path, pressure=0.06, rp=(0.0, 0.0, 70.0), contact_surfaces=surfaces)
+    fe.solver = make_solver(args.precision, args, load_steps=args.couple_increments, predictor=False)
+    fe.initialize()
+    p = fe.assembly.plan
+    for _ in range(warmup):
+        run_solve(fe, torch)
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        GC, t = run_solve(fe, torch)
+        lat.append(t)
+    elapsed = time.perf_counter() - t_start
+    stats = dict(fe.solver.stats)
+    gold = np.load(os.path.join(ROOT, 'tests', 'golden', 'gradient_C3D4_results.npy'), allow_pickle=False)
+    g = GC.cpu().numpy()
+    # roofline: the barrier gradient/Hessian kernel over the staged pairs at the solution
+    U = p.gc_to_u(GC)
+    Un = U[:3 * p.n_nodes]
+    inner = p.inner if hasattr(p, 'inner') else p._plan
+    groups = [gr for gr in inner.groups if gr.kind == 'barrier']
+    ks = inner.kstage(F64)
+    n_staged = sum(b.spi.size for b in p.barriers)
+    # staged pairs that are candidates at the solution: the ones the kernel evaluates
+    x = (inner.X.reshape(-1) + Un).reshape(-1, 3).cpu().numpy()
+    n_cand = 0
+    for b in p.barriers:
+        c1 = x[b.c1_np].sum(axis=1) / 3.0
+        c2 = x[b.c2_np].sum(axis=1) / 3.0
+        d2 = ((c1[b.spi] - c2[b.spj]) ** 2).sum(axis=1)
+        n_cand += int((d2 <= b.radius ** 2).sum())
+
+    def assemble():
+        for gr in groups:
+            gr.load.assemble(Un, gr.dst_k, ks, F64, gr.dst_r, inner.rstage)
+    t_asm = event_time(torch, assemble, 20)
+    flops = BARRIER_FLOP_PER_PAIR * n_cand
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_contact(fe, inp, g, stats['newton_iters'])
+    line = base_line(args, 'static Newton solves/s of the reference self-contact model', steps / elapsed,
+                     'solves/s', steps, warmup, elapsed, 'f64' if args.precision == 'fp64' else 'f32+f64',
+                     {'workload': 'gradient_test/C3D4.inp (reference): %d C3D4, pressure 0.06 on surface_1_All, '
+                                  'ContactSelf on surface_0..3_All, surface_0_Bottom fixed, surface_0_Head coupled to '
+                                  'a reference point at (0,0,70), %d increments, Newton rel tol %g'
+                                  % (inp.part['final_model'].n_elements(), args.couple_increments, args.rel_tol),
+                      'precision': args.precision, 'free_dof': p.n_free})
+    line['data'] = 'the reference\'s INP mesh (tests/gradient_test/C3D4.inp, committed gzipped as a fixture)'
+    line.update({'latency_p50_s': float(np.percentile(lat, 50)),
+                 'newton_iters_per_solve': stats['newton_iters'], 'cg_iters_per_solve': stats['cg_iters'],
+                 'pattern_rebuilds': p.rebuilds, 'staged_pairs': int(n_staged), 'candidate_pairs_at_solution': n_cand,
+                 'rel_diff_vs_reference_golden': float(np.abs(g - gold).max() / np.abs(gold).max()),
+                 'reference_point_v_z': g[-6:].tolist(),
+                 'roofline': roofline('k_bar_assemble_wg (fea_barrier_assemble), all staged pairs of the 4 surfaces',
+                                      flops / t_asm / 1e12, FP64_PEAK_TFLOPS, 'TFLOP/s', 'mfma',
+                                      alg_flop_per_launch=flops, avg_launch_us=t_asm * 1e6,
+                                      note='FP64 vector arithmetic (no FP64 MFMA form); %d of the %d staged '
+                                           'pairs are candidates at the solution' % (n_cand, n_staged)),
+                 'cpu_baseline': cpu})
+    return line
+
+
+def cpu_baseline_contact(fe, inp, GC, newton_iters):
+    """One FP64 oracle Newton iteration of the same model, bounded: the Couple-reduced
+    element + pressure assembly and a direct solve timed in full, the barrier terms timed on
+    a sample of 2 000 candidate pairs and scaled to all; solves/s extrapolated with the GPU
+    run's Newton count."""
+    from oracle import barrier as ob
+    from oracle import couple as cp
+    from oracle import solvers as osol
+    from tests.helpers import FACE_NODES, nh
+    part = inp.part['final_model']
+    X = part.nodes[:, 1:].numpy().astype(np.float64)
+    conn = part.elems['C3D4'][:, 1:].astype(np.int64)
+    fixed = np.zeros((X.shape[0], 3), dtype=bool)
+    fixed[sorted(part.sets_nodes['surface_0_Bottom'])] = True
+    pos = {eid: i for i, eid in enumerate(part.elems['C3D4'][:, 0])}
+    fac = lambda s: np.concatenate([conn[[pos[e] for e in ids]][:, FACE_NODES['C3D4'][f]]
+                                    for ids, f in part.surfaces[s]])
+    om = {'nodes': X, 'fixed': fixed,
+          'groups': [{'etype': 'C3D4', 'conn': conn, 'material': nh(*inp.material['Rubber'].mat_para)}],
+          'pressures': [{'stype': 'T3', 'conn': fac('surface_1_All'), 'p': 0.06}], 'tractions': [],
+          'rps': [{'node': (0.0, 0.0, 70.0)}],
+          'couples': [{'nodes': np.array(sorted(part.sets_nodes['surface_0_Head'])), 'rp': 0}]}
+    cp.assemble_reduced(om, GC)
+    t0 = time.perf_counter()
+    R, K = cp.assemble_reduced(om, GC)
+    osol.jacobi_scaled_solve(K, -R, np.zeros_like(R), linear='direct')
+    t_base = time.perf_counter() - t0
+    U, _ = cp.state(om, GC)
+    x = X + U
+    n_pairs, t_bar = 0, 0.0
+    for s in ['surface_%d_All' % i for i in range(4)]:
+        f = fac(s)
+        pairs = ob.candidate_pairs(ob.centroids(x, f), 3.0)
+        ratio = ob.self_ratio(X, f, pairs)
+        keep = ratio.sum((1, 2)) > 0
+        pairs, ratio = pairs[keep], ratio[keep]
+        n_pairs += pairs.shape[0]
+        m = min(2000, pairs.shape[0])
+        if m:
+            t1 = time.perf_counter()
+            ob.barrier((X, X), x, f, x, f, pairs[:m], ratio[:m], order=2)
+            t_bar += (time.perf_counter() - t1) * pairs.shape[0] / m
+    t_it = t_base + t_bar
+    return {'value': 1.0 / (t_it * newton_iters), 'unit': 'solves/s', 'cores': len(os.sched_getaffinity(0)),
+            'kind': 'port',
+            'sample': 'one FP64 oracle Newton iteration: reduced element/pressure/Couple assembly + direct solve '
+                      '%.2f s, barrier terms of %d active pairs %.2f s (timed on 2000 per surface, scaled), x %d '
+                      'Newton iterations of the GPU solve (extrapolated)' % (t_base, n_pairs, t_bar, newton_iters)}
+
+
 # ============================================================================ explicit (C3)
 def bench_explicit(args, torch):
     from fea_amd import workloads, solver
@@ -778,7 +906,8 @@ def main():
     import torch
     torch.set_num_threads(len(os.sched_getaffinity(0)))
     fn = {'beam': bench_beam, 'cube': bench_cube, 'explicit': bench_explicit, 'adjoint': bench_adjoint,
-          'serving': bench_serving, 'kernels': bench_kernels, 'couple': bench_couple}[args.workload]
+          'serving': bench_serving, 'kernels': bench_kernels, 'couple': bench_couple,
+          'contact': bench_contact}[args.workload]
     line = fn(args, torch)
     print(json.dumps(line), flush=True)
 

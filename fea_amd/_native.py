@@ -70,6 +70,14 @@ _SIGS = {
     'fea_gather_nodal': (C_INT, [C_INT, C_P, C_P, C_P, C_P]),
     'fea_pcg_system_size': (C_INT, []),
     'fea_couple_map_size': (C_INT, []),
+    'fea_barrier_search': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_DBL, C_INT, C_DBL, C_INT, C_P, C_P, C_P,
+                                   C_P, C_P]),
+    'fea_barrier_active': (C_INT, [C_P, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_barrier_missing': (C_INT, [C_P, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_INT, C_I64, C_P, C_P]),
+    'fea_barrier_energy_blocks': (C_INT, [C_INT]),
+    'fea_scan_i32': (C_INT, [C_INT, C_P, C_P, C_P]),
+    'fea_barrier_energy': (C_INT, [C_P, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_barrier_assemble': (C_INT, [C_P, C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_couple_kinematics': (C_INT, [C_P, C_P, C_INT, C_P]),
     'fea_couple_condense': (C_INT, [C_P, C_INT, C_P, C_P, C_DBL, C_P, C_P, C_P]),
     'fea_pcg_batch_layout': (C_INT, [C_INT, C_P, C_P, C_P]),

@@ -123,12 +123,30 @@ class Assembly:
         X = np.concatenate(X, axis=0) if X else np.zeros((0, 3))
         free = np.concatenate(masks) if masks else np.zeros(0, dtype=bool)
         self._coupled = bool(self._reference_points or self._constraints)
+        barrier_loads = [l for l in self._loads.values() if isinstance(l, loads.barrier._BarrierLoad)]
+        if barrier_loads:
+            from ..barrier import BarrierSet, LivePlan
+            base = self._base_plan_builder(X, free, groups, offsets)
+            Xd = torch.as_tensor(X, dtype=torch.float64, device=self.device).reshape(-1).contiguous()
+            sets = []
+            for name, l in self._loads.items():
+                if l in barrier_loads:
+                    c1, c2 = l.facets(self)
+                    sets.append(BarrierSet(c1, c2, Xd, l.params(), self.device, name=name))
+            self.plan = LivePlan(lambda extra: base(extra), sets, node_off)
+            self._K_indices = None
+            return
+        self.plan = self._base_plan_builder(X, free, groups, offsets)([])
+        self._K_indices = None
+
+    def _base_plan_builder(self, X, free, groups, offsets):
+        """A function of extra groups building the DevicePlan (or CoupledPlan) of the model."""
         if self._coupled:
             from ..couple import CoupledPlan
             rp_names = list(self._reference_points.keys())
             if not rp_names:
                 raise ValueError('a Couple needs a reference point')
-            if 3 * node_off != self.RGC_list_indexStart[self._reference_points[rp_names[0]]._RGC_index]:
+            if 3 * X.shape[0] != self.RGC_list_indexStart[self._reference_points[rp_names[0]]._RGC_index]:
                 raise RuntimeError('reference point RGC blocks must follow the instances')
             rps = []
             for rn in rp_names:
@@ -144,10 +162,8 @@ class Assembly:
                 if not isinstance(c, constraints.Couple):
                     raise NotImplementedError('constraint %s' % type(c).__name__)
                 couples.append((c._indexNodes + offsets[c.instance_name], rp_names.index(c.rp_name)))
-            self.plan = CoupledPlan(X, free, groups, couples, rps, device=self.device)
-        else:
-            self.plan = DevicePlan(X, free, groups, device=self.device)
-        self._K_indices = None
+            return lambda extra: CoupledPlan(X, free, groups + extra, couples, rps, device=self.device)
+        return lambda extra: DevicePlan(X, free, groups + extra, device=self.device)
 
     def initialize_dynamic(self):
         for ins in self._instances.values():
@@ -202,8 +218,10 @@ class Assembly:
 
     # ------------------------------------------------------------------ assembly
     def K_indices(self):
-        if self._K_indices is None:
+        v = getattr(self.plan, 'version', 0)
+        if self._K_indices is None or self._K_version != v:
             self._K_indices = self.plan.csr_indices_coo()
+            self._K_version = v
         return self._K_indices
 
     def assemble_Stiffness_Matrix(self, RGC=None, GC=None):

@@ -17,7 +17,7 @@ OUT_DIR = os.path.join(HERE, '_lib')
 LIB = os.path.join(OUT_DIR, 'libfea_amd.so')
 ARCH = os.environ.get('FEA_AMD_ARCH', 'gfx950')
 
-HIP_SOURCES = ['elements.hip', 'surfaces.hip', 'linalg.hip', 'pcg.hip', 'contact.hip', 'adjoint.hip', 'batched.hip', 'couple.hip']
+HIP_SOURCES = ['elements.hip', 'surfaces.hip', 'linalg.hip', 'pcg.hip', 'contact.hip', 'adjoint.hip', 'batched.hip', 'couple.hip', 'barrier.hip']
 CPP_SOURCES = ['pattern.cpp', 'api.cpp']
 
 # packed-FP32 instructions are disabled: the compiler may form them from scalar

@@ -67,7 +67,7 @@ class Group:
     """One contiguous batch of elements or facets sharing a kernel configuration."""
 
     def __init__(self, kind, ntype, conn, params=None, load=None, in_k=True, rho=1.0, name=''):
-        self.kind = kind            # 'elem' | 'pressure' | 'traction'
+        self.kind = kind            # 'elem' | 'pressure' | 'traction' | 'barrier' (contact pairs)
         self.ntype = int(ntype)     # nodes per element / facet (= C-ABI type id)
         self.conn_np = np.ascontiguousarray(conn, dtype=np.int32)
         self.n = int(self.conn_np.shape[0])
@@ -247,6 +247,8 @@ class DevicePlan:
                 t = np.asarray(g.load.traction, dtype=np.float64)
                 call('fea_surface_traction', g.ntype, g.n, g.conn, self.X, U,
                      t, float(load_scale), g.dst_r, self.rstage, None, s)
+            elif g.kind == 'barrier':
+                g.load.assemble(U, None, None, F64, g.dst_r, self.rstage)
 
     def tangent(self, U, load_scale=1.0, dtype=F64, with_residual=True, val_dtype=None):
         """Tangent CSR values (``val_dtype``, default ``dtype``) and optionally the float64 residual.
@@ -276,6 +278,8 @@ class DevicePlan:
                 t = np.asarray(g.load.traction, dtype=np.float64)
                 call('fea_surface_traction', g.ntype, g.n, g.conn, self.X, U,
                      t, float(load_scale), g.dst_r, self.rstage, None, s)
+            elif g.kind == 'barrier':
+                g.load.assemble(U, g.dst_k, ks, dtype, g.dst_r if with_residual else None, rs)
         val_dtype = dtype if val_dtype is None else val_dtype
         val = self.csr_values(val_dtype)
         call('fea_gather_csr', dtype, val_dtype, self.n_blocks, self.blk_a, self.blk_b, self.blk_colpos,
@@ -305,6 +309,8 @@ class DevicePlan:
                 t = np.asarray(g.load.traction, dtype=np.float64)
                 call('fea_surface_traction', g.ntype, g.n, g.conn, self.X, U,
                      t, 1.0, None, None, seg, s)
+            else:
+                continue                    # barrier energy: evaluated over the exact pairs by LivePlan
             call('fea_reduce_sum', nb, seg, self._eout[gi:gi + 1], s)
         return self._eout
 
@@ -313,7 +319,7 @@ class DevicePlan:
         terms = self.energy_terms(U, load_scale).cpu().numpy()
         E = 0.0
         for gi, g in enumerate(self.groups):
-            if g.n == 0:
+            if g.n == 0 or g.kind == 'barrier':
                 continue
             if g.kind == 'elem':
                 E += terms[gi]

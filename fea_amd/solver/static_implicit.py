@@ -112,8 +112,11 @@ class StaticImplicitSolver(BaseSolver):
 
     # ------------------------------------------------------------------ internals
     def _pcg_for(self, mat_dtype, vec_dtype):
-        key = (mat_dtype, vec_dtype)
+        # keyed on the plan's CSR version: a contact plan re-creates its pattern when the
+        # contact pairs leave the staged set
+        key = (mat_dtype, vec_dtype, getattr(self.assembly.plan, 'version', 0))
         if key not in self._pcg:
+            self._pcg = {k: v for k, v in self._pcg.items() if k[2] == key[2]}
             p = self.assembly.plan
             self._pcg[key] = PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, mat_dtype, p.device,
                                  vec_dtype=vec_dtype, grp=p.grp, n_groups=p.spmv_groups, chunk=self.pcg_chunk)

@@ -116,10 +116,11 @@ def void_block_pressure(m=10, k=5, ns=5, *, mu=1.0, kappa=100.0, pressure=0.05, 
                                  load=lambda nm: loads.Pressure(nm, 'void', pressure)), mesh
 
 
-def inp_couple(path, *, pressure=0.06, rp=(0.0, 0.0, 80.0), instance='final_model'):
-    """The reference's Couple benchmark set-up (tests/pressure_test/test_benchmark.py:24-41)
-    on the INP mesh at ``path``: follower pressure on surface_1_All, surface_0_Bottom fixed,
-    the surface_0_Head node set coupled to a reference point at ``rp``."""
+def inp_couple(path, *, pressure=0.06, rp=(0.0, 0.0, 80.0), instance='final_model', contact_surfaces=()):
+    """The reference's Couple set-up (tests/pressure_test/test_benchmark.py:24-41,
+    tests/gradient_test/test_grad.py:27-52) on the INP mesh at ``path``: follower pressure on
+    surface_1_All, surface_0_Bottom fixed, the surface_0_Head node set coupled to a reference
+    point at ``rp``, and a ContactSelf load on each of ``contact_surfaces``."""
     from .assemble import ReferencePoint, constraints
     from .inp import FEA_INP, from_inp
     inp = FEA_INP().read_inp(path)
@@ -128,6 +129,8 @@ def inp_couple(path, *, pressure=0.06, rp=(0.0, 0.0, 80.0), instance='final_mode
     a.add_load(loads.Pressure(instance_name=instance, surface_set='surface_1_All', pressure=pressure),
                name='pressure-1')
     a.add_boundary(boundarys.Boundary_Condition(instance_name=instance, set_nodes_name='surface_0_Bottom'))
+    for s in contact_surfaces:
+        a.add_load(loads.ContactSelf(instance_name=instance, surface_name=s))
     name = a.add_reference_point(ReferencePoint(list(rp)))
     a.add_constraint(constraints.Couple(instance_name=instance, set_nodes_name='surface_0_Head', rp_name=name))
     return fe, inp

@@ -440,6 +440,60 @@ int fea_couple_kinematics(const fea_couple_map *m, double *state, int with_deriv
 int fea_couple_condense(const fea_couple_map *m, int val_prec, const double *K_ext, const double *R_ext,
                         double load_scale, void *val_red, double *R_red, void *stream);
 
+/* ---------------------------------------------------------------- barrier --
+ * The reference's Gauss-point-pair contact barrier on T3 facets (ContactSelf /
+ * Contact, loads/contact.py:86-122, :300-711, :771-1207).  For facet pairs whose
+ * centroids lie within `radius` at the current configuration (FMA-free float64
+ * centroid ((a + b) + c) / 3 and squared distance, as fea_contact_search), and each
+ * of their 3 x 3 Gauss pairs (points (1/6,1/6), (2/3,1/6), (1/6,2/3), weight 1/6):
+ *   phi = ratio w_p w_q c_g exp(k_g D) S((s_f - M)/(s_f - e_f)) S((h_t - L)/(r_h h_t)),
+ * D = (n_p - n_q).(y_g - y_G)/2, M = n_p.n_q, L = |y_g - y_G|, S the clamped quintic
+ * smoothstep, w = |X_xi x X_eta|/6 from the reference coordinates; ratio = 1 for two
+ * surfaces, and for one surface (self_contact) the initial-geometry factor
+ * S((-(n0_g - n0_G).dy0/|dy0| - ign_min)/(ign_max - ign_min)) of contact.py:312-331.
+ * The barrier energy is + sum phi in the total potential energy, its gradient adds to
+ * the residual and its Hessian to the tangent. */
+typedef struct fea_barrier_params {
+  double c_g, k_g;        /* penalty_distance_g (1e-5), penalty_factor_g (20) */
+  double h_t, r_h;        /* penalty_threshold_h (1.5), penalty_ratio_h (0.5) */
+  double s_f, e_f;        /* penalty_start_f (-0.0), penalty_end_f (-0.8) */
+  double ign_min, ign_max;/* ignore_min_normal (0.8), ignore_max_normal (1.2), self contact */
+  double radius;          /* candidate radius: search ratio (2.0) x h_t */
+  int32_t self_contact;   /* 1: one surface (pairs j > i, ratio factor), 0: two surfaces */
+  int32_t reserved;
+} fea_barrier_params;
+/* Candidate pairs (i, j): query facets qconn [n_query x 3] against the facets of a grid
+ * built by fea_contact_grid_build (centroids, buckets; cell >= radius).  With
+ * self_contact only j > i.  Count pass (offsets == NULL): count[i].  Fill pass:
+ * pair_i/pair_j at offsets[i] (exclusive scan of the counts), deterministic order. */
+int fea_barrier_search(int n_query, const int32_t *qconn, const double *X, const double *U, const double *centroid,
+                       const int32_t *bucket_start, const int32_t *bucket_items, const double *origin_host,
+                       double cell, int table_bits, double radius, int self_contact, const int32_t *offsets,
+                       int32_t *count, int32_t *pair_i, int32_t *pair_j, void *stream);
+/* out[0..n] = exclusive prefix sum of in[0..n) (out[n] = total), one workgroup. */
+int fea_scan_i32(int n, const int32_t *in, int32_t *out, void *stream);
+/* flag[p] = 1 if some Gauss pair of pair p has a nonzero ratio (the pairs ContactSelf keeps). */
+int fea_barrier_active(const fea_barrier_params *P, int n_pairs, const int32_t *pair_i, const int32_t *pair_j,
+                       const int32_t *conn1, const int32_t *conn2, const double *X, int32_t *flag, void *stream);
+/* *missing += number of pairs with a nonzero ratio whose key i * stride + j is not in
+ * keys[n_keys] (ascending): the pairs a staged pattern does not cover. */
+int fea_barrier_missing(const fea_barrier_params *P, int n_pairs, const int32_t *pair_i, const int32_t *pair_j,
+                        const int32_t *conn1, const int32_t *conn2, const double *X, const int64_t *keys,
+                        int n_keys, int64_t stride, int32_t *missing, void *stream);
+int fea_barrier_energy_blocks(int n_pairs);
+/* per-block partial sums of phi over the listed pairs (all treated as candidates) */
+int fea_barrier_energy(const fea_barrier_params *P, int n_pairs, const int32_t *pair_i, const int32_t *pair_j,
+                       const int32_t *conn1, const int32_t *conn2, const double *X, const double *U,
+                       double *partial, void *stream);
+/* Gradient rows (dst_r: slot per pair node, 6 per pair, nodes = conn1[i] then conn2[j])
+ * and Hessian 3x3 blocks (dst_k: 36 per pair, row-major node pairs; stage_prec) of every
+ * listed pair that is a candidate at U; the slots of the others are zeroed.  One 64-lane
+ * workgroup per pair.  Either output may be NULL. */
+int fea_barrier_assemble(const fea_barrier_params *P, int stage_prec, int n_pairs, const int32_t *pair_i,
+                         const int32_t *pair_j, const int32_t *conn1, const int32_t *conn2, const double *X,
+                         const double *U, const int32_t *dst_k, void *kstage, const int32_t *dst_r, double *rstage,
+                         void *stream);
+
 #ifdef __cplusplus
 }
 #endif

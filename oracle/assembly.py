@@ -10,6 +10,8 @@ so global DOF = 3*node + component as in the flattened RGC of assembly.py:97-128
                  'rho': float (optional)}],
      'pressures': [{'stype': 'T3'|'T6'|'Q4', 'conn': [Es,ns] int, 'p': float}],
      'tractions': [{'stype': ..., 'conn': ..., 't': (tx,ty,tz)}],
+     'barriers': [{'conn1': [F1,3] int, 'conn2': [F2,3] int or None (self contact),
+                   'params': dict (optional, oracle.barrier.DEFAULTS)}],
      'fixed': [N,3] bool}
 """
 import numpy as np
@@ -63,6 +65,16 @@ def assemble_full(model, U, onlyforce=False):
             rows.append(idx[0]), cols.append(idx[1]), vals.append(-Kv)
     for t in model.get('tractions', []):
         R -= t['f'].reshape(-1)
+    for b in model.get('barriers', []):
+        Eb, gb, Hb, nodes = barrier_terms(model, b, U, 1 if onlyforce else 2)
+        if nodes is None:
+            continue
+        dof = (3 * nodes[:, :, None] + np.arange(3)[None, None, :]).reshape(nodes.shape[0], 18)
+        np.add.at(R, dof.reshape(-1), gb.reshape(-1))
+        if not onlyforce:
+            rows.append(np.repeat(dof, 18, axis=1).reshape(-1))
+            cols.append(np.tile(dof, (1, 18)).reshape(-1))
+            vals.append(Hb.reshape(-1))
     if onlyforce:
         return R, None
     return R, (np.concatenate(rows), np.concatenate(cols), np.concatenate(vals))
@@ -106,7 +118,39 @@ def total_energy(model, U):
         E -= -p['p'] * sf.pressure_volume(p['stype'], pos, p['conn'])     # pressure.py:159
     for t in model.get('tractions', []):
         E -= float((t['f'] * U).sum())
+    for b in model.get('barriers', []):
+        E += float(barrier_terms(model, b, U, 0)[0].sum())          # -(-sum phi), contact.py:386-387
     return E
+
+
+def barrier_terms(model, b, U, order):
+    """ContactSelf / Contact (contact.py:300-711, :771-1207) at the current configuration:
+    candidate pairs from the current centroids, the self-contact ratio from the initial one;
+    returns (energy per pair, gradient, Hessian, pair nodes [P, 6])."""
+    from . import barrier as ob
+    prm = dict(ob.DEFAULTS, **b.get('params', {}))
+    X0 = model['nodes']
+    x = X0 + U.reshape(-1, 3)
+    c1 = b['conn1']
+    c2 = c1 if b.get('conn2') is None else b['conn2']
+    r = prm['search_ratio'] * prm['h_t']
+    if b.get('conn2') is None:
+        pairs = ob.candidate_pairs(ob.centroids(x, c1), r)
+        ratio = ob.self_ratio(X0, c1, pairs, prm)
+        keep = ratio.sum((1, 2)) > 0
+        pairs, ratio = pairs[keep], ratio[keep]
+    else:
+        pairs = ob.candidate_pairs(ob.centroids(x, c1), r, ob.centroids(x, c2))
+        ratio = np.ones((pairs.shape[0], 3, 3))
+    if pairs.shape[0] == 0:
+        return np.zeros(0), None, None, None
+    out = ob.barrier((X0, X0), x, c1, x, c2, pairs, ratio, prm, order=order)
+    nodes = np.concatenate([c1[pairs[:, 0]], c2[pairs[:, 1]]], axis=1)
+    if order == 0:
+        return out, None, None, nodes
+    if order == 1:
+        return out[0], out[1], None, nodes
+    return out[0], out[1], out[2], nodes
 
 
 def gc_to_u(model, GC):
