@@ -44,11 +44,11 @@ def slotted_block(nx=6, ny=2, nz=5, size=(3.0, 1.0, 2.5), slot_x=1.0, slot_z=(1.
                 out[f] = sel
         return [(ids, f) for f, ids in out.items()]
     on = lambda z: (lambda fx: np.all(np.abs(fx[:, :, 2] - z) < tol, axis=1) & np.all(fx[:, :, 0] > slot_x - tol, axis=1))
-    slot = faces(on(slot_z[0])) + faces(on(slot_z[1]))
+    lo, up = faces(on(slot_z[0])), faces(on(slot_z[1]))
     top = faces(on(size[2]))
     mesh = {'nodes': X, 'conn': conn, 'etype': 'C3D4',
             'node_sets': {'z0': np.nonzero(np.abs(X[:, 2]) < tol)[0]},
-            'face_sets': {'slot': slot, 'arm_top': top}}
+            'face_sets': {'slot': lo + up, 'slot_lo': lo, 'slot_up': up, 'arm_top': top}}
     return mesh
 
 
@@ -135,3 +135,38 @@ def test_static_solve_closes_the_slot_like_the_oracle(gpu, prec):
     assert Eb > 1e-6                                  # the slot closed onto the barrier
     print('barrier: newton %d cg %d rebuilds %d' % (fe.solver.stats['newton_iters'], fe.solver.stats['cg_iters'],
                                                     fe.assembly.plan.rebuilds))
+
+
+def test_two_surface_contact_matches_oracle(gpu):
+    """Contact (contact.py:714-1207) between the slot's two faces as two surfaces: ratio 1
+    for every pair, pairs (upper facet, lower facet) within the candidate radius."""
+    mesh = slotted_block()
+    q = 0.05
+    fe = workloads._controller_from_mesh(mesh, materials.NeoHookean(1.0, 10.0), fixed_set='z0',
+                                         load=lambda nm: loads.Traction(nm, 'arm_top', (0.0, 0.0, -q)))
+    fe.assembly.add_load(loads.Contact(instance_name1='block', instance_name2='block', surface_name1='slot_up',
+                                       surface_name2='slot_lo'), 'contact')
+    om = oracle_model(mesh, nh(1.0, 10.0), 'z0', traction=(0.0, 0.0, -q), load_set='arm_top')
+    om['barriers'] = [{'conn1': facets(mesh, 'slot_up')[0], 'conn2': facets(mesh, 'slot_lo')[0]}]
+    a = fe.assembly
+    a.initialize()
+    U = closed_state(om, mesh, seed=2)
+    free = oasm.free_mask(om)
+    GC = U.reshape(-1)[free]
+    R, Kidx, Kval = a.assemble_Stiffness_Matrix(GC=torch.as_tensor(GC, device='cuda'))
+    R_o, K_o = oasm.assemble_reduced(om, U)
+    assert rel_err(R.cpu().numpy(), R_o) < 1e-11
+    K = np.zeros(K_o.shape)
+    Ki = Kidx.cpu().numpy()
+    K[Ki[0], Ki[1]] = Kval.cpu().numpy()
+    assert np.abs(K - K_o.toarray()).max() < 1e-11 * np.abs(K_o.data).max()
+    assert abs(float(a._total_Potential_Energy(GC=torch.as_tensor(GC, device='cuda'))) -
+               oasm.total_energy(om, U)) < 1e-12 * abs(oasm.total_energy(om, U))
+    fe.solver = solver.StaticImplicitSolver(precision='fp64', rel_tol=1e-10, load_steps=4, cg_tol=1e-24,
+                                            cg_max_iter=20000, armijo_slack=1e-12, predictor=False)
+    fe.solver.initialize(a)
+    GC = fe.solver.solve(GC0=torch.zeros_like(a.GC))
+    assert GC is not False
+    GC_o, _ = osol.static_solve(om, increments=4, rel_tol=1e-10, linear='direct', armijo_slack_rel=1e-12,
+                                predictor=False, max_iter=100)
+    assert rel_err(GC.cpu().numpy(), GC_o) < 1e-7
