@@ -20,7 +20,7 @@
 
 namespace fea {
 
-namespace {
+namespace bar {
 
 __device__ __forceinline__ double ad(double a, double b) { return __dadd_rn(a, b); }
 __device__ __forceinline__ double mu(double a, double b) { return __dmul_rn(a, b); }
@@ -505,10 +505,11 @@ __global__ __launch_bounds__(1024) void k_scan_i32(int n, const int32_t *__restr
   }
 }
 
-}  // namespace
+}  // namespace bar
 }  // namespace fea
 
 using namespace fea;
+using namespace fea::bar;
 
 extern "C" {
 

@@ -145,7 +145,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_update(int n_sys, const fea_pc
   if (ws[W_DONE] != 0.0) return;
   const int lb = blockIdx.x - vec_start[k];
   const int it = (int)ws[W_ITER];
-  const double rz_old = sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, S.vec_blocks);
+  // r.z of the previous iteration: stored by its direction stage (a re-sum of the same
+  // partials in the same order gives the same bits); summed here only in iteration 1
+  const double rz_old = it > 1 ? ws[W_RZH + ((it - 1) & 1)]
+                               : sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, S.vec_blocks);
   const double pq = sum_partials(ws + W_PQ, S.spmv_blocks);
   const TV alpha = (TV)(pq != 0.0 ? rz_old / pq : 0.0);  // b = A x exactly: no step
   const TV *dinv = (const TV *)S.dinv;
@@ -178,12 +181,16 @@ __global__ __launch_bounds__(kBlock) void k_batch_dir(int n_sys, const fea_pcg_s
   const int lb = blockIdx.x - vec_start[k];
   const int it = (int)ws[W_ITER];
   const double rz_new = sum_partials(ws + W_RZP + (it & 1) * PCG_G, S.vec_blocks);
-  const double rz_old = sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, S.vec_blocks);
-  const double bz = sum_partials(ws + W_BZP, S.vec_blocks);
+  const double rz_old = it > 1 ? ws[W_RZH + ((it - 1) & 1)]
+                               : sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, S.vec_blocks);
+  // b.z is rewritten with the same value by every direction stage: from iteration 2 on
+  // it is read back (workgroup 0 of this launch may be storing the identical bits)
+  const double bz = it > 1 ? ws[W_BZ] : sum_partials(ws + W_BZP, S.vec_blocks);
   const bool done = (rz_new == 0.0) || (rz_new / bz < S.tol) || (it >= S.max_iter) || !(rz_new == rz_new);
   // every workgroup of the system reads W_DONE at its start: only a later launch may see it set
   if (lb == 0 && threadIdx.x == 0) {
     ws[W_RZ] = rz_new;
+    ws[W_RZH + (it & 1)] = rz_new;
     ws[W_BZ] = bz;
     if (done) ws[W_DONE] = 1.0;
   }

@@ -102,7 +102,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const TV *__restri
   if (ws[W_DONE] != 0.0) return;
   const int it = (int)ws[W_ITER];
   const int g1 = (int)ws[W_G1];
-  const double rz_old = sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, gridDim.x);
+  // r.z of the previous iteration: stored by its direction stage (a re-sum of the same
+  // partials in the same order gives the same bits); summed here only in iteration 1
+  const double rz_old = it > 1 ? ws[W_RZH + ((it - 1) & 1)]
+                               : sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, gridDim.x);
   const double pq = sum_partials(ws + W_PQ, g1);
   const TV alpha = (TV)(pq != 0.0 ? rz_old / pq : 0.0);  // b = A x exactly: no step
   double rz = 0.0;
@@ -126,11 +129,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, const TV *__restrict_
   if (ws[W_DONE] != 0.0) return;
   const int it = (int)ws[W_ITER];
   const double rz_new = sum_partials(ws + W_RZP + (it & 1) * PCG_G, gridDim.x);
-  const double rz_old = sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, gridDim.x);
-  const double bz = sum_partials(ws + W_BZP, gridDim.x);
+  const double rz_old = it > 1 ? ws[W_RZH + ((it - 1) & 1)]
+                               : sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, gridDim.x);
+  // b.z is rewritten with the same value by every direction stage: from iteration 2 on
+  // it is read back (workgroup 0 of this launch may be storing the identical bits)
+  const double bz = it > 1 ? ws[W_BZ] : sum_partials(ws + W_BZP, gridDim.x);
   const bool done = (rz_new == 0.0) || (rz_new / bz < tol) || (it >= max_iter) || !(rz_new == rz_new);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ws[W_RZ] = rz_new;
+    ws[W_RZH + (it & 1)] = rz_new;
     ws[W_BZ] = bz;
     if (done) ws[W_DONE] = 1.0;
   }

@@ -13,6 +13,9 @@ enum : int {
   G1_MAX = 2048,  // max workgroups of the SpMV stage (p.q partials; 2048 measured best)
   PCG_G = 512,    // workgroups of the vector stages (r.z / b.z partials)
   W_ITER = 0, W_DONE = 1, W_RZ = 2, W_BZ = 3, W_G1 = 4,
+  W_RZH = 5,      // r.z of iterations of parity 0 / 1, written by the direction stage's
+                  // workgroup 0 (slot it & 1) and read from the next iteration on
+                  // (slot (it - 1) & 1) instead of re-summing the partials
   W_PQ = 16, W_RZP = W_PQ + G1_MAX, W_BZP = W_RZP + 2 * PCG_G, W_END = W_BZP + PCG_G
 };
 

@@ -301,7 +301,7 @@ class SolveServer:
 
     def __init__(self, precision='mixed', hbm_budget=None, max_batch=64, rel_tol=1e-10, step_tol=1e-12,
                  cg_tol=1e-8, cg_max_iter=20000, max_newton=200, chunk=32, use_graph=False,
-                 armijo_slack=1e-12, verbose=False):
+                 armijo_slack=1e-12, verbose=False, predictor=True):
         nat.require_gpu()
         if precision not in ('fp64', 'mixed'):
             raise ValueError("precision must be 'fp64' or 'mixed'")
@@ -316,6 +316,7 @@ class SolveServer:
         self.cg_tol, self.cg_max_iter = float(cg_tol), int(cg_max_iter)
         self.max_newton = int(max_newton)
         self.armijo_slack = float(armijo_slack)
+        self.predictor = bool(predictor)
         self.verbose = verbose
         self.pcg = BatchedPCG(self.mat_dtype, F64, 'cuda', chunk=chunk, use_graph=use_graph)
         self.active: list[_Slot] = []
@@ -388,10 +389,11 @@ class SolveServer:
         if s.k >= s.steps:
             self._retire(s, 'done', now())
             return
-        # U <- 2U - U_prev, U_prev <- old U (linear extrapolation of converged states)
-        step = s.U - s.U_prev
-        s.U_prev.copy_(s.U)
-        s.U.add_(step)
+        if self.predictor:
+            # U <- 2U - U_prev, U_prev <- old U (linear extrapolation of converged states)
+            step = s.U - s.U_prev
+            s.U_prev.copy_(s.U)
+            s.U.add_(step)
         s.k += 1
         s.state = 'tangent'
 
@@ -549,7 +551,7 @@ class SolveServer:
 
 
 def solve_one_at_a_time(requests, precision='fp64', rel_tol=1e-10, step_tol=1e-12, cg_tol=1e-8,
-                        cg_max_iter=20000, max_newton=200, armijo_slack=1e-12):
+                        cg_max_iter=20000, max_newton=200, armijo_slack=1e-12, predictor=True):
     """The plain path: each request built and solved alone by ``StaticImplicitSolver``,
     in arrival order.  Returns (GC list, per-request seconds including the model
     build, FIFO latencies under the requests' arrival times)."""
@@ -560,7 +562,8 @@ def solve_one_at_a_time(requests, precision='fp64', rel_tol=1e-10, step_tol=1e-1
         fe, _ = request_model(r.spec)
         fe.solver = StaticImplicitSolver(maximum_iteration=max_newton, precision=precision, rel_tol=rel_tol,
                                          step_tol=step_tol, load_steps=int(r.spec.get('load_steps', 1)),
-                                         cg_tol=cg_tol, cg_max_iter=cg_max_iter, armijo_slack=armijo_slack)
+                                         cg_tol=cg_tol, cg_max_iter=cg_max_iter, armijo_slack=armijo_slack,
+                                         predictor=predictor)
         fe.initialize()
         GC = fe.solver.solve(GC0=torch.zeros_like(fe.assembly.GC))
         torch.cuda.synchronize()
