@@ -57,6 +57,8 @@ def parse():
     ap.add_argument('--cg-tol', type=float, default=1e-8)
     ap.add_argument('--no-fp64-compare', action='store_true')
     ap.add_argument('--forcing', default='ew', choices=['ew', 'fixed'])
+    ap.add_argument('--predictor', default='auto',
+                    help="beam: load-increment predictor: none, linear, quadratic, cubic, auto or an order 0..6")
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--spmv-reps', type=int, default=200)
     ap.add_argument('--max-newton', type=int, default=400, help='Newton iterations per solve (all increments) before failing')
@@ -66,6 +68,7 @@ def parse():
     ap.add_argument('--cells', type=int, default=40, help='explicit: cells per block edge')
     ap.add_argument('--couple-pressure', type=float, default=0.06, help='couple: pressure on surface_1_All')
     ap.add_argument('--couple-increments', type=int, default=4, help='couple: load increments')
+    ap.add_argument('--couple-predictor', default='auto', help='couple / contact: load-increment predictor')
     return ap.parse_args()
 
 
@@ -130,6 +133,11 @@ def _progress(k, st):
     print('#   t=%.2fs increment %d newton %d cg %d alpha %.3g |R| %.3e' % (
         time.perf_counter() - _T0[0], k, st['newton_iters'], st['cg_iters'], st['alpha'], st['residual']),
         file=sys.stderr, flush=True)
+
+
+def _pred(args, name=None):
+    name = args.predictor if name is None else name
+    return int(name) if name.isdigit() else name
 
 
 def make_solver(precision, args, load_steps=None, progress=True, predictor=True):
@@ -232,7 +240,7 @@ def bench_beam(args, torch):
     steps = 2 if args.steps is None else args.steps
     warmup = 1 if args.warmup is None else args.warmup
     fe, mesh = workloads.beam_traction(args.nx, args.ny, args.nz)
-    fe.solver = make_solver(args.precision, args)
+    fe.solver = make_solver(args.precision, args, predictor=_pred(args))
     fe.initialize()
     p = fe.assembly.plan
     log('model: %d elements, %d nodes, %d free DOF, nnz %d' % (mesh['conn'].shape[0], mesh['nodes'].shape[0],
@@ -257,7 +265,7 @@ def bench_beam(args, torch):
     fp64 = {}
     if args.precision != 'fp64' and not args.no_fp64_compare:
         fe64, _ = workloads.beam_traction(args.nx, args.ny, args.nz)
-        fe64.solver = make_solver('fp64', args)
+        fe64.solver = make_solver('fp64', args, predictor=_pred(args))
         fe64.initialize()
         GC64, t64 = run_solve(fe64, torch)
         err = float((GC - GC64).abs().max() / GC64.abs().max())
@@ -281,10 +289,12 @@ def bench_beam(args, torch):
                      {'workload': 'cantilever beam 4x1x1, %dx%dx%d C3D8, Mooney-Rivlin C10=0.5 C01=0.1 D1=0.01, '
                                   'clamped x=0, dead traction (0,0,-0.01) on x=4, %d increments, Newton rel tol %g'
                                   % (args.nx, args.ny, args.nz, args.increments, args.rel_tol),
-                      'precision': args.precision, 'free_dof': p.n_free, 'nnz': p.nnz})
+                      'precision': args.precision, 'predictor': args.predictor, 'free_dof': p.n_free,
+                      'nnz': p.nnz})
     line.update({
         'latency_p50_s': float(np.percentile(lat, 50)), 'latency_p99_s': float(np.percentile(lat, 99)),
         'newton_iters_per_solve': stats.get('newton_iters'), 'cg_iters_per_solve': stats.get('cg_iters'),
+        'predictor_orders': stats.get('predictor_orders'),
         'energy_evals_per_solve': stats.get('energy_evals'),
         'elem_tangent_evals_per_s': elem_rate, 'tangent_assembly_ms': 1e3 * t_tan,
         'elem_tangent_evals_per_s_f32_contraction': elem_rate32,
@@ -487,6 +497,7 @@ def bench_cube(args, torch):
                                   'tol 1e-10, 5% interior jitter', 'precision': 'fp64'})
     line.update({'latency_p50_s': float(np.percentile(lat, 50)), 'latency_p99_s': float(np.percentile(lat, 99)),
                  'newton_iters_per_solve': stats['newton_iters'], 'cg_iters_per_solve': stats['cg_iters'],
+                 'predictor_orders': stats.get('predictor_orders'),
                  'rel_err_vs_oracle': err,
                  'roofline': roofline('k_pcg_spmv<double,double,%d,2> (fea_pcg_spmv)' % _triple_sw(p),
                                       alg / t_spmv / 1e9,
@@ -508,9 +519,10 @@ def bench_couple(args, torch):
     warmup = 0 if args.warmup is None else args.warmup
     path = os.path.join(ROOT, 'tests', 'golden', 'inp', 'C3D4Less.inp.gz')
     fe, inp = workloads.inp_couple(path, pressure=args.couple_pressure)
-    # each increment starts from the previous equilibrium: a linear extrapolation of the
-    # actuator's large rotation starts far from the next one
-    fe.solver = make_solver(args.precision, args, load_steps=args.couple_increments, predictor=False)
+    # a plain linear extrapolation of the actuator's large rotation starts far from the next
+    # equilibrium; 'auto' falls back to the previous one when the extrapolation's energy is higher
+    fe.solver = make_solver(args.precision, args, load_steps=args.couple_increments,
+                            predictor=_pred(args, args.couple_predictor))
     fe.initialize()
     p = fe.assembly.plan
     for _ in range(warmup):
@@ -535,10 +547,12 @@ def bench_couple(args, torch):
                                   'reference point at (0,0,80), %d increments, Newton rel tol %g'
                                   % (inp.part['final_model'].n_elements(), *inp.material['Rubber'].mat_para,
                                      args.couple_pressure, args.couple_increments, args.rel_tol),
-                      'precision': args.precision, 'free_dof': p.n_free, 'nnz': p.nnz})
+                      'precision': args.precision, 'predictor': args.couple_predictor, 'free_dof': p.n_free,
+                      'nnz': p.nnz})
     line['data'] = 'the reference\'s INP mesh (tests/pressure_test/C3D4Less.inp, committed gzipped as a fixture)'
     line.update({'latency_p50_s': float(np.percentile(lat, 50)),
                  'newton_iters_per_solve': stats['newton_iters'], 'cg_iters_per_solve': stats['cg_iters'],
+                 'predictor_orders': stats.get('predictor_orders'),
                  'reference_point_v_z': rp.tolist(),
                  'roofline': roofline('k_pcg_spmv (fea_pcg_spmv) on the reduced CSR', alg / t_spmv / 1e9,
                                       HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=alg,
@@ -598,7 +612,8 @@ def bench_contact(args, torch):
     path = os.path.join(ROOT, 'tests', 'golden', 'inp', 'gradient_C3D4.inp.gz')
     surfaces = ['surface_%d_All' % i for i in range(4)]
     fe, inp = workloads.inp_couple(path, pressure=0.06, rp=(0.0, 0.0, 70.0), contact_surfaces=surfaces)
-    fe.solver = make_solver(args.precision, args, load_steps=args.couple_increments, predictor=False)
+    fe.solver = make_solver(args.precision, args, load_steps=args.couple_increments,
+                            predictor=_pred(args, args.couple_predictor))
     fe.initialize()
     p = fe.assembly.plan
     for _ in range(warmup):
@@ -642,10 +657,11 @@ def bench_contact(args, torch):
                                   'ContactSelf on surface_0..3_All, surface_0_Bottom fixed, surface_0_Head coupled to '
                                   'a reference point at (0,0,70), %d increments, Newton rel tol %g'
                                   % (inp.part['final_model'].n_elements(), args.couple_increments, args.rel_tol),
-                      'precision': args.precision, 'free_dof': p.n_free})
+                      'precision': args.precision, 'predictor': args.couple_predictor, 'free_dof': p.n_free})
     line['data'] = 'the reference\'s INP mesh (tests/gradient_test/C3D4.inp, committed gzipped as a fixture)'
     line.update({'latency_p50_s': float(np.percentile(lat, 50)),
                  'newton_iters_per_solve': stats['newton_iters'], 'cg_iters_per_solve': stats['cg_iters'],
+                 'predictor_orders': stats.get('predictor_orders'),
                  'pattern_rebuilds': p.rebuilds, 'staged_pairs': int(n_staged), 'candidate_pairs_at_solution': n_cand,
                  'rel_diff_vs_reference_golden': float(np.abs(g - gold).max() / np.abs(gold).max()),
                  'reference_point_v_z': g[-6:].tolist(),

@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_update(int n_sys, const fea_pc
   // r.z of the previous iteration: stored by its direction stage (a re-sum of the same
   // partials in the same order gives the same bits); summed here only in iteration 1
   const double rz_old = it > 1 ? ws[W_RZH + ((it - 1) & 1)]
-                               : sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, S.vec_blocks);
+                               : sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_RING, S.vec_blocks);
   const double pq = sum_partials(ws + W_PQ, S.spmv_blocks);
   const TV alpha = (TV)(pq != 0.0 ? rz_old / pq : 0.0);  // b = A x exactly: no step
   const TV *dinv = (const TV *)S.dinv;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_update(int n_sys, const fea_pc
     rz += (double)ri * (double)zi;
   }
   const double s = block_sum(rz);
-  if (threadIdx.x == 0) ws[W_RZP + (it & 1) * PCG_G + lb] = s;
+  if (threadIdx.x == 0) ws[W_RZP + (it & 1) * PCG_RING + lb] = s;
 }
 
 // convergence test (r.z / b.(dinv b) < tol or max_iter) and p = z + beta p
@@ -180,9 +180,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_dir(int n_sys, const fea_pcg_s
   if (ws[W_DONE] != 0.0) return;
   const int lb = blockIdx.x - vec_start[k];
   const int it = (int)ws[W_ITER];
-  const double rz_new = sum_partials(ws + W_RZP + (it & 1) * PCG_G, S.vec_blocks);
+  const double rz_new = sum_partials(ws + W_RZP + (it & 1) * PCG_RING, S.vec_blocks);
   const double rz_old = it > 1 ? ws[W_RZH + ((it - 1) & 1)]
-                               : sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_G, S.vec_blocks);
+                               : sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_RING, S.vec_blocks);
   // b.z is rewritten with the same value by every direction stage: from iteration 2 on
   // it is read back (workgroup 0 of this launch may be storing the identical bits)
   const double bz = it > 1 ? ws[W_BZ] : sum_partials(ws + W_BZP, S.vec_blocks);

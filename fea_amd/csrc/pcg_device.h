@@ -16,7 +16,8 @@ enum : int {
   W_RZH = 5,      // r.z of iterations of parity 0 / 1, written by the direction stage's
                   // workgroup 0 (slot it & 1) and read from the next iteration on
                   // (slot (it - 1) & 1) instead of re-summing the partials
-  W_PQ = 16, W_RZP = W_PQ + G1_MAX, W_BZP = W_RZP + 2 * PCG_G, W_END = W_BZP + PCG_G
+  PCG_RING = 2048, // most workgroups of the init / update stages (r.z, b.z partials per parity)
+  W_PQ = 16, W_RZP = W_PQ + G1_MAX, W_BZP = W_RZP + 2 * PCG_RING, W_END = W_BZP + PCG_RING
 };
 
 // MODE 0: one row per group; MODE 1: node groups (rows share the column list);

@@ -26,7 +26,15 @@ Additions (keyword-only, defaults reproduce the reference):
   the line search does not reject full steps whose energy change is below
   float64 resolution near convergence (0 reproduces the reference).
 * ``predictor``: with load increments, start increment k from the linear
-  extrapolation 2 u_{k-1} - u_{k-2} of the converged states.
+  extrapolation 2 u_{k-1} - u_{k-2} of the converged states (True / 'linear'), the
+  quadratic one 3 u_{k-1} - 3 u_{k-2} + u_{k-3} ('quadratic', from the third
+  increment), the cubic one ('cubic') or any order m <= 6 (an int: the degree-m
+  polynomial through the last m + 1 converged states, lower while fewer exist), the
+  order chosen per increment ('auto': the order m <= 6 whose prediction of u_{k-1}
+  from u_{k-2}, u_{k-3}, ... misses it least in the 2-norm, raised by one when that is
+  the highest order testable, and replaced by u_{k-1} when the prediction's potential
+  energy at the new load exceeds u_{k-1}'s), or from u_{k-1} (False: for paths with large rotations,
+  where the extrapolation can start far from the next equilibrium).
 * ``forcing='ew'``: inexact-Newton forcing term of Eisenstat and Walker
   (choice 2, gamma = 0.9, alpha = 2 with the safeguard gamma eta_{k-1}^2)
   for the PCG relative tolerance eta_k, clamped to [sqrt(cg_tol), eta_max];
@@ -46,12 +54,28 @@ from ._linear_solver import PCG
 from .basesolver import BaseSolver
 
 
+_PREDICTORS = {False: 0, True: 1, 'none': 0, 'linear': 1, 'quadratic': 2, 'cubic': 3}
+AUTO_MAX_ORDER = 6
+
+
+def predictor_order(predictor):
+    """Extrapolation order of a ``predictor`` argument: False / 'none' 0, True / 'linear' 1,
+    'quadratic' 2, 'cubic' 3, or an int in [0, 6]; 'auto' gives the largest order it may pick."""
+    if predictor == 'auto':
+        return AUTO_MAX_ORDER
+    if isinstance(predictor, (bool, str)) and predictor in _PREDICTORS:
+        return _PREDICTORS[predictor]
+    if isinstance(predictor, int) and 0 <= predictor <= 6:
+        return int(predictor)
+    raise ValueError("predictor must be False, True, 'linear', 'quadratic', 'cubic' or an order 0..6")
+
+
 class StaticImplicitSolver(BaseSolver):
 
     def __init__(self, maximum_iteration: int = 10000, tol_error: float = 1e-5, *,
                  precision: str = 'fp64', rel_tol: float | None = None, load_steps: int = 1,
                  cg_tol: float = 1e-5, cg_max_iter: int = 1500, ir_steps: int = 1,
-                 armijo_slack: float = 0.0, predictor: bool = True, verbose: bool = False,
+                 armijo_slack: float = 0.0, predictor=True, verbose: bool = False,
                  pcg_chunk: int = 32, progress=None, step_tol: float = 1e-12, forcing: str = 'fixed',
                  eta_max: float = 1e-2) -> None:
         super().__init__()
@@ -74,7 +98,8 @@ class StaticImplicitSolver(BaseSolver):
         self.cg_max_iter = int(cg_max_iter)
         self.ir_steps = int(ir_steps)
         self.armijo_slack = float(armijo_slack)
-        self.predictor = bool(predictor)
+        self.predictor = predictor_order(predictor)
+        self.predictor_auto = predictor == 'auto'
         self.progress = progress      # callable(increment, stats) after each Newton iteration
         self.verbose = verbose
         self.pcg_chunk = int(pcg_chunk)
@@ -278,6 +303,32 @@ class StaticImplicitSolver(BaseSolver):
                 if self._converged_abs(dmax, rmax, tol_error):
                     return True
 
+    @staticmethod
+    def _extrapolate(hist, m, out):
+        """out <- equal-step Lagrange extrapolation of degree m through hist[0..m] (newest
+        first) to the next level: sum_j (-1)^j C(m+1, j+1) hist[j]."""
+        s = nat.stream_ptr()
+        out.zero_()
+        for j in range(m + 1):
+            call('fea_axpy', out.numel(), (-1.0) ** j * math.comb(m + 1, j + 1), F64, hist[j], out, s)
+
+    def _auto_order(self, p, hist, max_order):
+        """The order whose extrapolation from hist[1:] misses hist[0] least (2-norm); one
+        higher when the best is the highest order hist can test (the miss still falls)."""
+        top = min(max_order, len(hist) - 2)
+        if not hasattr(self, '_pred_tmp') or self._pred_tmp.numel() != hist[0].numel():
+            self._pred_tmp = torch.empty_like(hist[0])
+        t = self._pred_tmp
+        miss = []
+        for m in range(top + 1):
+            self._extrapolate(hist[1:], m, t)
+            call('fea_axpy', t.numel(), -1.0, F64, hist[0], t, nat.stream_ptr())
+            miss.append(float(p.dot(t, t).item()))
+        best = int(np.argmin(miss))
+        if best == top and top + 1 <= min(max_order, len(hist) - 1):
+            best = top + 1
+        return best
+
     def _solve_iteration(self, GC, tol_error):
         """static_implicit.py:179-295, once per load increment."""
         p = self.assembly.plan
@@ -289,18 +340,25 @@ class StaticImplicitSolver(BaseSolver):
             zero = torch.zeros_like(U)
             rss, _ = self._vec_stats(p.residual(zero, 1.0, F64))
             self._r_ref = math.sqrt(rss) if rss > 0 else None
-        U_prev = U.clone() if (self.predictor and self.load_steps > 1) else None
-        step = torch.empty_like(U) if U_prev is not None else None
-        s = nat.stream_ptr()
+        # converged states of the previous load levels, newest first (u_0 = the start)
+        order = self.predictor
+        keep = order + 2 if self.predictor_auto else order + 1
+        hist = [U.clone()] if order and self.load_steps > 1 else []
+        self.stats['predictor_orders'] = []
         for k in range(1, self.load_steps + 1):
-            if U_prev is not None and k >= 2:
-                # predictor: U <- 2 U - U_prev (linear extrapolation), U_prev <- old U
-                step.copy_(U)
-                call('fea_axpy', U.numel(), -1.0, F64, U_prev, step, s)
-                U_prev.copy_(U)
-                call('fea_axpy', U.numel(), 1.0, F64, step, U, s)
+            if hist and k >= 2:
+                m = self._auto_order(p, hist, order) if self.predictor_auto else min(order, len(hist) - 1)
+                self._extrapolate(hist, m, U)
                 if hasattr(p, 'sync'):
                     p.sync(U)                      # coupled nodes back onto their reference points
+                if self.predictor_auto and m > 0:
+                    # keep the extrapolation only where its potential energy at the new load
+                    # is below the last converged state's
+                    scale = k / self.load_steps
+                    if not self._energy(U, scale) <= self._energy(hist[0], scale):
+                        m = 0
+                        U.copy_(hist[0])
+                self.stats['predictor_orders'].append(m)
             self._increment = k
             ok = self._newton(U, k / self.load_steps, tol_error)
             self.stats['increments'] = k
@@ -308,5 +366,8 @@ class StaticImplicitSolver(BaseSolver):
                 if self.verbose:
                     print('load increment %d failed to converge' % k)
                 return False
+            if hist:
+                hist.insert(0, U.clone())
+                del hist[keep:]
         self.stats['newton_iters'] = self._iter_now
         return p.u_to_gc(U)

@@ -2,6 +2,8 @@
 
 Test infrastructure only (see oracle/__init__.py).
 """
+from math import comb
+
 import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
@@ -141,21 +143,53 @@ def newton(model, GC, tol_error=1e-5, rel_tol=None, max_iter=200, linear='cg',
     return GC, it
 
 
-def static_solve(model, increments=1, predictor=True, GC0=None, **kw):
-    """Load-incremented static solve: Newton per increment, started from the linear
-    extrapolation of the two previous converged states (``predictor``)."""
+def extrapolate(hist, order):
+    """Equal-step Lagrange extrapolation of degree m = min(order, len(hist) - 1) through
+    the states hist[0..m] (newest first) to the next level: sum_j (-1)^j C(m+1, j+1) hist[j]."""
+    m = min(order, len(hist) - 1)
+    return sum((-1.0) ** j * comb(m + 1, j + 1) * hist[j] for j in range(m + 1))
+
+
+def auto_order(hist, max_order):
+    """The order m <= max_order whose extrapolation from hist[1:] misses hist[0] least in the
+    2-norm; one higher when that is the highest order hist can test."""
+    top = min(max_order, len(hist) - 2)
+    miss = [np.linalg.norm(extrapolate(hist[1:], m) - hist[0]) for m in range(top + 1)]
+    best = int(np.argmin(miss))
+    return best + 1 if best == top and top + 1 <= min(max_order, len(hist) - 1) else best
+
+
+def static_solve(model, increments=1, predictor=True, GC0=None, orders=None, **kw):
+    """Load-incremented static solve: Newton per increment, started from the equal-step
+    Lagrange extrapolation of the previous converged states (``predictor`` True / 'linear':
+    2 g1 - g2; 'quadratic': 3 g1 - 3 g2 + g3; 'cubic': 4 g1 - 6 g2 + 4 g3 - g4; an int:
+    that order; lower orders while fewer states exist; 'auto': ``auto_order`` up to 6 per
+    increment, replaced by g1 when the prediction's potential energy at the new load
+    exceeds g1's; False: from g1).  ``orders`` (a list) receives the order used per increment."""
+    order = predictor if isinstance(predictor, int) and not isinstance(predictor, bool) else \
+        {False: 0, True: 1, 'linear': 1, 'quadratic': 2, 'cubic': 3, 'auto': 6}[predictor]
+    keep = order + 2 if predictor == 'auto' else order + 1
     _, _, force_fn, n = _system(model)
     GC = np.zeros(n) if GC0 is None else np.asarray(GC0, dtype=np.float64).copy()
     total = 0
     kw.setdefault('r_ref', float(np.linalg.norm(force_fn(np.zeros(n)))))
-    prev = GC.copy()
+    hist = [GC.copy()]                     # converged states, newest first
     for k in range(1, increments + 1):
-        if predictor and k >= 2:
-            GC, prev = GC + (GC - prev), GC
+        if order and k >= 2:
+            m = auto_order(hist, order) if predictor == 'auto' else order
+            GC = extrapolate(hist, m)
+            if predictor == 'auto' and m > 0:
+                # safeguard: the last equilibrium when its energy at the new load is lower
+                e = _system(scale_loads(model, k / increments))[0]
+                if not e(GC) <= e(hist[0]):
+                    m, GC = 0, hist[0].copy()
+            if orders is not None:
+                orders.append(min(m, len(hist) - 1))
         GC, its = newton(scale_loads(model, k / increments), GC, **kw)
         total += its
         if GC is False:
             return False, total
+        hist = [GC.copy()] + hist[:keep - 1]
     return GC, total
 
 

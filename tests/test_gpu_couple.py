@@ -152,13 +152,15 @@ def _c3d4less(pressure):
     return fe, om, np.array(sorted(part.sets_nodes['surface_0_Head']))
 
 
-@pytest.mark.parametrize('prec', ['mixed'])
-def test_reference_couple_model_c3d4less(gpu, prec):
+@pytest.mark.parametrize('predictor', [False, 'auto'])
+def test_reference_couple_model_c3d4less(gpu, predictor, prec='mixed'):
     """tests/pressure_test/test_benchmark.py's model (C3D4Less.inp) at half its pressure (0.03;
-    the full 0.06 curls the actuator through 2.4 rad and is the benchmark's workload)."""
+    the full 0.06 curls the actuator through 2.4 rad and is the benchmark's workload); with
+    'auto', the second increment starts from an extrapolation or, when its energy is higher,
+    from the first increment's equilibrium."""
     fe, om, head = _c3d4less(0.03)
     fe.solver = solver.StaticImplicitSolver(precision=prec, rel_tol=1e-9, load_steps=2, cg_tol=1e-16,
-                                            cg_max_iter=20000, armijo_slack=1e-12, forcing='ew', predictor=False)
+                                            cg_max_iter=20000, armijo_slack=1e-12, forcing='ew', predictor=predictor)
     fe.initialize()
     a = fe.assembly
     assert a.GC.numel() == cp.n_gc(om)
@@ -173,5 +175,6 @@ def test_reference_couple_model_c3d4less(gpu, prec):
     d1 = np.linalg.norm(x[:, None] - x[None], axis=2)
     assert np.abs(d1 - d0).max() < 1e-10 * d0.max()
     assert np.abs(rp[0]).max() > 1e-2                                          # the head moved
-    print('C3D4Less: newton %d, cg %d, rp (v, z) %s' % (fe.solver.stats['newton_iters'],
-                                                       fe.solver.stats['cg_iters'], np.array2string(rp[0], precision=4)))
+    print('C3D4Less: newton %d, cg %d, orders %s, rp (v, z) %s' % (
+        fe.solver.stats['newton_iters'], fe.solver.stats['cg_iters'], fe.solver.stats['predictor_orders'],
+        np.array2string(rp[0], precision=4)))

@@ -84,6 +84,26 @@ def test_static_beam_mixed_matches_fp64(gpu):
     assert rel_err(res['mixed-f32'], GC_o) < 1e-8
 
 
+@pytest.mark.parametrize('predictor', ['quadratic', 'cubic', 5, 'auto', False])
+def test_static_predictor_orders_match_oracle(gpu, predictor):
+    """Higher-order load-increment predictors start each increment elsewhere but converge
+    to the oracle's equilibrium (the oracle runs the same extrapolation and, for 'auto',
+    picks the same order per increment)."""
+    fe, mesh = workloads.beam_traction(8, 2, 2, traction=(0.0, 0.0, -0.01))
+    fe.solver = solver.StaticImplicitSolver(precision='mixed', rel_tol=1e-10, load_steps=5, cg_tol=1e-12,
+                                            armijo_slack=1e-12, predictor=predictor)
+    GC = fe.solve()
+    assert GC is not False
+    om = oracle_model(mesh, mr(0.5, 0.1, 0.01), 'x0', traction=(0.0, 0.0, -0.01), load_set='x1')
+    orders = []
+    GC_o, its_o = osol.static_solve(om, increments=5, predictor=predictor, orders=orders, rel_tol=1e-10,
+                                    linear='direct', armijo_slack_rel=1e-12)
+    assert rel_err(GC.cpu().numpy(), GC_o) < 1e-8
+    assert fe.solver.stats['predictor_orders'] == orders
+    print('predictor %s: orders %s newton %d (oracle %d)' % (predictor, orders, fe.solver.stats['newton_iters'],
+                                                             its_o))
+
+
 def test_reference_convergence_rule(gpu):
     # default constructor: absolute max-norm test of static_implicit.py:290
     fe, om = cube_models(2, 0.05)

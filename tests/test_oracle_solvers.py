@@ -113,3 +113,53 @@ def test_consistent_mass_closed_form():
             for d in range(3):
                 ref8[3 * a + d, 3 * b + d] = 2.0 * h ** 3 * w
     assert np.abs(M8 - ref8).max() < 1e-14 * np.abs(ref8).max()
+
+
+def test_extrapolation_is_exact_on_polynomials():
+    # degree-m Lagrange extrapolation reproduces any polynomial of degree <= m in the load level
+    rng = np.random.default_rng(3)
+    for order in (1, 2, 3):
+        c = rng.standard_normal((order + 1, 5))
+        f = lambda t: sum(c[d] * t ** d for d in range(order + 1))
+        hist = [f(k) for k in (4.0, 3.0, 2.0, 1.0)]
+        assert np.allclose(osol.extrapolate(hist, order), f(5.0), rtol=1e-12, atol=1e-12)
+        # one degree higher is not reproduced
+        g = lambda t: f(t) + t ** (order + 1)
+        assert not np.allclose(osol.extrapolate([g(k) for k in (4.0, 3.0, 2.0, 1.0)], order), g(5.0))
+    # with fewer states the degree drops: one state -> constant
+    assert np.array_equal(osol.extrapolate([np.ones(3)], 3), np.ones(3))
+
+
+def test_predictor_orders_reach_the_same_equilibrium():
+    m = meshgen.kuhn_tet_block(2, 2, 2, jitter=0.05, seed=1)
+    om = oracle_model(m, nh(1.0, 20.0), 'z0', pressure=0.1, load_set='z1')
+    ref, _ = osol.static_solve(om, increments=4, predictor=False, rel_tol=1e-10, linear='direct',
+                               armijo_slack_rel=1e-12)
+    for pred in ('linear', 'quadratic', 'cubic'):
+        GC, _ = osol.static_solve(om, increments=4, predictor=pred, rel_tol=1e-10, linear='direct',
+                                  armijo_slack_rel=1e-12)
+        assert np.abs(GC - ref).max() <= 1e-8 * np.abs(ref).max()
+
+
+def test_auto_order_choice():
+    # a smooth (analytic) path: every extra order misses less, so the choice climbs past the
+    # highest testable order, up to the cap
+    t = 0.1 * np.arange(9.0)[::-1]
+    hist = [np.exp(t[j]) * np.ones(2) for j in range(9)]
+    assert [osol.auto_order(hist[:h], 6) for h in (2, 3, 4, 5)] == [1, 2, 3, 4]
+    assert osol.auto_order(hist, 6) == 6 and osol.auto_order(hist, 3) == 3
+    # a zig-zag path: extrapolation amplifies the alternation, the constant guess misses least
+    zig = [np.array([0.1 * j + (-1.0) ** j]) for j in range(6)][::-1]
+    assert osol.auto_order(zig, 6) == 0
+
+
+def test_auto_predictor_reaches_the_same_equilibrium():
+    m = meshgen.kuhn_tet_block(2, 2, 2, jitter=0.05, seed=1)
+    om = oracle_model(m, nh(1.0, 20.0), 'z0', pressure=0.1, load_set='z1')
+    ref, its0 = osol.static_solve(om, increments=5, predictor=False, rel_tol=1e-10, linear='direct',
+                                  armijo_slack_rel=1e-12)
+    orders = []
+    GC, its = osol.static_solve(om, increments=5, predictor='auto', orders=orders, rel_tol=1e-10,
+                                linear='direct', armijo_slack_rel=1e-12)
+    assert np.abs(GC - ref).max() <= 1e-8 * np.abs(ref).max()
+    assert len(orders) == 4 and orders[0] == 1 and its < its0
