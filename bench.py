@@ -59,6 +59,8 @@ def parse():
     ap.add_argument('--forcing', default='ew', choices=['ew', 'fixed'])
     ap.add_argument('--predictor', default='auto',
                     help="beam: load-increment predictor: none, linear, quadratic, cubic, auto or an order 0..6")
+    ap.add_argument('--preconditioner', default='block', choices=['block', 'jacobi'],
+                    help='beam: PCG preconditioner (node-block or the reference\'s point Jacobi)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--spmv-reps', type=int, default=200)
     ap.add_argument('--max-newton', type=int, default=400, help='Newton iterations per solve (all increments) before failing')
@@ -140,13 +142,13 @@ def _pred(args, name=None):
     return int(name) if name.isdigit() else name
 
 
-def make_solver(precision, args, load_steps=None, progress=True, predictor=True):
+def make_solver(precision, args, load_steps=None, progress=True, predictor=True, preconditioner='jacobi'):
     from fea_amd import solver
     return solver.StaticImplicitSolver(maximum_iteration=args.max_newton, precision=precision,
                                        rel_tol=args.rel_tol, load_steps=load_steps or args.increments,
                                        cg_tol=args.cg_tol, cg_max_iter=20000, armijo_slack=1e-12,
                                        progress=_progress if progress else None, forcing=args.forcing,
-                                       predictor=predictor)
+                                       predictor=predictor, preconditioner=preconditioner)
 
 
 def run_solve(fe, torch):
@@ -240,7 +242,7 @@ def bench_beam(args, torch):
     steps = 2 if args.steps is None else args.steps
     warmup = 1 if args.warmup is None else args.warmup
     fe, mesh = workloads.beam_traction(args.nx, args.ny, args.nz)
-    fe.solver = make_solver(args.precision, args, predictor=_pred(args))
+    fe.solver = make_solver(args.precision, args, predictor=_pred(args), preconditioner=args.preconditioner)
     fe.initialize()
     p = fe.assembly.plan
     log('model: %d elements, %d nodes, %d free DOF, nnz %d' % (mesh['conn'].shape[0], mesh['nodes'].shape[0],
@@ -265,7 +267,7 @@ def bench_beam(args, torch):
     fp64 = {}
     if args.precision != 'fp64' and not args.no_fp64_compare:
         fe64, _ = workloads.beam_traction(args.nx, args.ny, args.nz)
-        fe64.solver = make_solver('fp64', args, predictor=_pred(args))
+        fe64.solver = make_solver('fp64', args, predictor=_pred(args), preconditioner=args.preconditioner)
         fe64.initialize()
         GC64, t64 = run_solve(fe64, torch)
         err = float((GC - GC64).abs().max() / GC64.abs().max())
@@ -289,8 +291,8 @@ def bench_beam(args, torch):
                      {'workload': 'cantilever beam 4x1x1, %dx%dx%d C3D8, Mooney-Rivlin C10=0.5 C01=0.1 D1=0.01, '
                                   'clamped x=0, dead traction (0,0,-0.01) on x=4, %d increments, Newton rel tol %g'
                                   % (args.nx, args.ny, args.nz, args.increments, args.rel_tol),
-                      'precision': args.precision, 'predictor': args.predictor, 'free_dof': p.n_free,
-                      'nnz': p.nnz})
+                      'precision': args.precision, 'predictor': args.predictor,
+                      'preconditioner': args.preconditioner, 'free_dof': p.n_free, 'nnz': p.nnz})
     line.update({
         'latency_p50_s': float(np.percentile(lat, 50)), 'latency_p99_s': float(np.percentile(lat, 99)),
         'newton_iters_per_solve': stats.get('newton_iters'), 'cg_iters_per_solve': stats.get('cg_iters'),

@@ -47,10 +47,11 @@ _SIGS = {
     'fea_spmv': (C_INT, [C_INT, C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_jacobi_inv': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_P]),
     'fea_pcg_ws_size': (C_I64, [C_INT]),
-    'fea_pcg_start': (C_INT, [C_INT, C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P,
-                              C_P, C_P, C_P, C_P]),
-    'fea_pcg_iterate': (C_INT, [C_INT, C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P,
-                                C_P, C_P, C_P, C_DBL, C_INT, C_INT, C_P, C_P]),
+    'fea_pcg_start': (C_INT, [C_INT, C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_INT, C_P, C_P, C_P,
+                              C_P, C_P, C_P, C_P, C_P]),
+    'fea_pcg_iterate': (C_INT, [C_INT, C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_INT, C_P, C_P, C_P,
+                                C_P, C_P, C_P, C_P, C_DBL, C_INT, C_INT, C_P, C_P]),
+    'fea_block_jacobi_inv': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_pcg_spmv': (C_INT, [C_INT, C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_spmv_residual_mixed': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_vec_blocks': (C_INT, [C_INT]),

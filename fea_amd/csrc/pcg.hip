@@ -194,6 +194,279 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, const TV *__restrict_
   for (int64_t i = i0 + kPre * stride; i < n; i += stride) p[i] = z[i] + beta * p[i];
 }
 
+// ---------------------------------------------------------------- node-block Jacobi
+// M = blockdiag(K_gg) over the row groups g (the free rows of a node, nr <= 3): z = M^-1 r
+// per group with the symmetric inverse stored as 6 floats (00, 01, 02, 11, 12, 22) -- the
+// bytes of the FP64 diagonal it replaces.  Entries of rows / columns beyond nr are zero.
+__device__ __forceinline__ void bj_apply(const float *__restrict__ B, const double r[3], double z[3]) {
+  const double b00 = B[0], b01 = B[1], b02 = B[2], b11 = B[3], b12 = B[4], b22 = B[5];
+  z[0] = b00 * r[0] + b01 * r[1] + b02 * r[2];
+  z[1] = b01 * r[0] + b11 * r[1] + b12 * r[2];
+  z[2] = b02 * r[0] + b12 * r[1] + b22 * r[2];
+}
+
+// binv[g] = (K_gg)^-1 from the group's diagonal block: entry (a, b) of row r0 + a sits at
+// diag_slot[r0 + a] + b - a when the row's sorted columns hold r0 .. r0 + nr - 1 (checked;
+// absent entries are zero).  A block that is not positive definite falls back to the
+// point-Jacobi inverse 1 / |K_ii| of its rows.
+template <typename TM>
+__global__ __launch_bounds__(kBlock) void k_block_jacobi_inv(int ng, const int32_t *__restrict__ grp,
+                                                             const int32_t *__restrict__ rowptr,
+                                                             const int32_t *__restrict__ col,
+                                                             const int32_t *__restrict__ diag_slot,
+                                                             const TM *__restrict__ val, float *__restrict__ binv) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const int r0 = grp[g];
+  const int nr = grp[g + 1] - r0;
+  double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int a = 0; a < nr; ++a) {
+    const int d = diag_slot[r0 + a];
+    if (d < 0) continue;
+    for (int b = 0; b < nr; ++b) {
+      const int sl = d + b - a;
+      if (sl >= rowptr[r0 + a] && sl < rowptr[r0 + a + 1] && col[sl] == r0 + b) A[a][b] = (double)val[sl];
+    }
+  }
+  // symmetric part (the tangent is symmetric; rounding of a float matrix is not)
+  for (int a = 0; a < 3; ++a)
+    for (int b = a + 1; b < 3; ++b) A[a][b] = A[b][a] = 0.5 * (A[a][b] + A[b][a]);
+  for (int a = nr; a < 3; ++a) A[a][a] = 1.0;  // pad: identity on absent rows
+  const double c00 = A[1][1] * A[2][2] - A[1][2] * A[1][2];
+  const double c01 = A[0][2] * A[1][2] - A[0][1] * A[2][2];
+  const double c02 = A[0][1] * A[1][2] - A[0][2] * A[1][1];
+  const double c11 = A[0][0] * A[2][2] - A[0][2] * A[0][2];
+  const double c12 = A[0][1] * A[0][2] - A[0][0] * A[1][2];
+  const double c22 = A[0][0] * A[1][1] - A[0][1] * A[0][1];
+  const double det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+  // positive definite: leading minors A00 > 0, c22 > 0, det > 0
+  const bool spd = A[0][0] > 0.0 && c22 > 0.0 && det > 0.0 && det == det;
+  float *B = binv + 6 * (int64_t)g;
+  double o[6];
+  if (spd) {
+    const double id = 1.0 / det;
+    o[0] = c00 * id; o[1] = c01 * id; o[2] = c02 * id; o[3] = c11 * id; o[4] = c12 * id; o[5] = c22 * id;
+  } else {
+    for (int k = 0; k < 6; ++k) o[k] = 0.0;
+    const double d0 = fabs(A[0][0]), d1 = fabs(A[1][1]), d2 = fabs(A[2][2]);
+    o[0] = d0 == 0.0 ? 1.0 : 1.0 / d0;
+    o[3] = d1 == 0.0 ? 1.0 : 1.0 / d1;
+    o[5] = d2 == 0.0 ? 1.0 : 1.0 / d2;
+  }
+  // absent rows / columns exactly zero
+  if (nr < 3) { o[2] = o[4] = o[5] = 0.0; }
+  if (nr < 2) { o[1] = o[3] = 0.0; }
+  for (int k = 0; k < 6; ++k) B[k] = (float)o[k];
+}
+
+template <typename TV>
+__device__ __forceinline__ void bj_load(const TV *__restrict__ v, int r0, int nr, double o[3]) {
+  o[0] = (double)v[r0];
+  o[1] = nr > 1 ? (double)v[r0 + 1] : 0.0;
+  o[2] = nr > 2 ? (double)v[r0 + 2] : 0.0;
+}
+
+// block-Jacobi start stage: z = M^-1 r, p = z; partials of r.z (ring slot 0) and b.(M^-1 b)
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_pcg_init_bj(int ng, const int32_t *__restrict__ grp,
+                                                        const float *__restrict__ binv, const TV *__restrict__ b,
+                                                        const TV *__restrict__ r, TV *__restrict__ z,
+                                                        TV *__restrict__ p, double *__restrict__ ws) {
+  double rz = 0.0, bz = 0.0;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += (int64_t)gridDim.x * blockDim.x) {
+    const int r0 = grp[g], nr = grp[g + 1] - r0;
+    double rr[3], bb[3], zz[3], zb[3];
+    bj_load(r, r0, nr, rr);
+    bj_load(b, r0, nr, bb);
+    bj_apply(binv + 6 * g, rr, zz);
+    bj_apply(binv + 6 * g, bb, zb);
+    for (int a = 0; a < nr; ++a) {
+      const TV zi = (TV)zz[a];
+      z[r0 + a] = zi;
+      p[r0 + a] = zi;
+      rz += rr[a] * (double)zi;
+      bz += bb[a] * zb[a];
+    }
+  }
+  write_partial(ws + W_RZP, rz);
+  write_partial(ws + W_BZP, bz);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ws[W_ITER] = 0.0;
+    ws[W_DONE] = 0.0;
+  }
+}
+
+// block-Jacobi update stage: as k_pcg_update with z = M^-1 r per row group.  With the
+// vector grid (~one group per thread on a large system) each thread loads its first
+// group's operands before the flag test and the partial sums, as k_pcg_update does.
+template <typename TV>
+__device__ __forceinline__ double bj_update_group(int r0, int nr, const float *__restrict__ B, TV alpha,
+                                                  const TV vx[3], const TV vp[3], const TV vr[3], const TV vq[3],
+                                                  TV *__restrict__ x, TV *__restrict__ r, TV *__restrict__ z) {
+  double rr[3] = {0.0, 0.0, 0.0}, zz[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (a < nr) {
+      x[r0 + a] = vx[a] + alpha * vp[a];
+      const TV ri = vr[a] - alpha * vq[a];
+      r[r0 + a] = ri;
+      rr[a] = (double)ri;
+    }
+  }
+  bj_apply(B, rr, zz);
+  double rz = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (a < nr) {
+      const TV zi = (TV)zz[a];
+      z[r0 + a] = zi;
+      rz += rr[a] * (double)zi;
+    }
+  }
+  return rz;
+}
+
+template <typename TV>
+__device__ __forceinline__ void bj_fetch(int64_t g, const int32_t *__restrict__ grp, const float *__restrict__ binv,
+                                         const TV *__restrict__ x, const TV *__restrict__ r,
+                                         const TV *__restrict__ p, const TV *__restrict__ q, int &r0, int &nr,
+                                         float B[6], TV vx[3], TV vp[3], TV vr[3], TV vq[3]) {
+  r0 = grp[g];
+  nr = grp[g + 1] - r0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (a < nr) {
+      vx[a] = x[r0 + a];
+      vp[a] = p[r0 + a];
+      vr[a] = r[r0 + a];
+      vq[a] = q[r0 + a];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) B[k] = binv[6 * g + k];
+}
+
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_pcg_update_bj(int ng, const int32_t *__restrict__ grp,
+                                                          const float *__restrict__ binv, TV *__restrict__ x,
+                                                          TV *__restrict__ r, TV *__restrict__ z,
+                                                          const TV *__restrict__ p, const TV *__restrict__ q,
+                                                          double *__restrict__ ws) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t g0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int r0 = 0, nr = 0;
+  float B[6];
+  TV vx[3], vp[3], vr[3], vq[3];
+  if (g0 < ng) bj_fetch(g0, grp, binv, x, r, p, q, r0, nr, B, vx, vp, vr, vq);
+  if (ws[W_DONE] != 0.0) return;
+  const int it = (int)ws[W_ITER];
+  const int g1 = (int)ws[W_G1];
+  const double rz_old = it > 1 ? ws[W_RZH + ((it - 1) & 1)]
+                               : sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_RING, gridDim.x);
+  const double pq = sum_partials(ws + W_PQ, g1);
+  const TV alpha = (TV)(pq != 0.0 ? rz_old / pq : 0.0);
+  double rz = 0.0;
+  if (g0 < ng) rz += bj_update_group(r0, nr, B, alpha, vx, vp, vr, vq, x, r, z);
+  for (int64_t g = g0 + stride; g < ng; g += stride) {
+    bj_fetch(g, grp, binv, x, r, p, q, r0, nr, B, vx, vp, vr, vq);
+    rz += bj_update_group(r0, nr, B, alpha, vx, vp, vr, vq, x, r, z);
+  }
+  write_partial(ws + W_RZP + (it & 1) * PCG_RING, rz);
+}
+
+// block-Jacobi update stage for triple-structured systems (every row group 3 rows: row i is
+// row i % 3 of group i / 3): one thread per row, coalesced like k_pcg_update, a block
+// stepping over RB = 255 rows (whole groups); the three new residuals of a group meet in
+// LDS for z = M^-1 r.  The first kPre steps are loaded before the flag and the partial sums.
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_pcg_update_bj3(int n, const float *__restrict__ binv,
+                                                           TV *__restrict__ x, TV *__restrict__ r,
+                                                           TV *__restrict__ z, const TV *__restrict__ p,
+                                                           const TV *__restrict__ q, double *__restrict__ ws) {
+  constexpr int RB = kBlock - kBlock % 3;
+  __shared__ double sr[kPre][kBlock];
+  const int t = threadIdx.x;
+  const int a = t % 3;                      // row of the group (RB and the block origin are multiples of 3)
+  // packed symmetric inverse (00, 01, 02, 11, 12, 22): row a is (ia0, ia1, ia2)
+  const int ia0 = a;
+  const int ia1 = a == 0 ? 1 : (a == 1 ? 3 : 4);
+  const int ia2 = a == 0 ? 2 : (a == 1 ? 4 : 5);
+  const bool lane = t < RB;
+  const int64_t step = (int64_t)gridDim.x * RB;
+  const int64_t i0 = (int64_t)blockIdx.x * RB + t;
+  TV vx[kPre], vp[kPre], vr[kPre], vq[kPre];
+  float b0[kPre], b1[kPre], b2[kPre];
+#pragma unroll
+  for (int k = 0; k < kPre; ++k) {
+    const int64_t i = i0 + k * step;
+    if (lane && i < n) {
+      vx[k] = x[i];
+      vp[k] = p[i];
+      vr[k] = r[i];
+      vq[k] = q[i];
+      const float *B = binv + 6 * (i / 3);
+      b0[k] = B[ia0];
+      b1[k] = B[ia1];
+      b2[k] = B[ia2];
+    }
+  }
+  if (ws[W_DONE] != 0.0) return;
+  const int it = (int)ws[W_ITER];
+  const int g1 = (int)ws[W_G1];
+  const double rz_old = it > 1 ? ws[W_RZH + ((it - 1) & 1)]
+                               : sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_RING, gridDim.x);
+  const double pq = sum_partials(ws + W_PQ, g1);
+  const TV alpha = (TV)(pq != 0.0 ? rz_old / pq : 0.0);
+  double rz = 0.0;
+#pragma unroll
+  for (int k = 0; k < kPre; ++k) {
+    const int64_t i = i0 + k * step;
+    if (lane && i < n) {
+      x[i] = vx[k] + alpha * vp[k];
+      vr[k] = vr[k] - alpha * vq[k];
+      r[i] = vr[k];
+      sr[k][t] = (double)vr[k];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPre; ++k) {
+    const int64_t i = i0 + k * step;
+    if (lane && i < n) {
+      const int tb = t - a;
+      const TV zi = (TV)((double)b0[k] * sr[k][tb] + (double)b1[k] * sr[k][tb + 1] + (double)b2[k] * sr[k][tb + 2]);
+      z[i] = zi;
+      rz += (double)vr[k] * (double)zi;
+    }
+  }
+  // rows beyond the prefetched steps (systems above kPre * gridDim.x * RB rows)
+  for (int64_t base = (int64_t)kPre * step; base < n; base += step) {
+    __syncthreads();                        // sr[0] of the previous step has been read
+    const int64_t i = base + (int64_t)blockIdx.x * RB + t;
+    const bool on = lane && i < n;
+    TV ri = TV(0);
+    float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+    if (on) {
+      x[i] += alpha * p[i];
+      ri = r[i] - alpha * q[i];
+      r[i] = ri;
+      sr[0][t] = (double)ri;
+      const float *B = binv + 6 * (i / 3);
+      c0 = B[ia0];
+      c1 = B[ia1];
+      c2 = B[ia2];
+    }
+    __syncthreads();
+    if (on) {
+      const int tb = t - a;
+      const TV zi = (TV)((double)c0 * sr[0][tb] + (double)c1 * sr[0][tb + 1] + (double)c2 * sr[0][tb + 2]);
+      z[i] = zi;
+      rz += (double)ri * (double)zi;
+    }
+  }
+  write_partial(ws + W_RZP + (it & 1) * PCG_RING, rz);
+}
+
 template <typename TM, typename TV>
 __global__ __launch_bounds__(kBlock) void k_jacobi_inv(int n, const int32_t *__restrict__ diag_slot,
                                                        const TM *__restrict__ val, TV *__restrict__ dinv) {
@@ -274,6 +547,7 @@ struct PcgArgs {
   const int32_t *grp, *rowptr, *col;
   const void *val, *dinv, *b;
   void *x, *r, *z, *p, *q;
+  int precond;  // 0: point Jacobi (dinv: vec_dtype[n]); 1: node-block Jacobi (dinv: float[6 n_groups])
   double *ws;
   double tol;
   int max_iter, iters;
@@ -286,8 +560,12 @@ static int pcg_start_t(const PcgArgs &a) {
   const SpmvShape s = shape_of(a.n, a.nnz, a.n_groups, a.grp);
   launch_spmv<TM, TV>(s, a.grp, a.rowptr, a.col, (const TM *)a.val, (const TV *)a.x, (const TV *)a.b,
                       (TV *)a.r, a.st);
-  k_pcg_init<TV><<<vec_grid(), kBlock, 0, a.st>>>(a.n, (const TV *)a.dinv, (const TV *)a.b, (const TV *)a.r,
-                                            (TV *)a.z, (TV *)a.p, a.ws);
+  if (a.precond == 1)
+    k_pcg_init_bj<TV><<<vec_grid(), kBlock, 0, a.st>>>(s.ngroups, a.grp, (const float *)a.dinv, (const TV *)a.b,
+                                                       (const TV *)a.r, (TV *)a.z, (TV *)a.p, a.ws);
+  else
+    k_pcg_init<TV><<<vec_grid(), kBlock, 0, a.st>>>(a.n, (const TV *)a.dinv, (const TV *)a.b, (const TV *)a.r,
+                                                    (TV *)a.z, (TV *)a.p, a.ws);
   FEA_CHECK_LAUNCH("fea_pcg_start");
   return FEA_OK;
 }
@@ -299,8 +577,15 @@ static int pcg_iterate_t(const PcgArgs &a) {
   for (int it = 0; it < a.iters; ++it) {
     launch_pcg_spmv<TM, TV>(s, a.grp, a.rowptr, a.col, (const TM *)a.val, (const TV *)a.p, (TV *)a.q,
                             a.ws, a.st);
-    k_pcg_update<TV><<<gu, kBlock, 0, a.st>>>(a.n, (const TV *)a.dinv, (TV *)a.x, (TV *)a.r, (TV *)a.z,
-                                             (const TV *)a.p, (const TV *)a.q, a.ws);
+    if (a.precond == 1 && s.mode == 2)
+      k_pcg_update_bj3<TV><<<gu, kBlock, 0, a.st>>>(a.n, (const float *)a.dinv, (TV *)a.x, (TV *)a.r, (TV *)a.z,
+                                                   (const TV *)a.p, (const TV *)a.q, a.ws);
+    else if (a.precond == 1)
+      k_pcg_update_bj<TV><<<gu, kBlock, 0, a.st>>>(s.ngroups, a.grp, (const float *)a.dinv, (TV *)a.x, (TV *)a.r,
+                                                  (TV *)a.z, (const TV *)a.p, (const TV *)a.q, a.ws);
+    else
+      k_pcg_update<TV><<<gu, kBlock, 0, a.st>>>(a.n, (const TV *)a.dinv, (TV *)a.x, (TV *)a.r, (TV *)a.z,
+                                               (const TV *)a.p, (const TV *)a.q, a.ws);
     k_pcg_dir<TV><<<PCG_G, kBlock, 0, a.st>>>(a.n, (const TV *)a.z, (TV *)a.p, a.ws, a.tol, a.max_iter, gu);
   }
   FEA_CHECK_LAUNCH("fea_pcg_iterate");
@@ -346,6 +631,24 @@ extern "C" int fea_spmv(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_
   return FEA_OK;
 }
 
+extern "C" int fea_block_jacobi_inv(int mat_dtype, int n, int n_groups, const int32_t *grp, const int32_t *rowptr,
+                                    const int32_t *col, const int32_t *diag_slot, const void *val, float *binv,
+                                    void *stream) {
+  FEA_CHECK_ARG(mat_dtype == FEA_F64 || mat_dtype == FEA_F32, "fea_block_jacobi_inv: bad matrix dtype");
+  const int ng = n_groups < 0 ? -n_groups : n_groups;
+  FEA_CHECK_ARG(n >= 0 && ng >= 0, "fea_block_jacobi_inv: bad sizes");
+  if (ng == 0) return FEA_OK;
+  FEA_CHECK_ARG(grp && rowptr && col && diag_slot && val && binv, "fea_block_jacobi_inv: null argument");
+  hipStream_t st = as_stream(stream);
+  const int g = blocks_for(ng);
+  if (mat_dtype == FEA_F64)
+    k_block_jacobi_inv<double><<<g, kBlock, 0, st>>>(ng, grp, rowptr, col, diag_slot, (const double *)val, binv);
+  else
+    k_block_jacobi_inv<float><<<g, kBlock, 0, st>>>(ng, grp, rowptr, col, diag_slot, (const float *)val, binv);
+  FEA_CHECK_LAUNCH("fea_block_jacobi_inv");
+  return FEA_OK;
+}
+
 extern "C" int fea_jacobi_inv(int mat_dtype, int vec_dtype, int n, const int32_t *diag_slot, const void *val,
                               void *dinv, void *stream) {
   FEA_CHECK_ARG(valid_combo(mat_dtype, vec_dtype), "fea_jacobi_inv: unsupported precision pair");
@@ -366,27 +669,31 @@ extern "C" int fea_jacobi_inv(int mat_dtype, int vec_dtype, int n, const int32_t
 
 extern "C" int fea_pcg_start(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups,
                              const int32_t *grp, const int32_t *rowptr, const int32_t *col, const void *val,
-                             const void *dinv, const void *b, void *x, void *r, void *z, void *p, double *ws,
-                             void *stream) {
+                             int precond, const void *dinv, const void *b, void *x, void *r, void *z, void *p,
+                             double *ws, void *stream) {
   FEA_CHECK_ARG(valid_combo(mat_dtype, vec_dtype), "fea_pcg_start: unsupported precision pair");
   FEA_CHECK_ARG(n > 0 && rowptr && col && val && dinv && b && x && r && z && p && ws,
                 "fea_pcg_start: bad arguments");
-  PcgArgs a{n, nnz, n_groups, grp, rowptr, col, val, dinv, b, x, r, z, p, nullptr, ws, 0.0, 0, 0, nullptr,
-            as_stream(stream)};
+  FEA_CHECK_ARG(precond == 0 || (precond == 1 && grp != nullptr && n_groups != 0),
+                "fea_pcg_start: block Jacobi needs the row groups");
+  PcgArgs a{n, nnz, n_groups, grp, rowptr, col, val, dinv, b, x, r, z, p, nullptr, precond, ws, 0.0, 0, 0,
+            nullptr, as_stream(stream)};
   return FEA_DISPATCH(mat_dtype, vec_dtype, pcg_start_t, a);
 }
 
 extern "C" int fea_pcg_iterate(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups,
                                const int32_t *grp, const int32_t *rowptr, const int32_t *col,
-                               const void *val, const void *dinv, void *x, void *r, void *z, void *p, void *q,
-                               double *ws, double tol, int max_iter, int iters, double *status_host,
-                               void *stream) {
+                               const void *val, int precond, const void *dinv, void *x, void *r, void *z,
+                               void *p, void *q, double *ws, double tol, int max_iter, int iters,
+                               double *status_host, void *stream) {
   FEA_CHECK_ARG(valid_combo(mat_dtype, vec_dtype), "fea_pcg_iterate: unsupported precision pair");
   FEA_CHECK_ARG(n > 0 && iters >= 0 && max_iter >= 1 && rowptr && col && val && dinv && x && r && z && p &&
                     q && ws,
                 "fea_pcg_iterate: bad arguments");
-  PcgArgs a{n, nnz, n_groups, grp, rowptr, col, val, dinv, nullptr, x, r, z, p, q, ws, tol, max_iter, iters,
-            status_host, as_stream(stream)};
+  FEA_CHECK_ARG(precond == 0 || (precond == 1 && grp != nullptr && n_groups != 0),
+                "fea_pcg_iterate: block Jacobi needs the row groups");
+  PcgArgs a{n, nnz, n_groups, grp, rowptr, col, val, dinv, nullptr, x, r, z, p, q, precond, ws, tol, max_iter,
+            iters, status_host, as_stream(stream)};
   return FEA_DISPATCH(mat_dtype, vec_dtype, pcg_iterate_t, a);
 }
 

@@ -27,10 +27,16 @@ class PCG:
 
     ``mat_dtype`` is the precision of the matrix values, ``vec_dtype`` that of
     the vectors and recurrences; ``grp`` (optional) the node row groups.
+    ``precond='block'`` replaces the point Jacobi M = |diag K| by the node-block
+    Jacobi M = blockdiag(K_gg) over the row groups (an addition: other iterates,
+    the same solution; needs ``grp``, else point Jacobi is used).
     """
 
     def __init__(self, n, nnz, rowptr, col, diag_slot, mat_dtype, device, vec_dtype=None, grp=None,
-                 n_groups=0, chunk=32):
+                 n_groups=0, chunk=32, precond='jacobi'):
+        if precond not in ('jacobi', 'block'):
+            raise ValueError("precond must be 'jacobi' or 'block'")
+        self.block = precond == 'block' and grp is not None and int(n_groups) != 0
         self.n, self.nnz, self.chunk = int(n), int(nnz), int(chunk)
         self.mat_dtype = mat_dtype
         self.vec_dtype = mat_dtype if vec_dtype is None else vec_dtype
@@ -38,7 +44,11 @@ class PCG:
         self.grp, self.n_groups = grp, int(n_groups)
         T = _DT[self.vec_dtype]
         mk = lambda: torch.zeros(max(1, self.n), dtype=T, device=device)[:self.n]
-        self.x, self.r, self.z, self.p, self.q, self.dinv, self.b = mk(), mk(), mk(), mk(), mk(), mk(), mk()
+        self.x, self.r, self.z, self.p, self.q, self.b = mk(), mk(), mk(), mk(), mk(), mk()
+        if self.block:
+            self.dinv = torch.zeros(6 * abs(self.n_groups), dtype=torch.float32, device=device)
+        else:
+            self.dinv = mk()
         self.ws = torch.zeros(int(nat.lib().fea_pcg_ws_size(self.n)), dtype=torch.float64, device=device)
         self.status = np.zeros(4, dtype=np.float64)
         self.iterations = 0
@@ -49,8 +59,12 @@ class PCG:
 
     def set_matrix(self, val):
         self.val = val
-        call('fea_jacobi_inv', self.mat_dtype, self.vec_dtype, self.n, self.diag_slot, val, self.dinv,
-             nat.stream_ptr())
+        if self.block:
+            call('fea_block_jacobi_inv', self.mat_dtype, self.n, self.n_groups, self.grp, self.rowptr, self.col,
+                 self.diag_slot, val, self.dinv, nat.stream_ptr())
+        else:
+            call('fea_jacobi_inv', self.mat_dtype, self.vec_dtype, self.n, self.diag_slot, val, self.dinv,
+                 nat.stream_ptr())
 
     def solve(self, b, x0=None, tol=1e-5, max_iter=1500):
         """Solve K x = b; returns x (vector precision)."""
@@ -62,13 +76,14 @@ class PCG:
             self.x.zero_()
         else:
             self.x.copy_(x0)
+        pc = 1 if self.block else 0
         call('fea_pcg_start', self.mat_dtype, self.vec_dtype, self.n, self.nnz, self.n_groups, self.grp,
-             self.rowptr, self.col, self.val, self.dinv, b, self.x, self.r, self.z, self.p, self.ws, s)
+             self.rowptr, self.col, self.val, pc, self.dinv, b, self.x, self.r, self.z, self.p, self.ws, s)
         done_iters = 0
         while True:
             step = min(self.chunk, max(1, max_iter - done_iters))
             call('fea_pcg_iterate', self.mat_dtype, self.vec_dtype, self.n, self.nnz, self.n_groups, self.grp,
-                 self.rowptr, self.col, self.val, self.dinv, self.x, self.r, self.z, self.p, self.q, self.ws,
+                 self.rowptr, self.col, self.val, pc, self.dinv, self.x, self.r, self.z, self.p, self.q, self.ws,
                  float(tol), int(max_iter), int(step), self.status, s)
             done_iters += step
             if self.status[1] != 0.0 or done_iters >= max_iter:

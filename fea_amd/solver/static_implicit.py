@@ -39,6 +39,8 @@ Additions (keyword-only, defaults reproduce the reference):
   (choice 2, gamma = 0.9, alpha = 2 with the safeguard gamma eta_{k-1}^2)
   for the PCG relative tolerance eta_k, clamped to [sqrt(cg_tol), eta_max];
   the default 'fixed' keeps the reference's constant cg_tol.
+* ``preconditioner='block'``: node-block (3x3) Jacobi instead of the reference's point
+  Jacobi scaling (other CG iterates, the same Newton solution).
 """
 from __future__ import annotations
 
@@ -77,8 +79,11 @@ class StaticImplicitSolver(BaseSolver):
                  cg_tol: float = 1e-5, cg_max_iter: int = 1500, ir_steps: int = 1,
                  armijo_slack: float = 0.0, predictor=True, verbose: bool = False,
                  pcg_chunk: int = 32, progress=None, step_tol: float = 1e-12, forcing: str = 'fixed',
-                 eta_max: float = 1e-2) -> None:
+                 eta_max: float = 1e-2, preconditioner: str = 'jacobi') -> None:
         super().__init__()
+        if preconditioner not in ('jacobi', 'block'):
+            raise ValueError("preconditioner must be 'jacobi' or 'block'")
+        self.preconditioner = preconditioner
         if precision not in ('fp64', 'mixed', 'mixed-f32', 'mixed-ir'):
             raise ValueError("precision must be 'fp64', 'mixed', 'mixed-f32' or 'mixed-ir'")
         self.maximum_iteration = maximum_iteration
@@ -144,7 +149,8 @@ class StaticImplicitSolver(BaseSolver):
             self._pcg = {k: v for k, v in self._pcg.items() if k[2] == key[2]}
             p = self.assembly.plan
             self._pcg[key] = PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, mat_dtype, p.device,
-                                 vec_dtype=vec_dtype, grp=p.grp, n_groups=p.spmv_groups, chunk=self.pcg_chunk)
+                                 vec_dtype=vec_dtype, grp=p.grp, n_groups=p.spmv_groups, chunk=self.pcg_chunk,
+                                 precond=self.preconditioner)
         return self._pcg[key]
 
     def _vec_stats(self, x):

@@ -196,6 +196,13 @@ int fea_reduce_sum(int n, const double *partial, double *out, void *stream);
 int fea_spmv(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
              const int32_t *rowptr, const int32_t *col, const void *val, const void *x, void *y,
              void *stream);
+/* Node-block Jacobi (an addition to the reference's point Jacobi): binv[6 g .. 6 g + 5]
+ * = the symmetric inverse (00, 01, 02, 11, 12, 22) of the diagonal block of row group g
+ * (the free rows of one node, consecutive columns), in float; a block that is not positive
+ * definite falls back to the point-Jacobi inverse of its rows. */
+int fea_block_jacobi_inv(int mat_dtype, int n, int n_groups, const int32_t *grp, const int32_t *rowptr,
+                         const int32_t *col, const int32_t *diag_slot, const void *val, float *binv,
+                         void *stream);
 /* Jacobi diagonal of static_implicit.py:314-317: dinv[r] = 1/|A_rr| (1 if 0). */
 int fea_jacobi_inv(int mat_dtype, int vec_dtype, int n, const int32_t *diag_slot, const void *val,
                    void *dinv, void *stream);
@@ -207,15 +214,20 @@ int fea_jacobi_inv(int mat_dtype, int vec_dtype, int n, const int32_t *diag_slot
  * r, z, p, q of length n.  fea_pcg_start sets r = b - A x, z = dinv r, p = z;
  * fea_pcg_iterate enqueues `iters` iterations (each a no-op once converged:
  * r.z / b.(dinv b) < tol, or max_iter reached) and, if status_host != NULL,
- * copies {iterations, done, r.z, b.(dinv b)} to it (synchronises the stream). */
+ * copies {iterations, done, r.z, b.(dinv b)} to it (synchronises the stream).
+ * precond 0: point Jacobi, dinv = vec_dtype[n] (fea_jacobi_inv; the reference's
+ * scaling).  precond 1: node-block Jacobi over the row groups (grp required), dinv =
+ * float[6 * |n_groups|] from fea_block_jacobi_inv; an addition that changes the
+ * iterates, not the solution. */
 int64_t fea_pcg_ws_size(int n);
 int fea_pcg_start(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
-                  const int32_t *rowptr, const int32_t *col, const void *val, const void *dinv,
-                  const void *b, void *x, void *r, void *z, void *p, double *ws, void *stream);
+                  const int32_t *rowptr, const int32_t *col, const void *val, int precond,
+                  const void *dinv, const void *b, void *x, void *r, void *z, void *p, double *ws,
+                  void *stream);
 int fea_pcg_iterate(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups,
                     const int32_t *grp, const int32_t *rowptr, const int32_t *col, const void *val,
-                    const void *dinv, void *x, void *r, void *z, void *p, void *q, double *ws,
-                    double tol, int max_iter, int iters, double *status_host, void *stream);
+                    int precond, const void *dinv, void *x, void *r, void *z, void *p, void *q,
+                    double *ws, double tol, int max_iter, int iters, double *status_host, void *stream);
 /* One SpMV stage of the PCG iteration alone: q = A p and the p.q partials
  * (the CSR `A @ p` of _linear_solver.py:32 fused with the dot of :33); a no-op
  * when ws marks the solve converged. */

@@ -32,6 +32,72 @@ def conjugate_gradient(A, b, x0=None, tol=1e-3, max_iter=1500):
     return x, it + 1
 
 
+def block_jacobi_inverse(A, grp):
+    """Node-block Jacobi blocks (an addition to the reference's point Jacobi): for each row
+    group g (rows grp[g] .. grp[g+1]-1) the inverse of the symmetric part of its diagonal
+    block by cofactors, rounded to float32 as stored; a block that is not positive definite
+    (leading minors) falls back to 1 / |A_ii|.  Returns [G, 3, 3] (zero outside nr x nr)."""
+    A = sp.csr_matrix(A)
+    out = np.zeros((len(grp) - 1, 3, 3))
+    for g in range(len(grp) - 1):
+        r0, nr = grp[g], grp[g + 1] - grp[g]
+        B = np.eye(3)
+        B[:nr, :nr] = A[r0:r0 + nr, r0:r0 + nr].toarray()
+        B = 0.5 * (B + B.T)
+        c = np.array([[B[1, 1] * B[2, 2] - B[1, 2] ** 2, B[0, 2] * B[1, 2] - B[0, 1] * B[2, 2],
+                       B[0, 1] * B[1, 2] - B[0, 2] * B[1, 1]],
+                      [0.0, B[0, 0] * B[2, 2] - B[0, 2] ** 2, B[0, 1] * B[0, 2] - B[0, 0] * B[1, 2]],
+                      [0.0, 0.0, B[0, 0] * B[1, 1] - B[0, 1] ** 2]])
+        c[1, 0], c[2, 0], c[2, 1] = c[0, 1], c[0, 2], c[1, 2]
+        det = B[0, 0] * c[0, 0] + B[0, 1] * c[0, 1] + B[0, 2] * c[0, 2]
+        if B[0, 0] > 0 and c[2, 2] > 0 and det > 0:
+            inv = c / det
+        else:
+            d = np.abs(np.diag(B))
+            inv = np.diag(np.where(d == 0, 1.0, 1.0 / np.where(d == 0, 1.0, d)))
+        inv[nr:, :] = 0.0
+        inv[:, nr:] = 0.0
+        out[g] = inv.astype(np.float32).astype(np.float64)
+    return out
+
+
+def pcg_block_jacobi(A, b, grp, x0=None, tol=1e-5, max_iter=1500):
+    """Preconditioned CG with M^-1 = blockdiag(block_jacobi_inverse): tol on r.z / b.(M^-1 b)."""
+    Minv = block_jacobi_inverse(A, grp)
+    rows = np.concatenate([np.arange(grp[g], grp[g] + 3) for g in range(len(grp) - 1)])
+    n = b.size
+
+    def apply(v):
+        vv = np.zeros(rows.size)
+        ok = rows < n
+        vv[ok] = v[rows[ok]]
+        zz = np.einsum('gab,gb->ga', Minv, vv.reshape(-1, 3)).reshape(-1)
+        z = np.zeros(n)
+        for g in range(len(grp) - 1):
+            nr = grp[g + 1] - grp[g]
+            z[grp[g]:grp[g] + nr] = zz[3 * g:3 * g + nr]
+        return z
+    x = np.zeros_like(b) if x0 is None else x0.copy()
+    bz = b @ apply(b)
+    r = b - A @ x
+    z = apply(r)
+    p = z.copy()
+    rz = r @ z
+    it = 0
+    for it in range(max_iter):
+        q = A @ p
+        alpha = rz / (p @ q)
+        x = x + alpha * p
+        r = r - alpha * q
+        z = apply(r)
+        rz_new = r @ z
+        if rz_new / bz < tol:
+            break
+        p = z + rz_new / rz * p
+        rz = rz_new
+    return x, it + 1
+
+
 def jacobi_scaled_solve(K, rhs, x0, linear='cg', tol=1e-5, max_iter=1500):
     """_solve_linear_equation (static_implicit.py:297-350).
 

@@ -9,7 +9,7 @@ import scipy.sparse.linalg as spla
 import torch
 
 from fea_amd import solver, workloads
-from fea_amd._native import F64
+from fea_amd._native import F32, F64
 from oracle import assembly as oasm
 from oracle import solvers as osol
 from tests.helpers import mr, nh, oracle_model, rel_err
@@ -46,6 +46,50 @@ def test_pcg_matches_direct_solve(gpu):
     _, its = osol.conjugate_gradient(As, b / D, np.zeros_like(b), tol=1e-10, max_iter=5000)
     pcg.solve(-R, None, tol=1e-10, max_iter=5000)
     assert abs(pcg.iterations - its) <= max(3, 0.05 * its)
+
+
+@pytest.mark.parametrize('mode', ['triple', 'node'])
+@pytest.mark.parametrize('prec', ['fp64', 'mixed'])
+def test_block_jacobi_pcg_matches_oracle(gpu, prec, mode):
+    """Node-block Jacobi PCG (fea_block_jacobi_inv + precond 1) against the oracle's
+    restatement after fixed iteration counts, and converged against a direct solve; with the
+    row-per-thread stage of triple-structured systems and the group-per-thread one."""
+    fe, om = cube_models()
+    fe.assembly.initialize()
+    p = fe.assembly.plan
+    mat = F64 if prec == 'fp64' else F32
+    U = p.gc_to_u(torch.as_tensor(0.01 * np.random.default_rng(3).standard_normal(p.n_free), device='cuda'))
+    R, K = p.tangent(U, 1.0, F64, val_dtype=mat)
+    A = sp.csr_matrix((K.double().cpu().numpy(), p.col_np, p.rowptr_np), shape=(p.n_free, p.n_free))
+    b = -R.cpu().numpy()
+    grp = p.grp.cpu().numpy()
+    assert np.diff(grp).max() == 3 and np.diff(grp).min() >= 1
+    assert p.spmv_groups < 0                                     # triple-structured
+    ng = p.spmv_groups if mode == 'triple' else -p.spmv_groups
+    pcg = solver.PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, mat, p.device, vec_dtype=F64, grp=p.grp,
+                     n_groups=ng, chunk=16, precond='block')
+    pcg.set_matrix(K)
+    for k in (1, 7, 25):
+        x = pcg.solve(-R, None, tol=0.0, max_iter=k).cpu().numpy()
+        y, _ = osol.pcg_block_jacobi(A, b, grp, tol=0.0, max_iter=k)
+        assert pcg.iterations == k
+        assert rel_err(x, y) < 1e-9
+    x = pcg.solve(-R, None, tol=1e-24, max_iter=5000).cpu().numpy()
+    assert rel_err(x, spla.spsolve(A.tocsc(), b)) < 1e-8
+    pcg.solve(-R, None, tol=1e-10, max_iter=5000)
+    _, its = osol.pcg_block_jacobi(A, b, grp, tol=1e-10, max_iter=5000)
+    assert abs(pcg.iterations - its) <= max(3, 0.05 * its)
+
+
+def test_static_block_jacobi_matches_oracle(gpu):
+    fe, mesh = workloads.beam_traction(8, 2, 2, traction=(0.0, 0.0, -0.01))
+    fe.solver = solver.StaticImplicitSolver(precision='mixed', rel_tol=1e-10, load_steps=4, cg_tol=1e-12,
+                                            armijo_slack=1e-12, preconditioner='block')
+    GC = fe.solve()
+    assert GC is not False
+    om = oracle_model(mesh, mr(0.5, 0.1, 0.01), 'x0', traction=(0.0, 0.0, -0.01), load_set='x1')
+    GC_o, _ = osol.static_solve(om, increments=4, rel_tol=1e-10, linear='direct', armijo_slack_rel=1e-12)
+    assert rel_err(GC.cpu().numpy(), GC_o) < 1e-8
 
 
 def test_reference_cg_signature(gpu):

@@ -163,3 +163,29 @@ def test_auto_predictor_reaches_the_same_equilibrium():
                                 linear='direct', armijo_slack_rel=1e-12)
     assert np.abs(GC - ref).max() <= 1e-8 * np.abs(ref).max()
     assert len(orders) == 4 and orders[0] == 1 and its < its0
+
+
+def test_block_jacobi_pcg():
+    # blocks of one row are the point Jacobi scaling: the same iterates as the reference's CG
+    # on D^-1/2 A D^-1/2 (up to the float32 rounding of the stored inverse)
+    rng = np.random.default_rng(5)
+    Q = np.linalg.qr(rng.standard_normal((30, 30)))[0]
+    A = Q @ np.diag(np.geomspace(1.0, 1e2, 30)) @ Q.T
+    A = sp.csr_matrix(A + np.diag(rng.uniform(1, 5, 30)))
+    b = rng.standard_normal(30)
+    D = np.sqrt(np.abs(A.diagonal()))
+    As = sp.diags(1 / D) @ A @ sp.diags(1 / D)
+    for k in (1, 3, 10):
+        x1, _ = osol.pcg_block_jacobi(A, b, np.arange(31), tol=0.0, max_iter=k)
+        y, _ = osol.conjugate_gradient(As, b / D, tol=0.0, max_iter=k)
+        assert np.abs(x1 - y / D).max() <= 1e-6 * np.abs(x1).max()
+    # the stored 3x3 inverses invert the symmetric diagonal blocks (to float32), groups of 3, 3, 2, 1 ...
+    grp = np.array([0, 3, 6, 8, 9, 12, 15, 18, 21, 24, 27, 30])
+    M = osol.block_jacobi_inverse(A, grp)
+    for g in range(len(grp) - 1):
+        r0, nr = grp[g], grp[g + 1] - grp[g]
+        B = A[r0:r0 + nr, r0:r0 + nr].toarray()
+        assert np.allclose(M[g, :nr, :nr] @ B, np.eye(nr), atol=1e-5)
+        assert np.all(M[g, nr:, :] == 0) and np.all(M[g, :, nr:] == 0)
+    x, _ = osol.pcg_block_jacobi(A, b, grp, tol=1e-26, max_iter=500)
+    assert np.allclose(x, np.linalg.solve(A.toarray(), b), rtol=1e-9, atol=1e-12)
