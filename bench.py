@@ -94,13 +94,16 @@ def event_time(torch, fn, reps, warm=3):
 
 
 def spmv_bytes(plan, val_bytes):
-    """Compulsory bytes of one node-grouped SpMV: values, one int32 column index per
-    entry of each node's shared column list, the input vector read and the output
-    written once (float64), the group heads' row pointers and the group table."""
-    lens = np.diff(plan.rowptr_np)
-    grp = plan.grp.cpu().numpy()
-    shared_cols = int(lens[grp[:-1]].sum())
-    return plan.nnz * val_bytes + shared_cols * 4 + plan.n_free * 16 + (plan.n_groups + 1) * 8
+    """Compulsory bytes of one node-grouped SpMV: values, the column stream (triple-blocked:
+    one int32 per neighbour triple of each group; otherwise one per entry of each node's
+    shared column list), the input vector read and the output written once (float64), the
+    group heads' row pointers and the group table."""
+    if plan.triple:
+        cols = int(plan.spmv_col.numel())
+    else:
+        lens = np.diff(plan.rowptr_np)
+        cols = int(lens[plan.grp.cpu().numpy()[:-1]].sum())
+    return plan.nnz * val_bytes + cols * 4 + plan.n_free * 16 + (plan.n_groups + 1) * 8
 
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE (one counter
@@ -109,7 +112,7 @@ def spmv_bytes(plan, val_bytes):
 # the summary and the commands are in profiles/traffic_beam128x32x32_kernels.md.  A kernel
 # whose workload or template differs from the measured one reports null.
 MEASURED_TRAFFIC = {
-    ('beam 128x32x32 C3D8', 'k_pcg_spmv<float,double,16,2>'): 201.4e6 + 3.72e6,
+    ('beam 128x32x32 C3D8', 'k_pcg_spmv<float,double,16,2>'): 156.13e6 + 3.59e6,
 }
 
 
@@ -178,7 +181,7 @@ def spmv_roofline(fe, torch, reps, mat_dtype):
     ws = torch.zeros(int(nat.lib().fea_pcg_ws_size(p.n_free)), dtype=torch.float64, device='cuda')
     s = nat.stream_ptr()
     t = event_time(torch, lambda: call('fea_pcg_spmv', mat_dtype, F64, p.n_free, p.nnz, p.spmv_groups, p.grp,
-                                       p.rowptr, p.col, val, x, q, ws, s), reps)
+                                       p.rowptr, p.spmv_col, val, x, q, ws, s), reps)
     return t, spmv_bytes(p, 4 if mat_dtype == F32 else 8)
 
 
@@ -905,7 +908,8 @@ def bench_kernels(args, torch):
     ws = torch.zeros(int(nat.lib().fea_pcg_ws_size(p.n_free)), dtype=torch.float64, device='cuda')
     s = nat.stream_ptr()
     for _ in range(20):
-        call('fea_pcg_spmv', F32, F64, p.n_free, p.nnz, p.spmv_groups, p.grp, p.rowptr, p.col, val, x, q, ws, s)
+        call('fea_pcg_spmv', F32, F64, p.n_free, p.nnz, p.spmv_groups, p.grp, p.rowptr, p.spmv_col, val, x, q, ws,
+             s)
     a = torch.empty(256 << 20, dtype=torch.uint8, device='cuda').fill_(1)
     b = torch.empty_like(a)
     for _ in range(20):

@@ -204,7 +204,7 @@ class StaticEquilibrium(torch.autograd.Function):
         _set_theta(plan, theta)
         _, K = plan.tangent(U, 1.0, F64)
         pcg = PCG(plan.n_free, plan.nnz, plan.rowptr, plan.col, plan.diag_slot, F64, plan.device,
-                  grp=plan.grp, n_groups=plan.spmv_groups)
+                  grp=plan.grp, n_groups=plan.spmv_groups, spmv_col=plan.spmv_col)
         pcg.set_matrix(K)
         rhs = plan.u_to_gc(_vec(g, plan)).contiguous()
         lam_free = pcg.solve(rhs, None, tol=ctx.tol, max_iter=ctx.max_iter)

@@ -23,7 +23,7 @@ import torch
 
 from . import _native as nat
 from ._native import F32, F64, call
-from .device import DevicePlan, _i32, row_group_tables
+from .device import DevicePlan, _i32, row_group_tables, triple_columns
 
 _TORCH_DT = {F32: torch.float32, F64: torch.float64}
 
@@ -213,11 +213,14 @@ class CoupledPlan:
         self.diag_slot = _i32(diag, dev)
         if grp is None:
             self.grp, self.n_groups, self.spmv_groups, self.triple = None, 0, 0, False
+            self.spmv_col = self.col
         else:
             self.grp = _i32(grp, dev)
             self.n_groups = int(grp.shape[0] - 1)
             self.triple = triple
             self.spmv_groups = -self.n_groups if triple else self.n_groups
+            # the column stream of the SpMV: compact per-triple columns on a triple-structured CSR
+            self.spmv_col = _i32(triple_columns(rowptr, col, grp), dev) if triple else self.col
         self._load = c['load']
         arr = dict(
             cnode_ptr=_i32(c['cnode_ptr'], dev), rp_state=_i32(3 * N + 6 * np.arange(len(couples)), dev),
@@ -349,7 +352,7 @@ class CoupledPlan:
     def spmv(self, val, x, y, mat_dtype, vec_dtype=None):
         vec_dtype = mat_dtype if vec_dtype is None else vec_dtype
         call('fea_spmv', mat_dtype, vec_dtype, self.n_free, self.nnz, self.spmv_groups, self.grp, self.rowptr,
-             self.col, val, x, y, self._s())
+             self.spmv_col, val, x, y, self._s())
         return y
 
     def csr_indices_coo(self):

@@ -485,7 +485,8 @@ struct SpmvShape {
 };
 
 // n_groups < 0 marks a triple-structured group table (every group: 3 rows whose
-// column list is whole neighbour triples); |n_groups| is the group count.
+// column list is whole neighbour triples; col is then the compact triple-column
+// array); |n_groups| is the group count.
 static SpmvShape shape_of(int n, int64_t nnz, int n_groups, const int32_t *grp) {
   SpmvShape s;
   s.mode = (grp == nullptr) ? 0 : (n_groups < 0 ? 2 : 1);

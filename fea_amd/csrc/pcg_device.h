@@ -24,7 +24,10 @@ enum : int {
 // MODE 2: node groups whose column list is made of whole neighbour-node triples
 // (3 consecutive columns per neighbour, 3 rows per group): lane j takes
 // neighbour j's 3 columns with one index load, one 3-wide x load and three
-// 3-wide value loads.
+// 3-wide value loads.  In MODE 2 `col` is the compact triple-column array: one entry
+// (the first column) per neighbour triple of each group, group g's at
+// rowptr[grp[g]] / 9 + j (every earlier group holds 3 rows of a multiple of 3 entries),
+// so the index stream is a third of the CSR column array.
 template <typename TM, typename TV, int SW, int MODE>
 __device__ __forceinline__ int group_spmv(int g, int lane, const int32_t *__restrict__ grp,
                                           const int32_t *__restrict__ rowptr,
@@ -39,7 +42,7 @@ __device__ __forceinline__ int group_spmv(int g, int lane, const int32_t *__rest
     const int nb = L / 3;
     for (int j = lane; j < nb; j += SW) {
       const int o = s0 + 3 * j;
-      const int c = col[o];
+      const int c = col[s0 / 9 + j];
       const TV x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
       a0 += (TV)val[o] * x0 + (TV)val[o + 1] * x1 + (TV)val[o + 2] * x2;
       a1 += (TV)val[o + L] * x0 + (TV)val[o + L + 1] * x1 + (TV)val[o + L + 2] * x2;

@@ -63,6 +63,20 @@ def row_group_tables(rowptr, col, sizes):
     return grp, triple
 
 
+def triple_columns(rowptr, col, grp):
+    """The compact column array of a triple-structured CSR that the triple-blocked SpMV reads:
+    the first column of each neighbour triple of each group's head row, group g's triples at
+    rowptr[grp[g]] / 9 + j (every group holds 3 rows whose lengths are multiples of 3)."""
+    heads = np.asarray(grp[:-1], dtype=np.int64)
+    s0 = np.asarray(rowptr, dtype=np.int64)[heads]
+    cnt = (np.asarray(rowptr, dtype=np.int64)[heads + 1] - s0) // 3
+    start = np.cumsum(cnt) - cnt
+    if np.any(s0 % 9) or not np.array_equal(start, s0 // 9):
+        raise ValueError('not a triple-structured CSR')
+    idx = np.repeat(s0, cnt) + 3 * (np.arange(int(cnt.sum())) - np.repeat(start, cnt))
+    return np.ascontiguousarray(np.asarray(col)[idx], dtype=np.int32)
+
+
 class Group:
     """One contiguous batch of elements or facets sharing a kernel configuration."""
 
@@ -159,6 +173,8 @@ class DevicePlan:
             self.triple = triple
             self.spmv_groups = -self.n_groups if triple else self.n_groups
             self.rowptr, self.col = _i32(rowptr, dev), _i32(col, dev)
+            # the column stream of the SpMV: compact per-triple columns on a triple-structured CSR
+            self.spmv_col = _i32(triple_columns(rowptr, col, grp), dev) if triple else self.col
             self.diag_slot = _i32(diag, dev)
             self.blk_a, self.blk_b, self.blk_colpos = _i32(blk_a, dev), _i32(blk_b, dev), _i32(blk_cp, dev)
             self.kseg, self.rseg = _i32(kseg, dev), _i32(rseg, dev)
@@ -397,7 +413,7 @@ class DevicePlan:
     def spmv(self, val, x, y, mat_dtype, vec_dtype=None):
         vec_dtype = mat_dtype if vec_dtype is None else vec_dtype
         call('fea_spmv', mat_dtype, vec_dtype, self.n_free, self.nnz, self.spmv_groups, self.grp, self.rowptr,
-             self.col, val, x, y, self._s())
+             self.spmv_col, val, x, y, self._s())
         return y
 
     def csr_indices_coo(self):

@@ -30,10 +30,12 @@ class PCG:
     ``precond='block'`` replaces the point Jacobi M = |diag K| by the node-block
     Jacobi M = blockdiag(K_gg) over the row groups (an addition: other iterates,
     the same solution; needs ``grp``, else point Jacobi is used).
+    ``spmv_col``: the SpMV's column stream, required for a triple-structured group
+    table (``n_groups < 0``: the compact per-triple columns, ``DevicePlan.spmv_col``).
     """
 
     def __init__(self, n, nnz, rowptr, col, diag_slot, mat_dtype, device, vec_dtype=None, grp=None,
-                 n_groups=0, chunk=32, precond='jacobi'):
+                 n_groups=0, chunk=32, precond='jacobi', spmv_col=None):
         if precond not in ('jacobi', 'block'):
             raise ValueError("precond must be 'jacobi' or 'block'")
         self.block = precond == 'block' and grp is not None and int(n_groups) != 0
@@ -41,6 +43,9 @@ class PCG:
         self.mat_dtype = mat_dtype
         self.vec_dtype = mat_dtype if vec_dtype is None else vec_dtype
         self.rowptr, self.col, self.diag_slot = rowptr, col, diag_slot
+        if int(n_groups) < 0 and spmv_col is None:
+            raise ValueError('a triple-structured group table needs spmv_col (the compact triple columns)')
+        self.spmv_col = col if int(n_groups) >= 0 else spmv_col
         self.grp, self.n_groups = grp, int(n_groups)
         T = _DT[self.vec_dtype]
         mk = lambda: torch.zeros(max(1, self.n), dtype=T, device=device)[:self.n]
@@ -78,12 +83,12 @@ class PCG:
             self.x.copy_(x0)
         pc = 1 if self.block else 0
         call('fea_pcg_start', self.mat_dtype, self.vec_dtype, self.n, self.nnz, self.n_groups, self.grp,
-             self.rowptr, self.col, self.val, pc, self.dinv, b, self.x, self.r, self.z, self.p, self.ws, s)
+             self.rowptr, self.spmv_col, self.val, pc, self.dinv, b, self.x, self.r, self.z, self.p, self.ws, s)
         done_iters = 0
         while True:
             step = min(self.chunk, max(1, max_iter - done_iters))
             call('fea_pcg_iterate', self.mat_dtype, self.vec_dtype, self.n, self.nnz, self.n_groups, self.grp,
-                 self.rowptr, self.col, self.val, pc, self.dinv, self.x, self.r, self.z, self.p, self.q, self.ws,
+                 self.rowptr, self.spmv_col, self.val, pc, self.dinv, self.x, self.r, self.z, self.p, self.q, self.ws,
                  float(tol), int(max_iter), int(step), self.status, s)
             done_iters += step
             if self.status[1] != 0.0 or done_iters >= max_iter:
@@ -190,7 +195,7 @@ class BatchedPCG:
                 raise ValueError('system %d: dtype mismatch' % k)
             s = arr[k]
             s.n, s.n_groups, s.max_iter, s.nnz, s.tol = v.n, int(plan.spmv_groups), int(max_iter), plan.nnz, float(tol)
-            s.grp, s.rowptr, s.col, s.diag_slot = (plan.grp.data_ptr(), plan.rowptr.data_ptr(), plan.col.data_ptr(),
+            s.grp, s.rowptr, s.col, s.diag_slot = (plan.grp.data_ptr(), plan.rowptr.data_ptr(), plan.spmv_col.data_ptr(),
                                                    plan.diag_slot.data_ptr())
             s.val, s.dinv, s.b, s.x = val.data_ptr(), v.dinv.data_ptr(), v.b.data_ptr(), v.x.data_ptr()
             s.r, s.z, s.p, s.q, s.ws = v.r.data_ptr(), v.z.data_ptr(), v.p.data_ptr(), v.q.data_ptr(), v.ws.data_ptr()

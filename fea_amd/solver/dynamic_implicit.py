@@ -103,7 +103,7 @@ class DynamicImplicitSolver(StaticImplicitSolver):
         U = p.gc_to_u(torch.as_tensor(GC0, dtype=torch.float64, device=p.device).contiguous())
         R = p.residual(U, 1.0, F64).clone()
         pcg = PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, F64, p.device, grp=p.grp,
-                  n_groups=p.spmv_groups)
+                  n_groups=p.spmv_groups, spmv_col=p.spmv_col)
         pcg.set_matrix(p.mass_matrix())
         return -pcg.solve(R, None, tol=1e-28, max_iter=20000).clone()
 

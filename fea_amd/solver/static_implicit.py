@@ -150,7 +150,7 @@ class StaticImplicitSolver(BaseSolver):
             p = self.assembly.plan
             self._pcg[key] = PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, mat_dtype, p.device,
                                  vec_dtype=vec_dtype, grp=p.grp, n_groups=p.spmv_groups, chunk=self.pcg_chunk,
-                                 precond=self.preconditioner)
+                                 precond=self.preconditioner, spmv_col=p.spmv_col)
         return self._pcg[key]
 
     def _vec_stats(self, x):

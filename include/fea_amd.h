@@ -190,7 +190,10 @@ int fea_reduce_sum(int n, const double *partial, double *out, void *stream);
  * rows of one node do in the fea_pattern CSR.  grp == NULL: one row per group.
  * n_groups < 0 additionally asserts that every group has 3 rows and a column
  * list made of whole neighbour-node triples (3 consecutive column indices),
- * which selects the triple-blocked SpMV.
+ * which selects the triple-blocked SpMV; `col` is then the compact triple-column
+ * array, not the CSR's: one entry (the first column) per neighbour triple of each
+ * group, group g's at rowptr[grp[g]] / 9 + j (nnz / 9 entries).  This also holds
+ * for the PCG entry points and the batched systems below.
  *
  * CSR SpMV y = A x (the CSR `A @ p` of _linear_solver.py:27,32). */
 int fea_spmv(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,

@@ -34,7 +34,7 @@ def test_pcg_matches_direct_solve(gpu):
     b = -R.cpu().numpy()
     x_ref = spla.spsolve(A.tocsc(), b)
     pcg = solver.PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, F64, p.device, grp=p.grp,
-                     n_groups=p.spmv_groups)
+                     n_groups=p.spmv_groups, spmv_col=p.spmv_col)
     pcg.set_matrix(K)
     x = pcg.solve(-R, None, tol=1e-24, max_iter=5000).cpu().numpy()
     assert rel_err(x, x_ref) < 1e-8
@@ -67,7 +67,7 @@ def test_block_jacobi_pcg_matches_oracle(gpu, prec, mode):
     assert p.spmv_groups < 0                                     # triple-structured
     ng = p.spmv_groups if mode == 'triple' else -p.spmv_groups
     pcg = solver.PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, mat, p.device, vec_dtype=F64, grp=p.grp,
-                     n_groups=ng, chunk=16, precond='block')
+                     n_groups=ng, chunk=16, precond='block', spmv_col=p.spmv_col)
     pcg.set_matrix(K)
     for k in (1, 7, 25):
         x = pcg.solve(-R, None, tol=0.0, max_iter=k).cpu().numpy()
@@ -190,7 +190,7 @@ def test_spmv_precision_pairs(gpu, pair, grouped):
     xd = torch.as_tensor(x, device='cuda', dtype=tv)
     y = torch.zeros_like(xd)
     call('fea_spmv', m, v, p.n_free, p.nnz, p.spmv_groups if grouped else 0, p.grp if grouped else None,
-         p.rowptr, p.col, val, xd, y, nat.stream_ptr())
+         p.rowptr, p.spmv_col if grouped else p.col, val, xd, y, nat.stream_ptr())
     ref = A @ x
     tol = 1e-13 if pair == 'f64/f64' else (1e-6 if pair == 'f32/f64' else 1e-5)
     assert rel_err(y.double().cpu().numpy(), ref) < tol
@@ -208,9 +208,11 @@ def test_spmv_node_modes_agree(gpu):
     _, K = p.tangent(U, 1.0, F64)
     x = torch.as_tensor(np.random.default_rng(6).standard_normal(p.n_free), device='cuda')
     outs = []
-    for ng, grp in ((0, None), (p.n_groups, p.grp), (-p.n_groups, p.grp)):
+    # the triple-blocked mode reads the compact per-triple columns (a third of the CSR's)
+    assert p.spmv_col.numel() * 9 == p.nnz
+    for ng, grp, col in ((0, None, p.col), (p.n_groups, p.grp, p.col), (-p.n_groups, p.grp, p.spmv_col)):
         y = torch.zeros_like(x)
-        call('fea_spmv', F64, F64, p.n_free, p.nnz, ng, grp, p.rowptr, p.col, K, x, y, nat.stream_ptr())
+        call('fea_spmv', F64, F64, p.n_free, p.nnz, ng, grp, p.rowptr, col, K, x, y, nat.stream_ptr())
         outs.append(y.cpu().numpy())
     assert rel_err(outs[1], outs[0]) < 1e-14 and rel_err(outs[2], outs[0]) < 1e-14
 

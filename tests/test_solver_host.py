@@ -18,3 +18,29 @@ def test_predictor_order_rejects(arg):
 
 def test_auto_predictor_caps_at_six():
     assert predictor_order('auto') == 6
+
+
+def test_triple_columns_index():
+    """The compact column array read by the triple-blocked SpMV: entry rowptr[head] / 9 + j of
+    a group is the first column of the head row's neighbour triple j."""
+    import numpy as np
+    import scipy.sparse as sp
+    from fea_amd import meshgen
+    from fea_amd.device import row_group_tables, triple_columns
+    m = meshgen.hex_block(3, 2, 2)
+    N = m['nodes'].shape[0]
+    e = m['conn']
+    r = np.repeat(e, e.shape[1], axis=1).ravel()
+    c = np.tile(e, (1, e.shape[1])).ravel()
+    B = sp.csr_matrix((np.ones(r.size), (r, c)), shape=(N, N))
+    A = sp.kron(B, np.ones((3, 3))).tocsr()
+    A.sort_indices()
+    grp, triple = row_group_tables(A.indptr, A.indices, np.full(N, 3))
+    assert triple
+    tc = triple_columns(A.indptr, A.indices, grp)
+    assert tc.size * 9 == A.nnz
+    for g in range(N):
+        s0, L = A.indptr[grp[g]], A.indptr[grp[g] + 1] - A.indptr[grp[g]]
+        assert np.array_equal(tc[s0 // 9 + np.arange(L // 3)], A.indices[s0 + 3 * np.arange(L // 3)])
+    with pytest.raises(ValueError):
+        triple_columns(A.indptr, A.indices, np.arange(N + 1))      # one-row groups: not triples
