@@ -71,6 +71,8 @@ def parse():
     ap.add_argument('--couple-pressure', type=float, default=0.06, help='couple: pressure on surface_1_All')
     ap.add_argument('--couple-increments', type=int, default=4, help='couple: load increments')
     ap.add_argument('--couple-predictor', default='auto', help='couple / contact: load-increment predictor')
+    ap.add_argument('--couple-preconditioner', default='block', choices=['block', 'jacobi'],
+                    help='couple / contact: PCG preconditioner')
     return ap.parse_args()
 
 
@@ -527,7 +529,8 @@ def bench_couple(args, torch):
     # a plain linear extrapolation of the actuator's large rotation starts far from the next
     # equilibrium; 'auto' falls back to the previous one when the extrapolation's energy is higher
     fe.solver = make_solver(args.precision, args, load_steps=args.couple_increments,
-                            predictor=_pred(args, args.couple_predictor))
+                            predictor=_pred(args, args.couple_predictor),
+                            preconditioner=args.couple_preconditioner)
     fe.initialize()
     p = fe.assembly.plan
     for _ in range(warmup):
@@ -552,7 +555,8 @@ def bench_couple(args, torch):
                                   'reference point at (0,0,80), %d increments, Newton rel tol %g'
                                   % (inp.part['final_model'].n_elements(), *inp.material['Rubber'].mat_para,
                                      args.couple_pressure, args.couple_increments, args.rel_tol),
-                      'precision': args.precision, 'predictor': args.couple_predictor, 'free_dof': p.n_free,
+                      'precision': args.precision, 'predictor': args.couple_predictor,
+                      'preconditioner': args.couple_preconditioner, 'free_dof': p.n_free,
                       'nnz': p.nnz})
     line['data'] = 'the reference\'s INP mesh (tests/pressure_test/C3D4Less.inp, committed gzipped as a fixture)'
     line.update({'latency_p50_s': float(np.percentile(lat, 50)),
@@ -618,7 +622,8 @@ def bench_contact(args, torch):
     surfaces = ['surface_%d_All' % i for i in range(4)]
     fe, inp = workloads.inp_couple(path, pressure=0.06, rp=(0.0, 0.0, 70.0), contact_surfaces=surfaces)
     fe.solver = make_solver(args.precision, args, load_steps=args.couple_increments,
-                            predictor=_pred(args, args.couple_predictor))
+                            predictor=_pred(args, args.couple_predictor),
+                            preconditioner=args.couple_preconditioner)
     fe.initialize()
     p = fe.assembly.plan
     for _ in range(warmup):
@@ -662,7 +667,8 @@ def bench_contact(args, torch):
                                   'ContactSelf on surface_0..3_All, surface_0_Bottom fixed, surface_0_Head coupled to '
                                   'a reference point at (0,0,70), %d increments, Newton rel tol %g'
                                   % (inp.part['final_model'].n_elements(), args.couple_increments, args.rel_tol),
-                      'precision': args.precision, 'predictor': args.couple_predictor, 'free_dof': p.n_free})
+                      'precision': args.precision, 'predictor': args.couple_predictor,
+                      'preconditioner': args.couple_preconditioner, 'free_dof': p.n_free})
     line['data'] = 'the reference\'s INP mesh (tests/gradient_test/C3D4.inp, committed gzipped as a fixture)'
     line.update({'latency_p50_s': float(np.percentile(lat, 50)),
                  'newton_iters_per_solve': stats['newton_iters'], 'cg_iters_per_solve': stats['cg_iters'],
