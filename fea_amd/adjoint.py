@@ -180,13 +180,23 @@ class StaticEquilibrium(torch.autograd.Function):
     dX = -(lambda . df_int/dX + p Hess(V) lambda), dtheta = -lambda . df_int/dtheta."""
 
     @staticmethod
-    def forward(ctx, X, theta, fe, adjoint_tol=1e-24, adjoint_max_iter=20000):
+    def forward(ctx, X, theta, fe, GC0=None, adjoint_tol=1e-24, adjoint_max_iter=20000):
         asm = fe.assembly
         plan = asm.plan
         _check_loads(plan)
         plan.set_coordinates(X.detach())
         _set_theta(plan, theta)
-        GC = fe.solver.solve(torch.zeros_like(asm.GC))
+        if GC0 is None:
+            GC = fe.solver.solve(torch.zeros_like(asm.GC))
+        else:
+            # warm start from a nearby equilibrium (a perturbed mesh or material): one Newton
+            # solve at the full load instead of the solver's load increments from rest
+            steps = fe.solver.load_steps
+            fe.solver.load_steps = 1
+            try:
+                GC = fe.solver.solve(GC0.detach().clone())
+            finally:
+                fe.solver.load_steps = steps
         if isinstance(GC, bool):
             raise RuntimeError('static solve did not converge')
         U = plan.gc_to_u(GC).clone()
@@ -217,17 +227,19 @@ class StaticEquilibrium(torch.autograd.Function):
             gx = -(ex + pressure_hvp(plan, U, lam))
         if need[1]:
             gth = -eth.to(theta.device)
-        return gx, gth, None, None, None
+        return gx, gth, None, None, None, None
 
 
-def compliance(fe, X=None, theta=None):
-    """J = f_ext(U*, X) . U* (pressure loads) as a differentiable torch scalar."""
+def compliance(fe, X=None, theta=None, GC0=None):
+    """J = f_ext(U*, X) . U* (pressure loads) as a differentiable torch scalar.  ``GC0``: start
+    the equilibrium solve from this state at the full load (a nearby equilibrium, e.g. for
+    finite differences) instead of ramping the load from rest."""
     plan = fe.assembly.plan
     if X is None:
         X = plan.X.reshape(-1).clone().requires_grad_(True)
     if theta is None:
         theta = _theta_of(plan).requires_grad_(True)
-    U = StaticEquilibrium.apply(X, theta, fe)
+    U = StaticEquilibrium.apply(X, theta, fe, GC0)
     f = PressureForce.apply(X, U, plan)
     return torch.sum(f * U), X, theta
 

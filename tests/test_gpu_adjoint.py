@@ -111,6 +111,13 @@ def test_compliance_shape_gradient_matches_finite_differences(gpu):
         an = float(torch.dot(gX, d))
         assert abs(fd - an) < 2e-6 * abs(an) + 1e-9 * J0, (fd, an)
     plan.set_coordinates(X0)
+    # a solve started from the base equilibrium at the full load lands on the same J
+    GC_base = fe.assembly.GC.clone()
+    d = torch.as_tensor(rng.uniform(-1, 1, X0.numel()), device='cuda')
+    Jc = adjoint.compliance(fe, X=X0 + 1e-4 * d)[0].item()
+    Jw = adjoint.compliance(fe, X=X0 + 1e-4 * d, GC0=GC_base)[0].item()
+    assert abs(Jw - Jc) < 1e-11 * abs(Jc)
+    plan.set_coordinates(X0)
     # material parameters: theta = (mu/2, 0, kappa)
     th0 = adjoint._theta_of(plan)
     for j, hj in ((0, 1e-6), (2, 1e-4)):
