@@ -11,20 +11,31 @@
 //
 // Converged systems (ws[W_DONE] set) return at once, so a batch keeps running
 // until its slowest member finishes without disturbing the others.
+#include <climits>
+
 #include "pcg_device.h"
 
 namespace fea {
 
 constexpr int kBatchSW = 16;  // sub-wave per node group in batched SpMV (fixed: batch-independent)
+constexpr int kSteps = 4;     // grid-stride steps per workgroup a system's grid is sized for
 
-// index k of the system whose workgroup range [start[k], start[k+1]) holds b
+// index k of the system whose workgroup range [start[k], start[k+1]) holds b: the number of
+// systems whose range ends at or before b.  Every lane loads its (up to 16) table entries
+// first, so a workgroup waits for one load latency, not one per 64 systems (n_sys is at
+// most FEA_PCG_BATCH_MAX, checked by every entry point).
 __device__ __forceinline__ int find_system(const int32_t *__restrict__ start, int n_sys, int b) {
-  int cnt = 0;
-  for (int base = 0; base < n_sys; base += 64) {
-    const int k = base + (int)(threadIdx.x & 63);
-    const bool passed = (k < n_sys) && (start[k + 1] <= b);
-    cnt += __popcll(__ballot(passed));
+  constexpr int kSteps = (FEA_PCG_BATCH_MAX + 63) / 64;
+  const int lane = (int)(threadIdx.x & 63);
+  int end[kSteps];
+#pragma unroll
+  for (int t = 0; t < kSteps; ++t) {
+    const int k = t * 64 + lane;
+    end[t] = k < n_sys ? start[k + 1] : INT_MAX;
   }
+  int cnt = 0;
+#pragma unroll
+  for (int t = 0; t < kSteps; ++t) cnt += __popcll(__ballot(end[t] <= b));
   return __builtin_amdgcn_readfirstlane(cnt);
 }
 
@@ -253,9 +264,11 @@ extern "C" int fea_pcg_batch_layout(int n_sys, fea_pcg_system *sys_host, int32_t
                       S.b && S.x && S.r && S.z && S.p && S.q && S.ws && S.max_iter >= 1,
                   "fea_pcg_batch_layout: system %d is incomplete", k);
     const int64_t ng = S.n_groups < 0 ? -(int64_t)S.n_groups : S.n_groups;
-    int64_t sb = (ng * kBatchSW + kBlock - 1) / kBlock;
+    // kSteps grid-stride steps per workgroup (as the single-system grids give), so a
+    // workgroup's fixed start (finding its system, loading the descriptor) is amortised
+    int64_t sb = (ng * kBatchSW + kSteps * kBlock - 1) / (kSteps * kBlock);
     sb = sb < 1 ? 1 : (sb > G1_MAX ? G1_MAX : sb);
-    int64_t vb = ((int64_t)S.n + kBlock - 1) / kBlock;
+    int64_t vb = ((int64_t)S.n + kSteps * kBlock - 1) / (kSteps * kBlock);
     vb = vb > PCG_G ? PCG_G : vb;
     S.spmv_blocks = (int32_t)sb;
     S.vec_blocks = (int32_t)vb;
