@@ -69,11 +69,29 @@ __global__ __launch_bounds__(kBlock) void k_batch_residual(int n_sys, const fea_
     const int r0 = sys_group_spmv<TM, TV>(S, g, lane, x, o);
     if (lane == 0) {
       const int nr = S.grp[g + 1] - r0;
-      for (int j = 0; j < nr; ++j) {
-        r[r0 + j] = b[r0 + j] - o[j];
-        const int s = S.diag_slot[r0 + j];
-        const TV d = (s >= 0) ? fabs((TV)val[s]) : TV(0);
-        dinv[r0 + j] = (d == TV(0)) ? TV(1) : TV(1) / d;
+      for (int j = 0; j < nr; ++j) r[r0 + j] = b[r0 + j] - o[j];
+      if (S.precond == 1) {
+        // the node's diagonal block: entry (a, c) at diag_slot[r0 + a] + c - a.  On a
+        // triple-structured system the node's own columns are one aligned triple of every
+        // row of the group; otherwise the CSR columns confirm each entry
+        double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int a = 0; a < nr; ++a) {
+          const int d = S.diag_slot[r0 + a];
+          if (d < 0) continue;
+          for (int c = 0; c < nr; ++c) {
+            const int sl = d + c - a;
+            const bool ok = S.n_groups < 0 ||
+                            (sl >= S.rowptr[r0 + a] && sl < S.rowptr[r0 + a + 1] && S.col[sl] == r0 + c);
+            if (ok) A[a][c] = (double)val[sl];
+          }
+        }
+        bj_invert_store(A, nr, (float *)S.dinv + 6 * (int64_t)g);
+      } else {
+        for (int j = 0; j < nr; ++j) {
+          const int s = S.diag_slot[r0 + j];
+          const TV d = (s >= 0) ? fabs((TV)val[s]) : TV(0);
+          dinv[r0 + j] = (d == TV(0)) ? TV(1) : TV(1) / d;
+        }
       }
     }
   }
@@ -92,12 +110,33 @@ __global__ __launch_bounds__(kBlock) void k_batch_init(int n_sys, const fea_pcg_
   TV *z = (TV *)S.z;
   TV *p = (TV *)S.p;
   double rz = 0.0, bz = 0.0;
-  for (int64_t i = (int64_t)lb * blockDim.x + threadIdx.x; i < S.n; i += (int64_t)S.vec_blocks * blockDim.x) {
-    const TV zi = dinv[i] * r[i];
-    z[i] = zi;
-    p[i] = zi;
-    rz += (double)r[i] * (double)zi;
-    bz += (double)b[i] * (double)dinv[i] * (double)b[i];
+  if (S.precond == 1) {
+    // node-block Jacobi: one row group per thread
+    const int ng = S.n_groups < 0 ? -S.n_groups : S.n_groups;
+    const float *binv = (const float *)S.dinv;
+    for (int64_t g = (int64_t)lb * blockDim.x + threadIdx.x; g < ng; g += (int64_t)S.vec_blocks * blockDim.x) {
+      const int r0 = S.grp[g], nr = S.grp[g + 1] - r0;
+      double rr[3], bb[3], zz[3], zb[3];
+      bj_load(r, r0, nr, rr);
+      bj_load(b, r0, nr, bb);
+      bj_apply(binv + 6 * g, rr, zz);
+      bj_apply(binv + 6 * g, bb, zb);
+      for (int a = 0; a < nr; ++a) {
+        const TV zi = (TV)zz[a];
+        z[r0 + a] = zi;
+        p[r0 + a] = zi;
+        rz += rr[a] * (double)zi;
+        bz += bb[a] * zb[a];
+      }
+    }
+  } else {
+    for (int64_t i = (int64_t)lb * blockDim.x + threadIdx.x; i < S.n; i += (int64_t)S.vec_blocks * blockDim.x) {
+      const TV zi = dinv[i] * r[i];
+      z[i] = zi;
+      p[i] = zi;
+      rz += (double)r[i] * (double)zi;
+      bz += (double)b[i] * (double)dinv[i] * (double)b[i];
+    }
   }
   double *ws = S.ws;
   const double srz = block_sum(rz);
@@ -169,13 +208,36 @@ __global__ __launch_bounds__(kBlock) void k_batch_update(int n_sys, const fea_pc
   TV *r = (TV *)S.r;
   TV *z = (TV *)S.z;
   double rz = 0.0;
-  for (int64_t i = (int64_t)lb * blockDim.x + threadIdx.x; i < S.n; i += (int64_t)S.vec_blocks * blockDim.x) {
-    x[i] += alpha * p[i];
-    const TV ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    const TV zi = dinv[i] * ri;
-    z[i] = zi;
-    rz += (double)ri * (double)zi;
+  if (S.precond == 1) {
+    // node-block Jacobi: one row group per thread
+    const int ng = S.n_groups < 0 ? -S.n_groups : S.n_groups;
+    const float *binv = (const float *)S.dinv;
+    for (int64_t g = (int64_t)lb * blockDim.x + threadIdx.x; g < ng; g += (int64_t)S.vec_blocks * blockDim.x) {
+      const int r0 = S.grp[g], nr = S.grp[g + 1] - r0;
+      double rr[3] = {0.0, 0.0, 0.0}, zz[3];
+      for (int a = 0; a < nr; ++a) {
+        const int i = r0 + a;
+        x[i] += alpha * p[i];
+        const TV ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        rr[a] = (double)ri;
+      }
+      bj_apply(binv + 6 * g, rr, zz);
+      for (int a = 0; a < nr; ++a) {
+        const TV zi = (TV)zz[a];
+        z[r0 + a] = zi;
+        rz += rr[a] * (double)zi;
+      }
+    }
+  } else {
+    for (int64_t i = (int64_t)lb * blockDim.x + threadIdx.x; i < S.n; i += (int64_t)S.vec_blocks * blockDim.x) {
+      x[i] += alpha * p[i];
+      const TV ri = r[i] - alpha * q[i];
+      r[i] = ri;
+      const TV zi = dinv[i] * ri;
+      z[i] = zi;
+      rz += (double)ri * (double)zi;
+    }
   }
   const double s = block_sum(rz);
   if (threadIdx.x == 0) ws[W_RZP + (it & 1) * PCG_RING + lb] = s;
@@ -263,6 +325,7 @@ extern "C" int fea_pcg_batch_layout(int n_sys, fea_pcg_system *sys_host, int32_t
     FEA_CHECK_ARG(S.n > 0 && S.n_groups != 0 && S.grp && S.rowptr && S.col && S.diag_slot && S.val && S.dinv &&
                       S.b && S.x && S.r && S.z && S.p && S.q && S.ws && S.max_iter >= 1,
                   "fea_pcg_batch_layout: system %d is incomplete", k);
+    FEA_CHECK_ARG(S.precond == 0 || S.precond == 1, "fea_pcg_batch_layout: system %d: precond must be 0 or 1", k);
     const int64_t ng = S.n_groups < 0 ? -(int64_t)S.n_groups : S.n_groups;
     // kSteps grid-stride steps per workgroup (as the single-system grids give), so a
     // workgroup's fixed start (finding its system, loading the descriptor) is amortised

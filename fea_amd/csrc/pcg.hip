@@ -194,17 +194,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, const TV *__restrict_
   for (int64_t i = i0 + kPre * stride; i < n; i += stride) p[i] = z[i] + beta * p[i];
 }
 
-// ---------------------------------------------------------------- node-block Jacobi
-// M = blockdiag(K_gg) over the row groups g (the free rows of a node, nr <= 3): z = M^-1 r
-// per group with the symmetric inverse stored as 6 floats (00, 01, 02, 11, 12, 22) -- the
-// bytes of the FP64 diagonal it replaces.  Entries of rows / columns beyond nr are zero.
-__device__ __forceinline__ void bj_apply(const float *__restrict__ B, const double r[3], double z[3]) {
-  const double b00 = B[0], b01 = B[1], b02 = B[2], b11 = B[3], b12 = B[4], b22 = B[5];
-  z[0] = b00 * r[0] + b01 * r[1] + b02 * r[2];
-  z[1] = b01 * r[0] + b11 * r[1] + b12 * r[2];
-  z[2] = b02 * r[0] + b12 * r[1] + b22 * r[2];
-}
-
 // binv[g] = (K_gg)^-1 from the group's diagonal block: entry (a, b) of row r0 + a sits at
 // diag_slot[r0 + a] + b - a when the row's sorted columns hold r0 .. r0 + nr - 1 (checked;
 // absent entries are zero).  A block that is not positive definite falls back to the
@@ -228,42 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_block_jacobi_inv(int ng, const int32
       if (sl >= rowptr[r0 + a] && sl < rowptr[r0 + a + 1] && col[sl] == r0 + b) A[a][b] = (double)val[sl];
     }
   }
-  // symmetric part (the tangent is symmetric; rounding of a float matrix is not)
-  for (int a = 0; a < 3; ++a)
-    for (int b = a + 1; b < 3; ++b) A[a][b] = A[b][a] = 0.5 * (A[a][b] + A[b][a]);
-  for (int a = nr; a < 3; ++a) A[a][a] = 1.0;  // pad: identity on absent rows
-  const double c00 = A[1][1] * A[2][2] - A[1][2] * A[1][2];
-  const double c01 = A[0][2] * A[1][2] - A[0][1] * A[2][2];
-  const double c02 = A[0][1] * A[1][2] - A[0][2] * A[1][1];
-  const double c11 = A[0][0] * A[2][2] - A[0][2] * A[0][2];
-  const double c12 = A[0][1] * A[0][2] - A[0][0] * A[1][2];
-  const double c22 = A[0][0] * A[1][1] - A[0][1] * A[0][1];
-  const double det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
-  // positive definite: leading minors A00 > 0, c22 > 0, det > 0
-  const bool spd = A[0][0] > 0.0 && c22 > 0.0 && det > 0.0 && det == det;
-  float *B = binv + 6 * (int64_t)g;
-  double o[6];
-  if (spd) {
-    const double id = 1.0 / det;
-    o[0] = c00 * id; o[1] = c01 * id; o[2] = c02 * id; o[3] = c11 * id; o[4] = c12 * id; o[5] = c22 * id;
-  } else {
-    for (int k = 0; k < 6; ++k) o[k] = 0.0;
-    const double d0 = fabs(A[0][0]), d1 = fabs(A[1][1]), d2 = fabs(A[2][2]);
-    o[0] = d0 == 0.0 ? 1.0 : 1.0 / d0;
-    o[3] = d1 == 0.0 ? 1.0 : 1.0 / d1;
-    o[5] = d2 == 0.0 ? 1.0 : 1.0 / d2;
-  }
-  // absent rows / columns exactly zero
-  if (nr < 3) { o[2] = o[4] = o[5] = 0.0; }
-  if (nr < 2) { o[1] = o[3] = 0.0; }
-  for (int k = 0; k < 6; ++k) B[k] = (float)o[k];
-}
-
-template <typename TV>
-__device__ __forceinline__ void bj_load(const TV *__restrict__ v, int r0, int nr, double o[3]) {
-  o[0] = (double)v[r0];
-  o[1] = nr > 1 ? (double)v[r0 + 1] : 0.0;
-  o[2] = nr > 2 ? (double)v[r0 + 2] : 0.0;
+  bj_invert_store(A, nr, binv + 6 * (int64_t)g);
 }
 
 // block-Jacobi start stage: z = M^-1 r, p = z; partials of r.z (ring slot 0) and b.(M^-1 b)

@@ -72,6 +72,58 @@ __device__ __forceinline__ int group_spmv(int g, int lane, const int32_t *__rest
   return r0;
 }
 
+// ---------------------------------------------------------------- node-block Jacobi
+// M = blockdiag(K_gg) over the row groups g (the free rows of a node, nr <= 3): z = M^-1 r
+// per group with the symmetric inverse stored as 6 floats (00, 01, 02, 11, 12, 22) -- the
+// bytes of the FP64 diagonal it replaces.  Entries of rows / columns beyond nr are zero.
+__device__ __forceinline__ void bj_apply(const float *__restrict__ B, const double r[3], double z[3]) {
+  const double b00 = B[0], b01 = B[1], b02 = B[2], b11 = B[3], b12 = B[4], b22 = B[5];
+  z[0] = b00 * r[0] + b01 * r[1] + b02 * r[2];
+  z[1] = b01 * r[0] + b11 * r[1] + b12 * r[2];
+  z[2] = b02 * r[0] + b12 * r[1] + b22 * r[2];
+}
+
+// B = the packed float inverse of the symmetric part of the nr x nr block A (entries beyond
+// nr ignored); a block that is not positive definite (leading minors) falls back to the
+// point-Jacobi inverse 1 / |A_ii| of its rows
+__device__ inline void bj_invert_store(double A[3][3], int nr, float *__restrict__ B) {
+  for (int a = 0; a < 3; ++a)
+    for (int b = a + 1; b < 3; ++b) A[a][b] = A[b][a] = 0.5 * (A[a][b] + A[b][a]);
+  for (int a = nr; a < 3; ++a) {
+    for (int b = 0; b < 3; ++b) A[a][b] = A[b][a] = 0.0;
+    A[a][a] = 1.0;  // pad: identity on absent rows
+  }
+  const double c00 = A[1][1] * A[2][2] - A[1][2] * A[1][2];
+  const double c01 = A[0][2] * A[1][2] - A[0][1] * A[2][2];
+  const double c02 = A[0][1] * A[1][2] - A[0][2] * A[1][1];
+  const double c11 = A[0][0] * A[2][2] - A[0][2] * A[0][2];
+  const double c12 = A[0][1] * A[0][2] - A[0][0] * A[1][2];
+  const double c22 = A[0][0] * A[1][1] - A[0][1] * A[0][1];
+  const double det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+  const bool spd = A[0][0] > 0.0 && c22 > 0.0 && det > 0.0 && det == det;
+  double o[6];
+  if (spd) {
+    const double id = 1.0 / det;
+    o[0] = c00 * id; o[1] = c01 * id; o[2] = c02 * id; o[3] = c11 * id; o[4] = c12 * id; o[5] = c22 * id;
+  } else {
+    for (int k = 0; k < 6; ++k) o[k] = 0.0;
+    const double d0 = fabs(A[0][0]), d1 = fabs(A[1][1]), d2 = fabs(A[2][2]);
+    o[0] = d0 == 0.0 ? 1.0 : 1.0 / d0;
+    o[3] = d1 == 0.0 ? 1.0 : 1.0 / d1;
+    o[5] = d2 == 0.0 ? 1.0 : 1.0 / d2;
+  }
+  if (nr < 3) { o[2] = o[4] = o[5] = 0.0; }
+  if (nr < 2) { o[1] = o[3] = 0.0; }
+  for (int k = 0; k < 6; ++k) B[k] = (float)o[k];
+}
+
+template <typename TV>
+__device__ __forceinline__ void bj_load(const TV *__restrict__ v, int r0, int nr, double o[3]) {
+  o[0] = (double)v[r0];
+  o[1] = nr > 1 ? (double)v[r0 + 1] : 0.0;
+  o[2] = nr > 2 ? (double)v[r0 + 2] : 0.0;
+}
+
 // sum of g partials, identical in every workgroup (all threads get the result)
 __device__ __forceinline__ double sum_partials(const double *__restrict__ part, int g) {
   __shared__ double res;

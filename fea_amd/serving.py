@@ -301,7 +301,7 @@ class SolveServer:
 
     def __init__(self, precision='mixed', hbm_budget=None, max_batch=64, rel_tol=1e-10, step_tol=1e-12,
                  cg_tol=1e-8, cg_max_iter=20000, max_newton=200, chunk=32, use_graph=False,
-                 armijo_slack=1e-12, verbose=False, predictor=True):
+                 armijo_slack=1e-12, verbose=False, predictor=True, preconditioner='block'):
         nat.require_gpu()
         if precision not in ('fp64', 'mixed'):
             raise ValueError("precision must be 'fp64' or 'mixed'")
@@ -318,7 +318,8 @@ class SolveServer:
         self.armijo_slack = float(armijo_slack)
         self.predictor = bool(predictor)
         self.verbose = verbose
-        self.pcg = BatchedPCG(self.mat_dtype, F64, 'cuda', chunk=chunk, use_graph=use_graph)
+        # node-block Jacobi by default (point Jacobi, the reference's scaling: 'jacobi')
+        self.pcg = BatchedPCG(self.mat_dtype, F64, 'cuda', chunk=chunk, use_graph=use_graph, precond=preconditioner)
         self.active: list[_Slot] = []
         self.used = 0
         self.rounds = 0
@@ -335,7 +336,7 @@ class SolveServer:
         U = torch.zeros(3 * plan.n_nodes, dtype=torch.float64, device=dev)
         slot = _Slot(req=req, fe=fe, plan=plan, U=U, U_prev=U.clone(), Unew=torch.empty_like(U),
                      dGC=torch.zeros(plan.n_free, dtype=torch.float64, device=dev),
-                     vec=PcgVectors(plan.n_free, F64, dev), r_ref=0.0,
+                     vec=PcgVectors(plan.n_free, F64, dev, n_groups=plan.n_groups), r_ref=0.0,
                      steps=int(req.spec.get('load_steps', 1)))
         reads = _Reads()
         _stats_launch(plan, plan.residual(U, 1.0, F64), reads)

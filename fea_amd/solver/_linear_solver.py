@@ -124,7 +124,7 @@ def conjugate_gradient(A_indices, A_values, b, x0=torch.zeros([0]), tol=1e-3, ma
 class PcgSystem(ctypes.Structure):
     """Mirror of ``fea_pcg_system`` (include/fea_amd.h)."""
     _fields_ = [('n', ctypes.c_int32), ('n_groups', ctypes.c_int32), ('spmv_blocks', ctypes.c_int32),
-                ('vec_blocks', ctypes.c_int32), ('max_iter', ctypes.c_int32), ('reserved', ctypes.c_int32),
+                ('vec_blocks', ctypes.c_int32), ('max_iter', ctypes.c_int32), ('precond', ctypes.c_int32),
                 ('nnz', ctypes.c_int64), ('tol', ctypes.c_double),
                 ('grp', ctypes.c_void_p), ('rowptr', ctypes.c_void_p), ('col', ctypes.c_void_p),
                 ('diag_slot', ctypes.c_void_p), ('val', ctypes.c_void_p), ('dinv', ctypes.c_void_p),
@@ -135,11 +135,15 @@ class PcgSystem(ctypes.Structure):
 class PcgVectors:
     """Device vectors and workspace of one system of a batched PCG."""
 
-    def __init__(self, n, vec_dtype, device):
+    def __init__(self, n, vec_dtype, device, n_groups=0):
         T = _DT[vec_dtype]
         mk = lambda: torch.zeros(max(1, n), dtype=T, device=device)[:n]
         self.n = int(n)
-        self.x, self.r, self.z, self.p, self.q, self.dinv, self.b = mk(), mk(), mk(), mk(), mk(), mk(), mk()
+        self.x, self.r, self.z, self.p, self.q, self.b = mk(), mk(), mk(), mk(), mk(), mk()
+        # the preconditioner: n diagonal inverses, or 6 floats per row group (node-block)
+        nb = max(self.n * T.itemsize, 24 * abs(int(n_groups)))
+        self.dinv = torch.zeros(max(1, (nb + 7) // 8), dtype=torch.float64, device=device)
+        self.block_ok = abs(int(n_groups)) > 0
         self.ws = torch.zeros(int(nat.lib().fea_pcg_ws_size(self.n)), dtype=torch.float64, device=device)
 
     def nbytes(self):
@@ -159,10 +163,15 @@ class BatchedPCG:
     the solution.  The host reads one status array per chunk of ``chunk``
     iterations for the whole batch.  With ``use_graph`` the chunk (3 launches
     per iteration plus the status kernel) is captured once per batch layout in
-    a HIP graph and replayed.
+    a HIP graph and replayed.  ``precond='block'``: node-block Jacobi for every
+    system whose vectors were made with its row-group count (``PcgVectors(...,
+    n_groups=)``), point Jacobi for the others.
     """
 
-    def __init__(self, mat_dtype, vec_dtype, device, chunk=32, use_graph=False):
+    def __init__(self, mat_dtype, vec_dtype, device, chunk=32, use_graph=False, precond='jacobi'):
+        if precond not in ('jacobi', 'block'):
+            raise ValueError("precond must be 'jacobi' or 'block'")
+        self.precond = precond
         self.mat_dtype, self.vec_dtype = mat_dtype, vec_dtype
         self.device = torch.device(device)
         self.chunk = int(chunk)
@@ -198,6 +207,8 @@ class BatchedPCG:
             s.grp, s.rowptr, s.col, s.diag_slot = (plan.grp.data_ptr(), plan.rowptr.data_ptr(), plan.spmv_col.data_ptr(),
                                                    plan.diag_slot.data_ptr())
             s.val, s.dinv, s.b, s.x = val.data_ptr(), v.dinv.data_ptr(), v.b.data_ptr(), v.x.data_ptr()
+            # node-block Jacobi needs the row groups and a preconditioner buffer sized for them
+            s.precond = 1 if (self.precond == 'block' and getattr(v, 'block_ok', False)) else 0
             s.r, s.z, s.p, s.q, s.ws = v.r.data_ptr(), v.z.data_ptr(), v.p.data_ptr(), v.q.data_ptr(), v.ws.data_ptr()
         ss = np.zeros(n + 1, dtype=np.int32)
         vs = np.zeros(n + 1, dtype=np.int32)

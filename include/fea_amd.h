@@ -253,7 +253,8 @@ int fea_pcg_spmv(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups,
  * the system array and both tables to the device and passes the totals
  * spmv_start[n_sys], vec_start[n_sys].  The same device copies serve every
  * start / iterate call until the batch changes.
- * fea_pcg_batch_start: dinv = 1/|diag A|, r = b - A x (x = warm start), z, p,
+ * fea_pcg_batch_start: dinv = 1/|diag A| (or the node-block inverses, precond 1),
+ * r = b - A x (x = warm start), z, p,
  * reset of the iteration count and convergence flag.
  * fea_pcg_batch_iterate: `iters` iterations of every unconverged system; with
  * status != NULL (device, 2*n_sys doubles) writes {iterations, done} per system.
@@ -266,9 +267,10 @@ typedef struct fea_pcg_system {
   int32_t spmv_blocks; /* set by fea_pcg_batch_layout */
   int32_t vec_blocks;  /* set by fea_pcg_batch_layout */
   int32_t max_iter;
-  int32_t reserved;
+  int32_t precond;     /* 0: point Jacobi (dinv: vec_dtype[n]); 1: node-block Jacobi (dinv:
+                          float[6 |n_groups|], written by fea_pcg_batch_start) */
   int64_t nnz;
-  double tol;          /* stop when r.z / b.(dinv b) < tol */
+  double tol;          /* stop when r.z / b.(M^-1 b) < tol */
   const int32_t *grp, *rowptr, *col, *diag_slot;
   const void *val;     /* mat_dtype */
   void *dinv;

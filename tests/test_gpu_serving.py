@@ -38,7 +38,7 @@ def _systems(n_sys, dtype):
 def _solve(bpcg, systems, tol=1e-26, max_iter=6000):
     entries, vecs = [], []
     for fe, p, R, K in systems:
-        v = PcgVectors(p.n_free, F64, 'cuda')
+        v = PcgVectors(p.n_free, F64, 'cuda', n_groups=p.n_groups)
         v.b.copy_(R).neg_()
         v.x.zero_()
         entries.append((p, K, v, tol, max_iter))
@@ -47,10 +47,11 @@ def _solve(bpcg, systems, tol=1e-26, max_iter=6000):
     return [v.x.clone() for v in vecs], iters
 
 
+@pytest.mark.parametrize('pc', ['jacobi', 'block'])
 @pytest.mark.parametrize('dtype', [F64, F32])
-def test_batched_pcg_matches_direct_and_alone(gpu, dtype):
+def test_batched_pcg_matches_direct_and_alone(gpu, dtype, pc):
     systems = _systems(5, dtype)
-    bp = BatchedPCG(dtype, F64, 'cuda', chunk=16)
+    bp = BatchedPCG(dtype, F64, 'cuda', chunk=16, precond=pc)
     xs, iters = _solve(bp, systems)
     for (fe, p, R, K), x, it in zip(systems, xs, iters):
         A = sp.csr_matrix((K.double().cpu().numpy(), p.col_np, p.rowptr_np), shape=(p.n_free, p.n_free))
@@ -62,11 +63,11 @@ def test_batched_pcg_matches_direct_and_alone(gpu, dtype):
         assert 0 < it < 6000
     # every system alone: identical iterates, so identical bits and counts
     for k, s in enumerate(systems):
-        xa, ia = _solve(BatchedPCG(dtype, F64, 'cuda', chunk=16), [s])
+        xa, ia = _solve(BatchedPCG(dtype, F64, 'cuda', chunk=16, precond=pc), [s])
         assert torch.equal(xa[0], xs[k]) and ia[0] == iters[k]
     # a different mix and order
     sub = [systems[3], systems[0], systems[4]]
-    xm, im = _solve(BatchedPCG(dtype, F64, 'cuda', chunk=7), sub)
+    xm, im = _solve(BatchedPCG(dtype, F64, 'cuda', chunk=7, precond=pc), sub)
     for x, k in zip(xm, (3, 0, 4)):
         assert torch.equal(x, xs[k])
 
