@@ -81,10 +81,14 @@ def test_block_jacobi_pcg_matches_oracle(gpu, prec, mode):
     assert abs(pcg.iterations - its) <= max(3, 0.05 * its)
 
 
-def test_static_block_jacobi_matches_oracle(gpu):
+@pytest.mark.parametrize('predictor', [True, 'auto'])
+def test_static_block_jacobi_matches_oracle(gpu, predictor):
+    """The bench's solver configuration (mixed precision, node-block Jacobi, Eisenstat-Walker
+    forcing, the 'auto' predictor) lands on the oracle's equilibrium."""
     fe, mesh = workloads.beam_traction(8, 2, 2, traction=(0.0, 0.0, -0.01))
     fe.solver = solver.StaticImplicitSolver(precision='mixed', rel_tol=1e-10, load_steps=4, cg_tol=1e-12,
-                                            armijo_slack=1e-12, preconditioner='block')
+                                            armijo_slack=1e-12, preconditioner='block', predictor=predictor,
+                                            forcing='ew' if predictor == 'auto' else 'fixed')
     GC = fe.solve()
     assert GC is not False
     om = oracle_model(mesh, mr(0.5, 0.1, 0.01), 'x0', traction=(0.0, 0.0, -0.01), load_set='x1')
