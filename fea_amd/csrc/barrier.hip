@@ -480,29 +480,7 @@ __global__ __launch_bounds__(64) void k_bar_assemble_wg(fea_barrier_params P, in
 
 // exclusive scan of n counts into out[0..n], one workgroup of 1024 lanes over chunks
 __global__ __launch_bounds__(1024) void k_scan_i32(int n, const int32_t *__restrict__ in, int32_t *__restrict__ out) {
-  __shared__ int32_t part[1024];
-  const int t = threadIdx.x;
-  const int chunk = (n + 1023) / 1024;
-  const int b0 = t * chunk, b1 = min(n, b0 + chunk);
-  int32_t s = 0;
-  for (int b = b0; b < b1; ++b) s += in[b];
-  part[t] = s;
-  __syncthreads();
-  if (t == 0) {
-    int32_t acc = 0;
-    for (int i = 0; i < 1024; ++i) {
-      const int32_t v = part[i];
-      part[i] = acc;
-      acc += v;
-    }
-    out[n] = acc;
-  }
-  __syncthreads();
-  int32_t acc = part[t];
-  for (int b = b0; b < b1; ++b) {
-    out[b] = acc;
-    acc += in[b];
-  }
+  scan_1024(n, in, out);
 }
 
 }  // namespace bar

@@ -91,4 +91,43 @@ __device__ __forceinline__ double block_sum(double v) {
   return s;
 }
 
+// exclusive scan of in[0..n) into out[0..n] (out[n] = the total) by one workgroup of 1024
+// threads: each thread sums a contiguous chunk, the 1024 chunk sums are scanned with wave
+// shuffles (16 waves) and a scan of the 16 wave totals, and each thread writes its chunk's
+// prefixes.  Integer and exact.
+__device__ inline void scan_1024(int n, const int32_t *__restrict__ in, int32_t *__restrict__ out) {
+  __shared__ int32_t wsum[16];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+  const int chunk = (n + 1023) / 1024;
+  const int b0 = t * chunk, b1 = min(n, b0 + chunk);
+  int32_t s = 0;
+  for (int b = b0; b < b1; ++b) s += in[b];
+  int32_t v = s;  // inclusive scan within the wave
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t u = __shfl_up(v, off, 64);
+    if (lane >= off) v += u;
+  }
+  if (lane == 63) wsum[wv] = v;
+  __syncthreads();
+  if (wv == 0) {
+    int32_t w = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      const int32_t u = __shfl_up(w, off, 64);
+      if (lane >= off) w += u;
+    }
+    if (lane < 16) wsum[lane] = w;  // inclusive over the waves
+  }
+  __syncthreads();
+  const int32_t incl = v + (wv > 0 ? wsum[wv - 1] : 0);
+  if (t == 1023) out[n] = incl;
+  int32_t acc = incl - s;
+  for (int b = b0; b < b1; ++b) {
+    out[b] = acc;
+    acc += in[b];
+  }
+}
+
 }  // namespace fea

@@ -63,32 +63,10 @@ __global__ __launch_bounds__(kBlock) void k_bucket_count(int n, const int32_t *_
   if (i < n && keys[i] >= 0) atomicAdd(&cnt[keys[i]], 1);
 }
 
-// exclusive scan of cnt[0..nb) into start[0..nb], single workgroup
+// exclusive scan of cnt[0..nb) into start[0..nb], one workgroup of 1024 threads
 __global__ __launch_bounds__(1024) void k_bucket_scan(int nb, const int32_t *__restrict__ cnt,
                                                       int32_t *__restrict__ start) {
-  __shared__ int32_t part[1024];
-  const int t = threadIdx.x;
-  const int chunk = (nb + 1023) / 1024;
-  const int b0 = t * chunk, b1 = min(nb, b0 + chunk);
-  int32_t s = 0;
-  for (int b = b0; b < b1; ++b) s += cnt[b];
-  part[t] = s;
-  __syncthreads();
-  if (t == 0) {
-    int32_t acc = 0;
-    for (int i = 0; i < 1024; ++i) {
-      const int32_t v = part[i];
-      part[i] = acc;
-      acc += v;
-    }
-    start[nb] = acc;
-  }
-  __syncthreads();
-  int32_t acc = part[t];
-  for (int b = b0; b < b1; ++b) {
-    start[b] = acc;
-    acc += cnt[b];
-  }
+  scan_1024(nb, cnt, start);
 }
 
 __global__ __launch_bounds__(kBlock) void k_bucket_fill(int n, const int32_t *__restrict__ keys,

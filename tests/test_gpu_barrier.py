@@ -170,3 +170,15 @@ def test_two_surface_contact_matches_oracle(gpu):
     GC_o, _ = osol.static_solve(om, increments=4, rel_tol=1e-10, linear='direct', armijo_slack_rel=1e-12,
                                 predictor=False, max_iter=100)
     assert rel_err(GC.cpu().numpy(), GC_o) < 1e-7
+
+
+@pytest.mark.parametrize('n', [1, 63, 1024, 1025, 70001])
+def test_scan_i32_matches_cumsum(gpu, n):
+    """fea_scan_i32 (the one-workgroup exclusive scan behind the pair search and the contact
+    buckets): out[k] = sum in[:k], out[n] = the total."""
+    from fea_amd import _native as nat
+    from fea_amd._native import call
+    a = np.random.default_rng(n).integers(0, 9, n).astype(np.int32)
+    out = torch.full((n + 1,), -1, dtype=torch.int32, device='cuda')
+    call('fea_scan_i32', n, torch.as_tensor(a, device='cuda'), out, nat.stream_ptr())
+    assert np.array_equal(out.cpu().numpy(), np.concatenate([[0], np.cumsum(a)]).astype(np.int32))
