@@ -63,10 +63,14 @@ __global__ __launch_bounds__(kBlock) void k_bucket_count(int n, const int32_t *_
   if (i < n && keys[i] >= 0) atomicAdd(&cnt[keys[i]], 1);
 }
 
-// exclusive scan of cnt[0..nb) into start[0..nb], one workgroup of 1024 threads
-__global__ __launch_bounds__(1024) void k_bucket_scan(int nb, const int32_t *__restrict__ cnt,
+// exclusive scan of cnt[0..nb) into start[0..nb], one workgroup of 1024 threads; cnt is
+// then cleared (it is reused as the fill cursor): each thread clears the chunk it alone read
+__global__ __launch_bounds__(1024) void k_bucket_scan(int nb, int32_t *__restrict__ cnt,
                                                       int32_t *__restrict__ start) {
   scan_1024(nb, cnt, start);
+  const int chunk = (nb + 1023) / 1024;
+  const int b0 = threadIdx.x * chunk, b1 = min(nb, b0 + chunk);
+  for (int b = b0; b < b1; ++b) cnt[b] = 0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_bucket_fill(int n, const int32_t *__restrict__ keys,
@@ -318,8 +322,7 @@ static int bucket_sort(int n, const int32_t *keys, int nb, int32_t *start, int32
                        hipStream_t st) {
   k_fill_int<<<blocks_for(nb), kBlock, 0, st>>>(nb, work, 0);
   k_bucket_count<<<blocks_for(n > 0 ? n : 1), kBlock, 0, st>>>(n, keys, work);
-  k_bucket_scan<<<1, 1024, 0, st>>>(nb, work, start);
-  k_fill_int<<<blocks_for(nb), kBlock, 0, st>>>(nb, work, 0);
+  k_bucket_scan<<<1, 1024, 0, st>>>(nb, work, start);  // leaves work zeroed for the fill cursor
   k_bucket_fill<<<blocks_for(n > 0 ? n : 1), kBlock, 0, st>>>(n, keys, start, work, items);
   k_bucket_sort<<<blocks_for(nb), kBlock, 0, st>>>(nb, start, items);
   FEA_CHECK_LAUNCH("bucket sort");
