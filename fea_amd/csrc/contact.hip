@@ -8,6 +8,8 @@
 //   pair (s, f) is a candidate iff d2 <= r2.
 // Deterministic throughout: bucket sorts order items by id inside each bucket
 // and every force is summed by one thread in a fixed order (no float atomics).
+#include <climits>
+
 #include "common.h"
 
 namespace fea {
@@ -194,21 +196,28 @@ __global__ __launch_bounds__(kBlock) void k_contact_search(
 constexpr double kInsideTol = 1e-10;
 constexpr double kExtendTol = 0.1;
 
+// One wave per slave node: lane i evaluates candidates i, i + 64, ... of the slave's sorted
+// candidate list and keeps its best by (tier ascending, key descending, index ascending); a
+// butterfly over the wave picks the best of the lanes by the same order, which is the
+// candidate a serial scan of the list keeps (the first of equal ones).
+constexpr int kPenWaves = kBlock / 64;
+
 __global__ __launch_bounds__(kBlock) void k_contact_penalty(
     int ns, const int32_t *__restrict__ slave, const double *__restrict__ X, const double *__restrict__ U,
     const int32_t *__restrict__ conn, const int32_t *__restrict__ cand, const int32_t *__restrict__ count,
     int kmax, double k_pen, const double *__restrict__ area, double depth_max, int32_t *__restrict__ chosen,
     double *__restrict__ force,
     double *__restrict__ weight, double *__restrict__ energy_partial) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int s = blockIdx.x * kPenWaves + (threadIdx.x >> 6);
   double e = 0.0;
-  if (s < ns) {
+  if (s < ns) {  // uniform over the wave
     double p[3];
     node_pos(X, U, slave[s], p);
-    int best = -1, btier = 2;
+    int bf = -1, bi = INT_MAX, btier = 2;
     double bg = 0.0, bkey = 0.0, bn[3] = {0, 0, 0}, bw[3] = {0, 0, 0};
     const int nc = min(count[s], kmax);
-    for (int i = 0; i < nc; ++i) {
+    for (int i = lane; i < nc; i += 64) {
       const int f = cand[(int64_t)s * kmax + i];
       double a[3], b[3], c[3];
       node_pos(X, U, conn[3 * (int64_t)f + 0], a);
@@ -246,7 +255,8 @@ __global__ __launch_bounds__(kBlock) void k_contact_penalty(
       const int tier = (v >= -kInsideTol) ? 0 : 1;
       const double key = (tier == 0) ? g : v;
       if (tier < btier || (tier == btier && key > bkey)) {
-        best = f;
+        bf = f;
+        bi = i;
         btier = tier;
         bkey = key;
         bg = g;
@@ -258,16 +268,41 @@ __global__ __launch_bounds__(kBlock) void k_contact_penalty(
         bw[2] = eta;
       }
     }
-    chosen[s] = best;
-    const double ks = (area != nullptr) ? k_pen * area[s] : k_pen;
-    const double fm = (best >= 0) ? -ks * bg : 0.0;
-    force[3 * (int64_t)s + 0] = fm * bn[0];
-    force[3 * (int64_t)s + 1] = fm * bn[1];
-    force[3 * (int64_t)s + 2] = fm * bn[2];
-    weight[3 * (int64_t)s + 0] = bw[0];
-    weight[3 * (int64_t)s + 1] = bw[1];
-    weight[3 * (int64_t)s + 2] = bw[2];
-    if (best >= 0) e = 0.5 * ks * bg * bg;
+    // the wave's best lane
+    int wl = lane;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const int ot = __shfl_xor(btier, off, 64);
+      const double ok = __shfl_xor(bkey, off, 64);
+      const int oi = __shfl_xor(bi, off, 64);
+      const int ol = __shfl_xor(wl, off, 64);
+      if (ot < btier || (ot == btier && (ok > bkey || (ok == bkey && oi < bi)))) {
+        btier = ot;
+        bkey = ok;
+        bi = oi;
+        wl = ol;
+      }
+    }
+    bf = __shfl(bf, wl, 64);
+    bg = __shfl(bg, wl, 64);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      bn[j] = __shfl(bn[j], wl, 64);
+      bw[j] = __shfl(bw[j], wl, 64);
+    }
+    if (lane == 0) {
+      const int best = btier < 2 ? bf : -1;
+      chosen[s] = best;
+      const double ks = (area != nullptr) ? k_pen * area[s] : k_pen;
+      const double fm = (best >= 0) ? -ks * bg : 0.0;
+      force[3 * (int64_t)s + 0] = fm * bn[0];
+      force[3 * (int64_t)s + 1] = fm * bn[1];
+      force[3 * (int64_t)s + 2] = fm * bn[2];
+      weight[3 * (int64_t)s + 0] = best >= 0 ? bw[0] : 0.0;
+      weight[3 * (int64_t)s + 1] = best >= 0 ? bw[1] : 0.0;
+      weight[3 * (int64_t)s + 2] = best >= 0 ? bw[2] : 0.0;
+      if (best >= 0) e = 0.5 * ks * bg * bg;
+    }
   }
   if (energy_partial != nullptr) {
     const double t = block_sum(e);
@@ -369,7 +404,9 @@ extern "C" int fea_contact_search(int n_slave, const int32_t *slave, const doubl
   return FEA_OK;
 }
 
-extern "C" int fea_contact_blocks(int n_slave) { return n_slave > 0 ? blocks_for(n_slave) : 0; }
+extern "C" int fea_contact_blocks(int n_slave) {
+  return n_slave > 0 ? (n_slave + kPenWaves - 1) / kPenWaves : 0;  // one wave per slave
+}
 
 extern "C" int fea_contact_forces(int n_slave, const int32_t *slave, const double *X, const double *U,
                                   const int32_t *conn, int n_facets, const int32_t *cand,
@@ -392,7 +429,7 @@ extern "C" int fea_contact_forces(int n_slave, const int32_t *slave, const doubl
   FEA_CHECK_ARG(slave && X && U && conn && cand && count && chosen && force && weight && sb_start && sb_items &&
                     work && (n_master == 0 || (master_nodes && nf_ptr && nf_facet && nf_vert)),
                 "fea_contact_forces: null argument");
-  k_contact_penalty<<<blocks_for(n_slave), kBlock, 0, st>>>(n_slave, slave, X, U, conn, cand, count, kmax,
+  k_contact_penalty<<<fea_contact_blocks(n_slave), kBlock, 0, st>>>(n_slave, slave, X, U, conn, cand, count, kmax,
                                                             penalty, slave_area, depth_max, chosen, force,
                                                             weight,
                                                             energy_partial);
