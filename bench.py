@@ -833,7 +833,8 @@ def bench_adjoint(args, torch):
     warmup = 1 if args.warmup is None else args.warmup
     fe, mesh = workloads.void_block_pressure()
     fe.solver = solver.StaticImplicitSolver(precision='fp64', rel_tol=1e-11, load_steps=2, cg_tol=1e-22,
-                                            cg_max_iter=20000, armijo_slack=1e-12)
+                                            cg_max_iter=20000, armijo_slack=1e-12,
+                                            preconditioner=args.couple_preconditioner)
     fe.initialize()
     p = fe.assembly.plan
     E = sum(g.n for g in p.groups if g.kind == 'elem')
