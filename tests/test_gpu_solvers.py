@@ -81,6 +81,38 @@ def test_block_jacobi_pcg_matches_oracle(gpu, prec, mode):
     assert abs(pcg.iterations - its) <= max(3, 0.05 * its)
 
 
+@pytest.mark.parametrize('prec', ['fp64', 'mixed'])
+def test_block_jacobi_inverse_and_fallback_match_oracle(gpu, prec):
+    """fea_block_jacobi_inv against the oracle's restatement on every node block, with some
+    blocks made indefinite (a negated diagonal entry): those fall back to 1 / |K_ii|."""
+    from fea_amd import _native as nat
+    from fea_amd._native import call
+    fe, om = cube_models()
+    fe.assembly.initialize()
+    p = fe.assembly.plan
+    mat = F64 if prec == 'fp64' else F32
+    U = p.gc_to_u(torch.as_tensor(0.01 * np.random.default_rng(4).standard_normal(p.n_free), device='cuda'))
+    _, K = p.tangent(U, 1.0, F64, val_dtype=mat)
+    K = K.clone()
+    grp = p.grp.cpu().numpy()
+    flip = grp[:-1][::7]                                    # the first row of every 7th group
+    ds = p.diag_slot.cpu().numpy()
+    K[torch.as_tensor(ds[flip], device='cuda', dtype=torch.int64)] *= -1
+    binv = torch.zeros(6 * p.n_groups, dtype=torch.float32, device='cuda')
+    call('fea_block_jacobi_inv', mat, p.n_free, p.spmv_groups, p.grp, p.rowptr, p.col, p.diag_slot, K, binv,
+         nat.stream_ptr())
+    A = sp.csr_matrix((K.double().cpu().numpy(), p.col_np, p.rowptr_np), shape=(p.n_free, p.n_free))
+    M = osol.block_jacobi_inverse(A, grp)
+    iu = ([0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2])
+    got = binv.cpu().numpy().reshape(-1, 6).astype(np.float64)
+    want = M[:, iu[0], iu[1]]
+    assert np.abs(got - want).max() <= 1e-6 * np.abs(want).max()
+    # the flipped groups took the point-Jacobi fallback (diagonal only)
+    fl = np.arange(0, p.n_groups, 7)
+    assert np.all(got[fl][:, [1, 2, 4]] == 0.0)
+    assert np.allclose(got[fl, 0], 1.0 / np.abs(A.diagonal()[grp[fl]]), rtol=1e-6)
+
+
 @pytest.mark.parametrize('predictor', [True, 'auto'])
 def test_static_block_jacobi_matches_oracle(gpu, predictor):
     """The bench's solver configuration (mixed precision, node-block Jacobi, Eisenstat-Walker
