@@ -72,6 +72,35 @@ def test_batched_pcg_matches_direct_and_alone(gpu, dtype, pc):
         assert torch.equal(x, xs[k])
 
 
+@pytest.mark.parametrize('pc', ['jacobi', 'block'])
+def test_batched_pcg_node_groups_of_one_and_two_rows(gpu, pc):
+    """Systems whose row groups are not all 3 rows (a face held in x only: its nodes have 2
+    free rows; the generic node-grouped SpMV, and for node-block Jacobi the blocks confirmed
+    against the CSR columns) beside a triple-structured one: the batched solve matches a
+    direct solve and each system's iterates do not depend on the batch."""
+    from fea_amd import boundarys
+    systems = []
+    for k, face in enumerate(('x0', 'y1')):
+        fe, _ = workloads.cube_pressure(3, pressure=0.2, jitter=0.05, seed=20 + k)
+        fe.assembly.add_boundary(boundarys.Boundary_Condition('block', face, indexDoF=[0]))
+        fe.assembly.initialize()
+        p = fe.assembly.plan
+        assert p.spmv_groups > 0 and np.diff(p.grp.cpu().numpy()).min() < 3     # not triple-structured
+        U = p.gc_to_u(torch.as_tensor(0.01 * np.random.default_rng(30 + k).standard_normal(p.n_free),
+                                      device='cuda'))
+        R, K = p.tangent(U, 1.0, F64, val_dtype=F64)
+        systems.append((fe, p, R.clone(), K.clone()))
+    systems += _systems(1, F64)
+    xs, iters = _solve(BatchedPCG(F64, F64, 'cuda', chunk=16, precond=pc), systems)
+    for (fe, p, R, K), x, it in zip(systems, xs, iters):
+        A = sp.csr_matrix((K.double().cpu().numpy(), p.col_np, p.rowptr_np), shape=(p.n_free, p.n_free))
+        b = -R.cpu().numpy()
+        assert rel_err(x.cpu().numpy(), spla.spsolve(A.tocsc(), b)) < 1e-7, it
+    for k, s in enumerate(systems):
+        xa, ia = _solve(BatchedPCG(F64, F64, 'cuda', chunk=16, precond=pc), [s])
+        assert torch.equal(xa[0], xs[k]) and ia[0] == iters[k]
+
+
 def test_batched_pcg_graph_replays_match_eager(gpu):
     systems = _systems(4, F32)
     eager, it_e = _solve(BatchedPCG(F32, F64, 'cuda', chunk=8), systems, tol=1e-18)
