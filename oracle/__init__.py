@@ -19,8 +19,13 @@ Pinning status (what independently checks each part; see tests/test_oracle_*.py)
   MR(C01=0) == NH(mu=2C10, kappa=2/D1), objectivity W(QF)=W(F).   -> pinned
 * ``pattern`` CSR (crow, col) of the reduced free-DOF matrix: brute-force
   dense construction from connectivity + free mask.                 -> pinned
-* ``surfaces`` pressure volume / dV / d2V: divergence theorem (closed
-  surface volume), FD / complex step of V; traction resultant = t*A0. -> pinned
+* ``surfaces`` T3/T6/Q4 tables and Gauss rules: N_a(node_b) = delta_ab,
+  monomial exactness of the rules; pressure volume / dV / d2V and traction
+  on closed off-origin box surfaces with non-affine facets: V = box volume
+  (divergence theorem), sum w*detJ0 = area, traction resultant = t*A0 and
+  its moment, zero net pressure force and moment, a third of the vector
+  area on an open face, central differences of V and of the force
+  (tests/test_oracle_surfaces.py).                                  -> pinned
 * ``solvers`` CG: vs numpy.linalg.solve; Newton: residual at solution,
   path independence vs a direct-solve Newton.                        -> pinned
 * ``explicit``: lumped mass totals = rho*V, 1-DOF oscillator closed form.

@@ -35,12 +35,16 @@ def surf_gauss(stype):
     raise ValueError(stype)
 
 
-def surf_shape(stype):
-    """shape_function_gaussian = [N [g,a], dN [g,m,a]] (basesurface.py:122-141)."""
+def surf_shape_at(stype, pts):
+    """N [g,a] and dN [g,m,a] of a facet type at parametric points pts [g,2].
+
+    N_a = sum_b table[a,b] * monomial_b(pts) over the monomials of
+    basesurface.py:122-132; dN from the derivative of each monomial
+    (basesurface.py:152-170 restated on exponents)."""
     tab = SURF_TABLES[stype]
     nm = tab.shape[1]
     ex = MONO2[:nm]
-    pts, w = surf_gauss(stype)
+    pts = np.asarray(pts, dtype=np.float64)
     mono = np.prod(pts[:, None, :] ** ex[None], axis=2)
     dmono = np.zeros((pts.shape[0], 2, nm))
     for m in range(2):
@@ -50,6 +54,13 @@ def surf_shape(stype):
         dmono[:, m] = coef[None] * np.prod(pts[:, None, :] ** e2[None], axis=2)
     N = mono @ tab.T
     dN = np.einsum('gmb,ab->gma', dmono, tab)
+    return N, dN
+
+
+def surf_shape(stype):
+    """shape_function_gaussian = [N [g,a], dN [g,m,a]] (basesurface.py:122-141)."""
+    pts, w = surf_gauss(stype)
+    N, dN = surf_shape_at(stype, pts)
     return N, dN, w
 
 
