@@ -5,8 +5,10 @@ Each time step minimises the incremental functional
     Pi_inc(GC) = Pi(GC) + (GC - GC_pre) . M (GC - GC_pre) / (2 beta dt^2)
 
 (get_incremental_energy / get_incremental_stiffness_matrix, :100-144) from the
-predictor GC_pre = GC_n + dt GV_n + 0.25 dt^2 GA_n (the literal 0.25 of :181)
-with the Newton / Armijo iteration of the static solver, the effective tangent
+predictor GC_pre = GC_n + dt GV_n + (1/2 - beta) dt^2 GA_n (:181 writes the
+literal 0.25, which is this value at the default beta = 1/4; any other beta
+needs 1/2 - beta for the converged step to satisfy M GA + R = 0 with the
+acceleration update of :261-281) with the Newton / Armijo iteration of the static solver, the effective tangent
 K + M / (beta dt^2) in the tangent's own CSR (the consistent mass M is
 assembled once into the same pattern, C3base.py:348-384 / assembly.py:421-443)
 and the reference's stopping rule max|dGC| < tol and max|R| < tol (:468); then
@@ -120,7 +122,7 @@ class DynamicImplicitSolver(StaticImplicitSolver):
         self._r_ref = None
         self._increment = 1
         while t < self._time_end - 1e-12 * max(1.0, self._time_end):
-            self._GC_pre = GC + dt * GV + 0.25 * dt ** 2 * GA
+            self._GC_pre = GC + dt * GV + (0.5 - self._beta) * dt ** 2 * GA
             U = p.gc_to_u(GC.contiguous())
             self._iter_now = 0
             if not self._newton(U, 1.0, self.tol_error):

@@ -6,7 +6,13 @@ squared scaled residual, 1500 / 6000 CG iterations, warm start from the
 previous direction, the low-alpha bookkeeping).  All heavy work runs in HIP
 kernels on the device plan; the host only takes the branch decisions.
 
-Additions (keyword-only, defaults reproduce the reference):
+Additions (keyword-only, defaults reproduce the reference's rules except for the
+round-off step below):
+
+* In every mode the line search takes the full Newton step when the predicted
+  energy decrease |dGC . R| is at most 1e-13 |E| (below the energy's float64
+  resolution, so the Armijo test cannot resolve it); the reference backtracks
+  alpha to 0 there and decides from max|R|.
 
 * ``precision``: 'fp64' (everything FP64); 'mixed' (FP64 element arithmetic,
   tangent stored once-rounded in FP32, PCG on the FP32 matrix with FP64
@@ -221,7 +227,9 @@ class StaticImplicitSolver(BaseSolver):
             raise ValueError('dGC has nan or inf')
         if abs(deltaE) <= 1e-13 * abs(energy0):
             # the predicted decrease is below the energy's rounding: the Armijo test
-            # cannot resolve it, so take the full Newton step (only near convergence)
+            # cannot resolve it, so take the full Newton step (only near convergence).
+            # A deviation in every mode: the reference backtracks alpha to 0 here and
+            # then decides convergence from max|R| (static_implicit.py:98-177, :290).
             p.dof_axpy(U, sign, dGC, Unew)
             self.stats['energy_evals'] = self.stats.get('energy_evals', 0) + 1
             return 1.0, self._energy(Unew, scale)

@@ -342,8 +342,8 @@ def newmark_steps(residual_tangent, energy, M, GC0, GV0, dt, nsteps, beta=0.25, 
     """Newmark-beta implicit dynamics (DynamicImplicitSolver, dynamic_implicit.py:146-281, :339-472)
     for generic callables: residual_tangent(GC) -> (R, K) with R = dPi/dGC, energy(GC) -> Pi.
 
-    Per step: predictor GC_pre = GC + dt GV + 0.25 dt^2 GA (the literal 0.25 of
-    :181); Newton with direct solves on the incremental functional
+    Per step: predictor GC_pre = GC + dt GV + (1/2 - beta) dt^2 GA (:181 has the
+    literal 0.25, the same at beta = 1/4); Newton with direct solves on the incremental functional
     Pi(GC) + (GC - GC_pre).M.(GC - GC_pre) / (2 beta dt^2) (:100-144), Armijo
     c1 = 0.3 with halving (:285-330), stop when max|dGC| and max|R| < tol_error
     (:468); then the Newmark velocity / acceleration update (:261-281).  The
@@ -356,7 +356,7 @@ def newmark_steps(residual_tangent, energy, M, GC0, GV0, dt, nsteps, beta=0.25, 
     GA = spla.spsolve(M.tocsc(), -R0) if M.shape[0] > 1 else -R0 / M.toarray()[0, 0]
     out = [(GC.copy(), GV.copy(), np.atleast_1d(GA).copy())]
     for _ in range(nsteps):
-        pre = GC + dt * GV + 0.25 * dt * dt * GA
+        pre = GC + dt * GV + (0.5 - beta) * dt * dt * GA
 
         def inc_energy(x):
             d = x - pre

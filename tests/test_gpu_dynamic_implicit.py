@@ -50,21 +50,29 @@ def _velocity(om, amp):
     return V.reshape(-1)[oasm.free_mask(om)]
 
 
-def test_newmark_matches_oracle(gpu):
+@pytest.mark.parametrize('beta,gamma', [(0.25, 0.5), (0.3, 0.6)])
+def test_newmark_matches_oracle(gpu, beta, gamma):
     fe, om, _ = model('C3D8')
     dt, n = 0.05, 6
     fe.solver = solver.DynamicImplicitSolver(deltaT=dt, time_end=n * dt, tol_error=1e-11, cg_tol=1e-26,
-                                             cg_max_iter=20000)
+                                             cg_max_iter=20000, beta=beta, gamma=gamma)
     fe.initialize()
     GV0 = _velocity(om, 0.3)
     GC0 = np.zeros_like(GV0)
     assert fe.solver.solve(GC0=torch.as_tensor(GC0, device='cuda'), GV0=torch.as_tensor(GV0, device='cuda'))
-    out = osol.newmark(om, GC0, GV0, dt, n, tol_error=1e-11)
+    out = osol.newmark(om, GC0, GV0, dt, n, tol_error=1e-11, beta=beta, gamma=gamma)
     assert len(fe.solver._GC_list) == n + 1
     for k in (1, n):
         assert rel_err(fe.solver._GC_list[k].cpu().numpy(), out[k][0]) < 1e-8
         assert rel_err(fe.solver._GV_list[k].cpu().numpy(), out[k][1]) < 1e-7
     assert np.abs(out[n][0]).max() > 1e-2                            # the block moved
+    # the equation of motion M GA + R(GC) = 0 holds at the last step
+    p = fe.assembly.plan
+    GC, GA = fe.solver._GC_list[n].contiguous(), fe.solver._GA_list[n].contiguous()
+    MA = torch.empty_like(GA)
+    p.spmv(p.mass_matrix(), GA, MA, 1)
+    R = p.residual(p.gc_to_u(GC), 1.0, 1)
+    assert float((MA + R).abs().max()) < 1e-6 * float(MA.abs().max())
 
 
 def test_initial_acceleration_of_a_loaded_state(gpu):

@@ -75,21 +75,25 @@ def test_batched_pcg_matches_direct_and_alone(gpu, dtype, pc):
 @pytest.mark.parametrize('pc', ['jacobi', 'block'])
 def test_batched_pcg_node_groups_of_one_and_two_rows(gpu, pc):
     """Systems whose row groups are not all 3 rows (a face held in x only: its nodes have 2
-    free rows; the generic node-grouped SpMV, and for node-block Jacobi the blocks confirmed
+    free rows; a face held in x and y: 1 free row; the generic node-grouped SpMV, and for node-block Jacobi the blocks confirmed
     against the CSR columns) beside a triple-structured one: the batched solve matches a
     direct solve and each system's iterates do not depend on the batch."""
     from fea_amd import boundarys
     systems = []
-    for k, face in enumerate(('x0', 'y1')):
+    sizes = set()
+    for k, (face, dofs) in enumerate((('x0', [0]), ('y1', [0]), ('y1', [0, 1]))):
         fe, _ = workloads.cube_pressure(3, pressure=0.2, jitter=0.05, seed=20 + k)
-        fe.assembly.add_boundary(boundarys.Boundary_Condition('block', face, indexDoF=[0]))
+        fe.assembly.add_boundary(boundarys.Boundary_Condition('block', face, indexDoF=dofs))
         fe.assembly.initialize()
         p = fe.assembly.plan
-        assert p.spmv_groups > 0 and np.diff(p.grp.cpu().numpy()).min() < 3     # not triple-structured
+        g = np.diff(p.grp.cpu().numpy())
+        sizes |= set(g.tolist())
+        assert p.spmv_groups > 0 and g.min() < 3                 # not triple-structured
         U = p.gc_to_u(torch.as_tensor(0.01 * np.random.default_rng(30 + k).standard_normal(p.n_free),
                                       device='cuda'))
         R, K = p.tangent(U, 1.0, F64, val_dtype=F64)
         systems.append((fe, p, R.clone(), K.clone()))
+    assert sizes == {1, 2, 3}                                    # 1-, 2- and 3-row node groups all occur
     systems += _systems(1, F64)
     xs, iters = _solve(BatchedPCG(F64, F64, 'cuda', chunk=16, precond=pc), systems)
     for (fe, p, R, K), x, it in zip(systems, xs, iters):

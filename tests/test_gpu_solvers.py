@@ -48,14 +48,29 @@ def test_pcg_matches_direct_solve(gpu):
     assert abs(pcg.iterations - its) <= max(3, 0.05 * its)
 
 
-@pytest.mark.parametrize('mode', ['triple', 'node'])
+def partial_models():
+    """The cube with a face held in x only and a face held in x and y: node groups of
+    1, 2 and 3 free rows (not triple-structured)."""
+    from fea_amd import boundarys
+    fe, om = cube_models()
+    fe.assembly.add_boundary(boundarys.Boundary_Condition('bx', 'x0', indexDoF=[0]))
+    fe.assembly.add_boundary(boundarys.Boundary_Condition('bxy', 'y1', indexDoF=[0, 1]))
+    fe.assembly.initialize()
+    return fe
+
+
+@pytest.mark.parametrize('mode', ['triple', 'node', 'partial'])
 @pytest.mark.parametrize('prec', ['fp64', 'mixed'])
 def test_block_jacobi_pcg_matches_oracle(gpu, prec, mode):
     """Node-block Jacobi PCG (fea_block_jacobi_inv + precond 1) against the oracle's
     restatement after fixed iteration counts, and converged against a direct solve; with the
-    row-per-thread stage of triple-structured systems and the group-per-thread one."""
-    fe, om = cube_models()
-    fe.assembly.initialize()
+    row-per-thread stage of triple-structured systems, the group-per-thread one, and a model
+    with 1- and 2-row node groups (column-checked block extraction, generic SpMV)."""
+    if mode == 'partial':
+        fe = partial_models()
+    else:
+        fe, om = cube_models()
+        fe.assembly.initialize()
     p = fe.assembly.plan
     mat = F64 if prec == 'fp64' else F32
     U = p.gc_to_u(torch.as_tensor(0.01 * np.random.default_rng(3).standard_normal(p.n_free), device='cuda'))
@@ -63,9 +78,14 @@ def test_block_jacobi_pcg_matches_oracle(gpu, prec, mode):
     A = sp.csr_matrix((K.double().cpu().numpy(), p.col_np, p.rowptr_np), shape=(p.n_free, p.n_free))
     b = -R.cpu().numpy()
     grp = p.grp.cpu().numpy()
-    assert np.diff(grp).max() == 3 and np.diff(grp).min() >= 1
-    assert p.spmv_groups < 0                                     # triple-structured
-    ng = p.spmv_groups if mode == 'triple' else -p.spmv_groups
+    if mode == 'partial':
+        assert set(np.diff(grp).tolist()) == {1, 2, 3}
+        assert p.spmv_groups > 0                                 # not triple-structured
+        ng = p.spmv_groups
+    else:
+        assert np.diff(grp).max() == 3 and np.diff(grp).min() >= 1
+        assert p.spmv_groups < 0                                 # triple-structured
+        ng = p.spmv_groups if mode == 'triple' else -p.spmv_groups
     pcg = solver.PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, mat, p.device, vec_dtype=F64, grp=p.grp,
                      n_groups=ng, chunk=16, precond='block', spmv_col=p.spmv_col)
     pcg.set_matrix(K)
@@ -81,14 +101,19 @@ def test_block_jacobi_pcg_matches_oracle(gpu, prec, mode):
     assert abs(pcg.iterations - its) <= max(3, 0.05 * its)
 
 
+@pytest.mark.parametrize('model', ['cube', 'partial'])
 @pytest.mark.parametrize('prec', ['fp64', 'mixed'])
-def test_block_jacobi_inverse_and_fallback_match_oracle(gpu, prec):
+def test_block_jacobi_inverse_and_fallback_match_oracle(gpu, prec, model):
     """fea_block_jacobi_inv against the oracle's restatement on every node block, with some
-    blocks made indefinite (a negated diagonal entry): those fall back to 1 / |K_ii|."""
+    blocks made indefinite (a negated diagonal entry): those fall back to 1 / |K_ii|. Also on
+    a model with 1- and 2-row node groups (blocks confirmed against the CSR columns)."""
     from fea_amd import _native as nat
     from fea_amd._native import call
-    fe, om = cube_models()
-    fe.assembly.initialize()
+    if model == 'partial':
+        fe = partial_models()
+    else:
+        fe, om = cube_models()
+        fe.assembly.initialize()
     p = fe.assembly.plan
     mat = F64 if prec == 'fp64' else F32
     U = p.gc_to_u(torch.as_tensor(0.01 * np.random.default_rng(4).standard_normal(p.n_free), device='cuda'))

@@ -2,6 +2,7 @@
 residual at the solution, lumped mass vs dense brute force and rho*V, explicit
 central difference vs the closed form of an undamped 1-DOF oscillator."""
 import numpy as np
+import pytest
 import scipy.sparse as sp
 
 from fea_amd import meshgen
@@ -75,6 +76,32 @@ def test_newmark_linear_oscillator_closed_form():
     v = np.array([o[1][0] for o in out])
     energy = 0.5 * m * v ** 2 + 0.5 * k * x ** 2
     assert np.abs(energy - energy[0]).max() < 1e-12 * energy[0]
+
+
+@pytest.mark.parametrize('beta,gamma', [(0.25, 0.5), (0.3, 0.6), (0.3025, 0.6)])
+def test_newmark_equation_of_motion_each_step(beta, gamma):
+    """After every converged Newmark step the acceleration of :261-281 satisfies the
+    equation of motion M GA + R(GC) = 0; with the predictor's dt^2 term written as
+    (1/2 - beta) this holds for any beta (a literal 0.25 breaks it unless beta = 1/4).
+    A 3-DOF nonlinear chain: R(x) = K x + c x^3 (elementwise), energy to match."""
+    import scipy.sparse as sp
+    K = np.array([[4.0, -1.5, 0.0], [-1.5, 3.0, -1.0], [0.0, -1.0, 2.5]])
+    c = 0.7
+    M = sp.csr_matrix(np.diag([1.3, 0.8, 2.0]) + 0.1 * np.ones((3, 3)))
+
+    def rt(x):
+        return K @ x + c * x ** 3, sp.csr_matrix(K + np.diag(3 * c * x ** 2))
+
+    def en(x):
+        return 0.5 * float(x @ K @ x) + 0.25 * c * float((x ** 4).sum())
+
+    out = osol.newmark_steps(rt, en, M, np.array([0.4, -0.2, 0.1]), np.array([0.0, 0.5, -0.3]), 0.2, 15,
+                             beta=beta, gamma=gamma, tol_error=1e-10)
+    # M GA + R(GC) equals the incremental residual at Newton's last iterate (~1e-10
+    # here, amplified 1/(beta dt^2) = 100 at most); the literal 0.25 leaves 0.46 at beta = 0.3
+    for GC, _, GA in out:
+        res = M @ GA + rt(GC)[0]
+        assert np.abs(res).max() < 1e-7, res
 
 
 def test_consistent_mass_closed_form():
