@@ -53,8 +53,8 @@ def partial_models():
     1, 2 and 3 free rows (not triple-structured)."""
     from fea_amd import boundarys
     fe, om = cube_models()
-    fe.assembly.add_boundary(boundarys.Boundary_Condition('bx', 'x0', indexDoF=[0]))
-    fe.assembly.add_boundary(boundarys.Boundary_Condition('bxy', 'y1', indexDoF=[0, 1]))
+    fe.assembly.add_boundary(boundarys.Boundary_Condition('block', 'x0', indexDoF=[0]))
+    fe.assembly.add_boundary(boundarys.Boundary_Condition('block', 'y1', indexDoF=[0, 1]))
     fe.assembly.initialize()
     return fe
 
