@@ -51,6 +51,8 @@ _SIGS = {
                               C_P, C_P, C_P, C_P, C_P]),
     'fea_pcg_iterate': (C_INT, [C_INT, C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_INT, C_P, C_P, C_P,
                                 C_P, C_P, C_P, C_P, C_DBL, C_INT, C_INT, C_P, C_P]),
+    'fea_pcg_run_noalloc': (C_INT, [C_INT, C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_INT, C_P, C_P, C_P,
+                                    C_P, C_P, C_P, C_P, C_DBL, C_INT, C_INT, C_P, C_P]),
     'fea_block_jacobi_inv': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_pcg_spmv': (C_INT, [C_INT, C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_spmv_residual_mixed': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),

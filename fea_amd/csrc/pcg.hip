@@ -12,6 +12,8 @@
 // reads each column index and gathers each x value once for all rows of the
 // node, cutting index + gather traffic by up to 3x.  grp == NULL selects one
 // row per group (plain CSR).
+#include <mutex>
+
 #include "pcg_device.h"
 
 namespace fea {
@@ -667,4 +669,74 @@ extern "C" int fea_pcg_spmv(int mat_dtype, int vec_dtype, int n, int64_t nnz, in
     launch_pcg_spmv<float, float>(s, grp, rowptr, col, (const float *)val, (const float *)p, (float *)q, ws, st);
   FEA_CHECK_LAUNCH("fea_pcg_spmv");
   return FEA_OK;
+}
+
+// ---------------------------------------------------------------- allocation-free PCG
+// The PCG workspace (scalars and the partial-sum rings, W_END doubles = 65 664 B) of
+// fea_pcg_run_noalloc lives in this file-scope device array: one copy per GPU, held
+// until the process ends.  Host code never passes `scratch` itself to a kernel: the
+// array's device address is looked up once per device with hipGetSymbolAddress and
+// kept in g_scratch_dev.
+namespace fea {
+constexpr int kNoallocWsDoubles = W_END;
+constexpr int kMaxDevices = 64;
+}  // namespace fea
+
+static __device__ __align__(256) double scratch[fea::kNoallocWsDoubles];
+
+namespace fea {
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kMaxDevices] = {};
+
+// the current device's copy of `scratch`, looked up on the first call on that device
+static int scratch_for_current_device(double **out) {
+  int dev = -1;
+  FEA_HIP(hipGetDevice(&dev), "fea_pcg_run_noalloc: hipGetDevice");
+  FEA_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "fea_pcg_run_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev] == nullptr) {
+    void *ptr = nullptr;
+    FEA_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "fea_pcg_run_noalloc: hipGetSymbolAddress");
+    FEA_CHECK_ARG(ptr != nullptr, "fea_pcg_run_noalloc: no device address for the scratch array");
+    g_scratch_dev[dev] = static_cast<double *>(ptr);
+  }
+  *out = g_scratch_dev[dev];
+  return FEA_OK;
+}
+
+// zero the scalar header of the workspace (iteration counter, done flag, stored r.z):
+// a fill kernel rather than a memset so that a captured graph replays it correctly
+__global__ void k_ws_clear_header(double *__restrict__ ws) {
+  if (threadIdx.x < W_PQ) ws[threadIdx.x] = 0.0;
+}
+}  // namespace fea
+
+extern "C" int fea_pcg_run_noalloc(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups,
+                                   const int32_t *grp, const int32_t *rowptr, const int32_t *col,
+                                   const void *val, int precond, const void *dinv, const void *b, void *x,
+                                   void *r, void *z, void *p, void *q, double tol, int max_iter, int iters,
+                                   double *status_host, void *stream) {
+  FEA_CHECK_ARG(valid_combo(mat_dtype, vec_dtype), "fea_pcg_run_noalloc: unsupported precision pair");
+  FEA_CHECK_ARG(n > 0 && iters >= 0 && max_iter >= 1 && rowptr && col && val && dinv && x && r && z && p && q,
+                "fea_pcg_run_noalloc: bad arguments");
+  FEA_CHECK_ARG(precond == 0 || (precond == 1 && grp != nullptr && n_groups != 0),
+                "fea_pcg_run_noalloc: block Jacobi needs the row groups");
+  FEA_CHECK_ARG(fea_pcg_ws_size(n) <= (int64_t)kNoallocWsDoubles,
+                "fea_pcg_run_noalloc: n = %d needs %lld workspace doubles, the static scratch holds %d", n,
+                (long long)fea_pcg_ws_size(n), kNoallocWsDoubles);
+  double *ws = nullptr;
+  const int rc = scratch_for_current_device(&ws);
+  if (rc != FEA_OK) return rc;
+  hipStream_t st = as_stream(stream);
+  if (b != nullptr) {
+    k_ws_clear_header<<<1, 64, 0, st>>>(ws);
+    FEA_CHECK_LAUNCH("fea_pcg_run_noalloc");
+    PcgArgs a0{n, nnz, n_groups, grp, rowptr, col, val, dinv, b, x, r, z, p, nullptr, precond, ws, 0.0, 0, 0,
+               nullptr, st};
+    const int rs = FEA_DISPATCH(mat_dtype, vec_dtype, pcg_start_t, a0);
+    if (rs != FEA_OK) return rs;
+  }
+  PcgArgs a{n, nnz, n_groups, grp, rowptr, col, val, dinv, nullptr, x, r, z, p, q, precond, ws, tol, max_iter,
+            iters, status_host, st};
+  return FEA_DISPATCH(mat_dtype, vec_dtype, pcg_iterate_t, a);
 }

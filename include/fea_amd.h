@@ -231,6 +231,21 @@ int fea_pcg_iterate(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_grou
                     const int32_t *grp, const int32_t *rowptr, const int32_t *col, const void *val,
                     int precond, const void *dinv, void *x, void *r, void *z, void *p, void *q,
                     double *ws, double tol, int max_iter, int iters, double *status_host, void *stream);
+/* Allocation-free PCG: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): fea_pcg_start / fea_pcg_iterate with the float64 workspace
+ * taken from a static device array of the library (one per GPU, 65 664 B, held until
+ * the process ends) instead of a `ws` argument.  b != NULL starts a solve (the start
+ * stage of fea_pcg_start, then `iters` iterations); b == NULL continues the solve the
+ * previous call on this GPU started.  Vectors, matrix and preconditioner are the
+ * caller's, as for fea_pcg_iterate; status_host as there (NULL: no host sync, so the
+ * call can be captured).  Calls that overlap on one GPU share the workspace and race;
+ * a size whose workspace exceeds the array is rejected with FEA_ERR_ARG.  Nothing is
+ * allocated or freed. */
+int fea_pcg_run_noalloc(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
+                        const int32_t *rowptr, const int32_t *col, const void *val, int precond,
+                        const void *dinv, const void *b, void *x, void *r, void *z, void *p, void *q,
+                        double tol, int max_iter, int iters, double *status_host, void *stream);
 /* One SpMV stage of the PCG iteration alone: q = A p and the p.q partials
  * (the CSR `A @ p` of _linear_solver.py:32 fused with the dot of :33); a no-op
  * when ws marks the solve converged. */
