@@ -180,14 +180,28 @@ __global__ __launch_bounds__(kBlock) void k_elem_r(int n_elem, const int32_t *__
   constexpr int NEN = Elem<ET>::nen, NGP = Elem<ET>::ngp;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_elem) return;
+  // In float, coordinates and displacements are taken relative to the element's first
+  // node, formed in double: the Jacobian and grad u only see differences (the shape
+  // function derivatives sum to zero), and a rigid translation of the element no longer
+  // costs float digits of the strain.  In double the absolute values are used.
+  constexpr bool kRel = sizeof(T) < sizeof(double);
+  const int64_t n0 = conn[(int64_t)e * NEN];
+  double x0[3] = {0.0, 0.0, 0.0}, u0[3] = {0.0, 0.0, 0.0};
+  if (kRel) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      x0[i] = X[3 * n0 + i];
+      u0[i] = U[3 * n0 + i];
+    }
+  }
   T x[NEN][3], u[NEN][3];
 #pragma unroll
   for (int c = 0; c < NEN; ++c) {
     const int64_t n = conn[(int64_t)e * NEN + c];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      x[c][i] = (T)X[3 * n + i];
-      u[c][i] = (T)U[3 * n + i];
+      x[c][i] = (T)(X[3 * n + i] - x0[i]);
+      u[c][i] = (T)(U[3 * n + i] - u0[i]);
     }
   }
   T R[NEN][3];
