@@ -62,6 +62,7 @@ def parse():
     ap.add_argument('--preconditioner', default='block', choices=['block', 'jacobi'],
                     help='beam: PCG preconditioner (node-block or the reference\'s point Jacobi)')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-plain', action='store_true', help='beam: skip the plain FP64 reference-algorithm solve')
     ap.add_argument('--spmv-reps', type=int, default=200)
     ap.add_argument('--max-newton', type=int, default=400, help='Newton iterations per solve (all increments) before failing')
     ap.add_argument('--serving-requests', type=int, default=None)
@@ -242,6 +243,46 @@ def cpu_baseline_beam(args, plan, stats):
             't_assembly_s': t_asm, 't_cg_iter_s': t_cg, 't_energy_s': t_energy}
 
 
+def cpu_direct_route_beam(args, plan, stats, t_asm, t_energy):
+    """The reference's CPU route: every Newton iteration assembles the tangent and solves
+    it directly (static_implicit.py:329 -> pypardiso; scipy's spsolve here).  The direct
+    solve of the full 418k-DOF beam does not fit a bounded sample, so it is timed on the
+    same 4:1:1 beam at three coarser resolutions and extrapolated with the fitted power of
+    the DOF count; the assembly and energy times come from the slice sample."""
+    import scipy.sparse.linalg as spla
+    from fea_amd import meshgen
+    from oracle import assembly as oasm
+    from oracle import materials as omat
+    sizes, times = [], []
+    for m in (4, 6, 8):
+        mesh = meshgen.hex_block(4 * m, m, m, size=(4.0, 1.0, 1.0))
+        N = mesh['nodes'].shape[0]
+        fixed = np.zeros((N, 3), dtype=bool)
+        fixed[mesh['node_sets']['x0']] = True
+        om = {'nodes': mesh['nodes'], 'fixed': fixed, 'pressures': [], 'tractions': [],
+              'groups': [{'etype': 'C3D8', 'conn': mesh['conn'], 'material': omat.MooneyRivlin(0.5, 0.1, 0.01)}]}
+        U = 1e-3 * np.random.default_rng(0).standard_normal(mesh['nodes'].shape)
+        R, K = oasm.assemble_reduced(om, U)
+        K = K.tocsc()
+        t0 = time.perf_counter()
+        spla.spsolve(K, -R)
+        times.append(time.perf_counter() - t0)
+        sizes.append(K.shape[0])
+    slope = float(np.polyfit(np.log(sizes), np.log(times), 1)[0])
+    t_direct = times[-1] * (plan.n_free / sizes[-1]) ** slope
+    n_newton = stats['newton_iters']
+    t_solve = n_newton * (t_asm + t_direct) + stats['energy_evals'] * t_energy
+    return {'value': 1.0 / t_solve, 'unit': 'solves/s', 'cores': 1, 'kind': 'port',
+            'sample': ('reference CPU route (Newton with a direct linear solve per iteration, '
+                       'static_implicit.py:329): FP64 numpy-oracle tangent assembly of a 1/%d slice scaled to the '
+                       'full beam (%.2f s) and scipy spsolve (SuperLU, single thread) timed on the 4:1:1 beam at '
+                       '%s DOF (%s s), extrapolated as DOF^%.2f to %d DOF (%.0f s); x %d Newton iterations of the '
+                       'GPU solve + %d energy evaluations: %.0f s per solve; single-threaded numpy/scipy'
+                       % (args.nz // max(1, args.nz // 16), t_asm, sizes, ['%.3f' % t for t in times], slope,
+                          plan.n_free, t_direct, n_newton, stats['energy_evals'], t_solve)),
+            't_direct_solve_s': t_direct, 'direct_solve_dof_power': slope}
+
+
 def bench_beam(args, torch):
     from fea_amd import workloads
     from fea_amd._native import F32, F64
@@ -283,6 +324,27 @@ def bench_beam(args, torch):
         log('fp64 solve %.3f s, rel err %.2e' % (t64, err))
         del fe64
 
+    plain = {}
+    if args.precision != 'fp64' and not args.no_plain:
+        # the reference's algorithm on the GPU in FP64: point-Jacobi (the reference's
+        # scaling) CG at a fixed tolerance, each increment started from the last converged
+        # state (no predictor), no inexact-Newton forcing
+        fep, _ = workloads.beam_traction(args.nx, args.ny, args.nz)
+        fep.solver = make_solver('fp64', args, predictor=False, preconditioner='jacobi')
+        fep.solver.forcing = 'fixed'
+        fep.initialize()
+        GCp, tp = run_solve(fep, torch)
+        plain = {'plain_fp64_solves_per_s': 1.0 / tp, 'plain_fp64_solve_s': tp,
+                 'plain_fp64_rel_err': float((GC - GCp).abs().max() / GCp.abs().max()),
+                 'plain_fp64_newton_iters': fep.solver.stats['newton_iters'],
+                 'plain_fp64_cg_iters': fep.solver.stats['cg_iters'],
+                 'speedup_vs_plain_fp64': tp / (elapsed / steps),
+                 'plain_fp64_config': 'fp64, point Jacobi, no predictor, fixed CG tolerance %g on the squared '
+                                      'scaled residual ratio, Newton rel tol %g' % (args.cg_tol, args.rel_tol)}
+        log('plain fp64 solve %.3f s (%d Newton, %d CG)' % (tp, plain['plain_fp64_newton_iters'],
+                                                            plain['plain_fp64_cg_iters']))
+        del fep
+
     mat = F64 if args.precision == 'fp64' else F32
     kname_t = 'float' if mat == F32 else 'double'
     t_spmv, alg_bytes = spmv_roofline(fe, torch, args.spmv_reps, mat)
@@ -290,7 +352,11 @@ def bench_beam(args, torch):
     elem_rate, t_tan = element_rate(fe, torch)
     elem_rate32, t_tan32 = element_rate(fe, torch, kernel_dtype=F32)
     serving = serving_sample(args, torch)
-    cpu = None if args.no_cpu else cpu_baseline_beam(args, p, stats)
+    cpu = None
+    cg_route = None
+    if not args.no_cpu:
+        cg_route = cpu_baseline_beam(args, p, stats)
+        cpu = cpu_direct_route_beam(args, p, stats, cg_route['t_assembly_s'], cg_route['t_energy_s'])
 
     line = base_line(args, 'static Newton solves/s (mixed FP32/FP64) at 1e-8 rel err', value, 'solves/s', steps,
                      warmup, elapsed, 'f64' if args.precision == 'fp64' else 'f32+f64',
@@ -312,9 +378,12 @@ def bench_beam(args, torch):
                              traffic_key=('beam %dx%dx%d C3D8' % (args.nx, args.ny, args.nz),
                                           'k_pcg_spmv<%s,double,%d,2>' % (kname_t, _triple_sw(p)))),
         'cpu_baseline': cpu,
+        'cpu_cg_route_estimate': None if cg_route is None else {
+            'value': cg_route['value'], 'unit': 'solves/s', 'sample': cg_route['sample']},
         'serving_sample': serving,
     })
     line.update(fp64)
+    line.update(plain)
     return line
 
 

@@ -113,6 +113,25 @@ def lib():
     return L
 
 
+TORCH_OPS_PATH = os.path.join(_HERE, '_lib', 'libfea_torch.so')
+_ops = None
+
+
+def ops():
+    """The PyTorch-ROCm operators (torch.ops.fea, TORCH_LIBRARY in csrc/torch_ops.cpp) over
+    the same C ABI, with device / dtype / contiguity / size checks; raises if not built."""
+    global _ops
+    if _ops is not None:
+        return _ops
+    lib()
+    if not os.path.exists(TORCH_OPS_PATH):
+        raise FeaNativeError('fea_amd torch operator library not found at %s: run `python -m fea_amd.build`'
+                             % TORCH_OPS_PATH)
+    torch.ops.load_library(TORCH_OPS_PATH)
+    _ops = torch.ops.fea
+    return _ops
+
+
 def exported_symbols():
     return list(_SIGS.keys())
 

@@ -242,8 +242,7 @@ class DevicePlan:
         """R = f_int(U) - f_ext on free DOFs (assemble_force, assembly.py:230-281)."""
         self.residual_stage(U, load_scale, dtype)
         R = self.R if out is None else out
-        call('fea_gather_residual', self.n_nodes, self.rseg, self.dof_map, self.rstage, R,
-             self._s())
+        nat.ops().gather_residual(self.rseg, self.dof_map, self.rstage, R)
         return R
 
     def residual_stage(self, U, load_scale=1.0, dtype=F64):
@@ -254,8 +253,8 @@ class DevicePlan:
             if g.n == 0:
                 continue
             if g.kind == 'elem':
-                call('fea_elem_residual', g.ntype, dtype, g.n, g.conn, self.X, U,
-                     g.params, g.dst_r, self.rstage, s)
+                nat.ops().elem_residual(g.ntype, dtype, g.conn, self.X, U, g.params.tolist(), g.dst_r,
+                                        self.rstage)
             elif g.kind == 'pressure':
                 call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, self.X, U,
                      float(g.load.pressure) * load_scale, None, None, g.dst_r, self.rstage, s)
@@ -283,9 +282,8 @@ class DevicePlan:
             if g.n == 0:
                 continue
             if g.kind == 'elem':
-                call('fea_elem_residual_tangent', g.ntype, dtype, g.n, g.conn, self.X, U,
-                     g.params, g.dst_k, ks,
-                     g.dst_r if with_residual else None, rs, s)
+                nat.ops().elem_residual_tangent(g.ntype, dtype, g.conn, self.X, U, g.params.tolist(), g.dst_k, ks,
+                                                g.dst_r if with_residual else None, rs)
             elif g.kind == 'pressure':
                 call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, self.X, U,
                      float(g.load.pressure) * load_scale, g.dst_k, ks,
@@ -298,13 +296,12 @@ class DevicePlan:
                 g.load.assemble(U, g.dst_k, ks, dtype, g.dst_r if with_residual else None, rs)
         val_dtype = dtype if val_dtype is None else val_dtype
         val = self.csr_values(val_dtype)
-        call('fea_gather_csr', dtype, val_dtype, self.n_blocks, self.blk_a, self.blk_b, self.blk_colpos,
-             self.kseg, self.dof_map, self.rowptr, ks, val, s)
+        nat.ops().gather_csr(dtype, val_dtype, self.blk_a, self.blk_b, self.blk_colpos, self.kseg, self.dof_map,
+                             self.rowptr, ks, val)
         R = None
         if with_residual:
             R = self.R
-            call('fea_gather_residual', self.n_nodes, self.rseg, self.dof_map, self.rstage,
-                 R, s)
+            nat.ops().gather_residual(self.rseg, self.dof_map, self.rstage, R)
         return R, val
 
     def energy_terms(self, U, load_scale=1.0):
@@ -317,8 +314,7 @@ class DevicePlan:
                 continue
             seg = self._epart[off:off + nb]
             if g.kind == 'elem':
-                call('fea_elem_energy', g.ntype, g.n, g.conn, self.X, U,
-                     g.params, seg, s)
+                nat.ops().elem_energy(g.ntype, g.conn, self.X, U, g.params.tolist(), seg)
             elif g.kind == 'pressure':
                 call('fea_surface_volume', g.ntype, g.n, g.conn, self.X, U, seg, s)
             elif g.kind == 'traction':

@@ -56,6 +56,7 @@ class PCG:
             self.dinv = mk()
         self.ws = torch.zeros(int(nat.lib().fea_pcg_ws_size(self.n)), dtype=torch.float64, device=device)
         self.status = np.zeros(4, dtype=np.float64)
+        self._status_t = torch.from_numpy(self.status)          # host status, shared with the numpy view
         self.iterations = 0
 
     @property
@@ -65,11 +66,10 @@ class PCG:
     def set_matrix(self, val):
         self.val = val
         if self.block:
-            call('fea_block_jacobi_inv', self.mat_dtype, self.n, self.n_groups, self.grp, self.rowptr, self.col,
-                 self.diag_slot, val, self.dinv, nat.stream_ptr())
+            nat.ops().block_jacobi_inv(self.mat_dtype, self.n_groups, self.grp, self.rowptr, self.col,
+                                       self.diag_slot, val, self.dinv)
         else:
-            call('fea_jacobi_inv', self.mat_dtype, self.vec_dtype, self.n, self.diag_slot, val, self.dinv,
-                 nat.stream_ptr())
+            nat.ops().jacobi_inv(self.mat_dtype, self.vec_dtype, self.diag_slot, val, self.dinv)
 
     def solve(self, b, x0=None, tol=1e-5, max_iter=1500):
         """Solve K x = b; returns x (vector precision)."""
@@ -82,14 +82,15 @@ class PCG:
         else:
             self.x.copy_(x0)
         pc = 1 if self.block else 0
-        call('fea_pcg_start', self.mat_dtype, self.vec_dtype, self.n, self.nnz, self.n_groups, self.grp,
-             self.rowptr, self.spmv_col, self.val, pc, self.dinv, b, self.x, self.r, self.z, self.p, self.ws, s)
+        O = nat.ops()
+        O.pcg_start(self.mat_dtype, self.vec_dtype, self.n_groups, self.grp, self.rowptr, self.spmv_col, self.val,
+                    pc, self.dinv, b, self.x, self.r, self.z, self.p, self.ws)
         done_iters = 0
         while True:
             step = min(self.chunk, max(1, max_iter - done_iters))
-            call('fea_pcg_iterate', self.mat_dtype, self.vec_dtype, self.n, self.nnz, self.n_groups, self.grp,
-                 self.rowptr, self.spmv_col, self.val, pc, self.dinv, self.x, self.r, self.z, self.p, self.q, self.ws,
-                 float(tol), int(max_iter), int(step), self.status, s)
+            O.pcg_iterate(self.mat_dtype, self.vec_dtype, self.n_groups, self.grp, self.rowptr, self.spmv_col,
+                          self.val, pc, self.dinv, self.x, self.r, self.z, self.p, self.q, self.ws, float(tol),
+                          int(max_iter), int(step), self._status_t)
             done_iters += step
             if self.status[1] != 0.0 or done_iters >= max_iter:
                 break

@@ -33,6 +33,25 @@ def test_binding_covers_header():
     assert set(header_functions()) <= set(_native.exported_symbols())
 
 
+TORCH_OPS = ['elem_residual_tangent', 'elem_residual', 'elem_energy', 'gather_csr', 'gather_residual', 'spmv',
+             'pcg_start', 'pcg_iterate', 'jacobi_inv', 'block_jacobi_inv']
+
+
+def test_torch_operators_registered():
+    """The TORCH_LIBRARY(fea, ...) library loads and registers every operator with a schema;
+    there is no CPU kernel, so CPU tensors are refused (no silent host fallback)."""
+    import torch
+    O = _native.ops()
+    for name in TORCH_OPS:
+        schema = getattr(O, name).default._schema
+        assert schema.name == 'fea::' + name
+    x = torch.zeros(3, dtype=torch.float64)
+    rowptr = torch.tensor([0, 1, 2, 3], dtype=torch.int32)
+    col = torch.tensor([0, 1, 2], dtype=torch.int32)
+    with pytest.raises((NotImplementedError, RuntimeError)):
+        O.spmv(1, 1, 0, None, rowptr, col, x, x, x.clone())
+
+
 def build_pattern(model):
     """Call fea_pattern_create directly (host code, no GPU)."""
     L = _native.lib()
