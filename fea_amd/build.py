@@ -21,14 +21,14 @@ TORCH_OPS_LIB = os.path.join(OUT_DIR, 'libfea_torch.so')
 ARCH = os.environ.get('FEA_AMD_ARCH', 'gfx950')
 
 HIP_SOURCES = ['elements.hip', 'surfaces.hip', 'linalg.hip', 'pcg.hip', 'contact.hip', 'adjoint.hip', 'batched.hip', 'couple.hip', 'barrier.hip']
-CPP_SOURCES = ['pattern.cpp', 'api.cpp']
+CPP_SOURCES = ['pattern.cpp', 'api.cpp', 'amg_setup.cpp']
 
 # packed-FP32 instructions are disabled: the compiler may form them from scalar
 # float code, and they are avoided on this target.
 HIP_FLAGS = ['--offload-arch=' + ARCH, '-O3', '-std=c++17', '-fPIC',
              '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
              '-munsafe-fp-atomics']
-CPP_FLAGS = ['-O3', '-std=c++17', '-fPIC']
+CPP_FLAGS = ['-O3', '-std=c++17', '-fPIC', '-pthread']
 
 
 def _hipcc():
@@ -63,7 +63,7 @@ def build(verbose=False):
     with concurrent.futures.ThreadPoolExecutor(workers) as ex:
         objs = list(ex.map(_compile, jobs))
     tmp = LIB + '.tmp'
-    link = [hipcc, '--offload-arch=' + ARCH, '-shared', '-fPIC', '-Wl,-soname,libfea_amd.so', '-o', tmp] + objs
+    link = [hipcc, '--offload-arch=' + ARCH, '-shared', '-fPIC', '-pthread', '-Wl,-soname,libfea_amd.so', '-o', tmp] + objs
     _compile(link)
     os.replace(tmp, LIB)
     if verbose:

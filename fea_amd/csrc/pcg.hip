@@ -12,7 +12,10 @@
 // reads each column index and gathers each x value once for all rows of the
 // node, cutting index + gather traffic by up to 3x.  grp == NULL selects one
 // row per group (plain CSR).
+#include <algorithm>
+#include <cmath>
 #include <mutex>
+#include <vector>
 
 #include "pcg_device.h"
 
@@ -739,4 +742,610 @@ extern "C" int fea_pcg_run_noalloc(int mat_dtype, int vec_dtype, int n, int64_t 
   PcgArgs a{n, nnz, n_groups, grp, rowptr, col, val, dinv, nullptr, x, r, z, p, q, precond, ws, tol, max_iter,
             iters, status_host, st};
   return FEA_DISPATCH(mat_dtype, vec_dtype, pcg_iterate_t, a);
+}
+
+// ---------------------------------------------------------------- smoothed-aggregation AMG
+// V-cycle of the hierarchy built on the host (amg_setup.cpp).  Level 0 is the PCG's own
+// matrix (node-grouped CSR, current values) smoothed with its node-block Jacobi inverses;
+// levels >= 1 hold the Galerkin operators as CSR whose rows come in groups of 6 (one
+// aggregate's coarse DOFs) sharing one column list, with 6x6 block inverses; the coarsest
+// level is solved with its dense inverse.  The smoother is Chebyshev of D^-1 A on
+// [lambda_max / 30, 1.1 lambda_max], the same polynomial before and after the coarse
+// correction, so the V-cycle is a symmetric positive definite preconditioner for CG.
+// Operators, prolongators, smoother blocks and the coarse inverse are stored in float;
+// every vector and every accumulation is float64: rounding the preconditioner's input
+// residual to float costs ~20 % more CG iterations on stiff systems (the error of a
+// residual dominated by stiff components lands on the smooth part of M^-1 r), while float
+// storage of the operators is a fixed symmetric perturbation that costs none.  Every
+// kernel of the iteration path returns at once when the solve is marked converged.
+namespace fea {
+
+constexpr int AMG_SW = 16;  // lanes per 6-row group / per aggregate in the coarse-level kernels
+
+static int amg_grid(int64_t items, int per_block) {
+  const int64_t g = (items + per_block - 1) / per_block;
+  return (int)(g < 1 ? 1 : (g > G1_MAX ? G1_MAX : g));
+}
+
+// o[m] = (A x)[6 g + m] for a level >= 1 operator: the group's 6 rows are consecutive CSR
+// rows of equal length L sharing the column list, row m at s0 + m L
+template <int SW>
+__device__ __forceinline__ void group6_spmv(int g, int lane, const int32_t *__restrict__ rowptr,
+                                            const int32_t *__restrict__ col, const float *__restrict__ val,
+                                            const double *__restrict__ x, double o[6]) {
+  const int s0 = rowptr[6 * g];
+  const int L = rowptr[6 * g + 1] - s0;
+  double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int t = lane; t < L; t += SW) {
+    const double xv = x[col[s0 + t]];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) a[m] += (double)val[s0 + m * L + t] * xv;
+  }
+#pragma unroll
+  for (int off = SW / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int m = 0; m < 6; ++m) a[m] += __shfl_xor(a[m], off, SW);
+#pragma unroll
+  for (int m = 0; m < 6; ++m) o[m] = a[m];
+}
+
+__device__ __forceinline__ bool amg_done(const double *done) { return done != nullptr && *done != 0.0; }
+
+// one Chebyshev step on level 0: r = b - A x (FIRST: x = 0, no product), d = c1 d + c2 M^-1 r,
+// xo = x + d with M the node-block diagonal (6 packed floats per group)
+template <typename TM, int SW, int MODE, int FIRST>
+__global__ __launch_bounds__(kBlock) void k_amg_cheb0(int ngroups, const int32_t *__restrict__ grp,
+                                                      const int32_t *__restrict__ rowptr,
+                                                      const int32_t *__restrict__ col, const TM *__restrict__ val,
+                                                      const float *__restrict__ binv, const double *__restrict__ b,
+                                                      const double *__restrict__ x, double *__restrict__ xo,
+                                                      double *__restrict__ d, double c1, double c2,
+                                                      const double *__restrict__ done) {
+  if (amg_done(done)) return;
+  constexpr int GPB = kBlock / SW;
+  const int lane = threadIdx.x % SW;
+  for (int64_t g0 = (int64_t)blockIdx.x * GPB; g0 < ngroups; g0 += (int64_t)gridDim.x * GPB) {
+    const int g = (int)(g0 + threadIdx.x / SW);
+    if (g >= ngroups) continue;
+    double o[3] = {0.0, 0.0, 0.0};
+    if (!FIRST) group_spmv<TM, double, SW, MODE>(g, lane, grp, rowptr, col, val, x, o);
+    if (lane == 0) {
+      const int r0 = grp[g], nr = grp[g + 1] - r0;
+      double rr[3] = {0.0, 0.0, 0.0}, w[3];
+      for (int a = 0; a < nr; ++a) rr[a] = b[r0 + a] - o[a];
+      bj_apply(binv + 6 * (int64_t)g, rr, w);
+      for (int a = 0; a < nr; ++a) {
+        const double dn = (c1 != 0.0 ? c1 * d[r0 + a] : 0.0) + c2 * w[a];
+        d[r0 + a] = dn;
+        xo[r0 + a] = (FIRST ? 0.0 : x[r0 + a]) + dn;
+      }
+    }
+  }
+}
+
+// r = b - A x on level 0
+template <typename TM, int SW, int MODE>
+__global__ __launch_bounds__(kBlock) void k_amg_resid0(int ngroups, const int32_t *__restrict__ grp,
+                                                       const int32_t *__restrict__ rowptr,
+                                                       const int32_t *__restrict__ col, const TM *__restrict__ val,
+                                                       const double *__restrict__ b, const double *__restrict__ x,
+                                                       double *__restrict__ r, const double *__restrict__ done) {
+  if (amg_done(done)) return;
+  constexpr int GPB = kBlock / SW;
+  const int lane = threadIdx.x % SW;
+  for (int64_t g0 = (int64_t)blockIdx.x * GPB; g0 < ngroups; g0 += (int64_t)gridDim.x * GPB) {
+    const int g = (int)(g0 + threadIdx.x / SW);
+    if (g >= ngroups) continue;
+    double o[3];
+    const int r0 = group_spmv<TM, double, SW, MODE>(g, lane, grp, rowptr, col, val, x, o);
+    if (lane == 0) {
+      const int nr = grp[g + 1] - r0;
+      for (int a = 0; a < nr; ++a) r[r0 + a] = b[r0 + a] - o[a];
+    }
+  }
+}
+
+// the same two kernels on a level >= 1 (6-row groups, 36-float block inverses)
+template <int FIRST>
+__global__ __launch_bounds__(kBlock) void k_amg_cheb_c(int ngroups, const int32_t *__restrict__ rowptr,
+                                                       const int32_t *__restrict__ col, const float *__restrict__ val,
+                                                       const float *__restrict__ dinv, const double *__restrict__ b,
+                                                       const double *__restrict__ x, double *__restrict__ xo,
+                                                       double *__restrict__ d, double c1, double c2,
+                                                       const double *__restrict__ done) {
+  if (amg_done(done)) return;
+  constexpr int GPB = kBlock / AMG_SW;
+  const int lane = threadIdx.x % AMG_SW;
+  for (int64_t g0 = (int64_t)blockIdx.x * GPB; g0 < ngroups; g0 += (int64_t)gridDim.x * GPB) {
+    const int g = (int)(g0 + threadIdx.x / AMG_SW);
+    if (g >= ngroups) continue;
+    double o[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (!FIRST) group6_spmv<AMG_SW>(g, lane, rowptr, col, val, x, o);
+    if (lane == 0) {
+      double rr[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) rr[m] = b[6 * g + m] - o[m];
+      const float *D = dinv + 36 * (int64_t)g;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        double w = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) w += (double)D[6 * m + k] * rr[k];
+        const double dn = (c1 != 0.0 ? c1 * d[6 * g + m] : 0.0) + c2 * w;
+        d[6 * g + m] = dn;
+        xo[6 * g + m] = (FIRST ? 0.0 : x[6 * g + m]) + dn;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_amg_resid_c(int ngroups, const int32_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ col,
+                                                        const float *__restrict__ val, const double *__restrict__ b,
+                                                        const double *__restrict__ x, double *__restrict__ r,
+                                                        const double *__restrict__ done) {
+  if (amg_done(done)) return;
+  constexpr int GPB = kBlock / AMG_SW;
+  const int lane = threadIdx.x % AMG_SW;
+  for (int64_t g0 = (int64_t)blockIdx.x * GPB; g0 < ngroups; g0 += (int64_t)gridDim.x * GPB) {
+    const int g = (int)(g0 + threadIdx.x / AMG_SW);
+    if (g >= ngroups) continue;
+    double o[6];
+    group6_spmv<AMG_SW>(g, lane, rowptr, col, val, x, o);
+    if (lane == 0)
+#pragma unroll
+      for (int m = 0; m < 6; ++m) r[6 * g + m] = b[6 * g + m] - o[m];
+  }
+}
+
+// bc = P^T r: one AMG_SW-lane sub-wave per coarse aggregate over its transposed entries
+__global__ __launch_bounds__(kBlock) void k_amg_restrict(int nagg, const int32_t *__restrict__ pt_ptr,
+                                                         const int32_t *__restrict__ pt_blk,
+                                                         const int32_t *__restrict__ pt_row,
+                                                         const float *__restrict__ pval, const double *__restrict__ r,
+                                                         double *__restrict__ bc, const double *__restrict__ done) {
+  if (amg_done(done)) return;
+  constexpr int GPB = kBlock / AMG_SW;
+  const int lane = threadIdx.x % AMG_SW;
+  for (int64_t a0 = (int64_t)blockIdx.x * GPB; a0 < nagg; a0 += (int64_t)gridDim.x * GPB) {
+    const int I = (int)(a0 + threadIdx.x / AMG_SW);
+    if (I >= nagg) continue;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int t = pt_ptr[I] + lane; t < pt_ptr[I + 1]; t += AMG_SW) {
+      const double rv = r[pt_row[t]];
+      const float *p = pval + 6 * (int64_t)pt_blk[t];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) acc[m] += (double)p[m] * rv;
+    }
+#pragma unroll
+    for (int off = AMG_SW / 2; off > 0; off >>= 1)
+#pragma unroll
+      for (int m = 0; m < 6; ++m) acc[m] += __shfl_xor(acc[m], off, AMG_SW);
+    if (lane == 0)
+#pragma unroll
+      for (int m = 0; m < 6; ++m) bc[6 * I + m] = acc[m];
+  }
+}
+
+// x += P xc: one thread per fine row over its 6-column blocks
+__global__ __launch_bounds__(kBlock) void k_amg_prolong(int n, const int32_t *__restrict__ prow,
+                                                        const int32_t *__restrict__ pagg,
+                                                        const float *__restrict__ pval, const double *__restrict__ xc,
+                                                        double *__restrict__ x, const double *__restrict__ done) {
+  if (amg_done(done)) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int k = prow[i]; k < prow[i + 1]; ++k) {
+      const float *p = pval + 6 * (int64_t)k;
+      const double *c = xc + 6 * (int64_t)pagg[k];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) s += (double)p[m] * c[m];
+    }
+    x[i] += s;
+  }
+}
+
+// x = inv b on the coarsest level: one wave per row
+__global__ __launch_bounds__(kBlock) void k_amg_dense(int n, const float *__restrict__ inv,
+                                                      const double *__restrict__ b, double *__restrict__ x,
+                                                      const double *__restrict__ done) {
+  if (amg_done(done)) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (row >= n) return;
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) s += (double)inv[row * n + j] * b[j];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) x[row] = s;
+}
+
+// Chebyshev step coefficients for lambda in [rho / 30, 1.1 rho]: step 0 (d = r / theta),
+// then d = c1 d + c2 r with the three-term recurrence
+static void cheb_coeffs(double rho, int deg, double *c1, double *c2) {
+  const double lmax = 1.1 * rho, lmin = rho / 30.0;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+  double rk = 1.0 / sigma;
+  c1[0] = 0.0;
+  c2[0] = 1.0 / theta;
+  for (int k = 1; k < deg; ++k) {
+    const double rn = 1.0 / (2.0 * sigma - rk);
+    c1[k] = rn * rk;
+    c2[k] = 2.0 * rn / delta;
+    rk = rn;
+  }
+}
+
+constexpr int AMG_MAX_LEVELS = 12, AMG_MAX_DEG = 8;
+
+template <typename TM>
+struct AmgFine {
+  SpmvShape s;
+  const int32_t *grp, *rowptr, *col;
+  const TM *val;
+  const float *binv;
+  double rho;
+};
+
+template <typename TM>
+static void cheb0(const AmgFine<TM> &f, bool first, const double *b, const double *x, double *xo, double *d,
+                  double c1, double c2, const double *done, hipStream_t st) {
+  const int grid = spmv_grid(f.s);
+  const auto &s = f.s;
+  if (first) {
+#define FEA_AC0(SW, MD) \
+  k_amg_cheb0<TM, SW, MD, 1><<<grid, kBlock, 0, st>>>(s.ngroups, f.grp, f.rowptr, f.col, f.val, f.binv, b, x, xo, d, c1, c2, done)
+    FEA_SW_SWITCH(FEA_AC0);
+#undef FEA_AC0
+  } else {
+#define FEA_AC1(SW, MD) \
+  k_amg_cheb0<TM, SW, MD, 0><<<grid, kBlock, 0, st>>>(s.ngroups, f.grp, f.rowptr, f.col, f.val, f.binv, b, x, xo, d, c1, c2, done)
+    FEA_SW_SWITCH(FEA_AC1);
+#undef FEA_AC1
+  }
+}
+
+template <typename TM>
+static void resid0(const AmgFine<TM> &f, const double *b, const double *x, double *r, const double *done,
+                   hipStream_t st) {
+  const int grid = spmv_grid(f.s);
+  const auto &s = f.s;
+#define FEA_AR0(SW, MD) \
+  k_amg_resid0<TM, SW, MD><<<grid, kBlock, 0, st>>>(s.ngroups, f.grp, f.rowptr, f.col, f.val, b, x, r, done)
+  FEA_SW_SWITCH(FEA_AR0);
+#undef FEA_AR0
+}
+
+static void cheb_c(const fea_amg_level &L, bool first, const double *b, const double *x, double *xo, double c1,
+                   double c2, const double *done, hipStream_t st) {
+  const int ng = L.n / 6;
+  const int grid = amg_grid(ng, kBlock / AMG_SW);
+  if (first)
+    k_amg_cheb_c<1><<<grid, kBlock, 0, st>>>(ng, L.rowptr, L.col, L.val, L.dinv, b, x, xo, L.d, c1, c2, done);
+  else
+    k_amg_cheb_c<0><<<grid, kBlock, 0, st>>>(ng, L.rowptr, L.col, L.val, L.dinv, b, x, xo, L.d, c1, c2, done);
+}
+
+// z = V(b) on the hierarchy (b, z: level-0 vectors; z must not alias b)
+template <typename TM>
+static void amg_vcycle(const AmgFine<TM> &f, const fea_amg_level *lv, int nl, int deg, const double *b0, double *z0,
+                       const double *done, hipStream_t st) {
+  double c1[AMG_MAX_DEG], c2[AMG_MAX_DEG];
+  double *cur[AMG_MAX_LEVELS], *oth[AMG_MAX_LEVELS];
+  for (int l = 0; l + 1 < nl; ++l) {  // down: pre-smooth, residual, restriction
+    const fea_amg_level &L = lv[l];
+    const double *bl = l == 0 ? b0 : L.b;
+    cheb_coeffs(l == 0 ? f.rho : L.rho, deg, c1, c2);
+    double *xa = L.x, *xb = L.x2;
+    if (l == 0)
+      cheb0(f, true, bl, nullptr, xa, L.d, c1[0], c2[0], done, st);
+    else
+      cheb_c(L, true, bl, nullptr, xa, c1[0], c2[0], done, st);
+    for (int k = 1; k < deg; ++k) {
+      if (l == 0)
+        cheb0(f, false, bl, xa, xb, L.d, c1[k], c2[k], done, st);
+      else
+        cheb_c(L, false, bl, xa, xb, c1[k], c2[k], done, st);
+      std::swap(xa, xb);
+    }
+    cur[l] = xa;
+    oth[l] = xb;
+    if (l == 0)
+      resid0(f, bl, xa, L.r, done, st);
+    else
+      k_amg_resid_c<<<amg_grid(L.n / 6, kBlock / AMG_SW), kBlock, 0, st>>>(L.n / 6, L.rowptr, L.col, L.val, bl, xa,
+                                                                           L.r, done);
+    k_amg_restrict<<<amg_grid(L.nagg, kBlock / AMG_SW), kBlock, 0, st>>>(L.nagg, L.pt_ptr, L.pt_blk, L.pt_row,
+                                                                        L.pval, L.r, lv[l + 1].b, done);
+  }
+  const fea_amg_level &C = lv[nl - 1];
+  k_amg_dense<<<(C.n + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, st>>>(C.n, C.dense_inv, C.b, C.x, done);
+  for (int l = nl - 2; l >= 0; --l) {  // up: prolongation, post-smooth
+    const fea_amg_level &L = lv[l];
+    const double *bl = l == 0 ? b0 : L.b;
+    const double *xc = (l + 1 == nl - 1) ? C.x : cur[l + 1];
+    k_amg_prolong<<<amg_grid(L.n, kBlock), kBlock, 0, st>>>(L.n, L.prow, L.pagg, L.pval, xc, cur[l], done);
+    cheb_coeffs(l == 0 ? f.rho : L.rho, deg, c1, c2);
+    double *x = cur[l], *o = oth[l];
+    for (int k = 0; k < deg; ++k) {
+      double *out = (l == 0 && k == deg - 1) ? z0 : o;
+      if (l == 0)
+        cheb0(f, false, bl, x, out, L.d, c1[k], c2[k], done, st);
+      else
+        cheb_c(L, false, bl, x, out, c1[k], c2[k], done, st);
+      o = x;
+      x = out;
+    }
+    cur[l] = x;
+  }
+}
+
+// ---- PCG stages around the V-cycle
+// partials of b.(M^-1 b) for the stopping test
+__global__ __launch_bounds__(kBlock) void k_amg_bz(int n, const double *__restrict__ b, const double *__restrict__ zb,
+                                                   double *__restrict__ ws) {
+  double bz = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    bz += b[i] * zb[i];
+  write_partial(ws + W_BZP, bz);
+}
+
+// p = z (= M^-1 r from the V-cycle), partials of r.z (ring slot 0); reset the counters
+__global__ __launch_bounds__(kBlock) void k_amg_pcg_init(int n, const double *__restrict__ r,
+                                                         const double *__restrict__ z, double *__restrict__ p,
+                                                         double *__restrict__ ws) {
+  double rz = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double zi = z[i];
+    p[i] = zi;
+    rz += r[i] * zi;
+  }
+  write_partial(ws + W_RZP, rz);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ws[W_ITER] = 0.0;
+    ws[W_DONE] = 0.0;
+  }
+}
+
+// alpha = r.z / p.q; x += alpha p; r -= alpha q
+__global__ __launch_bounds__(kBlock) void k_amg_pcg_update(int n, double *__restrict__ x, double *__restrict__ r,
+                                                           const double *__restrict__ p,
+                                                           const double *__restrict__ q, double *__restrict__ ws) {
+  if (ws[W_DONE] != 0.0) return;
+  const int it = (int)ws[W_ITER];
+  const int g1 = (int)ws[W_G1];
+  const double rz_old = it > 1 ? ws[W_RZH + ((it - 1) & 1)]
+                               : sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_RING, gridDim.x);
+  const double pq = sum_partials(ws + W_PQ, g1);
+  const double alpha = pq != 0.0 ? rz_old / pq : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    r[i] -= alpha * q[i];
+  }
+}
+
+// partials of r.z of this iteration (ring slot it & 1), z = M^-1 r from the V-cycle
+__global__ __launch_bounds__(kBlock) void k_amg_pcg_rz(int n, const double *__restrict__ r,
+                                                       const double *__restrict__ z, double *__restrict__ ws) {
+  if (ws[W_DONE] != 0.0) return;
+  const int it = (int)ws[W_ITER];
+  double rz = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    rz += r[i] * z[i];
+  write_partial(ws + W_RZP + (it & 1) * PCG_RING, rz);
+}
+
+// y = s D^-1 A x (float) and partials of y.y: one step of the power iteration for lambda_max
+template <typename TM, int SW, int MODE>
+__global__ __launch_bounds__(kBlock) void k_amg_power0(int ngroups, const int32_t *__restrict__ grp,
+                                                       const int32_t *__restrict__ rowptr,
+                                                       const int32_t *__restrict__ col, const TM *__restrict__ val,
+                                                       const float *__restrict__ binv, const float *__restrict__ x,
+                                                       float *__restrict__ y, float s, double *__restrict__ part) {
+  constexpr int GPB = kBlock / SW;
+  const int lane = threadIdx.x % SW;
+  double yy = 0.0;
+  for (int64_t g0 = (int64_t)blockIdx.x * GPB; g0 < ngroups; g0 += (int64_t)gridDim.x * GPB) {
+    const int g = (int)(g0 + threadIdx.x / SW);
+    if (g >= ngroups) continue;
+    float o[3];
+    const int r0 = group_spmv<TM, float, SW, MODE>(g, lane, grp, rowptr, col, val, x, o);
+    if (lane == 0) {
+      const int nr = grp[g + 1] - r0;
+      double rr[3] = {0.0, 0.0, 0.0}, w[3];
+      for (int a = 0; a < nr; ++a) rr[a] = (double)o[a];
+      bj_apply(binv + 6 * (int64_t)g, rr, w);
+      for (int a = 0; a < nr; ++a) {
+        const float v = s * (float)w[a];
+        y[r0 + a] = v;
+        yy += (double)v * (double)v;
+      }
+    }
+  }
+  write_partial(part, yy);
+}
+
+template <typename TM>
+static AmgFine<TM> amg_fine(int n, int64_t nnz, int n_groups, const int32_t *grp, const int32_t *rowptr,
+                            const int32_t *col, const void *val, const float *binv, double rho) {
+  AmgFine<TM> f;
+  f.s = shape_of(n, nnz, n_groups, grp);
+  f.grp = grp;
+  f.rowptr = rowptr;
+  f.col = col;
+  f.val = (const TM *)val;
+  f.binv = binv;
+  f.rho = rho;
+  return f;
+}
+
+struct AmgPcgArgs {
+  int n;
+  int64_t nnz;
+  int n_groups;
+  const int32_t *grp, *rowptr, *col;
+  const void *val;
+  const float *binv;
+  double rho0;
+  const fea_amg_level *lv;
+  int nl, deg;
+  const double *b;
+  double *x, *r, *z, *p, *q;
+  double *ws;
+  double tol;
+  int max_iter, iters;
+  double *status_host;
+  hipStream_t st;
+};
+
+template <typename TM>
+static int amg_pcg_start_t(const AmgPcgArgs &a) {
+  const AmgFine<TM> f = amg_fine<TM>(a.n, a.nnz, a.n_groups, a.grp, a.rowptr, a.col, a.val, a.binv, a.rho0);
+  const int gu = vec_grid();
+  launch_spmv<TM, double>(f.s, a.grp, a.rowptr, a.col, f.val, a.x, a.b, a.r, a.st);  // r = b - A x
+  amg_vcycle<TM>(f, a.lv, a.nl, a.deg, a.b, a.z, nullptr, a.st);                    // z = M^-1 b
+  k_amg_bz<<<gu, kBlock, 0, a.st>>>(a.n, a.b, a.z, a.ws);
+  amg_vcycle<TM>(f, a.lv, a.nl, a.deg, a.r, a.z, nullptr, a.st);                    // z = M^-1 r
+  k_amg_pcg_init<<<gu, kBlock, 0, a.st>>>(a.n, a.r, a.z, a.p, a.ws);
+  FEA_CHECK_LAUNCH("fea_pcg_start_amg");
+  return FEA_OK;
+}
+
+template <typename TM>
+static int amg_pcg_iterate_t(const AmgPcgArgs &a) {
+  const AmgFine<TM> f = amg_fine<TM>(a.n, a.nnz, a.n_groups, a.grp, a.rowptr, a.col, a.val, a.binv, a.rho0);
+  const int gu = vec_grid();
+  const double *done = a.ws + W_DONE;
+  for (int it = 0; it < a.iters; ++it) {
+    launch_pcg_spmv<TM, double>(f.s, a.grp, a.rowptr, a.col, f.val, a.p, a.q, a.ws, a.st);
+    k_amg_pcg_update<<<gu, kBlock, 0, a.st>>>(a.n, a.x, a.r, a.p, a.q, a.ws);
+    amg_vcycle<TM>(f, a.lv, a.nl, a.deg, a.r, a.z, done, a.st);
+    k_amg_pcg_rz<<<gu, kBlock, 0, a.st>>>(a.n, a.r, a.z, a.ws);
+    k_pcg_dir<double><<<PCG_G, kBlock, 0, a.st>>>(a.n, a.z, a.p, a.ws, a.tol, a.max_iter, gu);
+  }
+  FEA_CHECK_LAUNCH("fea_pcg_iterate_amg");
+  if (a.status_host != nullptr) {
+    double h[4];
+    FEA_HIP(hipMemcpyAsync(h, a.ws, sizeof(h), hipMemcpyDeviceToHost, a.st), "fea_pcg_iterate_amg");
+    FEA_HIP(hipStreamSynchronize(a.st), "fea_pcg_iterate_amg");
+    for (int i = 0; i < 4; ++i) a.status_host[i] = h[i];
+  }
+  return FEA_OK;
+}
+
+static int check_amg_levels(const fea_amg_level *lv, int nl, int deg, int n, const char *who) {
+  FEA_CHECK_ARG(lv != nullptr && nl >= 2 && nl <= AMG_MAX_LEVELS, "%s: 2 .. %d levels expected", who,
+                AMG_MAX_LEVELS);
+  FEA_CHECK_ARG(deg >= 1 && deg <= AMG_MAX_DEG, "%s: smoother degree 1 .. %d", who, AMG_MAX_DEG);
+  FEA_CHECK_ARG(lv[0].n == n, "%s: level 0 has %d rows, the matrix %d", who, lv[0].n, n);
+  for (int l = 0; l < nl; ++l) {
+    const fea_amg_level &L = lv[l];
+    FEA_CHECK_ARG(L.x && L.x2 && L.d && L.r && (l == 0 || L.b), "%s: level %d work vectors missing", who, l);
+    if (l + 1 < nl) {
+      FEA_CHECK_ARG(L.prow && L.pagg && L.pval && L.pt_ptr && L.pt_blk && L.pt_row && L.nagg * 6 == lv[l + 1].n,
+                    "%s: level %d prolongator inconsistent with level %d", who, l, l + 1);
+      FEA_CHECK_ARG(l == 0 || (L.rowptr && L.col && L.val && L.dinv && L.n % 6 == 0 && L.rho > 0.0),
+                    "%s: level %d operator / smoother missing", who, l);
+    } else {
+      FEA_CHECK_ARG(L.dense_inv != nullptr && L.n > 0, "%s: the coarsest level needs its dense inverse", who);
+    }
+  }
+  return FEA_OK;
+}
+
+}  // namespace fea
+
+extern "C" int fea_amg_level_size(void) { return (int)sizeof(fea_amg_level); }
+
+extern "C" int fea_amg_vcycle(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
+                              const int32_t *rowptr, const int32_t *col, const void *val, const float *binv,
+                              double rho0, const fea_amg_level *levels, int n_levels, int degree, const double *b,
+                              double *z, void *stream) {
+  FEA_CHECK_ARG(mat_dtype == FEA_F64 || mat_dtype == FEA_F32, "fea_amg_vcycle: bad matrix dtype");
+  FEA_CHECK_ARG(n > 0 && grp && n_groups != 0 && rowptr && col && val && binv && b && z && b != z && rho0 > 0.0,
+                "fea_amg_vcycle: bad arguments (node groups and block-Jacobi inverses are required)");
+  const int rc = check_amg_levels(levels, n_levels, degree, n, "fea_amg_vcycle");
+  if (rc != FEA_OK) return rc;
+  hipStream_t st = as_stream(stream);
+  if (mat_dtype == FEA_F64)
+    amg_vcycle<double>(amg_fine<double>(n, nnz, n_groups, grp, rowptr, col, val, binv, rho0), levels, n_levels,
+                       degree, b, z, nullptr, st);
+  else
+    amg_vcycle<float>(amg_fine<float>(n, nnz, n_groups, grp, rowptr, col, val, binv, rho0), levels, n_levels,
+                      degree, b, z, nullptr, st);
+  FEA_CHECK_LAUNCH("fea_amg_vcycle");
+  return FEA_OK;
+}
+
+extern "C" int fea_amg_power(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
+                             const int32_t *rowptr, const int32_t *col, const void *val, const float *binv, float *x,
+                             float *y, double *partial, int iters, double *rho_out, void *stream) {
+  FEA_CHECK_ARG(mat_dtype == FEA_F64 || mat_dtype == FEA_F32, "fea_amg_power: bad matrix dtype");
+  FEA_CHECK_ARG(n > 0 && grp && n_groups != 0 && rowptr && col && val && binv && x && y && partial && rho_out &&
+                    iters >= 1,
+                "fea_amg_power: bad arguments");
+  hipStream_t st = as_stream(stream);
+  const SpmvShape s = shape_of(n, nnz, n_groups, grp);
+  const int grid = spmv_grid(s);
+  std::vector<double> h(grid);
+  // x has unit 2-norm; every step maps a unit input to D^-1 A of it, whose norm estimates
+  // lambda_max, and the next step scales its image back by that norm
+  float scale = 1.f;
+  double lam = 0.0;
+  for (int it = 0; it < iters; ++it) {
+    float *in = (it & 1) ? y : x, *out = (it & 1) ? x : y;
+    if (mat_dtype == FEA_F64) {
+#define FEA_PW(SW, MD) \
+  k_amg_power0<double, SW, MD><<<grid, kBlock, 0, st>>>(s.ngroups, grp, rowptr, col, (const double *)val, binv, in, out, scale, partial)
+      FEA_SW_SWITCH(FEA_PW);
+#undef FEA_PW
+    } else {
+#define FEA_PW(SW, MD) \
+  k_amg_power0<float, SW, MD><<<grid, kBlock, 0, st>>>(s.ngroups, grp, rowptr, col, (const float *)val, binv, in, out, scale, partial)
+      FEA_SW_SWITCH(FEA_PW);
+#undef FEA_PW
+    }
+    FEA_CHECK_LAUNCH("fea_amg_power");
+    FEA_HIP(hipMemcpyAsync(h.data(), partial, sizeof(double) * grid, hipMemcpyDeviceToHost, st), "fea_amg_power");
+    FEA_HIP(hipStreamSynchronize(st), "fea_amg_power");
+    double yy = 0.0;
+    for (int k = 0; k < grid; ++k) yy += h[k];
+    const double ny = std::sqrt(yy);
+    lam = ny;
+    if (!(ny > 0.0)) break;
+    scale = (float)(1.0 / ny);
+  }
+  *rho_out = lam;
+  return FEA_OK;
+}
+
+extern "C" int fea_pcg_start_amg(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
+                                 const int32_t *rowptr, const int32_t *col, const void *val, const float *binv,
+                                 double rho0, const fea_amg_level *levels, int n_levels, int degree, const double *b,
+                                 double *x, double *r, double *z, double *p, double *ws, void *stream) {
+  FEA_CHECK_ARG(mat_dtype == FEA_F64 || mat_dtype == FEA_F32, "fea_pcg_start_amg: bad matrix dtype");
+  FEA_CHECK_ARG(n > 0 && grp && n_groups != 0 && rowptr && col && val && binv && b && x && r && z && p && ws &&
+                    rho0 > 0.0,
+                "fea_pcg_start_amg: bad arguments");
+  const int rc = check_amg_levels(levels, n_levels, degree, n, "fea_pcg_start_amg");
+  if (rc != FEA_OK) return rc;
+  AmgPcgArgs a{n, nnz, n_groups, grp, rowptr, col, val, binv, rho0, levels, n_levels, degree, b, x, r, z, p,
+               nullptr, ws, 0.0, 0, 0, nullptr, as_stream(stream)};
+  return mat_dtype == FEA_F64 ? amg_pcg_start_t<double>(a) : amg_pcg_start_t<float>(a);
+}
+
+extern "C" int fea_pcg_iterate_amg(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
+                                   const int32_t *rowptr, const int32_t *col, const void *val, const float *binv,
+                                   double rho0, const fea_amg_level *levels, int n_levels, int degree, double *x,
+                                   double *r, double *z, double *p, double *q, double *ws, double tol, int max_iter,
+                                   int iters, double *status_host, void *stream) {
+  FEA_CHECK_ARG(mat_dtype == FEA_F64 || mat_dtype == FEA_F32, "fea_pcg_iterate_amg: bad matrix dtype");
+  FEA_CHECK_ARG(n > 0 && grp && n_groups != 0 && rowptr && col && val && binv && x && r && z && p && q && ws &&
+                    rho0 > 0.0 && iters >= 0 && max_iter >= 1,
+                "fea_pcg_iterate_amg: bad arguments");
+  const int rc = check_amg_levels(levels, n_levels, degree, n, "fea_pcg_iterate_amg");
+  if (rc != FEA_OK) return rc;
+  AmgPcgArgs a{n, nnz, n_groups, grp, rowptr, col, val, binv, rho0, levels, n_levels, degree, nullptr, x, r, z, p,
+               q, ws, tol, max_iter, iters, status_host, as_stream(stream)};
+  return mat_dtype == FEA_F64 ? amg_pcg_iterate_t<double>(a) : amg_pcg_iterate_t<float>(a);
 }

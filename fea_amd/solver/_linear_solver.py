@@ -32,13 +32,22 @@ class PCG:
     the same solution; needs ``grp``, else point Jacobi is used).
     ``spmv_col``: the SpMV's column stream, required for a triple-structured group
     table (``n_groups < 0``: the compact per-triple columns, ``DevicePlan.spmv_col``).
+    ``precond='amg'``: the smoothed-aggregation V-cycle of ``amg`` (solver.amg.AMG, built
+    for the same pattern; float64 vectors), with the node-block Jacobi of the current
+    matrix on its finest level.
     """
 
     def __init__(self, n, nnz, rowptr, col, diag_slot, mat_dtype, device, vec_dtype=None, grp=None,
-                 n_groups=0, chunk=32, precond='jacobi', spmv_col=None):
-        if precond not in ('jacobi', 'block'):
-            raise ValueError("precond must be 'jacobi' or 'block'")
-        self.block = precond == 'block' and grp is not None and int(n_groups) != 0
+                 n_groups=0, chunk=32, precond='jacobi', spmv_col=None, amg=None):
+        if precond not in ('jacobi', 'block', 'amg'):
+            raise ValueError("precond must be 'jacobi', 'block' or 'amg'")
+        self.amg = amg if precond == 'amg' else None
+        if precond == 'amg':
+            if amg is None or grp is None or int(n_groups) == 0:
+                raise ValueError("precond='amg' needs the hierarchy (solver.amg.AMG) and the node row groups")
+            if (mat_dtype if vec_dtype is None else vec_dtype) != F64:
+                raise ValueError("precond='amg' runs with float64 vectors")
+        self.block = precond in ('block', 'amg') and grp is not None and int(n_groups) != 0
         self.n, self.nnz, self.chunk = int(n), int(nnz), int(chunk)
         self.mat_dtype = mat_dtype
         self.vec_dtype = mat_dtype if vec_dtype is None else vec_dtype
@@ -70,6 +79,9 @@ class PCG:
                                        self.diag_slot, val, self.dinv)
         else:
             nat.ops().jacobi_inv(self.mat_dtype, self.vec_dtype, self.diag_slot, val, self.dinv)
+        if self.amg is not None:
+            # the V-cycle's finest level smooths with this matrix: lambda_max of D^-1 A for it
+            self.amg.estimate_rho(self.mat_dtype, val, self.dinv)
 
     def solve(self, b, x0=None, tol=1e-5, max_iter=1500):
         """Solve K x = b; returns x (vector precision)."""
@@ -81,6 +93,8 @@ class PCG:
             self.x.zero_()
         else:
             self.x.copy_(x0)
+        if self.amg is not None:
+            return self._solve_amg(b, tol, max_iter)
         pc = 1 if self.block else 0
         O = nat.ops()
         O.pcg_start(self.mat_dtype, self.vec_dtype, self.n_groups, self.grp, self.rowptr, self.spmv_col, self.val,
@@ -91,6 +105,20 @@ class PCG:
             O.pcg_iterate(self.mat_dtype, self.vec_dtype, self.n_groups, self.grp, self.rowptr, self.spmv_col,
                           self.val, pc, self.dinv, self.x, self.r, self.z, self.p, self.q, self.ws, float(tol),
                           int(max_iter), int(step), self._status_t)
+            done_iters += step
+            if self.status[1] != 0.0 or done_iters >= max_iter:
+                break
+        self.iterations = int(self.status[0])
+        return self.x
+
+    def _solve_amg(self, b, tol, max_iter):
+        A = self.amg
+        A.pcg_start(self.mat_dtype, self.val, self.dinv, b, self.x, self.r, self.z, self.p, self.ws)
+        done_iters = 0
+        while True:
+            step = min(self.chunk, max(1, max_iter - done_iters))
+            A.pcg_iterate(self.mat_dtype, self.val, self.dinv, self.x, self.r, self.z, self.p, self.q, self.ws,
+                          float(tol), int(max_iter), int(step), self.status)
             done_iters += step
             if self.status[1] != 0.0 or done_iters >= max_iter:
                 break

@@ -46,7 +46,10 @@ round-off step below):
   for the PCG relative tolerance eta_k, clamped to [sqrt(cg_tol), eta_max];
   the default 'fixed' keeps the reference's constant cg_tol.
 * ``preconditioner='block'``: node-block (3x3) Jacobi instead of the reference's point
-  Jacobi scaling (other CG iterates, the same Newton solution).
+  Jacobi scaling (other CG iterates, the same Newton solution); ``'amg'``: a
+  smoothed-aggregation multigrid V-cycle (solver/amg.py) built once per model from the
+  tangent at u = 0, its finest level smoothed with the current tangent
+  (``amg_degree``: Chebyshev degree of the smoother).
 """
 from __future__ import annotations
 
@@ -85,11 +88,14 @@ class StaticImplicitSolver(BaseSolver):
                  cg_tol: float = 1e-5, cg_max_iter: int = 1500, ir_steps: int = 1,
                  armijo_slack: float = 0.0, predictor=True, verbose: bool = False,
                  pcg_chunk: int = 32, progress=None, step_tol: float = 1e-12, forcing: str = 'fixed',
-                 eta_max: float = 1e-2, preconditioner: str = 'jacobi') -> None:
+                 eta_max: float = 1e-2, preconditioner: str = 'jacobi', amg_degree: int = 2) -> None:
         super().__init__()
-        if preconditioner not in ('jacobi', 'block'):
-            raise ValueError("preconditioner must be 'jacobi' or 'block'")
+        if preconditioner not in ('jacobi', 'block', 'amg'):
+            raise ValueError("preconditioner must be 'jacobi', 'block' or 'amg'")
+        if preconditioner == 'amg' and precision == 'mixed-ir':
+            raise ValueError("preconditioner='amg' needs float64 PCG vectors (not 'mixed-ir')")
         self.preconditioner = preconditioner
+        self.amg_degree = int(amg_degree)
         if precision not in ('fp64', 'mixed', 'mixed-f32', 'mixed-ir'):
             raise ValueError("precision must be 'fp64', 'mixed', 'mixed-f32' or 'mixed-ir'")
         self.maximum_iteration = maximum_iteration
@@ -122,6 +128,19 @@ class StaticImplicitSolver(BaseSolver):
         if any(isinstance(l, NodeToSurfaceContact) for l in assembly._loads.values()):
             raise NotImplementedError('node-to-surface contact has no tangent; use DynamicExplicitSolver')
         self._pcg = {}
+        if self.preconditioner == 'amg':
+            self._build_amg()
+
+    def _build_amg(self):
+        """The multigrid hierarchy of this model, from its float64 tangent at u = 0 under the
+        full load (built once per model, like the sparsity pattern)."""
+        from .amg import AMG
+        p = self.assembly.plan
+        U = torch.zeros(3 * p.n_nodes, dtype=torch.float64, device=p.device)
+        _, K = p.tangent(U, 1.0, F64)
+        self.amg = AMG(p, K, degree=self.amg_degree)
+        self.amg.version = getattr(p, 'version', 0)
+        self.amg_setup_s = self.amg.setup_time
 
     # ------------------------------------------------------------------ API
     def solve(self, GC0=None, *args, **kwargs):
@@ -154,9 +173,14 @@ class StaticImplicitSolver(BaseSolver):
         if key not in self._pcg:
             self._pcg = {k: v for k, v in self._pcg.items() if k[2] == key[2]}
             p = self.assembly.plan
+            amg = None
+            if self.preconditioner == 'amg':
+                if getattr(self, 'amg', None) is None or self.amg.version != key[2]:
+                    self._build_amg()                # the pattern changed (contact pairs re-staged)
+                amg = self.amg
             self._pcg[key] = PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, mat_dtype, p.device,
                                  vec_dtype=vec_dtype, grp=p.grp, n_groups=p.spmv_groups, chunk=self.pcg_chunk,
-                                 precond=self.preconditioner, spmv_col=p.spmv_col)
+                                 precond=self.preconditioner, spmv_col=p.spmv_col, amg=amg)
         return self._pcg[key]
 
     def _vec_stats(self, x):

@@ -246,6 +246,72 @@ int fea_pcg_run_noalloc(int mat_dtype, int vec_dtype, int n, int64_t nnz, int n_
                         const int32_t *rowptr, const int32_t *col, const void *val, int precond,
                         const void *dinv, const void *b, void *x, void *r, void *z, void *p, void *q,
                         double tol, int max_iter, int iters, double *status_host, void *stream);
+/* ---- smoothed-aggregation multigrid preconditioner (an addition: the reference's CG uses
+ * point-Jacobi scaling, static_implicit.py:314-321; same solution, far fewer iterations).
+ * Host set-up, once per model: fea_amg_build aggregates the node groups of the free-DOF
+ * CSR (host arrays; rows grouped by node as `grp`, 1..3 rows each), builds the tentative
+ * prolongator from the six rigid-body modes (dof_xyz: coordinates of each row's node,
+ * dof_comp: its component 0..2), smooths it with one damped block-Jacobi step, forms the
+ * Galerkin coarse operators (6 DOFs per aggregate) until at most max_coarse DOFs remain
+ * and inverts the coarsest densely.  fea_amg_level_sizes / fea_amg_level_export copy a
+ * level out (sizes: rows, aggregates of the next level, prolongator blocks, operator
+ * nonzeros, row groups, dense order); any NULL output is skipped. */
+typedef struct fea_amg_host fea_amg_host;
+int fea_amg_build(int n, const int32_t *rowptr, const int32_t *col, const double *val, int n_groups,
+                  const int32_t *grp, const double *dof_xyz, const int32_t *dof_comp, int max_coarse,
+                  int max_levels, int n_threads, fea_amg_host **out);
+int fea_amg_num_levels(const fea_amg_host *h);
+int fea_amg_level_sizes(const fea_amg_host *h, int level, int64_t *sizes);
+int fea_amg_level_export(const fea_amg_host *h, int level, int32_t *prow, int32_t *pagg, float *pval,
+                         int32_t *pt_ptr, int32_t *pt_blk, int32_t *pt_row, int32_t *rowptr, int32_t *col,
+                         double *val, float *dinv36, double *rho, float *dense_inv);
+void fea_amg_destroy(fea_amg_host *h);
+
+/* A level of the hierarchy on the device (caller-owned buffers; a host array of these is
+ * passed to the calls below). */
+typedef struct fea_amg_level {
+  int32_t n;                  /* rows of this level */
+  int32_t nagg;               /* aggregates of the next level (its rows / 6); 0 on the coarsest */
+  double rho;                 /* lambda_max(D^-1 A) of the level's smoother (levels >= 1) */
+  const int32_t *rowptr;      /* levels >= 1: operator rows, in groups of 6 sharing one column list */
+  const int32_t *col;
+  const float *val;
+  const float *dinv;          /* levels >= 1: 6x6 inverse of each group's diagonal block */
+  const int32_t *prow;        /* prolongator to the next level: per row, blocks of 6 columns */
+  const int32_t *pagg;        /* aggregate of each block */
+  const float *pval;          /* 6 values per block */
+  const int32_t *pt_ptr;      /* transpose: per aggregate, its (block, row) entries */
+  const int32_t *pt_blk;
+  const int32_t *pt_row;
+  const float *dense_inv;     /* coarsest level: n x n inverse */
+  double *b, *x, *x2, *d, *r; /* work vectors of length n (b unused on level 0) */
+} fea_amg_level;
+int fea_amg_level_size(void);
+/* z = V-cycle(b): level 0 is the matrix (node-grouped CSR as for fea_spmv, current values)
+ * with its node-block Jacobi inverses binv (fea_block_jacobi_inv) and lambda_max estimate
+ * rho0 (fea_amg_power); Chebyshev smoothing of the given degree on [rho/30, 1.1 rho]. */
+int fea_amg_vcycle(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp, const int32_t *rowptr,
+                   const int32_t *col, const void *val, const float *binv, double rho0,
+                   const fea_amg_level *levels, int n_levels, int degree, const double *b, double *z,
+                   void *stream);
+/* lambda_max(D^-1 A) of the level-0 matrix by `iters` power steps from x (unit 2-norm;
+ * x, y float work vectors, partial: float64 scratch of the SpMV grid); synchronises. */
+int fea_amg_power(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp, const int32_t *rowptr,
+                  const int32_t *col, const void *val, const float *binv, float *x, float *y, double *partial,
+                  int iters, double *rho_out, void *stream);
+/* PCG preconditioned by the V-cycle: as fea_pcg_start / fea_pcg_iterate with float64
+ * vectors; the stopping test is r.z / b.(M^-1 b) < tol as for the Jacobi PCG.  The
+ * V-cycle stores operators in float and keeps every vector and sum in float64. */
+int fea_pcg_start_amg(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp, const int32_t *rowptr,
+                      const int32_t *col, const void *val, const float *binv, double rho0,
+                      const fea_amg_level *levels, int n_levels, int degree, const double *b, double *x, double *r,
+                      double *z, double *p, double *ws, void *stream);
+int fea_pcg_iterate_amg(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
+                        const int32_t *rowptr, const int32_t *col, const void *val, const float *binv, double rho0,
+                        const fea_amg_level *levels, int n_levels, int degree, double *x, double *r, double *z,
+                        double *p, double *q, double *ws, double tol, int max_iter, int iters,
+                        double *status_host, void *stream);
+
 /* One SpMV stage of the PCG iteration alone: q = A p and the p.q partials
  * (the CSR `A @ p` of _linear_solver.py:32 fused with the dot of :33); a no-op
  * when ws marks the solve converged. */
