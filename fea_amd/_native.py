@@ -53,6 +53,22 @@ _SIGS = {
                                 C_P, C_P, C_P, C_P, C_DBL, C_INT, C_INT, C_P, C_P]),
     'fea_pcg_run_noalloc': (C_INT, [C_INT, C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_INT, C_P, C_P, C_P,
                                     C_P, C_P, C_P, C_P, C_DBL, C_INT, C_INT, C_P, C_P]),
+    'fea_amg_build': (C_INT, [C_INT, C_P, C_P, C_P, C_INT, C_P, C_P, C_P, C_INT, C_INT,
+                                     C_INT, ctypes.POINTER(C_P)]),
+    'fea_amg_num_levels': (C_INT, [C_P]),
+    'fea_amg_level_sizes': (C_INT, [C_P, C_INT, C_P]),
+    'fea_amg_level_export': (C_INT, [C_P, C_INT] + [C_P] * 12),
+    'fea_amg_destroy': (None, [C_P]),
+    'fea_amg_level_size': (C_INT, []),
+    'fea_amg_vcycle': (C_INT, [C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P,
+                                      C_DBL, C_P, C_INT, C_INT, C_P, C_P, C_P]),
+    'fea_amg_power': (C_INT, [C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P,
+                                     C_P, C_P, C_INT, ctypes.POINTER(C_DBL), C_P]),
+    'fea_pcg_start_amg': (C_INT, [C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P,
+                                         C_DBL, C_P, C_INT, C_INT] + [C_P] * 7),
+    'fea_pcg_iterate_amg': (C_INT, [C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P,
+                                           C_P, C_DBL, C_P, C_INT, C_INT] + [C_P] * 6 +
+                            [C_DBL, C_INT, C_INT, C_P, C_P]),
     'fea_block_jacobi_inv': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_pcg_spmv': (C_INT, [C_INT, C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_spmv_residual_mixed': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),

@@ -34,35 +34,10 @@ class AmgLevel(ctypes.Structure):
                 ('b', _P), ('x', _P), ('x2', _P), ('d', _P), ('r', _P)]
 
 
-_SIGS = {
-    'fea_amg_build': (ctypes.c_int, [ctypes.c_int, _P, _P, _P, ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int,
-                                     ctypes.c_int, ctypes.POINTER(_P)]),
-    'fea_amg_num_levels': (ctypes.c_int, [_P]),
-    'fea_amg_level_sizes': (ctypes.c_int, [_P, ctypes.c_int, _P]),
-    'fea_amg_level_export': (ctypes.c_int, [_P, ctypes.c_int] + [_P] * 12),
-    'fea_amg_destroy': (None, [_P]),
-    'fea_amg_level_size': (ctypes.c_int, []),
-    'fea_amg_vcycle': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int, _P, _P, _P, _P, _P,
-                                      ctypes.c_double, _P, ctypes.c_int, ctypes.c_int, _P, _P, _P]),
-    'fea_amg_power': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int, _P, _P, _P, _P, _P, _P,
-                                     _P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), _P]),
-    'fea_pcg_start_amg': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int, _P, _P, _P, _P, _P,
-                                         ctypes.c_double, _P, ctypes.c_int, ctypes.c_int] + [_P] * 7),
-    'fea_pcg_iterate_amg': (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int, _P, _P, _P, _P,
-                                           _P, ctypes.c_double, _P, ctypes.c_int, ctypes.c_int] + [_P] * 6 +
-                            [ctypes.c_double, ctypes.c_int, ctypes.c_int, _P, _P]),
-}
 
 
 def _lib():
-    L = nat.lib()
-    if not getattr(L, '_amg_sigs', False):
-        for name, (res, args) in _SIGS.items():
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
-        L._amg_sigs = True
-    return L
+    return nat.lib()
 
 
 def _ptr(t):
