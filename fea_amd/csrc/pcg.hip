@@ -760,7 +760,8 @@ extern "C" int fea_pcg_run_noalloc(int mat_dtype, int vec_dtype, int n, int64_t 
 // kernel of the iteration path returns at once when the solve is marked converged.
 namespace fea {
 
-constexpr int AMG_SW = 16;  // lanes per 6-row group / per aggregate in the coarse-level kernels
+constexpr int AMG_SW = 64;  // lanes per 6-row group / per aggregate in the coarse-level kernels
+typedef _Float16 half_t;    // prolongator values: IEEE binary16
 
 static int amg_grid(int64_t items, int per_block) {
   const int64_t g = (items + per_block - 1) / per_block;
@@ -809,16 +810,16 @@ __global__ __launch_bounds__(kBlock) void k_amg_cheb0(int ngroups, const int32_t
     if (g >= ngroups) continue;
     double o[3] = {0.0, 0.0, 0.0};
     if (!FIRST) group_spmv<TM, double, SW, MODE>(g, lane, grp, rowptr, col, val, x, o);
-    if (lane == 0) {
-      const int r0 = grp[g], nr = grp[g + 1] - r0;
+    // every lane holds the group's products (butterfly sums): lane a updates row a
+    const int r0 = grp[g], nr = grp[g + 1] - r0;
+    if (lane < nr) {
       double rr[3] = {0.0, 0.0, 0.0}, w[3];
       for (int a = 0; a < nr; ++a) rr[a] = b[r0 + a] - o[a];
       bj_apply(binv + 6 * (int64_t)g, rr, w);
-      for (int a = 0; a < nr; ++a) {
-        const double dn = (c1 != 0.0 ? c1 * d[r0 + a] : 0.0) + c2 * w[a];
-        d[r0 + a] = dn;
-        xo[r0 + a] = (FIRST ? 0.0 : x[r0 + a]) + dn;
-      }
+      const int a = lane;
+      const double dn = (c1 != 0.0 ? c1 * d[r0 + a] : 0.0) + c2 * w[a];
+      d[r0 + a] = dn;
+      xo[r0 + a] = (FIRST ? 0.0 : x[r0 + a]) + dn;
     }
   }
 }
@@ -838,10 +839,7 @@ __global__ __launch_bounds__(kBlock) void k_amg_resid0(int ngroups, const int32_
     if (g >= ngroups) continue;
     double o[3];
     const int r0 = group_spmv<TM, double, SW, MODE>(g, lane, grp, rowptr, col, val, x, o);
-    if (lane == 0) {
-      const int nr = grp[g + 1] - r0;
-      for (int a = 0; a < nr; ++a) r[r0 + a] = b[r0 + a] - o[a];
-    }
+    if (lane < grp[g + 1] - r0) r[r0 + lane] = b[r0 + lane] - (lane == 0 ? o[0] : (lane == 1 ? o[1] : o[2]));
   }
 }
 
@@ -861,20 +859,18 @@ __global__ __launch_bounds__(kBlock) void k_amg_cheb_c(int ngroups, const int32_
     if (g >= ngroups) continue;
     double o[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (!FIRST) group6_spmv<AMG_SW>(g, lane, rowptr, col, val, x, o);
-    if (lane == 0) {
+    if (lane < 6) {  // lane m updates row m of the group
       double rr[6];
 #pragma unroll
-      for (int m = 0; m < 6; ++m) rr[m] = b[6 * g + m] - o[m];
-      const float *D = dinv + 36 * (int64_t)g;
+      for (int k = 0; k < 6; ++k) rr[k] = b[6 * g + k] - o[k];
+      const int m = lane;
+      const float *D = dinv + 36 * (int64_t)g + 6 * m;
+      double w = 0.0;
 #pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        double w = 0.0;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) w += (double)D[6 * m + k] * rr[k];
-        const double dn = (c1 != 0.0 ? c1 * d[6 * g + m] : 0.0) + c2 * w;
-        d[6 * g + m] = dn;
-        xo[6 * g + m] = (FIRST ? 0.0 : x[6 * g + m]) + dn;
-      }
+      for (int k = 0; k < 6; ++k) w += (double)D[k] * rr[k];
+      const double dn = (c1 != 0.0 ? c1 * d[6 * g + m] : 0.0) + c2 * w;
+      d[6 * g + m] = dn;
+      xo[6 * g + m] = (FIRST ? 0.0 : x[6 * g + m]) + dn;
     }
   }
 }
@@ -892,18 +888,23 @@ __global__ __launch_bounds__(kBlock) void k_amg_resid_c(int ngroups, const int32
     if (g >= ngroups) continue;
     double o[6];
     group6_spmv<AMG_SW>(g, lane, rowptr, col, val, x, o);
-    if (lane == 0)
+    if (lane < 6) {
+      double om = o[0];
 #pragma unroll
-      for (int m = 0; m < 6; ++m) r[6 * g + m] = b[6 * g + m] - o[m];
+      for (int m = 1; m < 6; ++m)
+        if (lane == m) om = o[m];
+      r[6 * g + lane] = b[6 * g + lane] - om;
+    }
   }
 }
 
-// bc = P^T r: one AMG_SW-lane sub-wave per coarse aggregate over its transposed entries
+// bc = P^T r: one wave per coarse aggregate over its transposed entries, whose 6 values
+// (ptval, binary16) are stored in entry order so a wave reads them contiguously
 __global__ __launch_bounds__(kBlock) void k_amg_restrict(int nagg, const int32_t *__restrict__ pt_ptr,
-                                                         const int32_t *__restrict__ pt_blk,
                                                          const int32_t *__restrict__ pt_row,
-                                                         const float *__restrict__ pval, const double *__restrict__ r,
-                                                         double *__restrict__ bc, const double *__restrict__ done) {
+                                                         const half_t *__restrict__ ptval,
+                                                         const double *__restrict__ r, double *__restrict__ bc,
+                                                         const double *__restrict__ done) {
   if (amg_done(done)) return;
   constexpr int GPB = kBlock / AMG_SW;
   const int lane = threadIdx.x % AMG_SW;
@@ -913,7 +914,7 @@ __global__ __launch_bounds__(kBlock) void k_amg_restrict(int nagg, const int32_t
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int t = pt_ptr[I] + lane; t < pt_ptr[I + 1]; t += AMG_SW) {
       const double rv = r[pt_row[t]];
-      const float *p = pval + 6 * (int64_t)pt_blk[t];
+      const half_t *p = ptval + 6 * (int64_t)t;
 #pragma unroll
       for (int m = 0; m < 6; ++m) acc[m] += (double)p[m] * rv;
     }
@@ -930,13 +931,13 @@ __global__ __launch_bounds__(kBlock) void k_amg_restrict(int nagg, const int32_t
 // x += P xc: one thread per fine row over its 6-column blocks
 __global__ __launch_bounds__(kBlock) void k_amg_prolong(int n, const int32_t *__restrict__ prow,
                                                         const int32_t *__restrict__ pagg,
-                                                        const float *__restrict__ pval, const double *__restrict__ xc,
+                                                        const half_t *__restrict__ pval, const double *__restrict__ xc,
                                                         double *__restrict__ x, const double *__restrict__ done) {
   if (amg_done(done)) return;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double s = 0.0;
     for (int k = prow[i]; k < prow[i + 1]; ++k) {
-      const float *p = pval + 6 * (int64_t)k;
+      const half_t *p = pval + 6 * (int64_t)k;
       const double *c = xc + 6 * (int64_t)pagg[k];
 #pragma unroll
       for (int m = 0; m < 6; ++m) s += (double)p[m] * c[m];
@@ -987,16 +988,33 @@ struct AmgFine {
   double rho;
 };
 
+// the first Chebyshev step from x = 0 on level 0: d = xo = c2 M^-1 b, one thread per group
+__global__ __launch_bounds__(kBlock) void k_amg_cheb0_first(int ngroups, const int32_t *__restrict__ grp,
+                                                            const float *__restrict__ binv,
+                                                            const double *__restrict__ b, double *__restrict__ xo,
+                                                            double *__restrict__ d, double c2,
+                                                            const double *__restrict__ done) {
+  if (amg_done(done)) return;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += (int64_t)gridDim.x * blockDim.x) {
+    const int r0 = grp[g], nr = grp[g + 1] - r0;
+    double rr[3], w[3];
+    bj_load(b, r0, nr, rr);
+    bj_apply(binv + 6 * g, rr, w);
+    for (int a = 0; a < nr; ++a) {
+      const double dn = c2 * w[a];
+      d[r0 + a] = dn;
+      xo[r0 + a] = dn;
+    }
+  }
+}
+
 template <typename TM>
 static void cheb0(const AmgFine<TM> &f, bool first, const double *b, const double *x, double *xo, double *d,
                   double c1, double c2, const double *done, hipStream_t st) {
   const int grid = spmv_grid(f.s);
   const auto &s = f.s;
   if (first) {
-#define FEA_AC0(SW, MD) \
-  k_amg_cheb0<TM, SW, MD, 1><<<grid, kBlock, 0, st>>>(s.ngroups, f.grp, f.rowptr, f.col, f.val, f.binv, b, x, xo, d, c1, c2, done)
-    FEA_SW_SWITCH(FEA_AC0);
-#undef FEA_AC0
+    k_amg_cheb0_first<<<amg_grid(s.ngroups, kBlock), kBlock, 0, st>>>(s.ngroups, f.grp, f.binv, b, xo, d, c2, done);
   } else {
 #define FEA_AC1(SW, MD) \
   k_amg_cheb0<TM, SW, MD, 0><<<grid, kBlock, 0, st>>>(s.ngroups, f.grp, f.rowptr, f.col, f.val, f.binv, b, x, xo, d, c1, c2, done)
@@ -1055,8 +1073,9 @@ static void amg_vcycle(const AmgFine<TM> &f, const fea_amg_level *lv, int nl, in
     else
       k_amg_resid_c<<<amg_grid(L.n / 6, kBlock / AMG_SW), kBlock, 0, st>>>(L.n / 6, L.rowptr, L.col, L.val, bl, xa,
                                                                            L.r, done);
-    k_amg_restrict<<<amg_grid(L.nagg, kBlock / AMG_SW), kBlock, 0, st>>>(L.nagg, L.pt_ptr, L.pt_blk, L.pt_row,
-                                                                        L.pval, L.r, lv[l + 1].b, done);
+    k_amg_restrict<<<amg_grid(L.nagg, kBlock / AMG_SW), kBlock, 0, st>>>(L.nagg, L.pt_ptr, L.pt_row,
+                                                                        (const half_t *)L.ptval, L.r, lv[l + 1].b,
+                                                                        done);
   }
   const fea_amg_level &C = lv[nl - 1];
   k_amg_dense<<<(C.n + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, st>>>(C.n, C.dense_inv, C.b, C.x, done);
@@ -1064,7 +1083,8 @@ static void amg_vcycle(const AmgFine<TM> &f, const fea_amg_level *lv, int nl, in
     const fea_amg_level &L = lv[l];
     const double *bl = l == 0 ? b0 : L.b;
     const double *xc = (l + 1 == nl - 1) ? C.x : cur[l + 1];
-    k_amg_prolong<<<amg_grid(L.n, kBlock), kBlock, 0, st>>>(L.n, L.prow, L.pagg, L.pval, xc, cur[l], done);
+    k_amg_prolong<<<amg_grid(L.n, kBlock), kBlock, 0, st>>>(L.n, L.prow, L.pagg, (const half_t *)L.pval, xc,
+                                                             cur[l], done);
     cheb_coeffs(l == 0 ? f.rho : L.rho, deg, c1, c2);
     double *x = cur[l], *o = oth[l];
     for (int k = 0; k < deg; ++k) {
@@ -1242,7 +1262,7 @@ static int check_amg_levels(const fea_amg_level *lv, int nl, int deg, int n, con
     const fea_amg_level &L = lv[l];
     FEA_CHECK_ARG(L.x && L.x2 && L.d && L.r && (l == 0 || L.b), "%s: level %d work vectors missing", who, l);
     if (l + 1 < nl) {
-      FEA_CHECK_ARG(L.prow && L.pagg && L.pval && L.pt_ptr && L.pt_blk && L.pt_row && L.nagg * 6 == lv[l + 1].n,
+      FEA_CHECK_ARG(L.prow && L.pagg && L.pval && L.pt_ptr && L.ptval && L.pt_row && L.nagg * 6 == lv[l + 1].n,
                     "%s: level %d prolongator inconsistent with level %d", who, l, l + 1);
       FEA_CHECK_ARG(l == 0 || (L.rowptr && L.col && L.val && L.dinv && L.n % 6 == 0 && L.rho > 0.0),
                     "%s: level %d operator / smoother missing", who, l);

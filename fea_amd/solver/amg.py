@@ -30,7 +30,7 @@ class AmgLevel(ctypes.Structure):
     """Mirror of ``fea_amg_level`` (include/fea_amd.h)."""
     _fields_ = [('n', ctypes.c_int32), ('nagg', ctypes.c_int32), ('rho', ctypes.c_double),
                 ('rowptr', _P), ('col', _P), ('val', _P), ('dinv', _P), ('prow', _P), ('pagg', _P), ('pval', _P),
-                ('pt_ptr', _P), ('pt_blk', _P), ('pt_row', _P), ('dense_inv', _P),
+                ('pt_ptr', _P), ('pt_row', _P), ('ptval', _P), ('dense_inv', _P),
                 ('b', _P), ('x', _P), ('x2', _P), ('d', _P), ('r', _P)]
 
 
@@ -123,14 +123,20 @@ class AMG:
                                          _np_ptr(g('rowptr')), _np_ptr(g('col')), _np_ptr(g('val')),
                                          _np_ptr(g('dinv')), ctypes.byref(rho), _np_ptr(g('dense'))),
                   'fea_amg_level_export')
-            d = {k: torch.as_tensor(v.astype(np.float32) if k == 'val' else v, device=dev) for k, v in host.items()}
+            if not last:
+                # prolongator values in binary16, and a copy in transposed-entry order for restriction
+                p6 = host.pop('pval').reshape(-1, 6)
+                host['pval16'] = p6.astype(np.float16).view(np.uint16).reshape(-1)
+                host['ptval16'] = p6[host.pop('pt_blk')].astype(np.float16).view(np.uint16).reshape(-1)
+            d = {k: torch.as_tensor(v.astype(np.float32) if k == 'val' else v.view(np.int16)
+                                    if v.dtype == np.uint16 else v, device=dev) for k, v in host.items()}
             for k in ('b', 'x', 'x2', 'd', 'r'):
                 d[k] = torch.zeros(max(1, n), dtype=torch.float64, device=dev)
             lv = arr[l]
             lv.n, lv.nagg, lv.rho = n, nagg if not last else 0, float(rho.value)
             lv.rowptr, lv.col, lv.val, lv.dinv = (_ptr(d.get(k)) for k in ('rowptr', 'col', 'val', 'dinv'))
-            lv.prow, lv.pagg, lv.pval = _ptr(d.get('prow')), _ptr(d.get('pagg')), _ptr(d.get('pval'))
-            lv.pt_ptr, lv.pt_blk, lv.pt_row = _ptr(d.get('pt_ptr')), _ptr(d.get('pt_blk')), _ptr(d.get('pt_row'))
+            lv.prow, lv.pagg, lv.pval = _ptr(d.get('prow')), _ptr(d.get('pagg')), _ptr(d.get('pval16'))
+            lv.pt_ptr, lv.pt_row, lv.ptval = _ptr(d.get('pt_ptr')), _ptr(d.get('pt_row')), _ptr(d.get('ptval16'))
             lv.dense_inv = _ptr(d.get('dense'))
             lv.b, lv.x, lv.x2, lv.d, lv.r = (_ptr(d[k]) for k in ('b', 'x', 'x2', 'd', 'r'))
             self.levels.append({'n': n, 'nagg': nagg, 'nnz': nnz, 'rho': float(rho.value), 'blocks': nblk})

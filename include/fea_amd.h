@@ -279,10 +279,10 @@ typedef struct fea_amg_level {
   const float *dinv;          /* levels >= 1: 6x6 inverse of each group's diagonal block */
   const int32_t *prow;        /* prolongator to the next level: per row, blocks of 6 columns */
   const int32_t *pagg;        /* aggregate of each block */
-  const float *pval;          /* 6 values per block */
-  const int32_t *pt_ptr;      /* transpose: per aggregate, its (block, row) entries */
-  const int32_t *pt_blk;
+  const uint16_t *pval;       /* 6 values per block, IEEE binary16 */
+  const int32_t *pt_ptr;      /* transpose: per aggregate, its entries (row, 6 binary16 values) */
   const int32_t *pt_row;
+  const uint16_t *ptval;
   const float *dense_inv;     /* coarsest level: n x n inverse */
   double *b, *x, *x2, *d, *r; /* work vectors of length n (b unused on level 0) */
 } fea_amg_level;

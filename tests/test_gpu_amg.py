@@ -1,7 +1,8 @@
 """GPU parity of the smoothed-aggregation multigrid preconditioner.
 
-* The device V-cycle (float operators, float64 vectors) against the oracle's V-cycle
-  (oracle/amg.py, float64) on the same host-built hierarchy, with the device's lambda_max estimate, for float64 and
+* The device V-cycle (float operators, binary16 prolongators, float64 vectors) against the
+  oracle's V-cycle (oracle/amg.py, float64, the prolongators rounded as stored) on the same
+  host-built hierarchy, with the device's lambda_max estimate, for float64 and
   float32 matrices, on a brick beam and on a tet cube with 1-, 2- and 3-row node groups.
 * The device power iteration against a dense eigensolve of D^-1 A.
 * AMG-PCG: converged against a direct solve, its iteration count against the oracle's
@@ -45,6 +46,9 @@ def _oracle_levels(p, K0):
                           *A.dof_geometry(p.dof_map_np, p.X.cpu().numpy()), max_coarse=60)
     for l in range(1, len(lv) - 1):
         lv[l]['Dinv'] = sp.block_diag(list(lv[l]['dinv']), format='csr')
+    for L in lv[:-1]:                    # the device stores the prolongators in binary16
+        L['P'] = sp.csr_matrix((L['P'].data.astype(np.float16).astype(np.float64), L['P'].indices, L['P'].indptr),
+                               shape=L['P'].shape)
     return lv
 
 
