@@ -98,6 +98,21 @@ def event_time(torch, fn, reps, warm=3):
     return e0.elapsed_time(e1) / 1e3 / reps
 
 
+def bound_call(name, *args):
+    """A C-ABI call with its arguments converted once (device addresses, scalars), so a
+    timing loop measures the kernels and not Python's per-call argument conversion."""
+    from fea_amd import _native as nat
+    fn = getattr(nat.lib(), name)
+    conv = tuple(nat._arg(a) for a in args)
+
+    def call():
+        rc = fn(*conv)
+        if rc != 0:
+            nat.check(rc, name)
+    call.keep = args
+    return call
+
+
 def spmv_bytes(plan, val_bytes):
     """Compulsory bytes of one node-grouped SpMV: values, the column stream (triple-blocked:
     one int32 per neighbour triple of each group; otherwise one per entry of each node's
@@ -118,6 +133,7 @@ def spmv_bytes(plan, val_bytes):
 # whose workload or template differs from the measured one reports null.
 MEASURED_TRAFFIC = {
     ('beam 128x32x32 C3D8', 'k_pcg_spmv<float,double,16,2>'): 156.13e6 + 3.59e6,
+    ('beam 128x32x32 C3D8', 'k_amg_cheb0<float,16,2,0>'): 166.38e6 + 6.74e6,
 }
 
 
@@ -185,9 +201,30 @@ def spmv_roofline(fe, torch, reps, mat_dtype):
     q = torch.empty_like(x)
     ws = torch.zeros(int(nat.lib().fea_pcg_ws_size(p.n_free)), dtype=torch.float64, device='cuda')
     s = nat.stream_ptr()
-    t = event_time(torch, lambda: call('fea_pcg_spmv', mat_dtype, F64, p.n_free, p.nnz, p.spmv_groups, p.grp,
-                                       p.rowptr, p.spmv_col, val, x, q, ws, s), reps)
+    t = event_time(torch, bound_call('fea_pcg_spmv', mat_dtype, F64, p.n_free, p.nnz, p.spmv_groups, p.grp,
+                                     p.rowptr, p.spmv_col, val, x, q, ws, s), reps)
     return t, spmv_bytes(p, 4 if mat_dtype == F32 else 8)
+
+
+def smooth_roofline(fe, torch, reps, mat_dtype):
+    """The multigrid V-cycle's dominant kernel: one level-0 Chebyshev step (fea_amg_smooth0:
+    r = b - A x with the tangent, node-block Jacobi, d and x updates) timed with HIP events.
+    Algorithmic bytes: the SpMV's (values, column stream, x read, group heads) plus b and d
+    read, d and the new x written (float64) and the 6-float block inverse per group."""
+    from fea_amd import _native as nat
+    from fea_amd._native import F32, call
+    p = fe.assembly.plan
+    val = p.csr_values(mat_dtype)
+    n = p.n_free
+    binv = torch.zeros(6 * abs(p.spmv_groups), dtype=torch.float32, device='cuda')
+    nat.ops().block_jacobi_inv(mat_dtype, p.spmv_groups, p.grp, p.rowptr, p.col, p.diag_slot, val, binv)
+    b, x, d = (torch.rand(n, dtype=torch.float64, device='cuda') for _ in range(3))
+    xo = torch.empty_like(x)
+    s = nat.stream_ptr()
+    t = event_time(torch, bound_call('fea_amg_smooth0', mat_dtype, n, p.nnz, p.spmv_groups, p.grp, p.rowptr,
+                                     p.spmv_col, val, binv, b, x, xo, d, 0.5, 0.25, s), reps)
+    alg = spmv_bytes(p, 4 if mat_dtype == F32 else 8) + 24 * n + 24 * p.n_groups
+    return t, alg
 
 
 def element_rate(fe, torch, reps=10, kernel_dtype=None):
@@ -350,6 +387,20 @@ def bench_beam(args, torch):
     kname_t = 'float' if mat == F32 else 'double'
     t_spmv, alg_bytes = spmv_roofline(fe, torch, args.spmv_reps, mat)
     achieved = alg_bytes / t_spmv / 1e9
+    spmv_line = roofline('k_pcg_spmv<%s,double,%d,2> (fea_pcg_spmv)' % (kname_t, _triple_sw(p)),
+                         achieved, HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=alg_bytes,
+                         avg_launch_us=t_spmv * 1e6,
+                         traffic_key=('beam %dx%dx%d C3D8' % (args.nx, args.ny, args.nz),
+                                      'k_pcg_spmv<%s,double,%d,2>' % (kname_t, _triple_sw(p))))
+    if args.preconditioner == 'amg':
+        t_sm, alg_sm = smooth_roofline(fe, torch, args.spmv_reps, mat)
+        dominant = roofline('k_amg_cheb0<%s,%d,2,0> (fea_amg_smooth0: one level-0 Chebyshev step of the '
+                            'multigrid V-cycle)' % (kname_t, _triple_sw(p)), alg_sm / t_sm / 1e9, HBM_PEAK_GBS,
+                            'GB/s', 'hbm', alg_bytes_per_launch=alg_sm, avg_launch_us=t_sm * 1e6,
+                            traffic_key=('beam %dx%dx%d C3D8' % (args.nx, args.ny, args.nz),
+                                         'k_amg_cheb0<%s,%d,2,0>' % (kname_t, _triple_sw(p))))
+    else:
+        dominant = spmv_line
     elem_rate, t_tan = element_rate(fe, torch)
     elem_rate32, t_tan32 = element_rate(fe, torch, kernel_dtype=F32)
     serving = serving_sample(args, torch)
@@ -378,11 +429,8 @@ def bench_beam(args, torch):
                     'pattern); every timed solve smooths with the current tangent'},
         'elem_tangent_evals_per_s': elem_rate, 'tangent_assembly_ms': 1e3 * t_tan,
         'elem_tangent_evals_per_s_f32_contraction': elem_rate32,
-        'roofline': roofline('k_pcg_spmv<%s,double,%d,2> (fea_pcg_spmv)' % (kname_t, _triple_sw(p)),
-                             achieved, HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=alg_bytes,
-                             avg_launch_us=t_spmv * 1e6,
-                             traffic_key=('beam %dx%dx%d C3D8' % (args.nx, args.ny, args.nz),
-                                          'k_pcg_spmv<%s,double,%d,2>' % (kname_t, _triple_sw(p)))),
+        'roofline': dominant,
+        'roofline_pcg_spmv': spmv_line,
         'cpu_baseline': cpu,
         'cpu_cg_route_estimate': None if cg_route is None else {
             'value': cg_route['value'], 'unit': 'solves/s', 'sample': cg_route['sample']},
@@ -984,8 +1032,9 @@ def cpu_baseline_adjoint(mesh, p, E, frac=8):
 def bench_kernels(args, torch):
     """A short fixed sequence for hardware-counter runs: the beam's tangent
     assembly (FP64 element kernel + CSR gather to FP32), 20 launches of the PCG
-    SpMV stage on the FP32 tangent, and 20 plain device copies of 256 MiB
-    (the known-size calibration for FETCH_SIZE / WRITE_SIZE)."""
+    SpMV stage on the FP32 tangent, 20 level-0 Chebyshev steps of the multigrid
+    V-cycle, and 20 plain device copies of 256 MiB (the known-size calibration for
+    FETCH_SIZE / WRITE_SIZE)."""
     from fea_amd import workloads
     from fea_amd._native import F32, F64, call
     from fea_amd import _native as nat
@@ -1003,6 +1052,12 @@ def bench_kernels(args, torch):
     for _ in range(20):
         call('fea_pcg_spmv', F32, F64, p.n_free, p.nnz, p.spmv_groups, p.grp, p.rowptr, p.spmv_col, val, x, q, ws,
              s)
+    binv = torch.zeros(6 * abs(p.spmv_groups), dtype=torch.float32, device='cuda')
+    nat.ops().block_jacobi_inv(F32, p.spmv_groups, p.grp, p.rowptr, p.col, p.diag_slot, val, binv)
+    xb, xd = torch.rand_like(x), torch.rand_like(x)
+    for _ in range(20):
+        call('fea_amg_smooth0', F32, p.n_free, p.nnz, p.spmv_groups, p.grp, p.rowptr, p.spmv_col, val, binv, x, xb, q,
+             xd, 0.5, 0.25, s)
     a = torch.empty(256 << 20, dtype=torch.uint8, device='cuda').fill_(1)
     b = torch.empty_like(a)
     for _ in range(20):

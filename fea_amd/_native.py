@@ -62,6 +62,8 @@ _SIGS = {
     'fea_amg_level_size': (C_INT, []),
     'fea_amg_vcycle': (C_INT, [C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P,
                                       C_DBL, C_P, C_INT, C_INT, C_P, C_P, C_P]),
+    'fea_amg_smooth0': (C_INT, [C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_DBL, C_DBL,
+                                C_P]),
     'fea_amg_power': (C_INT, [C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P,
                                      C_P, C_P, C_INT, ctypes.POINTER(C_DBL), C_P]),
     'fea_pcg_start_amg': (C_INT, [C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P,

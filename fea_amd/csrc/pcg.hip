@@ -1297,6 +1297,24 @@ extern "C" int fea_amg_vcycle(int mat_dtype, int n, int64_t nnz, int n_groups, c
   return FEA_OK;
 }
 
+extern "C" int fea_amg_smooth0(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
+                               const int32_t *rowptr, const int32_t *col, const void *val, const float *binv,
+                               const double *b, const double *x, double *xo, double *d, double c1, double c2,
+                               void *stream) {
+  FEA_CHECK_ARG(mat_dtype == FEA_F64 || mat_dtype == FEA_F32, "fea_amg_smooth0: bad matrix dtype");
+  FEA_CHECK_ARG(n > 0 && grp && n_groups != 0 && rowptr && col && val && binv && b && xo && d && x != xo,
+                "fea_amg_smooth0: bad arguments");
+  const AmgFine<float> ff = amg_fine<float>(n, nnz, n_groups, grp, rowptr, col, val, binv, 1.0);
+  const AmgFine<double> fd = amg_fine<double>(n, nnz, n_groups, grp, rowptr, col, val, binv, 1.0);
+  hipStream_t st = as_stream(stream);
+  if (mat_dtype == FEA_F64)
+    cheb0(fd, x == nullptr, b, x, xo, d, c1, c2, nullptr, st);
+  else
+    cheb0(ff, x == nullptr, b, x, xo, d, c1, c2, nullptr, st);
+  FEA_CHECK_LAUNCH("fea_amg_smooth0");
+  return FEA_OK;
+}
+
 extern "C" int fea_amg_power(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
                              const int32_t *rowptr, const int32_t *col, const void *val, const float *binv, float *x,
                              float *y, double *partial, int iters, double *rho_out, void *stream) {

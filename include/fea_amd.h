@@ -294,6 +294,12 @@ int fea_amg_vcycle(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_
                    const int32_t *col, const void *val, const float *binv, double rho0,
                    const fea_amg_level *levels, int n_levels, int degree, const double *b, double *z,
                    void *stream);
+/* One Chebyshev smoothing step of the V-cycle's level 0: d = c1 d + c2 M^-1 (b - A x),
+ * xo = x + d with M the node-block Jacobi inverses binv; x == NULL is the first step from
+ * x = 0 (no product, c1 ignored).  xo must not alias x. */
+int fea_amg_smooth0(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp, const int32_t *rowptr,
+                    const int32_t *col, const void *val, const float *binv, const double *b, const double *x,
+                    double *xo, double *d, double c1, double c2, void *stream);
 /* lambda_max(D^-1 A) of the level-0 matrix by `iters` power steps from x (unit 2-norm;
  * x, y float work vectors, partial: float64 scratch of the SpMV grid); synchronises. */
 int fea_amg_power(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp, const int32_t *rowptr,

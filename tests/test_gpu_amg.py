@@ -141,3 +141,37 @@ def test_static_amg_matches_oracle(gpu, precision):
     assert fe2.solve() is not False
     print('static %s: CG iterations amg %d, block Jacobi %d' % (precision, its_amg, fe2.solver.stats['cg_iters']))
     assert its_amg * 4 < fe2.solver.stats['cg_iters']
+
+
+@pytest.mark.parametrize('mat', [F64, F32])
+def test_smoothing_step_matches_numpy(gpu, mat):
+    """fea_amg_smooth0 (one level-0 Chebyshev step, the V-cycle's dominant kernel) against
+    d' = c1 d + c2 M^-1 (b - A x), x' = x + d' with M the stored block-Jacobi inverses."""
+    from fea_amd import _native as nat
+    from fea_amd._native import call
+    fe, p, amg, K0 = _model('beam')
+    _, K = p.tangent(_state(p, 0.02), 1.0, F64, val_dtype=mat)
+    binv = _binv(p, K, mat)
+    rng = np.random.default_rng(9)
+    n = p.n_free
+    b, x, d = (torch.as_tensor(rng.standard_normal(n), device='cuda') for _ in range(3))
+    xo = torch.zeros_like(x)
+    dd = d.clone()
+    call('fea_amg_smooth0', mat, n, p.nnz, p.spmv_groups, p.grp, p.rowptr, p.spmv_col, K, binv, b, x, xo, dd, 0.7,
+         0.3, nat.stream_ptr())
+    Kh = sp.csr_matrix((K.double().cpu().numpy(), p.col_np, p.rowptr_np), shape=(n, n))
+    grp = p.grp.cpu().numpy()
+    B = binv.cpu().numpy().astype(np.float64).reshape(-1, 6)
+    iu = ([0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2])
+    Ms = np.zeros((len(grp) - 1, 3, 3))
+    Ms[:, iu[0], iu[1]] = B
+    Ms[:, iu[1], iu[0]] = B
+    M = sp.block_diag([Ms[g, :grp[g + 1] - grp[g], :grp[g + 1] - grp[g]] for g in range(len(grp) - 1)], format='csr')
+    bh, xh, dh = b.cpu().numpy(), x.cpu().numpy(), d.cpu().numpy()
+    d_ref = 0.7 * dh + 0.3 * (M @ (bh - Kh @ xh))
+    assert rel_err(dd.cpu().numpy(), d_ref) < 1e-12
+    assert rel_err(xo.cpu().numpy(), xh + d_ref) < 1e-12
+    # the first step from x = 0: d = xo = c2 M^-1 b
+    call('fea_amg_smooth0', mat, n, p.nnz, p.spmv_groups, p.grp, p.rowptr, p.spmv_col, K, binv, b, None, xo, dd, 0.0,
+         0.3, nat.stream_ptr())
+    assert rel_err(xo.cpu().numpy(), 0.3 * (M @ bh)) < 1e-12
