@@ -208,7 +208,9 @@ def spmv_roofline(fe, torch, reps, mat_dtype):
 
 def smooth_roofline(fe, torch, reps, mat_dtype):
     """The multigrid V-cycle's dominant kernel: one level-0 Chebyshev step (fea_amg_smooth0:
-    r = b - A x with the tangent, node-block Jacobi, d and x updates) timed with HIP events.
+    r = b - A x with the tangent, node-block Jacobi, d and x updates) timed with HIP events
+    over back-to-back launches (rocprof gives these dispatches the same duration; inside the
+    solve, between the V-cycle's other kernels, the same kernel runs about 15 % faster).
     Algorithmic bytes: the SpMV's (values, column stream, x read, group heads) plus b and d
     read, d and the new x written (float64) and the 6-float block inverse per group."""
     from fea_amd import _native as nat
