@@ -470,17 +470,26 @@ def _serving_run(torch, reqs):
     srv, wall = serve(reqs, True)
     done = [r for r in reqs if r.status == 'done']
     lat = [r.latency for r in done]
-    secs, errs = [], []
+    secs, secs_amg, errs = [], [], []
     for r in sorted(reqs, key=lambda r: r.arrival):
         ref, sec, _ = serving.solve_one_at_a_time([r], precision='fp64')
         secs.append(sec[0])
         if r.status == 'done' and ref[r.rid] is not None:
             errs.append(float((r.GC - ref[r.rid]).abs().max() / ref[r.rid].abs().max()))
-        log('one-at-a-time fp64 request %d (%d elements): %.2f s' % (r.rid, r.n_elements(), sec[0]))
+        ref_a, sec_a, _ = serving.solve_one_at_a_time([r], precision='fp64', preconditioner='amg')
+        secs_amg.append(sec_a[0])
+        if r.status == 'done' and ref_a[r.rid] is not None:
+            errs.append(float((r.GC - ref_a[r.rid]).abs().max() / ref_a[r.rid].abs().max()))
+        log('one-at-a-time fp64 request %d (%d elements): %.2f s point Jacobi, %.2f s multigrid'
+            % (r.rid, r.n_elements(), sec[0], sec_a[0]))
     end, lat1 = 0.0, []
     for r, t in zip(sorted(reqs, key=lambda r: r.arrival), secs):
         end = max(end, r.arrival) + t
         lat1.append(end - r.arrival)
+    end_a, lat1a = 0.0, []
+    for r, t in zip(sorted(reqs, key=lambda r: r.arrival), secs_amg):
+        end_a = max(end_a, r.arrival) + t
+        lat1a.append(end_a - r.arrival)
     n_sat = sum(1 for r in sat if r.status == 'done')
     return {
         'requests': len(reqs), 'completed': len(done), 'qps_offered': None,
@@ -488,6 +497,8 @@ def _serving_run(torch, reqs):
         'saturated_solves_per_s': n_sat / wall_sat,
         'one_at_a_time_fp64_solves_per_s': len(reqs) / sum(secs),
         'saturated_throughput_ratio_vs_fp64_one_at_a_time': (n_sat / wall_sat) / (len(reqs) / sum(secs)),
+        'one_at_a_time_fp64_amg_solves_per_s': len(reqs) / sum(secs_amg),
+        'saturated_throughput_ratio_vs_fp64_amg_one_at_a_time': (n_sat / wall_sat) / (len(reqs) / sum(secs_amg)),
         'saturated_max_in_flight': int(max(srv_sat.batch_sizes)) if srv_sat.batch_sizes else 0,
         'poisson_solves_per_s': len(done) / wall, 'poisson_wall_s': wall,
         'latency_p50_s': float(np.percentile(lat, 50)) if lat else None,
@@ -495,6 +506,8 @@ def _serving_run(torch, reqs):
         'one_at_a_time_fp64_latency_p50_s': float(np.percentile(lat1, 50)),
         'one_at_a_time_fp64_latency_p99_s': float(np.percentile(lat1, 99)),
         'one_at_a_time_fp64_makespan_s': end,
+        'one_at_a_time_fp64_amg_latency_p50_s': float(np.percentile(lat1a, 50)),
+        'one_at_a_time_fp64_amg_latency_p99_s': float(np.percentile(lat1a, 99)),
         'max_in_flight': int(max(srv.batch_sizes)) if srv.batch_sizes else 0, 'ticks': srv.rounds,
         'peak_admitted_gb': srv.peak_used / 1e9,
         'max_rel_err_vs_fp64': max(errs) if errs else None,
@@ -577,10 +590,9 @@ def cpu_baseline_requests(reqs):
         mesh = serving.request_mesh(r.spec)
         m = r.spec['material']
         mat = nh(m[1], m[2]) if m[0] == 'NH' else mr(m[1], m[2], m[3])
-        face = serving._OPPOSITE[r.spec['fixed']]
         kind, val = r.spec['load']
         om = oracle_model(mesh, mat, r.spec['fixed'], pressure=val if kind == 'pressure' else None,
-                          traction=val if kind == 'traction' else None, load_set=face)
+                          traction=val if kind == 'traction' else None, load_set=serving.LOADED_SET)
         osol.static_solve(om, increments=r.spec['load_steps'], rel_tol=1e-10, linear='direct',
                           armijo_slack_rel=1e-12)
     t = time.perf_counter() - t0

@@ -16,7 +16,7 @@
 //   * the coarse operator is the Galerkin product P^T A P, in 6x6 blocks;
 //   * levels are added until at most `max_coarse` DOFs remain; the coarsest operator is
 //     inverted densely (Cholesky).
-// Everything here is host code (std::thread); the GPU applies the hierarchy (amg.hip).
+// Everything here is host code (std::thread); the GPU applies the hierarchy (pcg.hip).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -547,38 +547,92 @@ bool coarsen(Hierarchy &H, int l, Level &next) {
   return true;
 }
 
-// dense Cholesky inverse of the coarsest operator (symmetric positive definite)
-bool dense_inverse(const Csr &A, std::vector<double> &inv) {
+// rows 0..n-1 dealt round-robin to the threads (balances triangular loops)
+template <class F>
+void parallel_rows(int n, int threads, F f) {
+  if (threads <= 1 || n < 64) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> ts;
+  for (int t = 0; t < threads && t < n; ++t)
+    ts.emplace_back([&, t] {
+      for (int i = t; i < n; i += threads) f(i);
+    });
+  for (auto &th : ts) th.join();
+}
+
+// dense inverse of the coarsest operator (symmetric positive definite): a
+// right-looking Cholesky A = L L^T in column panels of 64 (row-major lower
+// triangle, every inner loop runs along a row), then W^T with W = L^-1 one
+// column of W per task, then A^-1 = W^T W as dot products of rows of W^T.
+bool dense_inverse(const Csr &A, std::vector<double> &inv, int threads) {
   const int n = A.n;
-  std::vector<double> M((size_t)n * n, 0.0);
+  const size_t N = (size_t)n;
+  std::vector<double> M(N * N, 0.0);
   for (int i = 0; i < n; ++i)
-    for (int k = A.rowptr[i]; k < A.rowptr[i + 1]; ++k) M[(size_t)i * n + A.col[k]] = A.val[k];
-  for (int j = 0; j < n; ++j) {  // in-place lower Cholesky
-    double d = M[(size_t)j * n + j];
-    for (int k = 0; k < j; ++k) d -= M[(size_t)j * n + k] * M[(size_t)j * n + k];
-    if (!(d > 0.0)) return false;
-    d = std::sqrt(d);
-    M[(size_t)j * n + j] = d;
-    for (int i = j + 1; i < n; ++i) {
-      double s = M[(size_t)i * n + j];
-      for (int k = 0; k < j; ++k) s -= M[(size_t)i * n + k] * M[(size_t)j * n + k];
-      M[(size_t)i * n + j] = s / d;
+    for (int k = A.rowptr[i]; k < A.rowptr[i + 1]; ++k) M[i * N + A.col[k]] = A.val[k];
+  constexpr int PB = 64;
+  for (int J = 0; J < n; J += PB) {
+    const int je = std::min(n, J + PB);
+    for (int j = J; j < je; ++j) {  // the diagonal block
+      double *rj = &M[j * N];
+      double d = rj[j];
+      for (int k = J; k < j; ++k) d -= rj[k] * rj[k];
+      if (!(d > 0.0)) return false;
+      rj[j] = std::sqrt(d);
+      for (int i = j + 1; i < je; ++i) {
+        double *ri = &M[i * N];
+        double t = ri[j];
+        for (int k = J; k < j; ++k) t -= ri[k] * rj[k];
+        ri[j] = t / rj[j];
+      }
     }
+    const int rest = n - je;
+    parallel_rows(rest, threads, [&](int r) {  // the panel below it, then the trailing update of row i
+      const int i = je + r;
+      double *ri = &M[i * N];
+      for (int j = J; j < je; ++j) {
+        const double *rj = &M[j * N];
+        double t = ri[j];
+        for (int k = J; k < j; ++k) t -= ri[k] * rj[k];
+        ri[j] = t / rj[j];
+      }
+    });
+    parallel_rows(rest, threads, [&](int r) {
+      const int i = je + r;
+      double *ri = &M[i * N];
+      for (int j = je; j <= i; ++j) {
+        const double *rj = &M[j * N];
+        double t = 0.0;
+        for (int k = J; k < je; ++k) t += ri[k] * rj[k];
+        ri[j] -= t;
+      }
+    });
   }
-  inv.assign((size_t)n * n, 0.0);
-  std::vector<double> y(n);
-  for (int c = 0; c < n; ++c) {  // solve L L^T x = e_c
-    for (int i = 0; i < n; ++i) {
-      double s = (i == c) ? 1.0 : 0.0;
-      for (int k = 0; k < i; ++k) s -= M[(size_t)i * n + k] * y[k];
-      y[i] = s / M[(size_t)i * n + i];
+  // W^T: row c holds column c of W = L^-1 (entries c..n-1), from L y = e_c
+  std::vector<double> Wt(N * N, 0.0);
+  parallel_rows(n, threads, [&](int c) {
+    double *y = &Wt[c * N];
+    for (int i = c; i < n; ++i) {
+      const double *ri = &M[i * N];
+      double t = (i == c) ? 1.0 : 0.0;
+      for (int k = c; k < i; ++k) t -= ri[k] * y[k];
+      y[i] = t / ri[i];
     }
-    for (int i = n - 1; i >= 0; --i) {
-      double s = y[i];
-      for (int k = i + 1; k < n; ++k) s -= M[(size_t)k * n + i] * inv[(size_t)k * n + c];
-      inv[(size_t)i * n + c] = s / M[(size_t)i * n + i];
+  });
+  inv.assign(N * N, 0.0);
+  parallel_rows(n, threads, [&](int a) {  // A^-1[a][b] = sum_{i >= max(a,b)} W[i][a] W[i][b]
+    const double *wa = &Wt[a * N];
+    for (int b = a; b < n; ++b) {
+      const double *wb = &Wt[b * N];
+      double t = 0.0;
+      for (int i = b; i < n; ++i) t += wa[i] * wb[i];
+      inv[a * N + b] = t;
     }
-  }
+  });
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < a; ++b) inv[a * N + b] = inv[b * N + a];
   return true;
 }
 
@@ -652,7 +706,7 @@ extern "C" int fea_amg_build(int n, const int32_t *rowptr, const int32_t *col, c
     fea::set_error("fea_amg_build: coarsest level has %d DOFs (more than 8192 for a dense inverse)", Lc.A.n);
     return FEA_ERR_STATE;
   }
-  if (!dense_inverse(Lc.A, H.coarse_inv)) {
+  if (!dense_inverse(Lc.A, H.coarse_inv, H.threads)) {
     fea::set_error("fea_amg_build: the coarsest operator is not positive definite");
     return FEA_ERR_STATE;
   }

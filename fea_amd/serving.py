@@ -1,7 +1,7 @@
 """Continuous-batching static solve server.
 
-Requests are independent static Newton solves (a mesh, a material, one loaded
-face and its opposite face fixed), each a full static_implicit.py:44-295 solve
+Requests are independent static Newton solves (a mesh, a material, one face
+clamped and a traction or pressure load), each a full static_implicit.py:44-295 solve
 of the reference.  The server keeps every admitted request's state on the
 device (its DevicePlan: CSR, staging, displacement, Newton direction, PCG
 vectors) and schedules at the granularity of one PCG chunk.  Each admitted
@@ -53,6 +53,7 @@ from .assemble import loads, materials
 from .solver._linear_solver import BatchedPCG, PcgVectors
 
 _OPPOSITE = {'x0': 'x1', 'x1': 'x0', 'y0': 'y1', 'y1': 'y0', 'z0': 'z1', 'z1': 'z0'}
+LOADED_SET = 'loaded'
 
 
 # ---------------------------------------------------------------------- requests
@@ -63,8 +64,8 @@ class SolveRequest:
     spec keys: ``etype`` ('C3D4' Kuhn-split tets or 'C3D8' bricks), ``cells``
     (nx, ny, nz), ``size`` (Lx, Ly, Lz), ``material`` ('NH', mu, kappa) or
     ('MR', C10, C01, D1), ``fixed`` face name ('x0' .. 'z1'), ``load``
-    ('pressure', p) or ('traction', (tx, ty, tz)) on the opposite face,
-    ``load_steps`` equal load increments.
+    ('pressure', p) on the five other faces or ('traction', (tx, ty, tz)) on
+    the opposite face, ``load_steps`` equal load increments.
     """
     rid: int
     spec: dict
@@ -90,27 +91,33 @@ class SolveRequest:
 
 
 def request_mesh(spec):
-    """The request's block.  For a pressure load the coordinate origin is put one
-    unit inside the block behind the centre of the loaded face (the face then
-    lies in the plane x.n = 1): the follower-pressure functional of an open
-    surface, V = 1/3 sum w det[x, x_xi, x_eta] (pressure.py:134-159), depends on
-    the origin, and its stiffness carries terms proportional to the in-plane
-    distance of the face points from it -- large enough on a big face with the
-    origin at a corner to make a mild compression unstable (the FP64 solver
-    diverges there).  A face plane through the origin is avoided because the
-    reference's form of the derivative (det A A^-T) is undefined there."""
+    """The request's block, centred on the coordinate origin, with the face set
+    ``LOADED_SET`` the request's load acts on.
+
+    A traction acts on the face opposite the clamped one.  A pressure acts on
+    the five faces other than the clamped one: the follower-pressure functional
+    V = 1/3 sum w det[x, x_xi, x_eta] (pressure.py:134-159) of an open surface
+    adds line loads on the surface's edge (an in-plane p d / 3 per unit length,
+    d the distance of the face plane from the origin, and a normal one that
+    cancels 2/3 of the net force), which sharpen under refinement.  Closed by the
+    clamped face, whose perimeter does not move, the surface's edge terms do no
+    work and the load is the exact follower pressure.  With the origin inside the
+    block no face plane passes through it, where the reference's form of the
+    derivative (det A A^-T) is undefined."""
     nx, ny, nz = spec['cells']
+    L = np.asarray(spec['size'], dtype=np.float64)
     if spec['etype'] == 'C3D4':
         mesh = meshgen.kuhn_tet_block(nx, ny, nz, size=tuple(spec['size']))
     elif spec['etype'] == 'C3D8':
         mesh = meshgen.hex_block(nx, ny, nz, size=tuple(spec['size']))
     else:
         raise ValueError('unsupported element type %r' % (spec['etype'],))
+    mesh['nodes'] = mesh['nodes'] - 0.5 * L
     if spec['load'][0] == 'pressure':
-        face = _OPPOSITE[spec['fixed']]
-        n_out = np.zeros(3)
-        n_out['xyz'.index(face[0])] = 1.0 if face[1] == '1' else -1.0
-        mesh['nodes'] = mesh['nodes'] - (mesh['nodes'][mesh['node_sets'][face]].mean(axis=0) - n_out)
+        faces = [f for f in _OPPOSITE if f != spec['fixed']]
+    else:
+        faces = [_OPPOSITE[spec['fixed']]]
+    mesh['face_sets'][LOADED_SET] = [entry for f in faces for entry in mesh['face_sets'][f]]
     return mesh
 
 
@@ -125,12 +132,11 @@ def request_model(spec):
         mat = materials.MooneyRivlin(m[1], m[2], m[3])
     else:
         raise ValueError('unsupported material %r' % (m[0],))
-    face = _OPPOSITE[spec['fixed']]
     kind, value = spec['load']
     if kind == 'pressure':
-        load = lambda nm: loads.Pressure(nm, face, float(value))
+        load = lambda nm: loads.Pressure(nm, LOADED_SET, float(value))
     elif kind == 'traction':
-        load = lambda nm: loads.Traction(nm, face, tuple(float(v) for v in value))
+        load = lambda nm: loads.Traction(nm, LOADED_SET, tuple(float(v) for v in value))
     else:
         raise ValueError('unsupported load %r' % (kind,))
     return _controller_from_mesh(mesh, mat, fixed_set=spec['fixed'], load=load), mesh
@@ -139,18 +145,20 @@ def request_model(spec):
 def estimate_footprint(spec, precision='mixed'):
     """Device bytes of a request before it is built (mesh sizes only).
 
-    Counts the staging of every element's node-pair blocks (the largest
-    term), the CSR (neighbour count per node: 27 for bricks, 15 for Kuhn tets),
-    the per-element maps and the nodal / free-DOF vectors."""
+    Counts the staging of every element's and loaded facet's node-pair blocks
+    (the largest term), the CSR (neighbour count per node: 27 for bricks, 15
+    for Kuhn tets), the per-element maps and the nodal / free-DOF vectors."""
     nx, ny, nz = spec['cells']
     N = (nx + 1) * (ny + 1) * (nz + 1)
+    cells = {'x': ny * nz, 'y': nx * nz, 'z': nx * ny}
+    faces = [f for f in _OPPOSITE if f != spec['fixed']] if spec['load'][0] == 'pressure' else [_OPPOSITE[spec['fixed']]]
     if spec['etype'] == 'C3D8':
-        E, nen, nb = nx * ny * nz, 8, 27
+        E, nen, nb, S, snen = nx * ny * nz, 8, 27, sum(cells[f[0]] for f in faces), 4
     else:
-        E, nen, nb = 6 * nx * ny * nz, 4, 15
+        E, nen, nb, S, snen = 6 * nx * ny * nz, 4, 15, 2 * sum(cells[f[0]] for f in faces), 3
     nnz = 9 * nb * N
-    kstage = E * nen * nen * 9 * 8
-    maps = E * nen * nen * 4 + 2 * E * nen * 4
+    kstage = E * nen * nen * 9 * 8 + S * snen * snen * 9 * 8
+    maps = E * nen * nen * 4 + 2 * E * nen * 4 + S * snen * snen * 4 + 2 * S * snen * 4
     csr = nnz * (4 + (8 if precision == 'fp64' else 4)) + 4 * N * 3 + 3 * 4 * nb * N
     vectors = 3 * N * (8 * 12 + 4)
     return int(kstage + maps + csr + vectors + 64 * 1024)
@@ -163,12 +171,10 @@ def random_requests(n, qps, seed=0, e_range=(2e3, 2e5), aspect_range=(1.0, 4.0),
     ``e_range``, side ratios Lx/Lz and Ly/Lz uniform in ``aspect_range``;
     NeoHookean (mu, kappa) or Mooney-Rivlin (C10, C01, D1) with parameters
     uniform within +-50% of mu = 1, kappa = 100 (C10 = mu/2 split 5:1 with
-    C01, D1 = 2/kappa); a random face fixed and the opposite face loaded by a
-    compressive follower pressure 0.05 mu (+-50%; the open-surface pressure
-    functional of pressure.py:134-159 gives a net normal force p A / 3, below
-    the Euler load of the most slender block) or a transverse dead traction
-    sized for a tip deflection of about 5% of the length, at most 0.03 mu
-    (+-50%).
+    C01, D1 = 2/kappa); a random face clamped and either a compressive
+    follower pressure 0.01 kappa (+-50%, a volume change of about 1%) on the
+    five other faces or, on the opposite face, a transverse dead traction sized
+    for a tip deflection of about 5% of the length, at most 0.03 mu (+-50%).
     """
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / qps, size=n)) if qps > 0 else np.zeros(n)
@@ -191,7 +197,7 @@ def random_requests(n, qps, seed=0, e_range=(2e3, 2e5), aspect_range=(1.0, 4.0),
         fixed = ['x0', 'x1', 'y0', 'y1', 'z0', 'z1'][int(rng.integers(6))]
         ax = 'xyz'.index(fixed[0])
         if rng.random() < 0.5:
-            load = ('pressure', 0.05 * mu * rng.uniform(0.5, 1.5))
+            load = ('pressure', 0.01 * kappa * rng.uniform(0.5, 1.5))
         else:
             others = [a for a in range(3) if a != ax]
             d_ax = min(others, key=lambda a: L[a])          # bend about the thinner side
@@ -552,10 +558,12 @@ class SolveServer:
 
 
 def solve_one_at_a_time(requests, precision='fp64', rel_tol=1e-10, step_tol=1e-12, cg_tol=1e-8,
-                        cg_max_iter=20000, max_newton=200, armijo_slack=1e-12, predictor=True):
+                        cg_max_iter=20000, max_newton=200, armijo_slack=1e-12, predictor=True,
+                        preconditioner='jacobi'):
     """The plain path: each request built and solved alone by ``StaticImplicitSolver``,
-    in arrival order.  Returns (GC list, per-request seconds including the model
-    build, FIFO latencies under the requests' arrival times)."""
+    in arrival order (``preconditioner`` as there: 'jacobi' is the reference's PCG).
+    Returns (GC list, per-request seconds including the model build and, for 'amg',
+    the multigrid set-up, FIFO latencies under the requests' arrival times)."""
     from .solver import StaticImplicitSolver
     out, secs = [], []
     for r in sorted(requests, key=lambda r: (r.arrival, r.rid)):
@@ -564,7 +572,7 @@ def solve_one_at_a_time(requests, precision='fp64', rel_tol=1e-10, step_tol=1e-1
         fe.solver = StaticImplicitSolver(maximum_iteration=max_newton, precision=precision, rel_tol=rel_tol,
                                          step_tol=step_tol, load_steps=int(r.spec.get('load_steps', 1)),
                                          cg_tol=cg_tol, cg_max_iter=cg_max_iter, armijo_slack=armijo_slack,
-                                         predictor=predictor)
+                                         predictor=predictor, preconditioner=preconditioner)
         fe.initialize()
         GC = fe.solver.solve(GC0=torch.zeros_like(fe.assembly.GC))
         torch.cuda.synchronize()
