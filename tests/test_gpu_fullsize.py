@@ -14,12 +14,17 @@
   element VJP at the converged state against the torch-autograd oracle.
 * Two 10^3-cell Kuhn-tet blocks (6 000 C3D4 each) in penalty contact: 60 explicit
   central-difference steps at 0.9 of the stability limit against the oracle's loop.
+* The largest pressure request of the 16-request serving stream (89 488 C3D8,
+  NeoHookean, clamped on y0, follower pressure on the five other faces): the solve server
+  (mixed) and the one-at-a-time multigrid FP64 solve converge to within 1e-7 of each
+  other, and the oracle's FP64 residual at the server's displacement is <= 1e-9 of the
+  pressure load (2-norms).
 """
 import numpy as np
 import pytest
 import torch
 
-from fea_amd import adjoint, solver, workloads
+from fea_amd import adjoint, serving, solver, workloads
 from oracle import assembly as oasm
 from oracle import autodiff as ad
 from oracle import contact as oc
@@ -144,3 +149,30 @@ def test_explicit_contact_ten_cubed_blocks(gpu, precision, tol):
 
 
 _ORACLE_EXPLICIT = {}
+
+
+def test_serving_pressure_request_full_size_oracle_residual(gpu):
+    req = serving.random_requests(16, 2.0, seed=0, e_range=(2e3, 2e5))[8]
+    spec = req.spec
+    assert spec['etype'] == 'C3D8' and spec['cells'] == (68, 47, 28) and spec['load'][0] == 'pressure'
+    serving.SolveServer(precision='mixed').run([req], realtime=False)
+    assert req.status == 'done', req.error
+    ref, _, _ = serving.solve_one_at_a_time([req], precision='fp64', preconditioner='amg')
+    assert ref[req.rid] is not None
+    torch.cuda.empty_cache()
+    mesh = serving.request_mesh(spec)
+    _, mu, kappa = spec['material']
+    p = spec['load'][1]
+    om = oracle_model(mesh, nh(mu, kappa), spec['fixed'], pressure=p, load_set=serving.LOADED_SET)
+    free = oasm.free_mask(om)
+    conn, stype = facets(mesh, serving.LOADED_SET)
+    u = oasm.gc_to_u(om, req.GC.numpy())
+    F, _ = osurf.pressure_force_stiffness(stype, mesh['nodes'] + u, conn, p, onlyforce=True)
+    f = np.zeros(mesh['nodes'].size)
+    np.add.at(f, osurf.pressure_force_idx(conn), F)
+    ratio = np.linalg.norm(oasm.assemble_force(om, u)) / np.linalg.norm(f[free])
+    err = rel_err(req.GC.numpy(), ref[req.rid].numpy())
+    print('serving request 8 (%d C3D8): server |R_oracle(u)| / |f_p| = %.2e in %d Newton iterations; '
+          'rel diff vs the one-at-a-time FP64 multigrid solve %.2e' % (req.n_elements(), ratio, req.newton_iters, err))
+    assert ratio <= 1e-9
+    assert err < 1e-7
