@@ -69,6 +69,7 @@ def parse():
     ap.add_argument('--serving-requests', type=int, default=None)
     ap.add_argument('--serving-qps', type=float, default=2.0)
     ap.add_argument('--serving-emax', type=float, default=None)
+    ap.add_argument('--serving-emin', type=float, default=2e3)
     ap.add_argument('--cells', type=int, default=40, help='explicit: cells per block edge')
     ap.add_argument('--fd-directions', type=int, default=64, help='adjoint: central-difference check directions')
     ap.add_argument('--couple-pressure', type=float, default=0.06, help='couple: pressure on surface_1_All')
@@ -526,11 +527,11 @@ def serving_sample(args, torch):
     if n <= 0:
         return None
     emax = 3e4 if args.serving_emax is None else args.serving_emax
-    reqs = serving.random_requests(n, args.serving_qps, seed=1, e_range=(2e3, emax))
+    reqs = serving.random_requests(n, args.serving_qps, seed=1, e_range=(args.serving_emin, emax))
     out, _ = _serving_run(torch, reqs)
     out['qps_offered'] = args.serving_qps
-    out['sample'] = '%d requests, elements log-uniform in [2e3, %.0e], Poisson arrivals at %.1f/s' % (
-        n, emax, args.serving_qps)
+    out['sample'] = '%d requests, elements log-uniform in [%.0e, %.0e], Poisson arrivals at %.1f/s' % (
+        n, args.serving_emin, emax, args.serving_qps)
     return out
 
 
@@ -541,7 +542,7 @@ def bench_serving(args, torch):
     from fea_amd.solver._linear_solver import BatchedPCG, PcgVectors
     n = 24 if args.serving_requests is None else args.serving_requests
     emax = 2e5 if args.serving_emax is None else args.serving_emax
-    reqs = serving.random_requests(n, args.serving_qps, seed=0, e_range=(2e3, emax))
+    reqs = serving.random_requests(n, args.serving_qps, seed=0, e_range=(args.serving_emin, emax))
     out, srv = _serving_run(torch, reqs)
     out['qps_offered'] = args.serving_qps
     # roofline: the batched SpMV stage over the stream's systems at once (tangents at u = 0)
@@ -569,9 +570,9 @@ def bench_serving(args, torch):
     line = base_line(args, 'static solves/s through the continuous-batching server (mixed FP32/FP64), saturated',
                      out['saturated_solves_per_s'], 'solves/s', n, 1, n / out['saturated_solves_per_s'], 'f32+f64',
                      {'workload': 'Poisson stream of %d static solve requests at %.1f/s: C3D4/C3D8 blocks, '
-                                  'elements log-uniform in [2e3, %.0e], aspect ratios U[1,4], NH/MR +-50%%, '
-                                  'pressure or traction on one face, opposite face fixed, 2 increments'
-                                  % (n, args.serving_qps, emax)})
+                                  'elements log-uniform in [%.0e, %.0e], aspect ratios U[1,4], NH/MR +-50%%, '
+                                  'one face clamped, follower pressure on the five others or a traction on the '
+                                  'opposite face, 2 increments' % (n, args.serving_qps, args.serving_emin, emax)})
     line.update(out)
     line['roofline'] = roofline('k_batch_spmv<float,double> (fea_pcg_batch_spmv), all %d systems in one launch' % nsys,
                                 total_bytes / t / 1e9, HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=total_bytes,

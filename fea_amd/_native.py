@@ -161,11 +161,15 @@ def check(rc, what=''):
 
 
 def _arg(a):
+    # pointer parameters are declared c_void_p, which takes the address as an int
     if isinstance(a, torch.Tensor):
-        return ctypes.c_void_p(a.data_ptr())
+        return a.data_ptr()
     if isinstance(a, np.ndarray):
-        return ctypes.c_void_p(a.ctypes.data)
+        return a.ctypes.data
     return a
+
+
+_FN = {}
 
 
 def call(name, *args):
@@ -175,8 +179,14 @@ def call(name, *args):
     them referenced until the call returns (and the kernels are enqueued), so a
     temporary cannot be freed and its memory reused before the launch.
     """
-    rc = getattr(lib(), name)(*[_arg(a) for a in args])
-    check(rc, name)
+    fn = _FN.get(name)
+    if fn is None:
+        if name not in _SIGS:        # an int address needs a declared pointer parameter
+            raise FeaNativeError('%s has no declared signature' % name)
+        fn = _FN[name] = getattr(lib(), name)
+    rc = fn(*[_arg(a) for a in args])
+    if rc != 0:
+        check(rc, name)
 
 
 def ptr(t):
@@ -186,7 +196,14 @@ def ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+_raw_stream = torch._C._cuda_getCurrentRawStream
+_cur_device = torch._C._cuda_getDevice
+
+
 def stream_ptr(device=None):
+    """The current torch stream of ``device`` (default: the current device) as a handle."""
+    if device is None:
+        return ctypes.c_void_p(_raw_stream(_cur_device()))
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 

@@ -83,15 +83,19 @@ int build(fea_pattern &P, int n_nodes, const int32_t *dof_map, const std::vector
   std::vector<int32_t> adj;
   adj.reserve((size_t)inc_ptr[n_nodes] * 4);
   std::vector<int32_t> tmp;
+  std::vector<int32_t> stamp(n_nodes, -1);  // stamp[B] == A: B already listed for A
   for (int A = 0; A < n_nodes; ++A) {
     tmp.clear();
     for (int64_t s = inc_ptr[A]; s < inc_ptr[A + 1]; ++s) {
       const auto &g = G[inc[s].first];
       const int32_t *c = g.conn + (int64_t)inc[s].second * g.nen;
-      for (int b = 0; b < g.nen; ++b) tmp.push_back(c[b]);
+      for (int b = 0; b < g.nen; ++b)
+        if (stamp[c[b]] != A) {
+          stamp[c[b]] = A;
+          tmp.push_back(c[b]);
+        }
     }
     std::sort(tmp.begin(), tmp.end());
-    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
     adj.insert(adj.end(), tmp.begin(), tmp.end());
     adj_ptr[A + 1] = (int64_t)adj.size();
   }
@@ -148,34 +152,39 @@ int build(fea_pattern &P, int n_nodes, const int32_t *dof_map, const std::vector
     }
   }
 
-  auto block_of = [&](int A, int B) -> int64_t {
-    if (nfree[A] == 0 || nfree[B] == 0) return -1;
-    // blocks of A are in adjacency order restricted to free B
-    int64_t lo = blk_start[A], hi = blk_start[A + 1];
-    while (lo < hi) {
-      const int64_t mid = (lo + hi) / 2;
-      if (P.blk_b[mid] < B) lo = mid + 1; else hi = mid;
-    }
-    return (lo < blk_start[A + 1] && P.blk_b[lo] == B) ? lo : -1;
-  };
-
-  // ---- K staging: count contributions per block, then place in (group, e, a, b) order
+  // ---- K staging: count contributions per block, then place in (group, e, a, b) order.
+  // The block of (c[a], c[b]) is looked up node by node: while node A is current, bid[B]
+  // holds the block (A, B) for every free neighbour B (bstamp[B] == A marks it valid).
   std::vector<int64_t> cnt(nb + 1, 0);
   P.dst_k.assign(G.size(), {});
   P.dst_r.assign(G.size(), {});
-  for (int gi = 0; gi < (int)G.size(); ++gi) {
-    const auto &g = G[gi];
-    if (!g.in_k) continue;
-    auto &dk = P.dst_k[gi];
-    dk.assign((size_t)g.n_elem * g.nen * g.nen, -1);
-    for (int e = 0; e < g.n_elem; ++e) {
-      const int32_t *c = g.conn + (int64_t)e * g.nen;
-      for (int a = 0; a < g.nen; ++a)
-        for (int b = 0; b < g.nen; ++b) {
-          const int64_t q = block_of(c[a], c[b]);
-          dk[((size_t)e * g.nen + a) * g.nen + b] = (int32_t)q;   // temporarily the block id
-          if (q >= 0) ++cnt[q + 1];
+  for (int gi = 0; gi < (int)G.size(); ++gi)
+    if (G[gi].in_k) P.dst_k[gi].assign((size_t)G[gi].n_elem * G[gi].nen * G[gi].nen, -1);
+  {
+    std::vector<int32_t> bstamp(n_nodes, -1);
+    std::vector<int64_t> bid(n_nodes, -1);
+    for (int A = 0; A < n_nodes; ++A) {
+      if (nfree[A] == 0) continue;  // rows of a fixed node stay -1
+      for (int64_t q = blk_start[A]; q < blk_start[A + 1]; ++q) {
+        bstamp[P.blk_b[q]] = A;
+        bid[P.blk_b[q]] = q;
+      }
+      for (int64_t s = inc_ptr[A]; s < inc_ptr[A + 1]; ++s) {
+        const int gi = inc[s].first;
+        const auto &g = G[gi];
+        const int e = inc[s].second;
+        const int32_t *c = g.conn + (int64_t)e * g.nen;
+        auto &dk = P.dst_k[gi];
+        for (int a = 0; a < g.nen; ++a) {
+          if (c[a] != A) continue;
+          for (int b = 0; b < g.nen; ++b) {
+            const int32_t B = c[b];
+            const int64_t q = bstamp[B] == A ? bid[B] : -1;
+            dk[((size_t)e * g.nen + a) * g.nen + b] = (int32_t)q;   // temporarily the block id
+            if (q >= 0) ++cnt[q + 1];
+          }
         }
+      }
     }
   }
   for (int64_t q = 0; q < nb; ++q) cnt[q + 1] += cnt[q];

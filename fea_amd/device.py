@@ -46,20 +46,15 @@ def row_group_tables(rowptr, col, sizes):
         sel = sizes > j
         if not np.array_equal(rl[grp[:-1][sel] + j], rl[grp[:-1][sel]]):
             return None, False
-    heads = grp[:-1]
     nnz = int(rowptr[-1])
     triple = bool(np.all(sizes == 3)) and nnz > 0
     if triple:
-        triple = bool(np.all(rl[heads] % 3 == 0))
+        triple = bool(np.all(rl % 3 == 0))
     if triple:
-        hrow = np.zeros(n, dtype=bool)
-        hrow[heads] = True
-        rows = np.repeat(np.arange(n), rl)
-        pos = np.arange(nnz) - rowptr[rows]
-        first = hrow[rows] & (pos % 3 == 0)
-        c0 = col[:nnz][first]
-        idx = np.nonzero(first)[0]
-        triple = bool(np.all(col[idx + 1] == c0 + 1) and np.all(col[idx + 2] == c0 + 2))
+        # every row a whole number of triples, so the triples of all rows are the rows of
+        # col reshaped [nnz/3, 3]: each must read c, c+1, c+2
+        C = np.asarray(col[:nnz]).reshape(-1, 3)
+        triple = bool(np.all(C[:, 1] == C[:, 0] + 1) and np.all(C[:, 2] == C[:, 0] + 2))
     return grp, triple
 
 

@@ -8,23 +8,26 @@ vectors) and schedules at the granularity of one PCG chunk.  Each admitted
 request is a state machine (tangent -> PCG -> line search -> tangent ...);
 one *tick* of the server
 
-* launches the tangent + residual of every request that needs one,
-* starts the PCG of requests whose linear system is new and runs ONE batched
-  chunk of Jacobi-PCG iterations (``BatchedPCG``: a single launch per CG stage
-  for all requests in the PCG stage, each stopping on its own tolerance),
-* launches the line-search evaluations (directional derivative, energies,
-  step norms) of requests whose direction is ready,
-* reads every result of the tick with one host synchronisation and advances
-  each request's state.
+* launches the tangent + residual of every request that needs one and the
+  line-search evaluations (directional derivative, energies, step norms) of
+  requests whose direction is ready, and starts the PCG of requests whose
+  linear system is new,
+* queues one asynchronous copy of all of the tick's results, including the
+  status of the previous PCG chunk,
+* queues ONE batched chunk of Jacobi-PCG iterations (``BatchedPCG``: a single
+  launch per CG stage for all requests in the PCG stage, each stopping on its
+  own tolerance), which the device runs while the host waits for the copy,
+  admits arrivals and advances each request's state.
 
 A request is never held back by another's PCG: a small request runs its
-Newton iterations while a large one is still iterating.  Between ticks
-finished requests leave (their memory returns to the budget) and arrived
-requests are admitted in arrival order while the HBM budget and the
-batch-size cap allow, so requests that arrive while others are on the GPU
-join at the next tick and their work overlaps on the device.  Admission
-estimates a request's device footprint from its mesh sizes before anything
-is allocated.
+Newton iterations while a large one is still iterating.  Finished requests
+leave (their memory returns to the budget) and arrived requests are admitted
+in arrival order while the HBM budget and the batch-size cap allow.  Admission
+estimates a request's device footprint from its mesh sizes before anything is
+allocated; building the request (mesh, sparsity pattern, uploads) is host work
+done while the device runs the tick's PCG chunk, for a bounded time per tick,
+with its uploads on a side stream that the main stream then waits for; a built
+request joins at the next tick.
 
 Every per-request reduction has a fixed shape that depends only on that
 request, and every decision is the request's own, so each result is
@@ -224,15 +227,35 @@ class _Reads:
         return len(self.items) - 1
 
     def fetch(self):
-        if not self.items:
+        return self.fetch_async().wait()
+
+    def fetch_async(self):
+        """Queue one copy of every collected tensor into pinned host memory; ``wait()``
+        on the result blocks until that copy (not later work) is done."""
+        return _Pending(self)
+
+
+class _Pending:
+    def __init__(self, reads):
+        self.sizes = [t.numel() for t in reads.items]
+        self.host = None
+        if self.sizes:
+            dev = torch.cat(reads.items)
+            self.host = torch.empty(dev.numel(), dtype=dev.dtype, pin_memory=True)
+            self.host.copy_(dev, non_blocking=True)
+            self.event = torch.cuda.Event()
+            self.event.record()
+        reads.items = []
+
+    def wait(self):
+        if self.host is None:
             return []
-        sizes = [t.numel() for t in self.items]
-        host = torch.cat(self.items).cpu().numpy()
+        self.event.synchronize()
+        host = self.host.numpy()
         out, o = [], 0
-        for s in sizes:
+        for s in self.sizes:
             out.append(host[o:o + s])
             o += s
-        self.items = []
         return out
 
 
@@ -301,13 +324,14 @@ class SolveServer:
     """Continuous-batching server of static solves on one GPU.
 
     hbm_budget: bytes of device memory admitted requests may hold (estimated
-    before admission); max_batch: most requests in flight; the remaining
-    arguments are those of ``StaticImplicitSolver``.
+    before admission); max_batch: most requests in flight; admit_budget: host
+    seconds per tick spent building arrived requests (at least one is built);
+    the remaining arguments are those of ``StaticImplicitSolver``.
     """
 
     def __init__(self, precision='mixed', hbm_budget=None, max_batch=64, rel_tol=1e-10, step_tol=1e-12,
                  cg_tol=1e-8, cg_max_iter=20000, max_newton=200, chunk=32, use_graph=False,
-                 armijo_slack=1e-12, verbose=False, predictor=True, preconditioner='block'):
+                 armijo_slack=1e-12, verbose=False, predictor=True, preconditioner='block', admit_budget=0.004):
         nat.require_gpu()
         if precision not in ('fp64', 'mixed'):
             raise ValueError("precision must be 'fp64' or 'mixed'")
@@ -327,14 +351,19 @@ class SolveServer:
         # node-block Jacobi by default (point Jacobi, the reference's scaling: 'jacobi')
         self.pcg = BatchedPCG(self.mat_dtype, F64, 'cuda', chunk=chunk, use_graph=use_graph, precond=preconditioner)
         self.active: list[_Slot] = []
+        self._side = torch.cuda.Stream()     # admission uploads (default priority)
+        self._retired = []
+        self.admit_budget = float(admit_budget)
         self.used = 0
         self.rounds = 0
         self.batch_sizes = []
         self.peak_used = 0
 
     # -------------------------------------------------------------- admission
-    def _admit(self, req, now):
-        req.footprint_est = estimate_footprint(req.spec, self.precision)
+    def _admit(self, req):
+        """Build the request's plan and read its full-load residual norm (the reference
+        of the relative residual test); returns the slot.  Runs with the admission stream
+        current, so it waits for none of the main stream's work."""
         fe, _ = request_model(req.spec)
         fe.assembly.initialize()
         plan = fe.assembly.plan
@@ -344,16 +373,12 @@ class SolveServer:
                      dGC=torch.zeros(plan.n_free, dtype=torch.float64, device=dev),
                      vec=PcgVectors(plan.n_free, F64, dev, n_groups=plan.n_groups), r_ref=0.0,
                      steps=int(req.spec.get('load_steps', 1)))
-        reads = _Reads()
-        _stats_launch(plan, plan.residual(U, 1.0, F64), reads)
-        rss, _ = _stats_value(reads.fetch()[0])
+        rd = _Reads()
+        _stats_launch(plan, plan.residual(U, 1.0, F64), rd)
+        rss, _ = _stats_value(rd.fetch()[0])          # waits for this stream's work only
         slot.r_ref = math.sqrt(rss)
         req.footprint = self._actual_footprint(slot)
-        req.t_admit = now
-        req.status = 'running'
-        self.used += req.footprint_est
-        self.peak_used = max(self.peak_used, self.used)
-        self.active.append(slot)
+        return slot
 
     def _actual_footprint(self, s):
         p = s.plan
@@ -372,9 +397,15 @@ class SolveServer:
         req.t_done = now
         req.newton_iters = slot.it_total
         if status == 'done':
-            req.GC = slot.plan.u_to_gc(slot.U).cpu()
+            # an asynchronous copy: complete before the next read (and before run returns)
+            gc = slot.plan.u_to_gc(slot.U)
+            req.GC = torch.empty(gc.shape, dtype=gc.dtype, pin_memory=True)
+            req.GC.copy_(gc, non_blocking=True)
         self.used -= req.footprint_est
         self.active.remove(slot)
+        # its buffers came from the admission stream; they are released after the next
+        # read, when no queued main-stream work can still touch them
+        self._retired.append(slot)
 
     # -------------------------------------------------------------- scheduling
     # Every admitted request is a small state machine advanced one stage per tick:
@@ -407,8 +438,9 @@ class SolveServer:
     def _pcg_entry(self, s):
         return (s.plan, s.K, s.vec, self.cg_tol, s.max_iter)
 
-    def _tick(self, now):
-        reads = _Reads()
+    def _launch(self, reads):
+        """Queue the device work of every admitted request's current stage; the
+        results land on ``reads``.  Returns what ``_decide`` needs."""
         by = {}
         for s in self.active:
             by.setdefault(s.state, []).append(s)
@@ -425,14 +457,30 @@ class SolveServer:
             st = s.ls
             s.plan.dof_axpy(s.U, st['sign'] * st['alpha'], s.dGC, s.Unew)
             s.rd = _energy_launch(s.plan, s.Unew, reads)
+        # status of the chunk queued at the end of the previous tick (complete in stream
+        # order before anything queued now); a start does not touch it
+        prev, prev_rd = self._chunk, None
+        if prev:
+            prev_rd = reads.add(self.pcg.status(len(prev)))
         new = by.get('pcg_new', [])
         self.pcg.start([self._pcg_entry(s) for s in new])
-        in_pcg = by.get('pcg', []) + new
-        pcg_rd = None
+        for s in new:
+            s.state = 'pcg'
+        return by, prev, prev_rd
+
+    def _launch_chunk(self):
+        """One batched chunk of PCG iterations for every request in its PCG stage.  It is
+        queued after the tick's read, so the device runs it while the host decides; its
+        status is read at the next tick.  A request whose PCG finished in the previous
+        chunk is still listed here (its result is not read yet): the chunk skips it."""
+        in_pcg = [s for s in self.active if s.state == 'pcg']
         if in_pcg:
-            pcg_rd = reads.add(self.pcg.iterate([self._pcg_entry(s) for s in in_pcg]))
-        host = reads.fetch()
-        # ---- decisions
+            self.pcg.iterate([self._pcg_entry(s) for s in in_pcg])
+        self._chunk = in_pcg
+
+    def _decide(self, tick, host, now):
+        """Advance every request's state machine from the tick's results."""
+        by, prev, prev_rd = tick
         for s in by.get('tangent', []):
             rss, _ = _stats_value(host[s.rd])
             if s.it_total >= self.max_newton or not math.isfinite(rss):
@@ -450,10 +498,11 @@ class SolveServer:
             s.vec.x.copy_(s.dGC)
             s.max_iter = 4 * self.cg_max_iter if s.it_total % 20 == 0 or s.low_alpha > 0 else self.cg_max_iter
             s.state = 'pcg_new'
-        if in_pcg:
-            status = host[pcg_rd].reshape(-1, 2)
-            for s, (its, done) in zip(in_pcg, status):
-                s.state = 'pcg'
+        if prev:
+            status = host[prev_rd].reshape(-1, 2)
+            for s, (its, done) in zip(prev, status):
+                if s.state != 'pcg':         # its finish was read at the previous tick
+                    continue
                 if done != 0.0:
                     s.dGC.copy_(s.vec.x)
                     s.req.cg_iters += int(its)
@@ -530,29 +579,55 @@ class SolveServer:
             for r in queue:
                 r.arrival = 0.0
         qi = 0
+        main = torch.cuda.current_stream()
+        self._chunk = []
         while qi < len(queue) or self.active:
+            # one tick: the stage work of every request, one asynchronous read of its
+            # results, then the next PCG chunk, which the device runs while the host admits
+            # arrivals and decides
+            reads = _Reads()
+            tick = None
+            if self.active:
+                self.batch_sizes.append(len(self.active))
+                tick = self._launch(reads)
+            pending = reads.fetch_async()
+            self._launch_chunk()
+            # admission in arrival order while slots and HBM budget allow, for at most
+            # admit_budget seconds per tick (at least one request): building a request is
+            # host work that overlaps the chunk; its uploads go on a side stream that the
+            # main stream then waits for, and it joins at the next tick
+            admitted = []
             t = now()
-            # admission in arrival order while slots and HBM budget allow
-            while qi < len(queue) and queue[qi].arrival <= t and len(self.active) < self.max_batch:
-                r = queue[qi]
-                est = estimate_footprint(r.spec, self.precision)
-                if self.active and self.used + est > self.hbm_budget:
-                    break
-                if est > self.hbm_budget:
-                    r.status, r.error, r.t_done = 'rejected', 'footprint %d B exceeds the budget' % est, now()
+            with torch.cuda.stream(self._side):
+                while (qi < len(queue) and queue[qi].arrival <= t
+                       and len(self.active) + len(admitted) < self.max_batch
+                       and (not admitted or now() - t < self.admit_budget)):
+                    r = queue[qi]
+                    est = estimate_footprint(r.spec, self.precision)
+                    if (self.active or admitted) and self.used + est > self.hbm_budget:
+                        break
                     qi += 1
-                    continue
-                qi += 1
-                self._admit(r, now())
-            if not self.active:
+                    if est > self.hbm_budget:
+                        r.status, r.error, r.t_done = 'rejected', 'footprint %d B exceeds the budget' % est, now()
+                        continue
+                    r.footprint_est, r.t_admit, r.status = est, now(), 'running'
+                    admitted.append(self._admit(r))
+                    self.used += est
+                    self.peak_used = max(self.peak_used, self.used)
+            if admitted:
+                main.wait_stream(self._side)
+            if tick is None and not admitted:
                 if qi < len(queue):
                     time.sleep(max(0.0, min(queue[qi].arrival - now(), 0.05)))
                 continue
-            self.batch_sizes.append(len(self.active))
-            self._tick(now)
-            self.rounds += 1
-            if on_round is not None:
-                on_round(self)
+            host = pending.wait()
+            self._retired = []
+            if tick is not None:
+                self._decide(tick, host, now)
+                self.rounds += 1
+                if on_round is not None:
+                    on_round(self)
+            self.active.extend(admitted)
         torch.cuda.synchronize()
         return requests
 

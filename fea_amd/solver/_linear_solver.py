@@ -215,38 +215,65 @@ class BatchedPCG:
         self._ss_dev = torch.zeros(self.max_sys + 1, dtype=torch.int32, device=self.device)
         self._vs_dev = torch.zeros(self.max_sys + 1, dtype=torch.int32, device=self.device)
         self._status_dev = torch.zeros(2 * self.max_sys, dtype=torch.float64, device=self.device)
+        self._table = None             # the bytes of the table on the device
         self._graph = None
         self._graph_key = None
         self.iterations = []
         self.chunks = 0
 
+    def _entry(self, plan, val, v, tol, max_iter):
+        """The system's table entry (bytes) and its SpMV / vector workgroup counts, cached
+        on its vectors under every pointer and value the entry holds that can change."""
+        key = (id(plan), plan.rowptr.data_ptr(), plan.spmv_col.data_ptr(), plan.grp.data_ptr(),
+               plan.diag_slot.data_ptr(), val.data_ptr(), float(tol), int(max_iter), self.precond, self.mat_dtype)
+        c = getattr(v, '_pcg_entry', None)
+        if c is not None and c[0] == key:
+            return c[1]
+        if plan.grp is None or plan.n_free != v.n:
+            raise ValueError('vectors do not match the plan')
+        if val.dtype != _DT[self.mat_dtype] or v.x.dtype != _DT[self.vec_dtype]:
+            raise ValueError('dtype mismatch')
+        s = PcgSystem()
+        s.n, s.n_groups, s.max_iter, s.nnz, s.tol = v.n, int(plan.spmv_groups), int(max_iter), plan.nnz, float(tol)
+        s.grp, s.rowptr, s.col, s.diag_slot = (plan.grp.data_ptr(), plan.rowptr.data_ptr(), plan.spmv_col.data_ptr(),
+                                               plan.diag_slot.data_ptr())
+        s.val, s.dinv, s.b, s.x = val.data_ptr(), v.dinv.data_ptr(), v.b.data_ptr(), v.x.data_ptr()
+        # node-block Jacobi needs the row groups and a preconditioner buffer sized for them
+        s.precond = 1 if (self.precond == 'block' and getattr(v, 'block_ok', False)) else 0
+        s.r, s.z, s.p, s.q, s.ws = v.r.data_ptr(), v.z.data_ptr(), v.p.data_ptr(), v.q.data_ptr(), v.ws.data_ptr()
+        ss = np.zeros(2, dtype=np.int32)
+        vs = np.zeros(2, dtype=np.int32)
+        nat.check(nat.lib().fea_pcg_batch_layout(1, ctypes.byref(s), ss.ctypes.data, vs.ctypes.data),
+                  'fea_pcg_batch_layout')
+        out = (bytes(s), int(ss[1]), int(vs[1]))
+        v._pcg_entry = (key, out)
+        return out
+
     def _layout(self, entries):
+        """Device table of the listed systems; uploaded only when it differs from the
+        table already on the device (the kernels only read it)."""
         n = len(entries)
         if n > self.max_sys:
             raise ValueError('batch of %d systems exceeds %d' % (n, self.max_sys))
-        arr = (PcgSystem * n)()
-        for k, (plan, val, v, tol, max_iter) in enumerate(entries):
-            if plan.grp is None or plan.n_free != v.n:
-                raise ValueError('system %d: vectors do not match the plan' % k)
-            want = _DT[self.mat_dtype]
-            if val.dtype != want or v.x.dtype != _DT[self.vec_dtype]:
-                raise ValueError('system %d: dtype mismatch' % k)
-            s = arr[k]
-            s.n, s.n_groups, s.max_iter, s.nnz, s.tol = v.n, int(plan.spmv_groups), int(max_iter), plan.nnz, float(tol)
-            s.grp, s.rowptr, s.col, s.diag_slot = (plan.grp.data_ptr(), plan.rowptr.data_ptr(), plan.spmv_col.data_ptr(),
-                                                   plan.diag_slot.data_ptr())
-            s.val, s.dinv, s.b, s.x = val.data_ptr(), v.dinv.data_ptr(), v.b.data_ptr(), v.x.data_ptr()
-            # node-block Jacobi needs the row groups and a preconditioner buffer sized for them
-            s.precond = 1 if (self.precond == 'block' and getattr(v, 'block_ok', False)) else 0
-            s.r, s.z, s.p, s.q, s.ws = v.r.data_ptr(), v.z.data_ptr(), v.p.data_ptr(), v.q.data_ptr(), v.ws.data_ptr()
+        try:
+            parts = [self._entry(*e) for e in entries]
+        except ValueError as err:
+            raise ValueError('batch entry: %s' % err) from None
         ss = np.zeros(n + 1, dtype=np.int32)
         vs = np.zeros(n + 1, dtype=np.int32)
-        nat.check(nat.lib().fea_pcg_batch_layout(n, ctypes.byref(arr), ss.ctypes.data, vs.ctypes.data),
-                  'fea_pcg_batch_layout')
-        raw = np.frombuffer(bytes(arr), dtype=np.uint8)
-        self._sys_dev[:raw.size].copy_(torch.from_numpy(raw.copy()))
-        self._ss_dev[:n + 1].copy_(torch.from_numpy(ss))
-        self._vs_dev[:n + 1].copy_(torch.from_numpy(vs))
+        ss[1:] = np.cumsum([p[1] for p in parts])
+        vs[1:] = np.cumsum([p[2] for p in parts])
+        if ss[-1] >= (1 << 30) or vs[-1] >= (1 << 30):
+            raise ValueError('batch too large')
+        blob = b''.join(p[0] for p in parts)
+        if blob != self._table:
+            # stream-ordered copies from pinned memory: the host does not wait for the
+            # kernels queued before them, which still read the previous table
+            raw = np.frombuffer(blob, dtype=np.uint8)
+            self._sys_dev[:raw.size].copy_(torch.from_numpy(raw.copy()).pin_memory(), non_blocking=True)
+            self._ss_dev[:n + 1].copy_(torch.from_numpy(ss).pin_memory(), non_blocking=True)
+            self._vs_dev[:n + 1].copy_(torch.from_numpy(vs).pin_memory(), non_blocking=True)
+            self._table = blob
         return n, int(ss[-1]), int(vs[-1])
 
     def _iterate(self, n, st, vt, stream):
@@ -263,10 +290,15 @@ class BatchedPCG:
 
     def iterate(self, entries):
         """Enqueue one chunk of iterations of the listed (started) systems; returns the
-        device status tensor [n, 2] = (iterations, done) written after the chunk."""
+        device status tensor [n, 2] = (iterations, done) written after the chunk (the
+        next chunk overwrites it; systems already done are skipped by the chunk)."""
         n, st, vt = self._layout(entries)
         self._iterate(n, st, vt, nat.stream_ptr())
         self.chunks += 1
+        return self._status_dev[:2 * n].view(n, 2)
+
+    def status(self, n):
+        """The status [n, 2] the last chunk wrote for its n systems (a device view)."""
         return self._status_dev[:2 * n].view(n, 2)
 
     def solve(self, entries):

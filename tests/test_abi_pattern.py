@@ -134,6 +134,19 @@ def test_staging_maps_are_a_permutation(ci):
     e, a, b = np.nonzero(dk >= 0)
     q = blk_of_slot[dk[e, a, b]]
     assert np.array_equal(P['blk'][0][q], conn[e, a]) and np.array_equal(P['blk'][1][q], conn[e, b])
+    # inside each block's segment the slots follow (group, element, a, b) order: the fixed
+    # order of the staging reduction
+    key, slot = [], []
+    for gi, g in enumerate(P['groups']):
+        d = g[0]
+        t = np.nonzero(d >= 0)[0]
+        key.append(gi * (1 << 40) + t)
+        slot.append(d[t])
+    key, slot = np.concatenate(key), np.concatenate(slot)
+    order = np.argsort(slot)
+    k_sorted, blk_sorted = key[order], blk_of_slot[slot[order]]
+    same = blk_sorted[1:] == blk_sorted[:-1]
+    assert np.all(k_sorted[1:][same] > k_sorted[:-1][same])
 
 
 def test_pattern_rejects_bad_dof_map():

@@ -44,3 +44,9 @@ def test_triple_columns_index():
         assert np.array_equal(tc[s0 // 9 + np.arange(L // 3)], A.indices[s0 + 3 * np.arange(L // 3)])
     with pytest.raises(ValueError):
         triple_columns(A.indptr, A.indices, np.arange(N + 1))      # one-row groups: not triples
+    # 3-row groups with row lengths a multiple of 3 whose second run of columns is not
+    # consecutive: not triple-structured; the same rows with consecutive runs are
+    for cols, want in (([0, 1, 2, 3, 5, 7], False), ([0, 1, 2, 4, 5, 6], True)):
+        rowptr = np.arange(0, 6 * 9 + 1, 6)
+        grp, triple = row_group_tables(rowptr, np.tile(cols, 9), np.full(3, 3))
+        assert grp is not None and triple == want
