@@ -171,22 +171,33 @@ __device__ __forceinline__ void stress(const GPState<T> &s, const MatParams &m, 
   for (int I = 0; I < 6; ++I) S[I] = sc * s.C[I] + si * s.Ci[I] + (I < 3 ? s1 : T(0));
 }
 
-// Material tangent C = 2 dS/dC in Voigt form (engineering shear strain), full 6x6.
-//   C = cii Ci(x)Ci + c1i (1(x)Ci + Ci(x)1) + cci (C(x)Ci + Ci(x)C) + c11 1(x)1
-//       + cI  Isym + cIi I_{Ci},   (I_{Ci})_ABCD = (Ci_AC Ci_BD + Ci_AD Ci_BC)/2
+// Material tangent CC = 2 dS/dC of W(I1bar, I2bar, J):
+//   CC = cii Ci(x)Ci + c1i (1(x)Ci + Ci(x)1) + cci (C(x)Ci + Ci(x)C) + c11 1(x)1
+//        + cI  Isym + cIi I_{Ci},   (I_{Ci})_ABCD = (Ci_AC Ci_BD + Ci_AD Ci_BC)/2
+struct TangentCoeffs {
+  double cii, c1i, cci, c11, cI, cIi;
+};
 template <typename T>
-__device__ __forceinline__ void tangent(const GPState<T> &s, const MatParams &m, T D[6][6]) {
+__device__ __forceinline__ TangentCoeffs tangent_coeffs(const GPState<T> &s, const MatParams &m) {
   const double Jm23 = pow(s.J, -2.0 / 3.0);
   const double Jm43 = Jm23 * Jm23;
   const double I1 = (double)s.I1, I2 = (double)s.I2;
   const double g1 = 4.0 * m.w1 * Jm23, g2 = 4.0 * m.w2 * Jm43;
   const double up = m.kv * s.Jm1, upp = m.kv;
-  const T cii = T(g1 * I1 / 9.0 + (4.0 / 9.0) * g2 * I2 + s.J * (up + s.J * upp));
-  const T c1i = T(-g1 / 3.0 - (2.0 / 3.0) * g2 * I1);
-  const T cci = T((2.0 / 3.0) * g2);
-  const T c11 = T(g2);
-  const T cI = T(-g2);
-  const T cIi = T(g1 * I1 / 3.0 + (2.0 / 3.0) * g2 * I2 - 2.0 * s.J * up);
+  TangentCoeffs c;
+  c.cii = g1 * I1 / 9.0 + (4.0 / 9.0) * g2 * I2 + s.J * (up + s.J * upp);
+  c.c1i = -g1 / 3.0 - (2.0 / 3.0) * g2 * I1;
+  c.cci = (2.0 / 3.0) * g2;
+  c.c11 = g2;
+  c.cI = -g2;
+  c.cIi = g1 * I1 / 3.0 + (2.0 / 3.0) * g2 * I2 - 2.0 * s.J * up;
+  return c;
+}
+// CC in Voigt form (engineering shear strain), full 6x6: D[I][J] = CC_{vA(I) vB(I) vA(J) vB(J)}.
+template <typename T>
+__device__ __forceinline__ void tangent(const GPState<T> &s, const MatParams &m, T D[6][6]) {
+  const TangentCoeffs tc = tangent_coeffs(s, m);
+  const T cii = T(tc.cii), c1i = T(tc.c1i), cci = T(tc.cci), c11 = T(tc.c11), cI = T(tc.cI), cIi = T(tc.cIi);
   T Cif[9];  // full C^-1
 #pragma unroll
   for (int A = 0; A < 3; ++A)
@@ -206,7 +217,67 @@ __device__ __forceinline__ void tangent(const GPState<T> &s, const MatParams &m,
     }
   }
 }
-
+// First elasticity tensor times w, AA_{jB,kD} = w (sum_{A,C} F_jA CC_ABCD F_kC + delta_jk S_BD),
+// from the structure of CC above (no 6 x 6 intermediate): with G = F Ci, H = F C,
+// b = F F^T and Q = G F^T,
+//   F CC F^T = G (x) X + Y (x) G + c11 F (x) F + cI/2 (b_jk delta_BD + F_jD F_kB)
+//              + cIi/2 (Q_jk Ci_BD + G_jD G_kB),
+// X = cii G + c1i F + cci H, Y = c1i F + cci H ((P (x) R)_{jB,kD} = P_jB R_kD).
+// Written as the packed upper triangle of the symmetric 9 x 9 matrix (index 3j+B).
+template <typename T, typename F9>
+__device__ __forceinline__ void first_elasticity_packed(const GPState<double> &s, const TangentCoeffs &tc,
+                                                        const double S[6], double w, F9 &&put) {
+  double Ci[9], Cf[9];
+#pragma unroll
+  for (int A = 0; A < 3; ++A)
+#pragma unroll
+    for (int B = 0; B < 3; ++B) {
+      Ci[A * 3 + B] = s.Ci[vidx(A, B)];
+      Cf[A * 3 + B] = s.C[vidx(A, B)];
+    }
+  double G[9], X[9], Y[9];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int B = 0; B < 3; ++B) {
+      double g = 0.0, h = 0.0;
+#pragma unroll
+      for (int A = 0; A < 3; ++A) {
+        g += s.F[j * 3 + A] * Ci[A * 3 + B];
+        h += s.F[j * 3 + A] * Cf[A * 3 + B];
+      }
+      G[j * 3 + B] = g;
+      Y[j * 3 + B] = w * (tc.c1i * s.F[j * 3 + B] + tc.cci * h);
+      X[j * 3 + B] = w * tc.cii * g + Y[j * 3 + B];
+    }
+  double b[9], Q[9];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double bb = 0.0, q = 0.0;
+#pragma unroll
+      for (int A = 0; A < 3; ++A) {
+        bb += s.F[j * 3 + A] * s.F[k * 3 + A];
+        q += G[j * 3 + A] * s.F[k * 3 + A];
+      }
+      b[j * 3 + k] = bb;
+      Q[j * 3 + k] = q;
+    }
+  const double w11 = w * tc.c11, wI = 0.5 * w * tc.cI, wIi = 0.5 * w * tc.cIi;
+#pragma unroll
+  for (int I = 0; I < 9; ++I)
+#pragma unroll
+    for (int J = I; J < 9; ++J) {
+      const int j = I / 3, B = I % 3, k = J / 3, D = J % 3;
+      double v = G[j * 3 + B] * X[k * 3 + D] + Y[j * 3 + B] * G[k * 3 + D] +
+                 w11 * s.F[j * 3 + B] * s.F[k * 3 + D] +
+                 wI * ((B == D ? b[j * 3 + k] : 0.0) + s.F[j * 3 + D] * s.F[k * 3 + B]) +
+                 wIi * (Q[j * 3 + k] * Ci[B * 3 + D] + G[j * 3 + D] * G[k * 3 + B]);
+      if (j == k) v += w * S[vidx(B, D)];
+      put(I * 9 - (I * (I - 1)) / 2 + (J - I), v);
+    }
+}
 // Strain energy density (hyperelastic.py:25-33 generalised), float64.
 __device__ __forceinline__ double energy_density(double I1, double I2, double J, double Jm1,
                                                  const MatParams &m) {

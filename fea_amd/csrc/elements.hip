@@ -4,169 +4,154 @@
 
 namespace fea {
 
-// Residual + tangent.  A workgroup takes EPW elements, one thread per (element,
-// node a).  The element's nodal coordinates / displacements and, per Gauss
-// point, dN/dX, F, w S and the packed upper triangle of w D are staged in LDS:
-// thread (e, g) computes Gauss point g's state once (float64), instead of every
-// node thread recomputing all of them.  Thread (e, a) then forms its row block
-// K_a. (3 x 3 nen) and R_a from the staged state, in passes over BH columns b
-// at a time to bound the live accumulators.  The B^T D B contraction runs in T.
+// Residual + tangent.  The stiffness is written with the first elasticity tensor of each
+// Gauss point,
+//   A_{jB,kD} = sum_{A,C} F_jA CC_ABCD F_kC + delta_jk S_BD   (times w det J),
+// as K_ab[j][k] = sum_g sum_{B,D} dN_a,B A_{jB,kD} dN_b,D, which needs a third of the
+// arithmetic of B^T D B formed per node pair.  A workgroup holds three waves and EPW
+// elements (EPW nen <= 64): wave j serves displacement row j of every (element, node a)
+// pair, one lane each, so j is uniform in a wave and each row's contraction is compiled
+// with its own constant LDS offsets.
+//   Phase 1: lane (e, g) of wave 0 forms the Gauss point's dN/dX, w P = w F S and w A in
+// float64 (near incompressibility the deviatoric tangent is a small difference of
+// bulk-sized terms) and stages them in LDS; A is symmetric under (jB) <-> (kD) and kept as
+// its packed upper triangle (45 values).  (Splitting this phase over the three waves by
+// entries of A made every part hold the registers of all three: 228-244 VGPRs.)
+//   Phase 2: lane (e, a) of wave j forms v_kD = sum_B dN_a,B A_{jB,kD} (27 FMAs) and adds
+// v . dN_b to K_ab[j][.] for every b (9 FMAs each), in T; its residual entry
+// R_aj = sum_g sum_B (w P)_jB dN_a,B accumulates in float64.
 template <int ET>
 struct RkCfg;
 template <>
 struct RkCfg<4> {
-  static constexpr int EPW = 16, BH = 4;
+  static constexpr int EPW = 16;
 };
 template <>
 struct RkCfg<8> {
-  static constexpr int EPW = 8, BH = 4;
+  static constexpr int EPW = 8;
 };
 template <>
 struct RkCfg<10> {
-  static constexpr int EPW = 6, BH = 5;
+  static constexpr int EPW = 6;
 };
+constexpr int kRkThreads = 192;  // three waves
+
+// packed upper-triangle index of the symmetric 9 x 9 matrix A (I, J in 0..8)
+__device__ __forceinline__ constexpr int sym9(int I, int J) {
+  return I <= J ? I * 9 - (I * (I - 1)) / 2 + (J - I) : J * 9 - (J * (J - 1)) / 2 + (I - J);
+}
+
+template <typename T, int ET, int JR>
+__device__ __forceinline__ void rk_contract(const double (*dn)[Elem<ET>::nen][3], const double (*wp)[9],
+                                            const T *A, int a, T K[][3], double &Rj) {
+  constexpr int NEN = Elem<ET>::nen, NGP = Elem<ET>::ngp;
+#pragma unroll 1
+  for (int g = 0; g < NGP; ++g) {
+    const T *Ag = A + g * 45;
+    const double d0 = dn[g][a][0], d1 = dn[g][a][1], d2 = dn[g][a][2];
+    Rj += wp[g][JR * 3 + 0] * d0 + wp[g][JR * 3 + 1] * d1 + wp[g][JR * 3 + 2] * d2;
+    const T t0 = T(d0), t1 = T(d1), t2 = T(d2);
+    T v[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+      v[q] = t0 * Ag[sym9(3 * JR + 0, q)] + t1 * Ag[sym9(3 * JR + 1, q)] + t2 * Ag[sym9(3 * JR + 2, q)];
+#pragma unroll
+    for (int b = 0; b < NEN; ++b) {
+      const T e0 = T(dn[g][b][0]), e1 = T(dn[g][b][1]), e2 = T(dn[g][b][2]);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) K[b][k] += v[3 * k + 0] * e0 + v[3 * k + 1] * e1 + v[3 * k + 2] * e2;
+    }
+  }
+}
 
 template <typename T, int ET>
-__global__ __launch_bounds__(RkCfg<ET>::EPW * Elem<ET>::nen) void k_elem_rk(
+__global__ __launch_bounds__(kRkThreads) void k_elem_rk(
     int n_elem, const int32_t *__restrict__ conn, const double *__restrict__ X, const double *__restrict__ U,
     MatParams mp, const int32_t *__restrict__ dst_k, T *__restrict__ kstage, const int32_t *__restrict__ dst_r,
     double *__restrict__ rstage) {
   constexpr int NEN = Elem<ET>::nen, NGP = Elem<ET>::ngp;
-  constexpr int EPW = RkCfg<ET>::EPW, BH = RkCfg<ET>::BH;
-  constexpr int O_F = 3 * NEN, O_S = O_F + 9, O_D = O_S + 6, GS = O_D + 21;  // per-GP layout
+  constexpr int EPW = RkCfg<ET>::EPW;
+  static_assert(EPW * NEN <= 64 && EPW * NGP <= 64, "one lane per (element, node) and per Gauss point");
+  // per-element strides padded by one entry, so the elements of a wave start on
+  // different banks
+  constexpr int SA = NGP * 45 + 1;
   __shared__ double sx[EPW][NEN][3], su[EPW][NEN][3];
-  __shared__ T sg[EPW][NGP][GS];
-  const int le = threadIdx.x / NEN, a = threadIdx.x - le * NEN;
+  __shared__ double sdn[EPW][NGP * NEN * 3 + 1];
+  __shared__ double swp[EPW][NGP * 9 + 1];
+  __shared__ T sA[EPW][SA];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // the row j of this wave
+  const int r = threadIdx.x & 63;
+  const int le = r / NEN, a = r - le * NEN;
   const int64_t e = (int64_t)blockIdx.x * EPW + le;
-  const bool valid = e < n_elem;
+  const bool valid = r < EPW * NEN && e < n_elem;
   if (valid) {
     const int64_t n = conn[e * NEN + a];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      sx[le][a][i] = X[3 * n + i];
-      su[le][a][i] = U[3 * n + i];
-    }
+    sx[le][a][wv] = X[3 * n + wv];
+    su[le][a][wv] = U[3 * n + wv];
   }
   __syncthreads();
-  if (valid && a < NGP) {
-    // Gauss point g = a of element e: state in float64 (near incompressibility
-    // the deviatoric part of the tangent is a small difference of bulk-sized terms)
-    double dNd[NEN][3];
-    const double wd = gp_geometry<double, ET>(a, sx[le], dNd);
-    double Hd[9];
-    disp_grad<double, NEN>(su[le], dNd, Hd);
-    GPState<double> sd;
-    kinematics(Hd, sd);
-    double Sd[6];
-    stress(sd, mp, Sd);
-    double Dd[6][6];
-    tangent(sd, mp, Dd);
-    T *o = sg[le][a];
+  if (wv == 0 && r < EPW * NGP) {
+    const int ge = r / NGP, g = r - ge * NGP;
+    if ((int64_t)blockIdx.x * EPW + ge < n_elem) {
+      double dNd[NEN][3];
+      const double wd = gp_geometry<double, ET>(g, sx[ge], dNd);
+      double Hd[9];
+      disp_grad<double, NEN>(su[ge], dNd, Hd);
+      GPState<double> sd;
+      kinematics(Hd, sd);
+      double Sd[6];
+      stress(sd, mp, Sd);
+      const TangentCoeffs tc = tangent_coeffs(sd, mp);
+      double *dn = sdn[ge] + g * NEN * 3;
 #pragma unroll
-    for (int c = 0; c < NEN; ++c)
+      for (int c = 0; c < NEN; ++c)
 #pragma unroll
-      for (int i = 0; i < 3; ++i) o[3 * c + i] = (T)dNd[c][i];
+        for (int i = 0; i < 3; ++i) dn[c * 3 + i] = dNd[c][i];
+      double *wp = swp[ge] + g * 9;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) o[O_F + k] = (T)sd.F[k];
+      for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
-    for (int I = 0; I < 6; ++I) o[O_S + I] = (T)(wd * Sd[I]);
-    int q = 0;
+        for (int B = 0; B < 3; ++B) {
+          double acc = 0.0;
 #pragma unroll
-    for (int I = 0; I < 6; ++I)
-#pragma unroll
-      for (int J = I; J < 6; ++J) o[O_D + q++] = (T)(wd * Dd[I][J]);
+          for (int A = 0; A < 3; ++A) acc += sd.F[jj * 3 + A] * Sd[vidx(A, B)];
+          wp[jj * 3 + B] = wd * acc;
+        }
+      T *o = sA[ge] + g * 45;
+      first_elasticity_packed<T>(sd, tc, Sd, wd, [&](int i, double v) { o[i] = (T)v; });
+    }
   }
   __syncthreads();
   if (!valid) return;
-  double Ra[3] = {0.0, 0.0, 0.0};
-#pragma unroll 1
-  for (int b0 = 0; b0 < NEN; b0 += BH) {
-    T K[BH][9];
+  double Rj = 0.0;
+  T K[NEN][3];
 #pragma unroll
-    for (int bb = 0; bb < BH; ++bb)
+  for (int b = 0; b < NEN; ++b)
 #pragma unroll
-      for (int k = 0; k < 9; ++k) K[bb][k] = T(0);
-    for (int g = 0; g < NGP; ++g) {
-      const T *st = sg[le][g];
-      T F[9], dNa[3];
+    for (int k = 0; k < 3; ++k) K[b][k] = T(0);
+  const auto dn = reinterpret_cast<const double (*)[NEN][3]>(sdn[le]);
+  const auto wp = reinterpret_cast<const double (*)[9]>(swp[le]);
+  if (wv == 0)
+    rk_contract<T, ET, 0>(dn, wp, sA[le], a, K, Rj);
+  else if (wv == 1)
+    rk_contract<T, ET, 1>(dn, wp, sA[le], a, K, Rj);
+  else
+    rk_contract<T, ET, 2>(dn, wp, sA[le], a, K, Rj);
+  if (kstage != nullptr) {
+    const int64_t base = (e * NEN + a) * NEN;
 #pragma unroll
-      for (int k = 0; k < 9; ++k) F[k] = st[O_F + k];
+    for (int b = 0; b < NEN; ++b) {
+      const int d = dst_k[base + b];
+      if (d >= 0) {
+        T *o = kstage + (int64_t)d * 9 + 3 * wv;
 #pragma unroll
-      for (int i = 0; i < 3; ++i) dNa[i] = st[3 * a + i];
-      T Ba[6][3];
-      bmat(F, dNa, Ba);
-      if (b0 == 0) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          T acc = T(0);
-#pragma unroll
-          for (int I = 0; I < 6; ++I) acc += st[O_S + I] * Ba[I][j];
-          Ra[j] += (double)acc;
-        }
-      }
-      // DB = (w D) Ba, D read from the packed upper triangle
-      T DB[6][3];
-#pragma unroll
-      for (int I = 0; I < 6; ++I)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) DB[I][j] = T(0);
-#pragma unroll
-      for (int I = 0; I < 6; ++I)
-#pragma unroll
-        for (int L = 0; L < 6; ++L) {
-          const int r = I < L ? I : L, c = I < L ? L : I;
-          const T d = st[O_D + r * 6 - (r * (r - 1)) / 2 + (c - r)];
-#pragma unroll
-          for (int j = 0; j < 3; ++j) DB[I][j] += d * Ba[L][j];
-        }
-      // (w S) dN_a (full symmetric S)
-      T SdNa[3];
-#pragma unroll
-      for (int A = 0; A < 3; ++A)
-        SdNa[A] = st[O_S + vidx(A, 0)] * dNa[0] + st[O_S + vidx(A, 1)] * dNa[1] + st[O_S + vidx(A, 2)] * dNa[2];
-#pragma unroll
-      for (int bb = 0; bb < BH; ++bb) {
-        const int b = b0 + bb;
-        if (b < NEN) {
-          const T dNb[3] = {st[3 * b + 0], st[3 * b + 1], st[3 * b + 2]};
-          T Bb[6][3];
-          bmat(F, dNb, Bb);
-          const T geo = SdNa[0] * dNb[0] + SdNa[1] * dNb[1] + SdNa[2] * dNb[2];
-#pragma unroll
-          for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-              T acc = (j == k) ? geo : T(0);
-#pragma unroll
-              for (int I = 0; I < 6; ++I) acc += DB[I][j] * Bb[I][k];
-              K[bb][j * 3 + k] += acc;
-            }
-        }
-      }
-    }
-    if (kstage != nullptr) {
-      const int64_t base = (e * NEN + a) * NEN;
-#pragma unroll
-      for (int bb = 0; bb < BH; ++bb) {
-        const int b = b0 + bb;
-        if (b < NEN) {
-          const int d = dst_k[base + b];
-          if (d >= 0) {
-            T *o = kstage + (int64_t)d * 9;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) o[k] = K[bb][k];
-          }
-        }
+        for (int k = 0; k < 3; ++k) o[k] = K[b][k];
       }
     }
   }
   if (rstage != nullptr) {
     const int d = dst_r[e * NEN + a];
-    if (d >= 0) {
-      rstage[(int64_t)d * 3 + 0] = Ra[0];
-      rstage[(int64_t)d * 3 + 1] = Ra[1];
-      rstage[(int64_t)d * 3 + 2] = Ra[2];
-    }
+    if (d >= 0) rstage[(int64_t)d * 3 + wv] = Rj;
   }
 }
 
@@ -399,7 +384,7 @@ static void launch_rk_t(int n, const int32_t *conn, const double *X, const doubl
                         const int32_t *dk, T *ks, const int32_t *dr, double *rs, hipStream_t st) {
   constexpr int EPW = RkCfg<ET>::EPW;
   const int grid = (int)(((int64_t)n + EPW - 1) / EPW);
-  k_elem_rk<T, ET><<<grid, EPW * Elem<ET>::nen, 0, st>>>(n, conn, X, U, mp, dk, ks, dr, rs);
+  k_elem_rk<T, ET><<<grid, kRkThreads, 0, st>>>(n, conn, X, U, mp, dk, ks, dr, rs);
 }
 
 template <typename T>

@@ -264,10 +264,12 @@ class DevicePlan:
         """Tangent CSR values (``val_dtype``, default ``dtype``) and optionally the float64 residual.
 
         Replaces _assemble_generalized_Matrix + _assemble_reduced_Matrix
-        (assembly.py:311-387).  ``dtype`` is the element kernels' arithmetic and
-        staging precision; the staged blocks are summed in it and rounded once
-        to ``val_dtype``.  With ``with_residual`` the residual comes from the
-        same element pass (``dtype`` arithmetic, float64 output).
+        (assembly.py:311-387).  ``dtype`` is the precision of the element
+        contraction and of the staging; the staged blocks are summed in it and
+        rounded once to ``val_dtype``.  With ``with_residual`` the residual comes
+        from the same element pass, accumulated in float64 whatever ``dtype``;
+        a pressure facet computes in ``dtype``, so with ``dtype`` F32 its
+        residual comes from a second, float64 residual-only launch.
         """
         self._check_u(U)
         s = self._s()
@@ -280,9 +282,15 @@ class DevicePlan:
                 nat.ops().elem_residual_tangent(g.ntype, dtype, g.conn, self.X, U, g.params.tolist(), g.dst_k, ks,
                                                 g.dst_r if with_residual else None, rs)
             elif g.kind == 'pressure':
-                call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, self.X, U,
-                     float(g.load.pressure) * load_scale, g.dst_k, ks,
-                     g.dst_r if with_residual else None, rs, s)
+                pr = float(g.load.pressure) * load_scale
+                if dtype == F64 or not with_residual:
+                    call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, self.X, U, pr, g.dst_k, ks,
+                         g.dst_r if with_residual else None, rs, s)
+                else:
+                    call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, self.X, U, pr, g.dst_k, ks, None,
+                         None, s)
+                    call('fea_surface_pressure', g.ntype, F64, g.n, g.conn, self.X, U, pr, None, None, g.dst_r,
+                         rs, s)
             elif g.kind == 'traction' and with_residual:
                 t = np.asarray(g.load.traction, dtype=np.float64)
                 call('fea_surface_traction', g.ntype, g.n, g.conn, self.X, U,

@@ -16,8 +16,8 @@ round-off step below):
 
 * ``precision``: 'fp64' (everything FP64); 'mixed' (FP64 element arithmetic,
   tangent stored once-rounded in FP32, PCG on the FP32 matrix with FP64
-  vectors and recurrences); 'mixed-f32' (as 'mixed' but the element B^T D B
-  contraction in FP32 arithmetic); 'mixed-ir' (FP32 element contraction, FP32
+  vectors and recurrences); 'mixed-f32' (as 'mixed' but the element stiffness
+  contraction and its staging in FP32); 'mixed-ir' (FP32 element contraction, FP32
   inner PCG inside an FP64 iterative-refinement loop of ``ir_steps`` passes).
   In every mode the residual, energy, convergence test and J-terms are FP64.
 * ``rel_tol``: converge on ||R||_2 <= rel_tol ||R_ref||_2 with R_ref the
@@ -199,9 +199,8 @@ class StaticImplicitSolver(BaseSolver):
             return p.tangent(U, scale, F64)
         if self.precision == 'mixed':
             return p.tangent(U, scale, F64, val_dtype=F32)
-        R = p.residual(U, scale, F64)
-        _, K = p.tangent(U, scale, F32, with_residual=False)
-        return R, K
+        # FP32 contraction and staging; the residual of the same pass is float64
+        return p.tangent(U, scale, F32, val_dtype=F32)
 
     def _solve_linear(self, K, R, dGC0, iter_now, alpha0, tol=None):
         """static_implicit.py:297-350 (CG branch; the Pardiso branch is replaced by PCG)."""
