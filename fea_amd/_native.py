@@ -30,6 +30,10 @@ _SIGS = {
     'fea_pattern_export_blocks': (C_INT, [C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_pattern_export_group': (C_INT, [C_P, C_INT, C_P, C_P]),
     'fea_pattern_destroy': (None, [C_P]),
+    'fea_pattern_export_group_colpos': (C_INT, [C_P, C_INT, C_P]),
+    'fea_color_elements': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P]),
+    'fea_elem_tangent_direct': (C_INT, [C_INT, C_INT, C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P,
+                                        C_INT, C_P, C_P, C_P]),
     'fea_elem_residual_tangent': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_elem_residual': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_elem_energy': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P]),

@@ -64,11 +64,17 @@ __device__ __forceinline__ void rk_contract(const double (*dn)[Elem<ET>::nen][3]
   }
 }
 
-template <typename T, int ET>
+// DIRECT: the elements are order[0 .. n_elem) (one colour: no two share a node) and each
+// lane adds its row blocks into the CSR values (V) at rowptr[row] + colpos[(e,a,b)] + the
+// free-column offset, instead of writing them to the staging; every CSR row has one
+// writer per launch, and colours run one after another, so the sums are deterministic.
+template <typename T, typename V, int ET, bool DIRECT>
 __global__ __launch_bounds__(kRkThreads) void k_elem_rk(
-    int n_elem, const int32_t *__restrict__ conn, const double *__restrict__ X, const double *__restrict__ U,
-    MatParams mp, const int32_t *__restrict__ dst_k, T *__restrict__ kstage, const int32_t *__restrict__ dst_r,
-    double *__restrict__ rstage) {
+    int n_elem, const int32_t *__restrict__ order, const int32_t *__restrict__ conn,
+    const double *__restrict__ X, const double *__restrict__ U, MatParams mp, const int32_t *__restrict__ dst_k,
+    T *__restrict__ kstage, const int32_t *__restrict__ dst_r, double *__restrict__ rstage,
+    const int32_t *__restrict__ dof_map, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colpos,
+    V *__restrict__ val, bool triple) {
   constexpr int NEN = Elem<ET>::nen, NGP = Elem<ET>::ngp;
   constexpr int EPW = RkCfg<ET>::EPW;
   static_assert(EPW * NEN <= 64 && EPW * NGP <= 64, "one lane per (element, node) and per Gauss point");
@@ -82,12 +88,40 @@ __global__ __launch_bounds__(kRkThreads) void k_elem_rk(
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // the row j of this wave
   const int r = threadIdx.x & 63;
   const int le = r / NEN, a = r - le * NEN;
-  const int64_t e = (int64_t)blockIdx.x * EPW + le;
-  const bool valid = r < EPW * NEN && e < n_elem;
+  const int64_t i = (int64_t)blockIdx.x * EPW + le;
+  const bool valid = r < EPW * NEN && i < n_elem;
+  const int64_t e = !valid ? 0 : (DIRECT ? (int64_t)order[i] : i);
+  int64_t n = 0;
   if (valid) {
-    const int64_t n = conn[e * NEN + a];
+    n = conn[e * NEN + a];
     sx[le][a][wv] = X[3 * n + wv];
     su[le][a][wv] = U[3 * n + wv];
+  }
+  // DIRECT: the CSR entries this lane adds to, and their values, loaded now so the
+  // loads complete during the element arithmetic (no other lane of this launch writes
+  // them: one colour, one writer per row)
+  int64_t rb = -1;
+  int cp[NEN];
+  V old[NEN][3];
+  if (DIRECT && valid) {
+    const int row = dof_map[3 * n + wv];
+    if (row >= 0) {
+      rb = rowptr[row];
+      const int64_t base = (e * NEN + a) * NEN;
+#pragma unroll
+      for (int b = 0; b < NEN; ++b) cp[b] = colpos[base + b];
+#pragma unroll
+      for (int b = 0; b < NEN; ++b)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) old[b][k] = V(0);
+      if (triple) {
+#pragma unroll
+        for (int b = 0; b < NEN; ++b)
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            if (cp[b] >= 0) old[b][k] = val[rb + cp[b] + k];
+      }
+    }
   }
   __syncthreads();
   if (wv == 0 && r < EPW * NGP) {
@@ -137,7 +171,36 @@ __global__ __launch_bounds__(kRkThreads) void k_elem_rk(
     rk_contract<T, ET, 1>(dn, wp, sA[le], a, K, Rj);
   else
     rk_contract<T, ET, 2>(dn, wp, sA[le], a, K, Rj);
-  if (kstage != nullptr) {
+  if (DIRECT) {
+    if (rb >= 0) {
+      if (triple) {
+#pragma unroll
+        for (int b = 0; b < NEN; ++b)
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            if (cp[b] >= 0) val[rb + cp[b] + k] = (V)((T)old[b][k] + K[b][k]);
+      } else {
+        // partly fixed nodes: the free-column offset of each entry from dof_map
+        int64_t at[NEN][3];
+#pragma unroll
+        for (int b = 0; b < NEN; ++b) {
+          const int64_t nb = conn[e * NEN + b];
+          int lk = 0;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) at[b][k] = (cp[b] >= 0 && dof_map[3 * nb + k] >= 0) ? rb + cp[b] + lk++ : -1;
+        }
+#pragma unroll
+        for (int b = 0; b < NEN; ++b)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) old[b][k] = at[b][k] >= 0 ? val[at[b][k]] : V(0);
+#pragma unroll
+        for (int b = 0; b < NEN; ++b)
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            if (at[b][k] >= 0) val[at[b][k]] = (V)((T)old[b][k] + K[b][k]);
+      }
+    }
+  } else if (kstage != nullptr) {
     const int64_t base = (e * NEN + a) * NEN;
 #pragma unroll
     for (int b = 0; b < NEN; ++b) {
@@ -384,7 +447,31 @@ static void launch_rk_t(int n, const int32_t *conn, const double *X, const doubl
                         const int32_t *dk, T *ks, const int32_t *dr, double *rs, hipStream_t st) {
   constexpr int EPW = RkCfg<ET>::EPW;
   const int grid = (int)(((int64_t)n + EPW - 1) / EPW);
-  k_elem_rk<T, ET><<<grid, kRkThreads, 0, st>>>(n, conn, X, U, mp, dk, ks, dr, rs);
+  k_elem_rk<T, T, ET, false><<<grid, kRkThreads, 0, st>>>(n, nullptr, conn, X, U, mp, dk, ks, dr, rs, nullptr,
+                                                          nullptr, nullptr, nullptr, false);
+}
+
+template <typename T, typename V, int ET>
+static void launch_rk_direct_t(int n, const int32_t *order, const int32_t *conn, const double *X, const double *U,
+                               MatParams mp, const int32_t *dr, double *rs, const int32_t *dof_map,
+                               const int32_t *rowptr, const int32_t *colpos, V *val, bool triple, hipStream_t st) {
+  constexpr int EPW = RkCfg<ET>::EPW;
+  const int grid = (int)(((int64_t)n + EPW - 1) / EPW);
+  k_elem_rk<T, V, ET, true><<<grid, kRkThreads, 0, st>>>(n, order, conn, X, U, mp, nullptr, nullptr, dr, rs,
+                                                         dof_map, rowptr, colpos, val, triple);
+}
+
+template <typename T, typename V>
+static int launch_rk_direct(int et, int n, const int32_t *order, const int32_t *conn, const double *X,
+                            const double *U, MatParams mp, const int32_t *dr, double *rs, const int32_t *dof_map,
+                            const int32_t *rowptr, const int32_t *colpos, V *val, bool triple, hipStream_t st) {
+  switch (et) {
+    case 4: launch_rk_direct_t<T, V, 4>(n, order, conn, X, U, mp, dr, rs, dof_map, rowptr, colpos, val, triple, st); break;
+    case 8: launch_rk_direct_t<T, V, 8>(n, order, conn, X, U, mp, dr, rs, dof_map, rowptr, colpos, val, triple, st); break;
+    case 10: launch_rk_direct_t<T, V, 10>(n, order, conn, X, U, mp, dr, rs, dof_map, rowptr, colpos, val, triple, st); break;
+  }
+  FEA_CHECK_LAUNCH("fea_elem_tangent_direct");
+  return FEA_OK;
 }
 
 template <typename T>
@@ -435,6 +522,32 @@ extern "C" int fea_elem_residual_tangent(int elem_type, int dtype, int n_elem, c
                              rstage, as_stream(stream));
   return launch_rk<float>(elem_type, n_elem, conn, X, U, mp, dst_k, (float *)kstage, dst_r, rstage,
                           as_stream(stream));
+}
+
+extern "C" int fea_elem_tangent_direct(int elem_type, int dtype, int val_dtype, int n_elem, const int32_t *order,
+                                       const int32_t *conn, const double *X, const double *U,
+                                       const double *params_host, const int32_t *dof_map, const int32_t *rowptr,
+                                       const int32_t *colpos, void *val, int triple, const int32_t *dst_r,
+                                       double *rstage, void *stream) {
+  FEA_CHECK_ARG(valid_elem(elem_type), "fea_elem_tangent_direct: bad element type %d", elem_type);
+  FEA_CHECK_ARG((dtype == FEA_F64 && (val_dtype == FEA_F64 || val_dtype == FEA_F32)) ||
+                    (dtype == FEA_F32 && val_dtype == FEA_F32),
+                "fea_elem_tangent_direct: unsupported precision pair %d/%d", dtype, val_dtype);
+  FEA_CHECK_ARG(n_elem >= 0 && params_host != nullptr, "fea_elem_tangent_direct: bad arguments");
+  FEA_CHECK_ARG((rstage == nullptr) == (dst_r == nullptr), "fea_elem_tangent_direct: rstage/dst_r mismatch");
+  if (n_elem == 0) return FEA_OK;
+  FEA_CHECK_ARG(order && conn && X && U && dof_map && rowptr && colpos && val,
+                "fea_elem_tangent_direct: null input");
+  const MatParams mp = mat_from(params_host);
+  hipStream_t st = as_stream(stream);
+  if (dtype == FEA_F32)
+    return launch_rk_direct<float, float>(elem_type, n_elem, order, conn, X, U, mp, dst_r, rstage, dof_map, rowptr,
+                                          colpos, (float *)val, triple != 0, st);
+  if (val_dtype == FEA_F32)
+    return launch_rk_direct<double, float>(elem_type, n_elem, order, conn, X, U, mp, dst_r, rstage, dof_map, rowptr,
+                                           colpos, (float *)val, triple != 0, st);
+  return launch_rk_direct<double, double>(elem_type, n_elem, order, conn, X, U, mp, dst_r, rstage, dof_map, rowptr,
+                                          colpos, (double *)val, triple != 0, st);
 }
 
 extern "C" int fea_elem_residual(int elem_type, int dtype, int n_elem, const int32_t *conn,

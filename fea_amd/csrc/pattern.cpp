@@ -327,4 +327,80 @@ extern "C" int fea_pattern_export_group(const fea_pattern *p, int group, int32_t
   return FEA_OK;
 }
 
+// Column position of every (element, a, b) block of a group in its CSR rows: the block's
+// blk_colpos, found from the staging slot through kseg; -1 where the staging map is -1.
+extern "C" int fea_pattern_export_group_colpos(const fea_pattern *p, int group, int32_t *colpos_host) {
+  if (p == nullptr || group < 0 || group >= (int)p->dst_k.size() || colpos_host == nullptr) {
+    fea::set_error("fea_pattern_export_group_colpos: bad argument");
+    return FEA_ERR_ARG;
+  }
+  const auto &dk = p->dst_k[group];
+  const auto &ks = p->kseg;
+  for (size_t t = 0; t < dk.size(); ++t) {
+    const int32_t d = dk[t];
+    if (d < 0) {
+      colpos_host[t] = -1;
+      continue;
+    }
+    // the block q with kseg[q] <= d < kseg[q + 1]
+    const int64_t q = (int64_t)(std::upper_bound(ks.begin(), ks.end(), d) - ks.begin()) - 1;
+    colpos_host[t] = p->blk_colpos[q];
+  }
+  return FEA_OK;
+}
+
 extern "C" void fea_pattern_destroy(fea_pattern *p) { delete p; }
+
+// Greedy colouring of elements: no two elements of one colour share a node; elements
+// are taken in order and get the smallest colour none of their already coloured
+// neighbours holds (a structured brick grid in natural order gets 8).
+extern "C" int fea_color_elements(int n_nodes, int n_elem, int nen, const int32_t *conn, int32_t *color,
+                                  int32_t *n_colors) {
+  if (n_nodes < 0 || n_elem < 0 || nen <= 0 || (n_elem > 0 && (conn == nullptr || color == nullptr)) ||
+      n_colors == nullptr) {
+    fea::set_error("fea_color_elements: bad arguments");
+    return FEA_ERR_ARG;
+  }
+  try {
+    std::vector<int64_t> ptr(n_nodes + 1, 0);
+    for (int64_t t = 0; t < (int64_t)n_elem * nen; ++t) {
+      const int32_t n = conn[t];
+      if (n < 0 || n >= n_nodes) {
+        fea::set_error("fea_color_elements: node id %d out of range", (int)n);
+        return FEA_ERR_ARG;
+      }
+      ++ptr[n + 1];
+    }
+    for (int i = 0; i < n_nodes; ++i) ptr[i + 1] += ptr[i];
+    std::vector<int32_t> inc(ptr[n_nodes]);
+    {
+      std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+      for (int e = 0; e < n_elem; ++e)
+        for (int a = 0; a < nen; ++a) inc[fill[conn[(int64_t)e * nen + a]]++] = e;
+    }
+    std::vector<int32_t> stamp;  // stamp[c] == e: colour c is taken by a neighbour of e
+    int32_t nc = 0;
+    for (int e = 0; e < n_elem; ++e) color[e] = -1;
+    for (int e = 0; e < n_elem; ++e) {
+      for (int a = 0; a < nen; ++a) {
+        const int32_t n = conn[(int64_t)e * nen + a];
+        for (int64_t s = ptr[n]; s < ptr[n + 1]; ++s) {
+          const int32_t c = color[inc[s]];
+          if (c >= 0) stamp[c] = e;
+        }
+      }
+      int32_t c = 0;
+      while (c < nc && stamp[c] == e) ++c;
+      if (c == nc) {
+        stamp.push_back(-1);
+        ++nc;
+      }
+      color[e] = c;
+    }
+    *n_colors = nc;
+  } catch (const std::bad_alloc &) {
+    fea::set_error("fea_color_elements: out of host memory");
+    return FEA_ERR_STATE;
+  }
+  return FEA_OK;
+}

@@ -88,6 +88,42 @@ void elem_residual_tangent(int64_t elem_type, int64_t dtype, const Tensor &conn,
                "elem_residual_tangent");
 }
 
+void elem_tangent_direct(int64_t elem_type, int64_t dtype, int64_t val_dtype, const Tensor &order, const Tensor &conn,
+                         const Tensor &X, const Tensor &U, c10::ArrayRef<double> params, const Tensor &dof_map,
+                         const Tensor &rowptr, const Tensor &colpos, const Tensor &val, bool triple,
+                         const c10::optional<Tensor> &dst_r, const c10::optional<Tensor> &rstage) {
+  const int nen = nen_of(elem_type);
+  float_type(dtype);
+  check_dev(order, at::kInt, X, "order");
+  check_dev(conn, at::kInt, X, "conn");
+  check_dev(X, at::kDouble, X, "X");
+  check_dev(U, at::kDouble, X, "U");
+  check_dev(dof_map, at::kInt, X, "dof_map");
+  check_dev(rowptr, at::kInt, X, "rowptr");
+  check_dev(colpos, at::kInt, X, "colpos");
+  check_dev(val, float_type(val_dtype), X, "val");
+  check_nodes(X, U);
+  check_params(params);
+  const int64_t n_all = elems_of(conn, nen);
+  TORCH_CHECK(colpos.numel() == n_all * nen * nen, "colpos: one entry per element node pair");
+  TORCH_CHECK(dof_map.numel() == X.numel(), "dof_map: one entry per nodal DOF");
+  TORCH_CHECK(order.numel() <= n_all, "order: at most one entry per element");
+  const bool with_r = dst_r.has_value() && dst_r->defined();
+  TORCH_CHECK(with_r == (rstage.has_value() && rstage->defined()), "dst_r and rstage: both or neither");
+  if (with_r) {
+    check_dev(*dst_r, at::kInt, X, "dst_r");
+    check_dev(*rstage, at::kDouble, X, "rstage");
+    TORCH_CHECK(dst_r->numel() == n_all * nen, "dst_r: one slot per element node");
+  }
+  check_status(fea_elem_tangent_direct((int)elem_type, (int)dtype, (int)val_dtype, (int)order.numel(),
+                                       order.data_ptr<int32_t>(), conn.data_ptr<int32_t>(), X.data_ptr<double>(),
+                                       U.data_ptr<double>(), params.data(), dof_map.data_ptr<int32_t>(),
+                                       rowptr.data_ptr<int32_t>(), colpos.data_ptr<int32_t>(), val.data_ptr(),
+                                       triple ? 1 : 0, with_r ? dst_r->data_ptr<int32_t>() : nullptr,
+                                       with_r ? rstage->data_ptr<double>() : nullptr, stream_of(X)),
+               "elem_tangent_direct");
+}
+
 void elem_residual(int64_t elem_type, int64_t dtype, const Tensor &conn, const Tensor &X, const Tensor &U,
                    c10::ArrayRef<double> params, const Tensor &dst_r, const Tensor &rstage) {
   const int nen = nen_of(elem_type);
@@ -291,6 +327,9 @@ void block_jacobi_inv(int64_t mat_dtype, int64_t n_groups, const Tensor &grp, co
 TORCH_LIBRARY(fea, m) {
   m.def("elem_residual_tangent(int elem_type, int dtype, Tensor conn, Tensor X, Tensor U, float[] params, "
         "Tensor dst_k, Tensor(a!) kstage, Tensor? dst_r, Tensor(b!)? rstage) -> ()");
+  m.def("elem_tangent_direct(int elem_type, int dtype, int val_dtype, Tensor order, Tensor conn, Tensor X, "
+        "Tensor U, float[] params, Tensor dof_map, Tensor rowptr, Tensor colpos, Tensor(a!) val, bool triple, "
+        "Tensor? dst_r, Tensor(b!)? rstage) -> ()");
   m.def("elem_residual(int elem_type, int dtype, Tensor conn, Tensor X, Tensor U, float[] params, Tensor dst_r, "
         "Tensor(a!) rstage) -> ()");
   m.def("elem_energy(int elem_type, Tensor conn, Tensor X, Tensor U, float[] params, Tensor(a!) partial) -> ()");
@@ -313,6 +352,7 @@ TORCH_LIBRARY(fea, m) {
 TORCH_LIBRARY_IMPL(fea, CUDA, m) {
   m.impl("elem_residual_tangent", &elem_residual_tangent);
   m.impl("elem_residual", &elem_residual);
+  m.impl("elem_tangent_direct", &elem_tangent_direct);
   m.impl("elem_energy", &elem_energy);
   m.impl("gather_csr", &gather_csr);
   m.impl("gather_residual", &gather_residual);

@@ -26,6 +26,12 @@ from ._native import F32, F64, call, ptr
 
 _TORCH_DT = {F32: torch.float32, F64: torch.float64}
 
+# Direct element assembly (fea_elem_tangent_direct: element colours adding their blocks
+# into the CSR, no element staging) for plans built while this is True.  Off by default:
+# on the 131k-brick beam the eight colour launches (0.83 ms per tangent) are slower than
+# staging and the gather (0.62 ms), although they move a fraction of the bytes.
+DIRECT_ASSEMBLY = False
+
 
 def _i32(a, device):
     return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
@@ -173,6 +179,7 @@ class DevicePlan:
             self.diag_slot = _i32(diag, dev)
             self.blk_a, self.blk_b, self.blk_colpos = _i32(blk_a, dev), _i32(blk_b, dev), _i32(blk_cp, dev)
             self.kseg, self.rseg = _i32(kseg, dev), _i32(rseg, dev)
+            rest_dk = {}
             for gi, g in enumerate(self.groups):
                 dk = np.zeros(max(1, g.n * g.ntype * g.ntype), dtype=np.int32) if g.in_k else None
                 dr = np.zeros(max(1, g.n * g.ntype), dtype=np.int32)
@@ -181,6 +188,39 @@ class DevicePlan:
                 g.conn = _i32(g.conn_np if g.n else np.zeros((1, g.ntype)), dev)
                 g.dst_k = None if dk is None else _i32(dk, dev)
                 g.dst_r = _i32(dr, dev)
+                g.colpos = g.order = None
+                if DIRECT_ASSEMBLY and g.kind == 'elem' and g.in_k and g.n > 0:
+                    # direct assembly: the CSR column of each (element, a, b) block and a
+                    # colouring of the elements (no two of a colour share a node)
+                    cp = np.zeros(g.n * g.ntype * g.ntype, dtype=np.int32)
+                    nat.check(L.fea_pattern_export_group_colpos(handle, gi, cp.ctypes.data),
+                              'fea_pattern_export_group_colpos')
+                    color = np.zeros(g.n, dtype=np.int32)
+                    nc = ctypes.c_int32(0)
+                    nat.check(L.fea_color_elements(self.n_nodes, g.n, g.ntype, g.conn_np.ctypes.data,
+                                                   color.ctypes.data, ctypes.byref(nc)), 'fea_color_elements')
+                    g.colpos = _i32(cp, dev)
+                    g.order = _i32(np.argsort(color, kind='stable'), dev)
+                    g.color_ptr = np.concatenate([[0], np.cumsum(np.bincount(color, minlength=nc.value))]).tolist()
+                elif g.in_k and dk is not None:
+                    rest_dk[gi] = dk
+            # direct assembly of the element stiffness when every element group has its
+            # colouring (set False to use the staged path on such a plan)
+            elems = [g for g in self.groups if g.kind == 'elem' and g.n > 0]
+            self.direct_assembly = bool(elems) and all(g.colpos is not None for g in elems)
+            # the other stiffness groups (pressure / traction facets, contact pairs) then keep
+            # a staging of their own: their slots ranked in order, which keeps each block's
+            # (group, element, a, b) order, with kseg_rest the rank of each block's start
+            self.n_kstage_rest, self.kseg_rest = 0, None
+            for g in self.groups:
+                g.dst_k_rest = None
+            if self.direct_assembly:
+                slots = np.sort(np.concatenate([d[d >= 0] for d in rest_dk.values()]) if rest_dk
+                                else np.zeros(0, dtype=np.int32))
+                self.n_kstage_rest = int(slots.size)
+                self.kseg_rest = _i32(np.searchsorted(slots, kseg), dev)
+                for gi, d in rest_dk.items():
+                    self.groups[gi].dst_k_rest = _i32(np.where(d >= 0, np.searchsorted(slots, d), -1), dev)
         finally:
             L.fea_pattern_destroy(handle)
 
@@ -211,6 +251,14 @@ class DevicePlan:
             raise ValueError('X must have %d entries' % (3 * self.n_nodes))
         self.X.view(-1).copy_(X)
         self._mass = None
+
+    def kstage_rest(self, dtype):
+        """Staging of the non-element stiffness groups under direct assembly."""
+        key = ('rest', dtype)
+        if key not in self._kstage:
+            self._kstage[key] = torch.zeros(max(1, 9 * self.n_kstage_rest), dtype=_TORCH_DT[dtype],
+                                            device=self.device)
+        return self._kstage[key]
 
     def kstage(self, dtype):
         if dtype not in self._kstage:
@@ -273,21 +321,25 @@ class DevicePlan:
         """
         self._check_u(U)
         s = self._s()
-        ks = self.kstage(dtype)
+        direct = self.direct_assembly
+        val_dtype = dtype if val_dtype is None else val_dtype
+        ks = self.kstage_rest(dtype) if direct else self.kstage(dtype)
         rs = self.rstage if with_residual else None
         for g in self.groups:
             if g.n == 0:
                 continue
+            dk = g.dst_k_rest if direct else g.dst_k
             if g.kind == 'elem':
-                nat.ops().elem_residual_tangent(g.ntype, dtype, g.conn, self.X, U, g.params.tolist(), g.dst_k, ks,
-                                                g.dst_r if with_residual else None, rs)
+                if not direct:
+                    nat.ops().elem_residual_tangent(g.ntype, dtype, g.conn, self.X, U, g.params.tolist(), g.dst_k,
+                                                    ks, g.dst_r if with_residual else None, rs)
             elif g.kind == 'pressure':
                 pr = float(g.load.pressure) * load_scale
                 if dtype == F64 or not with_residual:
-                    call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, self.X, U, pr, g.dst_k, ks,
+                    call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, self.X, U, pr, dk, ks,
                          g.dst_r if with_residual else None, rs, s)
                 else:
-                    call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, self.X, U, pr, g.dst_k, ks, None,
+                    call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, self.X, U, pr, dk, ks, None,
                          None, s)
                     call('fea_surface_pressure', g.ntype, F64, g.n, g.conn, self.X, U, pr, None, None, g.dst_r,
                          rs, s)
@@ -296,11 +348,24 @@ class DevicePlan:
                 call('fea_surface_traction', g.ntype, g.n, g.conn, self.X, U,
                      t, float(load_scale), g.dst_r, self.rstage, None, s)
             elif g.kind == 'barrier':
-                g.load.assemble(U, g.dst_k, ks, dtype, g.dst_r if with_residual else None, rs)
-        val_dtype = dtype if val_dtype is None else val_dtype
+                g.load.assemble(U, dk, ks, dtype, g.dst_r if with_residual else None, rs)
         val = self.csr_values(val_dtype)
-        nat.ops().gather_csr(dtype, val_dtype, self.blk_a, self.blk_b, self.blk_colpos, self.kseg, self.dof_map,
-                             self.rowptr, ks, val)
+        if direct and self.n_kstage_rest == 0:
+            val.zero_()                     # the element colours add into it
+        else:
+            # staged sums (under direct assembly only the non-element groups', zero elsewhere:
+            # this also clears the values for the element colours, which add to them)
+            nat.ops().gather_csr(dtype, val_dtype, self.blk_a, self.blk_b, self.blk_colpos,
+                                 self.kseg_rest if direct else self.kseg, self.dof_map, self.rowptr, ks, val)
+        if direct:
+            for g in self.groups:
+                if g.kind != 'elem' or g.n == 0:
+                    continue
+                params = g.params.tolist()
+                for c in range(len(g.color_ptr) - 1):
+                    nat.ops().elem_tangent_direct(g.ntype, dtype, val_dtype, g.order[g.color_ptr[c]:g.color_ptr[c + 1]],
+                                                  g.conn, self.X, U, params, self.dof_map, self.rowptr, g.colpos, val,
+                                                  self.triple, g.dst_r if with_residual else None, rs)
         R = None
         if with_residual:
             R = self.R

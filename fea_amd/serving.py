@@ -383,11 +383,13 @@ class SolveServer:
     def _actual_footprint(self, s):
         p = s.plan
         ts = [p.X, p.dof_map, p.rowptr, p.col, p.diag_slot, p.grp, p.blk_a, p.blk_b, p.blk_colpos, p.kseg,
-              p.rseg, p.rstage, p.R, s.U, s.U_prev, s.Unew, s.dGC]
+              p.kseg_rest, p.rseg, p.rstage, p.R, s.U, s.U_prev, s.Unew, s.dGC]
         for g in p.groups:
-            ts += [t for t in (g.conn, g.dst_k, g.dst_r) if t is not None]
-        nb = sum(t.numel() * t.element_size() for t in ts)
-        nb += 9 * p.n_kstage * 8 + p.nnz * (8 if self.precision == 'fp64' else 4)   # F64 staging, CSR values
+            ts += [g.conn, g.dst_k, g.dst_r, g.colpos, g.order, g.dst_k_rest]
+        nb = sum(t.numel() * t.element_size() for t in ts if t is not None)
+        # F64 staging (the non-element groups' only, under direct assembly), CSR values
+        stage = p.n_kstage_rest if p.direct_assembly else p.n_kstage
+        nb += 9 * stage * 8 + p.nnz * (8 if self.precision == 'fp64' else 4)
         return int(nb + s.vec.nbytes())
 
     def _retire(self, slot, status, now, reason=''):

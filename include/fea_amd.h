@@ -90,7 +90,17 @@ int fea_pattern_export_blocks(const fea_pattern *p, int32_t *blk_a_host, int32_t
  * free x free entry), dst_r[n_elem*nen] (-1 = node with no free DOF) */
 int fea_pattern_export_group(const fea_pattern *p, int group, int32_t *dst_k_host,
                              int32_t *dst_r_host);
+/* colpos[n_elem*nen*nen] of an in_k group: the column position of its (element, a, b)
+ * block inside the CSR rows of node a (blk_colpos of the block), -1 where dst_k is -1;
+ * the map of fea_elem_tangent_direct.  Replaces, with the colouring below, the
+ * scatter-add of element blocks into the global matrix (C3base.py:449,
+ * assembly.py:347-380). */
+int fea_pattern_export_group_colpos(const fea_pattern *p, int group, int32_t *colpos_host);
 void fea_pattern_destroy(fea_pattern *p);
+/* Greedy element colouring: color[e] in 0..n_colors-1, no two elements of one colour
+ * share a node (elements in order, smallest colour free of coloured neighbours). */
+int fea_color_elements(int n_nodes, int n_elem, int nen, const int32_t *conn, int32_t *color,
+                       int32_t *n_colors);
 
 /* ---------------------------------------------------------------- elements --
  * fea_elem_residual_tangent replaces Element_3D.structural_Force
@@ -107,6 +117,21 @@ int fea_elem_residual_tangent(int elem_type, int dtype, int n_elem, const int32_
                               const double *X, const double *U, const double *params_host,
                               const int32_t *dst_k, void *kstage, const int32_t *dst_r,
                               double *rstage, void *stream);
+
+/* fea_elem_residual_tangent for the elements order[0 .. n_elem) of one colour (no
+ * two share a node), adding each 3x3 block K_ab into the CSR values val (val_dtype)
+ * at rowptr[row] + colpos[(e*nen + a)*nen + b] + free-column offset instead of
+ * staging it; the residual goes to rstage as there.  Colours launched one after
+ * another on one stream give deterministic sums.  dtype/val_dtype: F64/F64, F64/F32
+ * (float64 contraction, each block added into float32 values) or F32/F32.  triple != 0
+ * asserts every node is fully free or fully fixed (the free-column offset of a block
+ * entry is then its component; dof_map is read for the rows only).  Replaces
+ * structural_Force + the scatter-add of C3base.py:413-452 and assembly.py:347-380. */
+int fea_elem_tangent_direct(int elem_type, int dtype, int val_dtype, int n_elem, const int32_t *order,
+                            const int32_t *conn, const double *X, const double *U,
+                            const double *params_host, const int32_t *dof_map, const int32_t *rowptr,
+                            const int32_t *colpos, void *val, int triple, const int32_t *dst_r,
+                            double *rstage, void *stream);
 
 /* Residual only (force branch of structural_Force, C3base.py:428-431), float64 or
  * float32 arithmetic, float64 output into rstage. */

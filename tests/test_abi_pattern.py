@@ -33,7 +33,7 @@ def test_binding_covers_header():
     assert set(header_functions()) <= set(_native.exported_symbols())
 
 
-TORCH_OPS = ['elem_residual_tangent', 'elem_residual', 'elem_energy', 'gather_csr', 'gather_residual', 'spmv',
+TORCH_OPS = ['elem_residual_tangent', 'elem_tangent_direct', 'elem_residual', 'elem_energy', 'gather_csr', 'gather_residual', 'spmv',
              'pcg_start', 'pcg_iterate', 'jacobi_inv', 'block_jacobi_inv']
 
 
@@ -84,8 +84,10 @@ def build_pattern(model):
     for gi, c in enumerate(conns):
         dk = np.zeros(c.shape[0] * c.shape[1] ** 2, dtype=np.int32)
         dr = np.zeros(c.shape[0] * c.shape[1], dtype=np.int32)
+        cp = np.zeros(c.shape[0] * c.shape[1] ** 2, dtype=np.int32)
         L.fea_pattern_export_group(h, gi, dk.ctypes.data, dr.ctypes.data)
-        groups.append((dk, dr))
+        _native.check(L.fea_pattern_export_group_colpos(h, gi, cp.ctypes.data), 'colpos')
+        groups.append((dk, dr, cp))
     L.fea_pattern_destroy(h)
     return dict(n_free=n_free, nnz=nnz, rowptr=rowptr, col=col, diag=diag, blk=blk, kseg=kseg, rseg=rseg,
                 groups=groups, dof=dof, nks=nks, nrs=nrs)
@@ -171,3 +173,52 @@ def test_pattern_empty_model():
     L.fea_pattern_sizes(h, sizes.ctypes.data)
     assert sizes.sum() == 0
     L.fea_pattern_destroy(h)
+
+
+def color_elements(conn, n_nodes):
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    color = np.zeros(conn.shape[0], dtype=np.int32)
+    nc = ctypes.c_int32(0)
+    _native.check(_native.lib().fea_color_elements(n_nodes, conn.shape[0], conn.shape[1], conn.ctypes.data,
+                                                   color.ctypes.data, ctypes.byref(nc)), 'color')
+    return color, nc.value
+
+
+@pytest.mark.parametrize('ci', range(3))
+def test_element_colours_and_direct_columns(ci):
+    """fea_color_elements: no two elements of a colour share a node (a structured brick
+    grid gets its 8 parity colours); fea_pattern_export_group_colpos: rowptr[row] +
+    colpos + the free-column offset of every (element, a, b) block is the CSR entry of
+    that row and column -- the addresses fea_elem_tangent_direct adds into."""
+    model = cases()[ci]
+    conn = model['groups'][0]['conn']
+    N = model['nodes'].shape[0]
+    color, nc = color_elements(conn, N)
+    assert color.min() == 0 and color.max() == nc - 1
+    for c in range(nc):
+        nodes = conn[color == c].reshape(-1)
+        assert np.unique(nodes).size == nodes.size
+    if ci == 1:
+        assert nc == 8
+    P = build_pattern(model)
+    dof, rowptr, col = P['dof'], P['rowptr'], P['col']
+    dk, _, cp = P['groups'][0]
+    nen = conn.shape[1]
+    cp = cp.reshape(-1, nen, nen)
+    assert np.array_equal(cp >= 0, dk.reshape(-1, nen, nen) >= 0)
+    checked = 0
+    for e in range(conn.shape[0]):
+        for a in range(nen):
+            for b in range(nen):
+                if cp[e, a, b] < 0:
+                    continue
+                A, B = conn[e, a], conn[e, b]
+                fc = [dof[3 * B + k] for k in range(3) if dof[3 * B + k] >= 0]
+                for j in range(3):
+                    r = dof[3 * A + j]
+                    if r < 0:
+                        continue
+                    base = rowptr[r] + cp[e, a, b]
+                    assert list(col[base:base + len(fc)]) == fc
+                    checked += 1
+    assert checked > 0

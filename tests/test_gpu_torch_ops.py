@@ -39,8 +39,14 @@ def test_tangent_ops_match_ctypes(gpu, etype, dtype, val_dtype):
         if g.kind == 'elem':
             call('fea_elem_residual_tangent', g.ntype, dtype, g.n, g.conn, p.X, U, g.params, g.dst_k, ks, g.dst_r, rs, s)
         elif g.kind == 'pressure':
-            call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, p.X, U, float(g.load.pressure), g.dst_k, ks,
-                 g.dst_r, rs, s)
+            if dtype == F64:
+                call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, p.X, U, float(g.load.pressure), g.dst_k,
+                     ks, g.dst_r, rs, s)
+            else:                           # float32 stiffness, float64 residual
+                call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, p.X, U, float(g.load.pressure), g.dst_k,
+                     ks, None, None, s)
+                call('fea_surface_pressure', g.ntype, F64, g.n, g.conn, p.X, U, float(g.load.pressure), None, None,
+                     g.dst_r, rs, s)
         elif g.kind == 'traction':
             call('fea_surface_traction', g.ntype, g.n, g.conn, p.X, U, np.asarray(g.load.traction, dtype=np.float64),
                  1.0, g.dst_r, rs, None, s)
