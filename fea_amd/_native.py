@@ -14,6 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, '_lib', 'libfea_amd.so')
 
 F32, F64 = 0, 1
+F64_STAGE_F32 = 2      # element tangent: float64 arithmetic, float32 staged blocks
 C_INT, C_I64, C_DBL, C_P = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
 
 

@@ -64,6 +64,7 @@ __device__ __forceinline__ void rk_contract(const double (*dn)[Elem<ET>::nen][3]
   }
 }
 
+// V is the type of the staged blocks (staged path) or of the CSR values (DIRECT).
 // DIRECT: the elements are order[0 .. n_elem) (one colour: no two share a node) and each
 // lane adds its row blocks into the CSR values (V) at rowptr[row] + colpos[(e,a,b)] + the
 // free-column offset, instead of writing them to the staging; every CSR row has one
@@ -72,7 +73,7 @@ template <typename T, typename V, int ET, bool DIRECT>
 __global__ __launch_bounds__(kRkThreads) void k_elem_rk(
     int n_elem, const int32_t *__restrict__ order, const int32_t *__restrict__ conn,
     const double *__restrict__ X, const double *__restrict__ U, MatParams mp, const int32_t *__restrict__ dst_k,
-    T *__restrict__ kstage, const int32_t *__restrict__ dst_r, double *__restrict__ rstage,
+    V *__restrict__ kstage, const int32_t *__restrict__ dst_r, double *__restrict__ rstage,
     const int32_t *__restrict__ dof_map, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ colpos,
     V *__restrict__ val, bool triple) {
   constexpr int NEN = Elem<ET>::nen, NGP = Elem<ET>::ngp;
@@ -206,9 +207,9 @@ __global__ __launch_bounds__(kRkThreads) void k_elem_rk(
     for (int b = 0; b < NEN; ++b) {
       const int d = dst_k[base + b];
       if (d >= 0) {
-        T *o = kstage + (int64_t)d * 9 + 3 * wv;
+        V *o = kstage + (int64_t)d * 9 + 3 * wv;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) o[k] = K[b][k];
+        for (int k = 0; k < 3; ++k) o[k] = (V)K[b][k];
       }
     }
   }
@@ -442,12 +443,12 @@ static bool valid_elem(int et) { return et == 4 || et == 8 || et == 10; }
 
 static MatParams mat_from(const double *p) { return MatParams{p[0], p[1], p[2]}; }
 
-template <typename T, int ET>
+template <typename T, typename S, int ET>
 static void launch_rk_t(int n, const int32_t *conn, const double *X, const double *U, MatParams mp,
-                        const int32_t *dk, T *ks, const int32_t *dr, double *rs, hipStream_t st) {
+                        const int32_t *dk, S *ks, const int32_t *dr, double *rs, hipStream_t st) {
   constexpr int EPW = RkCfg<ET>::EPW;
   const int grid = (int)(((int64_t)n + EPW - 1) / EPW);
-  k_elem_rk<T, T, ET, false><<<grid, kRkThreads, 0, st>>>(n, nullptr, conn, X, U, mp, dk, ks, dr, rs, nullptr,
+  k_elem_rk<T, S, ET, false><<<grid, kRkThreads, 0, st>>>(n, nullptr, conn, X, U, mp, dk, ks, dr, rs, nullptr,
                                                           nullptr, nullptr, nullptr, false);
 }
 
@@ -474,14 +475,14 @@ static int launch_rk_direct(int et, int n, const int32_t *order, const int32_t *
   return FEA_OK;
 }
 
-template <typename T>
+template <typename T, typename S>
 static int launch_rk(int et, int n, const int32_t *conn, const double *X, const double *U,
-                     MatParams mp, const int32_t *dk, T *ks, const int32_t *dr, double *rs,
+                     MatParams mp, const int32_t *dk, S *ks, const int32_t *dr, double *rs,
                      hipStream_t st) {
   switch (et) {
-    case 4: launch_rk_t<T, 4>(n, conn, X, U, mp, dk, ks, dr, rs, st); break;
-    case 8: launch_rk_t<T, 8>(n, conn, X, U, mp, dk, ks, dr, rs, st); break;
-    case 10: launch_rk_t<T, 10>(n, conn, X, U, mp, dk, ks, dr, rs, st); break;
+    case 4: launch_rk_t<T, S, 4>(n, conn, X, U, mp, dk, ks, dr, rs, st); break;
+    case 8: launch_rk_t<T, S, 8>(n, conn, X, U, mp, dk, ks, dr, rs, st); break;
+    case 10: launch_rk_t<T, S, 10>(n, conn, X, U, mp, dk, ks, dr, rs, st); break;
   }
   FEA_CHECK_LAUNCH("fea_elem_residual_tangent");
   return FEA_OK;
@@ -510,7 +511,8 @@ extern "C" int fea_elem_residual_tangent(int elem_type, int dtype, int n_elem, c
                                          void *kstage, const int32_t *dst_r, double *rstage,
                                          void *stream) {
   FEA_CHECK_ARG(valid_elem(elem_type), "fea_elem_residual_tangent: bad element type %d", elem_type);
-  FEA_CHECK_ARG(dtype == FEA_F32 || dtype == FEA_F64, "fea_elem_residual_tangent: bad dtype %d", dtype);
+  FEA_CHECK_ARG(dtype == FEA_F32 || dtype == FEA_F64 || dtype == FEA_F64_STAGE_F32,
+                "fea_elem_residual_tangent: bad dtype %d", dtype);
   FEA_CHECK_ARG(n_elem >= 0 && params_host != nullptr, "fea_elem_residual_tangent: bad arguments");
   FEA_CHECK_ARG((kstage == nullptr) == (dst_k == nullptr), "fea_elem_residual_tangent: kstage/dst_k mismatch");
   FEA_CHECK_ARG((rstage == nullptr) == (dst_r == nullptr), "fea_elem_residual_tangent: rstage/dst_r mismatch");
@@ -518,10 +520,13 @@ extern "C" int fea_elem_residual_tangent(int elem_type, int dtype, int n_elem, c
   FEA_CHECK_ARG(conn && X && U, "fea_elem_residual_tangent: null input");
   const MatParams mp = mat_from(params_host);
   if (dtype == FEA_F64)
-    return launch_rk<double>(elem_type, n_elem, conn, X, U, mp, dst_k, (double *)kstage, dst_r,
-                             rstage, as_stream(stream));
-  return launch_rk<float>(elem_type, n_elem, conn, X, U, mp, dst_k, (float *)kstage, dst_r, rstage,
-                          as_stream(stream));
+    return launch_rk<double, double>(elem_type, n_elem, conn, X, U, mp, dst_k, (double *)kstage, dst_r,
+                                     rstage, as_stream(stream));
+  if (dtype == FEA_F64_STAGE_F32)
+    return launch_rk<double, float>(elem_type, n_elem, conn, X, U, mp, dst_k, (float *)kstage, dst_r,
+                                    rstage, as_stream(stream));
+  return launch_rk<float, float>(elem_type, n_elem, conn, X, U, mp, dst_k, (float *)kstage, dst_r, rstage,
+                                 as_stream(stream));
 }
 
 extern "C" int fea_elem_tangent_direct(int elem_type, int dtype, int val_dtype, int n_elem, const int32_t *order,

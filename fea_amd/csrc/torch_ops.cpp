@@ -68,7 +68,7 @@ void elem_residual_tangent(int64_t elem_type, int64_t dtype, const Tensor &conn,
   check_dev(X, at::kDouble, X, "X");
   check_dev(U, at::kDouble, X, "U");
   check_dev(dst_k, at::kInt, X, "dst_k");
-  check_dev(kstage, float_type(dtype), X, "kstage");
+  check_dev(kstage, dtype == FEA_F64_STAGE_F32 ? at::kFloat : float_type(dtype), X, "kstage");
   check_nodes(X, U);
   check_params(params);
   const int64_t n = elems_of(conn, nen);

@@ -308,22 +308,27 @@ class DevicePlan:
             elif g.kind == 'barrier':
                 g.load.assemble(U, None, None, F64, g.dst_r, self.rstage)
 
-    def tangent(self, U, load_scale=1.0, dtype=F64, with_residual=True, val_dtype=None):
+    def tangent(self, U, load_scale=1.0, dtype=F64, with_residual=True, val_dtype=None, stage_dtype=None):
         """Tangent CSR values (``val_dtype``, default ``dtype``) and optionally the float64 residual.
 
         Replaces _assemble_generalized_Matrix + _assemble_reduced_Matrix
         (assembly.py:311-387).  ``dtype`` is the precision of the element
-        contraction and of the staging; the staged blocks are summed in it and
-        rounded once to ``val_dtype``.  With ``with_residual`` the residual comes
-        from the same element pass, accumulated in float64 whatever ``dtype``;
-        a pressure facet computes in ``dtype``, so with ``dtype`` F32 its
-        residual comes from a second, float64 residual-only launch.
+        contraction, ``stage_dtype`` (default ``dtype``; F32 under F64 rounds each
+        element block once as it is staged) that of the staged blocks, which are
+        summed in it and rounded once to ``val_dtype``.  With ``with_residual`` the
+        residual comes from the same element pass, accumulated in float64 whatever
+        ``dtype``; a pressure facet computes in the staging precision, so with F32
+        staging its residual comes from a second, float64 residual-only launch.
         """
         self._check_u(U)
         s = self._s()
         direct = self.direct_assembly
         val_dtype = dtype if val_dtype is None else val_dtype
-        ks = self.kstage_rest(dtype) if direct else self.kstage(dtype)
+        stage = dtype if stage_dtype is None else stage_dtype
+        if stage != dtype and not (dtype == F64 and stage == F32):
+            raise ValueError('staging precision F%d under element arithmetic F%d' % (stage, dtype))
+        elem_code = nat.F64_STAGE_F32 if stage != dtype else dtype
+        ks = self.kstage_rest(stage) if direct else self.kstage(stage)
         rs = self.rstage if with_residual else None
         for g in self.groups:
             if g.n == 0:
@@ -331,15 +336,15 @@ class DevicePlan:
             dk = g.dst_k_rest if direct else g.dst_k
             if g.kind == 'elem':
                 if not direct:
-                    nat.ops().elem_residual_tangent(g.ntype, dtype, g.conn, self.X, U, g.params.tolist(), g.dst_k,
-                                                    ks, g.dst_r if with_residual else None, rs)
+                    nat.ops().elem_residual_tangent(g.ntype, elem_code, g.conn, self.X, U, g.params.tolist(),
+                                                    g.dst_k, ks, g.dst_r if with_residual else None, rs)
             elif g.kind == 'pressure':
                 pr = float(g.load.pressure) * load_scale
-                if dtype == F64 or not with_residual:
-                    call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, self.X, U, pr, dk, ks,
+                if stage == F64 or not with_residual:
+                    call('fea_surface_pressure', g.ntype, stage, g.n, g.conn, self.X, U, pr, dk, ks,
                          g.dst_r if with_residual else None, rs, s)
                 else:
-                    call('fea_surface_pressure', g.ntype, dtype, g.n, g.conn, self.X, U, pr, dk, ks, None,
+                    call('fea_surface_pressure', g.ntype, stage, g.n, g.conn, self.X, U, pr, dk, ks, None,
                          None, s)
                     call('fea_surface_pressure', g.ntype, F64, g.n, g.conn, self.X, U, pr, None, None, g.dst_r,
                          rs, s)
@@ -348,14 +353,14 @@ class DevicePlan:
                 call('fea_surface_traction', g.ntype, g.n, g.conn, self.X, U,
                      t, float(load_scale), g.dst_r, self.rstage, None, s)
             elif g.kind == 'barrier':
-                g.load.assemble(U, dk, ks, dtype, g.dst_r if with_residual else None, rs)
+                g.load.assemble(U, dk, ks, stage, g.dst_r if with_residual else None, rs)
         val = self.csr_values(val_dtype)
         if direct and self.n_kstage_rest == 0:
             val.zero_()                     # the element colours add into it
         else:
             # staged sums (under direct assembly only the non-element groups', zero elsewhere:
             # this also clears the values for the element colours, which add to them)
-            nat.ops().gather_csr(dtype, val_dtype, self.blk_a, self.blk_b, self.blk_colpos,
+            nat.ops().gather_csr(stage, val_dtype, self.blk_a, self.blk_b, self.blk_colpos,
                                  self.kseg_rest if direct else self.kseg, self.dof_map, self.rowptr, ks, val)
         if direct:
             for g in self.groups:

@@ -37,6 +37,9 @@ extern "C" {
 /* precision of a kernel's arithmetic and of its floating outputs */
 #define FEA_F32 0
 #define FEA_F64 1
+/* fea_elem_residual_tangent only: float64 element arithmetic, each staged block
+ * rounded once to float32 (kstage is float32) */
+#define FEA_F64_STAGE_F32 2
 
 /* element types = nodes per element */
 #define FEA_C3D4 4
@@ -107,7 +110,8 @@ int fea_color_elements(int n_nodes, int n_elem, int nen, const int32_t *conn, in
  * (C3base.py:413-452) + components_Solid (:454-480) +
  * NeoHookean.material_Constitutive_C3 (hyperelastic.py:35-84):
  * per element and Gauss point it forms F = I + grad u from X and U, the
- * stress and consistent tangent, and writes the 3x3 blocks K_ab (dtype) to
+ * stress and consistent tangent, and writes the 3x3 blocks K_ab (dtype; float32
+ * for FEA_F64_STAGE_F32) to
  * kstage[dst_k[..]*9 ..] and the nodal forces R_a (float64) to
  * rstage[dst_r[..]*3 ..].  kstage/dst_k may be NULL (residual only);
  * rstage/dst_r may be NULL (tangent only).  Gauss-point geometry (dN/dX,
