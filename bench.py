@@ -918,8 +918,14 @@ def bench_explicit(args, torch):
     g = [g for g in p.groups if g.kind == 'elem']
     E = sum(x.n for x in g)
 
+    tl = sol._tiles
+    fn = nat.raw('fea_elem_residual_tiles')
+
     def residual():
-        p.residual_stage(U, 1.0, F32)
+        s = nat.stream_ptr()
+        for a in tl['elem_args']:
+            nat.check_rc(fn(a[0], a[1], a[2], a[3], a[4], U.data_ptr(), a[5], a[6], a[7], a[8], a[9], s),
+                         'fea_elem_residual_tiles')
     t_res = event_time(torch, residual, 200)
     alg = E * 4 * 4 + p.n_nodes * 3 * 8 * 3                 # conn + X + U read, nodal force written
     cpu = None
@@ -932,7 +938,8 @@ def bench_explicit(args, torch):
                                   '(%.3e)' % (args.cells, E, dt), 'precision': 'fp32 element forces, fp64 state'})
     line.update({'dt': dt, 'simulated_time': dt * (steps + warmup),
                  'energy': {'initial': hist[0], 'final': hist[-1], 'drift_rel_to_initial_kinetic': drift},
-                 'roofline': roofline('k_elem_r<float,4> (fea_elem_residual, one step\'s element pass)',
+                 'roofline': roofline('k_elem_r_tiles<float,4> (fea_elem_residual_tiles, one step\'s element pass: '
+                                      'both blocks)',
                                       alg / t_res / 1e9, HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=alg,
                                       avg_launch_us=t_res * 1e6),
                  'cpu_baseline': cpu})

@@ -86,6 +86,12 @@ _SIGS = {
     'fea_axpy': (C_INT, [C_INT, C_DBL, C_INT, C_P, C_P, C_P]),
     'fea_vec_stats': (C_INT, [C_INT, C_P, C_P, C_P]),
     'fea_explicit_update': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_DBL, C_DBL, C_P, C_P, C_P, C_P]),
+    'fea_explicit_update_tiles': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_DBL, C_DBL, C_P, C_P, C_P, C_P]),
+    'fea_elem_tile_size': (C_INT, [C_INT]),
+    'fea_elem_residual_tiles': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_contact_set_size': (C_INT, []),
+    'fea_contact_rebuild': (C_INT, [C_P, C_P, C_P, C_P]),
+    'fea_contact_eval': (C_INT, [C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_contact_grid_build': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_DBL, C_INT, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_contact_search': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_DBL, C_INT, C_DBL, C_INT, C_P,
                                    C_P, C_P, C_P]),
@@ -190,6 +196,21 @@ def call(name, *args):
             raise FeaNativeError('%s has no declared signature' % name)
         fn = _FN[name] = getattr(lib(), name)
     rc = fn(*[_arg(a) for a in args])
+    if rc != 0:
+        check(rc, name)
+
+
+def raw(name):
+    """The ctypes function of a declared entry point (for pre-bound integer arguments)."""
+    fn = _FN.get(name)
+    if fn is None:
+        if name not in _SIGS:
+            raise FeaNativeError('%s has no declared signature' % name)
+        fn = _FN[name] = getattr(lib(), name)
+    return fn
+
+
+def check_rc(rc, name):
     if rc != 0:
         check(rc, name)
 

@@ -10,6 +10,8 @@ Gauss-point-pair barrier with a cKDTree broad phase; this load replaces its role
 in explicit dynamics (whose reference force-only branches are broken,
 contact.py:658-659, :1140-1142) with the uniform-grid search of the HIP library.
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -22,7 +24,7 @@ class NodeToSurfaceContact(BaseLoad):
 
     def __init__(self, slave_instance: str, slave_surface: str, master_instance: str, master_surface: str,
                  penalty: float = 1e3, radius: float = None, depth_max: float = None, kmax: int = 64,
-                 search_interval: int = 1) -> None:
+                 search_interval: int = 1, skin: float = None) -> None:
         super().__init__()
         self.slave_instance, self.slave_surface = slave_instance, slave_surface
         self.master_instance, self.master_surface = master_instance, master_surface
@@ -31,6 +33,7 @@ class NodeToSurfaceContact(BaseLoad):
         self.depth_max = depth_max
         self.kmax = int(kmax)
         self.search_interval = int(search_interval)
+        self.skin = skin                # explicit steps: candidate lists reused within skin / 2 of motion
         self._state = None
 
     def initialize(self, assembly):
@@ -57,6 +60,8 @@ class NodeToSurfaceContact(BaseLoad):
             self.depth_max = edge
         if self.radius is None:
             self.radius = float(self.depth_max) + 1.25 * rho
+        if self.skin is None:
+            self.skin = 0.25 * float(self.radius)
         self.origin = tri.reshape(-1, 3).min(0) - self.radius
         self._state = None
 
@@ -111,6 +116,20 @@ def _tributary_areas(instance, surface):
     return nodes, area
 
 
+class ContactSet(ctypes.Structure):
+    """fea_contact_set of include/fea_amd.h (sizes, constants and device buffers of
+    one contact evaluation)."""
+    _fields_ = [('n_slave', ctypes.c_int32), ('n_facets', ctypes.c_int32), ('n_cnode', ctypes.c_int32),
+                ('kmax', ctypes.c_int32), ('table_bits', ctypes.c_int32), ('n_master', ctypes.c_int32),
+                ('cell', ctypes.c_double), ('radius', ctypes.c_double), ('skin', ctypes.c_double),
+                ('penalty', ctypes.c_double), ('depth_max', ctypes.c_double), ('origin', ctypes.c_double * 3)] + \
+               [(n, ctypes.c_void_p) for n in ('slave', 'conn', 'cnode', 'cn_slave', 'cn_master', 'nf_ptr',
+                                               'nf_facet', 'nf_vert', 'slave_area', 'centroid', 'force', 'weight',
+                                               'x0', 'dmax', 'key', 'bucket_start', 'bucket_items', 'work', 'cand',
+                                               'count', 'max_count', 'chosen', 'sup_max', 'inv_start', 'inv_items', 'm_ptr',
+                                               'm_items')]
+
+
 class ContactState:
     """Device buffers and the search / force pipeline of one contact pair set."""
 
@@ -152,6 +171,28 @@ class ContactState:
         self.origin = np.ascontiguousarray(load.origin, dtype=np.float64)
         self.steps = 0
         self.searched = False
+        # contact nodes (slaves and master nodes, each once) with their slave / master index
+        cnode = np.union1d(load.slave_nodes, mnodes)
+        cn_slave = np.full(cnode.shape[0], -1, dtype=np.int32)
+        cn_master = np.full(cnode.shape[0], -1, dtype=np.int32)
+        cn_slave[np.searchsorted(cnode, load.slave_nodes)] = np.arange(S, dtype=np.int32)
+        cn_master[np.searchsorted(cnode, mnodes)] = np.arange(mnodes.shape[0], dtype=np.int32)
+        self.cnode, self.cn_slave, self.cn_master = i32(cnode), i32(cn_slave), i32(cn_master)
+        # reused candidate lists (explicit steps): the rebuild's state and the host's poll of dmax
+        self.x0 = z(3 * cnode.shape[0], torch.float64)
+        self.vcand = z(S * load.kmax, torch.int32)
+        self.vcount = z(S, torch.int32)
+        self.dmax = z(1, torch.int64)
+        self.sup_max = z(1, torch.int32)
+        self.inv_start = z(F + 1, torch.int32)
+        self.inv_items = z(S * load.kmax, torch.int32)
+        self.m_ptr = z(mnodes.shape[0] + 1, torch.int32)
+        self.m_items = z(6 * S * load.kmax, torch.int32)
+        self._set = None
+        self._need_rebuild = True
+        self._pin = torch.zeros(1, dtype=torch.int64).pin_memory() if torch.cuda.is_available() else None
+        self._poll = None
+        self.rebuilds = 0
 
     def search(self, U, check=True):
         """Rebuild the facet grid at U and find every slave's candidate facets.
@@ -190,6 +231,71 @@ class ContactState:
              L.kmax, L.penalty, self.area, float(L.depth_max), int(self.mnodes.numel()), self.mnodes, self.nf_ptr,
              self.nf_facet, self.nf_vert, self.chosen, self.force, self.weight, self.sb_start, self.sb_items,
              self.work, 3 * self.plan.n_nodes, self.fc, self.epart, nat.stream_ptr())
+        return self.fc
+
+    def contact_set(self):
+        """The fea_contact_set of this pair set (built once; the buffers never move)."""
+        if self._set is None:
+            L = self.load
+            if L.kmax > 64:
+                raise ValueError('kmax above 64')
+            if nat.lib().fea_contact_set_size() != ctypes.sizeof(ContactSet):
+                raise nat.FeaNativeError('fea_contact_set layout mismatch')
+            c = ContactSet()
+            c.n_slave, c.n_facets, c.n_cnode = self.S, self.F, int(self.cnode.numel())
+            c.n_master = int(self.mnodes.numel())
+            c.kmax, c.table_bits = L.kmax, self.bits
+            c.radius, c.skin = float(L.radius), float(L.skin)
+            c.cell = c.radius + c.skin
+            c.penalty, c.depth_max = float(L.penalty), float(L.depth_max)
+            c.origin[:] = [float(v) for v in self.origin]
+            for name, t in (('slave', self.slave), ('conn', self.conn), ('cnode', self.cnode),
+                            ('cn_slave', self.cn_slave), ('cn_master', self.cn_master), ('nf_ptr', self.nf_ptr),
+                            ('nf_facet', self.nf_facet), ('nf_vert', self.nf_vert), ('slave_area', self.area),
+                            ('centroid', self.cent), ('force', self.force), ('weight', self.weight),
+                            ('x0', self.x0), ('dmax', self.dmax), ('key', self.key), ('bucket_start', self.bstart),
+                            ('bucket_items', self.bitems), ('work', self.work), ('cand', self.vcand),
+                            ('count', self.vcount), ('max_count', self.max_count), ('chosen', self.chosen),
+                            ('sup_max', self.sup_max), ('inv_start', self.inv_start),
+                            ('inv_items', self.inv_items), ('m_ptr', self.m_ptr), ('m_items', self.m_items)):
+                setattr(c, name, t.data_ptr())
+            self._set = c
+        return self._set
+
+    def rebuild(self, U):
+        """Search with radius + skin at U and store the candidate and facet -> slave lists."""
+        call('fea_contact_rebuild', ctypes.addressof(self.contact_set()), self.plan.X, U, nat.stream_ptr())
+        self._need_rebuild = False
+        self._poll = None
+        self.rebuilds += 1
+
+    def step_forces(self, U, poll_every=8):
+        """Penalty forces at U from the reused candidate lists (fea_contact_eval): the forces
+        of ``forces(U, search=True)``, written only at the contact nodes of ``fc`` (its other
+        entries stay zero).  Every ``poll_every`` steps the largest contact-node displacement
+        since the last rebuild is copied to pinned memory; the next poll waits for that copy
+        (so the host runs at most ``poll_every`` steps ahead of what it has read) and, once the
+        displacement exceeds skin/4 (half the budget), the step rebuilds the lists.  The kmax
+        overflow check is left to ``check()``."""
+        s = nat.stream_ptr()
+        if self.steps % poll_every == 0:
+            if self._poll is not None:
+                self._poll.synchronize()
+                d2 = float(self._pin.numpy().view(np.float64)[0])
+                self._poll = None
+                if d2 > (0.25 * self.load.skin) ** 2:
+                    self._need_rebuild = True
+            if self._need_rebuild:
+                self.rebuild(U)
+        elif self._need_rebuild:
+            self.rebuild(U)
+        call('fea_contact_eval', ctypes.addressof(self.contact_set()), self.plan.X, U, self.fc, self.epart, s)
+        self.searched = True
+        self.steps += 1
+        if self.steps % poll_every == 0:
+            self._pin.copy_(self.dmax, non_blocking=True)
+            self._poll = torch.cuda.Event()
+            self._poll.record()
         return self.fc
 
     def energy(self):

@@ -94,8 +94,9 @@ __device__ __forceinline__ double block_sum(double v) {
 // exclusive scan of in[0..n) into out[0..n] (out[n] = the total) by one workgroup of 1024
 // threads: each thread sums a contiguous chunk, the 1024 chunk sums are scanned with wave
 // shuffles (16 waves) and a scan of the 16 wave totals, and each thread writes its chunk's
-// prefixes.  Integer and exact.
-__device__ inline void scan_1024(int n, const int32_t *__restrict__ in, int32_t *__restrict__ out) {
+// prefixes.  Integer and exact.  in == out is allowed (each thread reads an entry of its own
+// chunk before it overwrites it; out[n] lies outside every chunk).
+__device__ inline void scan_1024(int n, const int32_t *in, int32_t *out) {
   __shared__ int32_t wsum[16];
   const int t = threadIdx.x;
   const int lane = t & 63, wv = t >> 6;
@@ -125,8 +126,9 @@ __device__ inline void scan_1024(int n, const int32_t *__restrict__ in, int32_t 
   if (t == 1023) out[n] = incl;
   int32_t acc = incl - s;
   for (int b = b0; b < b1; ++b) {
+    const int32_t v = in[b];
     out[b] = acc;
-    acc += in[b];
+    acc += v;
   }
 }
 

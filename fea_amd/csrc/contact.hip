@@ -113,17 +113,14 @@ constexpr int kMaxCand = 64;
 // serial scan of the 27 cells followed by a sort.
 constexpr int kSearchWaves = kBlock / 64;
 
-__global__ __launch_bounds__(kBlock) void k_contact_search(
-    int ns, const int32_t *__restrict__ slave, const double *__restrict__ X, const double *__restrict__ U,
-    const double *__restrict__ cent, const int32_t *__restrict__ bstart, const int32_t *__restrict__ bitems,
-    double ox, double oy, double oz, double inv_h, unsigned int mask, double r2, int kmax,
-    int32_t *__restrict__ cand, int32_t *__restrict__ count, int32_t *__restrict__ max_count) {
-  __shared__ int32_t buf[kSearchWaves][kMaxCand];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int s = blockIdx.x * kSearchWaves + wave;  // uniform across the wave
-  if (s >= ns) return;
-  double p[3];
-  node_pos(X, U, slave[s], p);
+// The candidates of the wave's slave position p (wave-uniform): returns this lane's entry of
+// the ascending candidate list (INT32_MAX past the kept ones) and the total count in `total`.
+// buf is the wave's kMaxCand-entry LDS row.
+__device__ __forceinline__ int32_t wave_candidates(const double p[3], const double *__restrict__ cent,
+                                                   const int32_t *__restrict__ bstart,
+                                                   const int32_t *__restrict__ bitems, double ox, double oy,
+                                                   double oz, double inv_h, unsigned int mask, double r2,
+                                                   int32_t *buf, int lane, int &total) {
   const int64_t cx = cell_of(p[0], ox, inv_h), cy = cell_of(p[1], oy, inv_h), cz = cell_of(p[2], oz, inv_h);
   const bool active = lane < 27;
   unsigned int k = 0xffffffffu;
@@ -152,30 +149,54 @@ __global__ __launch_bounds__(kBlock) void k_contact_search(
     const int y = __shfl_up(incl, off);
     if (lane >= off) incl += y;
   }
-  const int total = __shfl(incl, 63);
+  total = __shfl(incl, 63);
   int pos = incl - cnt;
   for (int it = b0; it < b1; ++it) {
     const int f = bitems[it];
     if (hit(f)) {
-      if (pos < kMaxCand) buf[wave][pos] = f;
+      if (pos < kMaxCand) buf[pos] = f;
       ++pos;
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   const int nloc = total < kMaxCand ? total : kMaxCand;
-  int32_t v = lane < nloc ? buf[wave][lane] : INT32_MAX;
+  int32_t v = lane < nloc ? buf[lane] : INT32_MAX;
   for (int size = 2; size <= 64; size <<= 1)
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
       const int32_t o = __shfl_xor(v, stride);
       const bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
       v = keep_min ? (o < v ? o : v) : (o > v ? o : v);
     }
+  return v;
+}
+
+// the slave's candidate list and count (and the overflow maximum)
+__device__ __forceinline__ void store_candidates(int s, int lane, int32_t v, int total, int kmax,
+                                                 int32_t *__restrict__ cand, int32_t *__restrict__ count,
+                                                 int32_t *__restrict__ max_count) {
+  const int nloc = total < kMaxCand ? total : kMaxCand;
   const int nout = nloc < kmax ? nloc : kmax;
   if (lane < nout) cand[(int64_t)s * kmax + lane] = v;
   if (lane == 0) {
     count[s] = total;
     if (max_count != nullptr && total > kmax) atomicMax(max_count, total);
   }
+}
+
+__global__ __launch_bounds__(kBlock) void k_contact_search(
+    int ns, const int32_t *__restrict__ slave, const double *__restrict__ X, const double *__restrict__ U,
+    const double *__restrict__ cent, const int32_t *__restrict__ bstart, const int32_t *__restrict__ bitems,
+    double ox, double oy, double oz, double inv_h, unsigned int mask, double r2, int kmax,
+    int32_t *__restrict__ cand, int32_t *__restrict__ count, int32_t *__restrict__ max_count) {
+  __shared__ int32_t buf[kSearchWaves][kMaxCand];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = blockIdx.x * kSearchWaves + wave;  // uniform across the wave
+  if (s >= ns) return;
+  double p[3];
+  node_pos(X, U, slave[s], p);
+  int total = 0;
+  const int32_t v = wave_candidates(p, cent, bstart, bitems, ox, oy, oz, inv_h, mask, r2, buf[wave], lane, total);
+  store_candidates(s, lane, v, total, kmax, cand, count, max_count);
 }
 
 // ---------------------------------------------------------------- penalty
@@ -195,6 +216,105 @@ __global__ __launch_bounds__(kBlock) void k_contact_search(
 // tributary area, 1 when area is null).
 constexpr double kInsideTol = 1e-10;
 constexpr double kExtendTol = 0.1;
+
+// The penalty terms of slave position p against facet f: false when the slave does not
+// penetrate the facet's plane within depth_max or projects outside the extended facet.
+struct PenCand {
+  int tier;
+  double key, g, n[3], w[3];
+};
+
+__device__ __forceinline__ bool pen_eval(const double p[3], const double *__restrict__ X,
+                                         const double *__restrict__ U, const int32_t *__restrict__ conn, int f,
+                                         double depth_max, PenCand &out) {
+  double a[3], b[3], c[3];
+  node_pos(X, U, conn[3 * (int64_t)f + 0], a);
+  node_pos(X, U, conn[3 * (int64_t)f + 1], b);
+  node_pos(X, U, conn[3 * (int64_t)f + 2], c);
+  double e1[3], e2[3], ap[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    e1[j] = b[j] - a[j];
+    e2[j] = c[j] - a[j];
+    ap[j] = p[j] - a[j];
+  }
+  double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+  const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  if (!(nn > 0.0)) return false;
+  n[0] /= nn;
+  n[1] /= nn;
+  n[2] /= nn;
+  const double g = ap[0] * n[0] + ap[1] * n[1] + ap[2] * n[2];
+  if (!(g < 0.0) || !(g > -depth_max)) return false;
+  double q[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) q[j] = ap[j] - g * n[j];
+  const double d00 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+  const double d01 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+  const double d11 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+  const double d20 = q[0] * e1[0] + q[1] * e1[1] + q[2] * e1[2];
+  const double d21 = q[0] * e2[0] + q[1] * e2[1] + q[2] * e2[2];
+  const double den = d00 * d11 - d01 * d01;
+  const double xi = (d11 * d20 - d01 * d21) / den;
+  const double eta = (d00 * d21 - d01 * d20) / den;
+  const double w0 = 1.0 - xi - eta;
+  const double v = fmin(w0, fmin(xi, eta));
+  if (!(v >= -kExtendTol)) return false;
+  out.tier = (v >= -kInsideTol) ? 0 : 1;
+  out.key = (out.tier == 0) ? g : v;
+  out.g = g;
+  out.n[0] = n[0];
+  out.n[1] = n[1];
+  out.n[2] = n[2];
+  out.w[0] = w0;
+  out.w[1] = xi;
+  out.w[2] = eta;
+  return true;
+}
+
+// The wave's best lane by (tier ascending, key descending, index ascending); lane 0 writes
+// the slave's choice, force and weights and returns its penalty energy (0 on other lanes).
+__device__ __forceinline__ double pen_finish(int s, int lane, int bf, int bi, int btier, double bkey, double bg,
+                                             double bn[3], double bw[3], double k_pen,
+                                             const double *__restrict__ area, int32_t *__restrict__ chosen,
+                                             double *__restrict__ force, double *__restrict__ weight) {
+  int wl = lane;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int ot = __shfl_xor(btier, off, 64);
+    const double ok = __shfl_xor(bkey, off, 64);
+    const int oi = __shfl_xor(bi, off, 64);
+    const int ol = __shfl_xor(wl, off, 64);
+    if (ot < btier || (ot == btier && (ok > bkey || (ok == bkey && oi < bi)))) {
+      btier = ot;
+      bkey = ok;
+      bi = oi;
+      wl = ol;
+    }
+  }
+  bf = __shfl(bf, wl, 64);
+  bg = __shfl(bg, wl, 64);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    bn[j] = __shfl(bn[j], wl, 64);
+    bw[j] = __shfl(bw[j], wl, 64);
+  }
+  double e = 0.0;
+  if (lane == 0) {
+    const int best = btier < 2 ? bf : -1;
+    chosen[s] = best;
+    const double ks = (area != nullptr) ? k_pen * area[s] : k_pen;
+    const double fm = (best >= 0) ? -ks * bg : 0.0;
+    force[3 * (int64_t)s + 0] = fm * bn[0];
+    force[3 * (int64_t)s + 1] = fm * bn[1];
+    force[3 * (int64_t)s + 2] = fm * bn[2];
+    weight[3 * (int64_t)s + 0] = best >= 0 ? bw[0] : 0.0;
+    weight[3 * (int64_t)s + 1] = best >= 0 ? bw[1] : 0.0;
+    weight[3 * (int64_t)s + 2] = best >= 0 ? bw[2] : 0.0;
+    if (best >= 0) e = 0.5 * ks * bg * bg;
+  }
+  return e;
+}
 
 // One wave per slave node: lane i evaluates candidates i, i + 64, ... of the slave's sorted
 // candidate list and keeps its best by (tier ascending, key descending, index ascending); a
@@ -219,90 +339,23 @@ __global__ __launch_bounds__(kBlock) void k_contact_penalty(
     const int nc = min(count[s], kmax);
     for (int i = lane; i < nc; i += 64) {
       const int f = cand[(int64_t)s * kmax + i];
-      double a[3], b[3], c[3];
-      node_pos(X, U, conn[3 * (int64_t)f + 0], a);
-      node_pos(X, U, conn[3 * (int64_t)f + 1], b);
-      node_pos(X, U, conn[3 * (int64_t)f + 2], c);
-      double e1[3], e2[3], ap[3];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        e1[j] = b[j] - a[j];
-        e2[j] = c[j] - a[j];
-        ap[j] = p[j] - a[j];
-      }
-      double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-      const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-      if (!(nn > 0.0)) continue;
-      n[0] /= nn;
-      n[1] /= nn;
-      n[2] /= nn;
-      const double g = ap[0] * n[0] + ap[1] * n[1] + ap[2] * n[2];
-      if (!(g < 0.0) || !(g > -depth_max)) continue;
-      double q[3];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) q[j] = ap[j] - g * n[j];
-      const double d00 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
-      const double d01 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
-      const double d11 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
-      const double d20 = q[0] * e1[0] + q[1] * e1[1] + q[2] * e1[2];
-      const double d21 = q[0] * e2[0] + q[1] * e2[1] + q[2] * e2[2];
-      const double den = d00 * d11 - d01 * d01;
-      const double xi = (d11 * d20 - d01 * d21) / den;
-      const double eta = (d00 * d21 - d01 * d20) / den;
-      const double w0 = 1.0 - xi - eta;
-      const double v = fmin(w0, fmin(xi, eta));
-      if (!(v >= -kExtendTol)) continue;
-      const int tier = (v >= -kInsideTol) ? 0 : 1;
-      const double key = (tier == 0) ? g : v;
-      if (tier < btier || (tier == btier && key > bkey)) {
+      PenCand c;
+      if (!pen_eval(p, X, U, conn, f, depth_max, c)) continue;
+      if (c.tier < btier || (c.tier == btier && c.key > bkey)) {
         bf = f;
         bi = i;
-        btier = tier;
-        bkey = key;
-        bg = g;
-        bn[0] = n[0];
-        bn[1] = n[1];
-        bn[2] = n[2];
-        bw[0] = w0;
-        bw[1] = xi;
-        bw[2] = eta;
+        btier = c.tier;
+        bkey = c.key;
+        bg = c.g;
+        bn[0] = c.n[0];
+        bn[1] = c.n[1];
+        bn[2] = c.n[2];
+        bw[0] = c.w[0];
+        bw[1] = c.w[1];
+        bw[2] = c.w[2];
       }
     }
-    // the wave's best lane
-    int wl = lane;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const int ot = __shfl_xor(btier, off, 64);
-      const double ok = __shfl_xor(bkey, off, 64);
-      const int oi = __shfl_xor(bi, off, 64);
-      const int ol = __shfl_xor(wl, off, 64);
-      if (ot < btier || (ot == btier && (ok > bkey || (ok == bkey && oi < bi)))) {
-        btier = ot;
-        bkey = ok;
-        bi = oi;
-        wl = ol;
-      }
-    }
-    bf = __shfl(bf, wl, 64);
-    bg = __shfl(bg, wl, 64);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      bn[j] = __shfl(bn[j], wl, 64);
-      bw[j] = __shfl(bw[j], wl, 64);
-    }
-    if (lane == 0) {
-      const int best = btier < 2 ? bf : -1;
-      chosen[s] = best;
-      const double ks = (area != nullptr) ? k_pen * area[s] : k_pen;
-      const double fm = (best >= 0) ? -ks * bg : 0.0;
-      force[3 * (int64_t)s + 0] = fm * bn[0];
-      force[3 * (int64_t)s + 1] = fm * bn[1];
-      force[3 * (int64_t)s + 2] = fm * bn[2];
-      weight[3 * (int64_t)s + 0] = best >= 0 ? bw[0] : 0.0;
-      weight[3 * (int64_t)s + 1] = best >= 0 ? bw[1] : 0.0;
-      weight[3 * (int64_t)s + 2] = best >= 0 ? bw[2] : 0.0;
-      if (best >= 0) e = 0.5 * ks * bg * bg;
-    }
+    e = pen_finish(s, lane, bf, bi, btier, bkey, bg, bn, bw, k_pen, area, chosen, force, weight);
   }
   if (energy_partial != nullptr) {
     const double t = block_sum(e);
@@ -351,6 +404,267 @@ __global__ __launch_bounds__(kBlock) void k_contact_slave(int ns, const int32_t 
 __global__ __launch_bounds__(kBlock) void k_fill_dbl(int64_t n, double *__restrict__ a) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0.0;
+}
+
+// ---------------------------------------------------------------- reused candidate lists
+// Between rebuilds each slave keeps the candidates of a search with the enlarged radius
+// r + skin, and facet -> slave lists of those pairs.  While no contact node has moved more
+// than skin/2 since the rebuild, every pair within r now was within r + skin then, so the
+// stored lists hold every exact candidate; each step tests the exact predicate on them.
+// Otherwise (the host rebuilds late) the kernels search every facet / slave instead.
+// dmax holds the largest squared displacement of a contact node since the rebuild, as the
+// bits of a non-negative double (ordered like the values).
+
+__device__ __forceinline__ void facet_centroid(const double *__restrict__ X, const double *__restrict__ U,
+                                               const int32_t *__restrict__ conn, int f, double m[3]) {
+  double a[3], b[3], c[3];
+  node_pos(X, U, conn[3 * (int64_t)f + 0], a);
+  node_pos(X, U, conn[3 * (int64_t)f + 1], b);
+  node_pos(X, U, conn[3 * (int64_t)f + 2], c);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) m[i] = dadd(dadd(a[i], b[i]), c[i]) / 3.0;
+}
+
+// the exact candidate predicate of k_contact_search at the current centroid
+__device__ __forceinline__ bool exact_candidate(const double p[3], const double m[3], double r2) {
+  const double ddx = p[0] - m[0], ddy = p[1] - m[1], ddz = p[2] - m[2];
+  return dadd(dadd(dmul(ddx, ddx), dmul(ddy, ddy)), dmul(ddz, ddz)) <= r2;
+}
+
+// dmax = max over contact nodes of |x - x0|^2
+__global__ __launch_bounds__(kBlock) void k_contact_track(int nc, const int32_t *__restrict__ cnode,
+                                                          const double *__restrict__ X,
+                                                          const double *__restrict__ U,
+                                                          const double *__restrict__ x0,
+                                                          unsigned long long *__restrict__ dmax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double d2 = 0.0;
+  if (i < nc) {
+    double p[3];
+    node_pos(X, U, cnode[i], p);
+    const double dx = p[0] - x0[3 * (int64_t)i], dy = p[1] - x0[3 * (int64_t)i + 1], dz = p[2] - x0[3 * (int64_t)i + 2];
+    d2 = dx * dx + dy * dy + dz * dz;
+  }
+  for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_down(d2, off, 64));
+  if ((threadIdx.x & 63) == 0 && d2 > 0.0) atomicMax(dmax, (unsigned long long)__double_as_longlong(d2));
+}
+
+// x0 = current positions of the contact nodes, dmax = 0 (after a rebuild's search)
+__global__ __launch_bounds__(kBlock) void k_contact_snapshot(int nc, const int32_t *__restrict__ cnode,
+                                                             const double *__restrict__ X,
+                                                             const double *__restrict__ U, double *__restrict__ x0,
+                                                             unsigned long long *__restrict__ dmax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *dmax = 0ull;
+  if (i >= nc) return;
+  double p[3];
+  node_pos(X, U, cnode[i], p);
+  x0[3 * (int64_t)i + 0] = p[0];
+  x0[3 * (int64_t)i + 1] = p[1];
+  x0[3 * (int64_t)i + 2] = p[2];
+}
+
+__device__ __forceinline__ bool lists_valid(const unsigned long long *dmax, double half_skin2, const int32_t *sup_max,
+                                            int kmax) {
+  return __longlong_as_double((long long)*dmax) <= half_skin2 && *sup_max <= kmax;
+}
+
+// One wave per slave: candidates from the slave's stored list (or every facet), the exact
+// predicate, then the penalty choice with ties broken by facet id (the order of the exact
+// search's sorted list).  count_exact[s] = the exact candidate count; counts above kmax raise
+// max_count as the search does.
+__global__ __launch_bounds__(kBlock) void k_contact_penalty_v(
+    int ns, const int32_t *__restrict__ slave, const double *__restrict__ X, const double *__restrict__ U,
+    const int32_t *__restrict__ conn, int nf, const int32_t *__restrict__ cand, const int32_t *__restrict__ count,
+    int kmax, double r2, const unsigned long long *__restrict__ dmax, double half_skin2,
+    const int32_t *__restrict__ sup_max, int32_t *__restrict__ max_count, double k_pen,
+    const double *__restrict__ area, double depth_max, int32_t *__restrict__ chosen, double *__restrict__ force,
+    double *__restrict__ weight, double *__restrict__ energy_partial) {
+  const int lane = threadIdx.x & 63;
+  const int s = blockIdx.x * kPenWaves + (threadIdx.x >> 6);
+  double e = 0.0;
+  if (s < ns) {
+    const bool valid = lists_valid(dmax, half_skin2, sup_max, kmax);
+    double p[3];
+    node_pos(X, U, slave[s], p);
+    int bf = -1, bi = INT_MAX, btier = 2, hits = 0;
+    double bg = 0.0, bkey = 0.0, bn[3] = {0, 0, 0}, bw[3] = {0, 0, 0};
+    const int n = valid ? min(count[s], kmax) : nf;
+    for (int i = lane; i < n; i += 64) {
+      const int f = valid ? cand[(int64_t)s * kmax + i] : i;
+      double m[3];
+      facet_centroid(X, U, conn, f, m);
+      if (!exact_candidate(p, m, r2)) continue;
+      ++hits;
+      PenCand c;
+      if (!pen_eval(p, X, U, conn, f, depth_max, c)) continue;
+      if (c.tier < btier || (c.tier == btier && c.key > bkey)) {
+        bf = f;
+        bi = f;
+        btier = c.tier;
+        bkey = c.key;
+        bg = c.g;
+        bn[0] = c.n[0];
+        bn[1] = c.n[1];
+        bn[2] = c.n[2];
+        bw[0] = c.w[0];
+        bw[1] = c.w[1];
+        bw[2] = c.w[2];
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) hits += __shfl_xor(hits, off, 64);
+    if (lane == 0 && max_count != nullptr && hits > kmax) atomicMax(max_count, hits);
+    e = pen_finish(s, lane, bf, bi, btier, bkey, bg, bn, bw, k_pen, area, chosen, force, weight);
+  }
+  if (energy_partial != nullptr) {
+    const double t = block_sum(e);
+    if (threadIdx.x == 0) energy_partial[blockIdx.x] = t;
+  }
+}
+
+// Contact node forces, one wave per contact node.  Its master items (slave s, facet f,
+// local vertex v), in facet order and ascending slaves within a facet, come from the stored
+// per-master-node lists (valid lists: every slave that chose one of the node's facets is
+// there) or, otherwise, from a scan of every slave per facet.  Lanes test items chosen[s] == f
+// in chunks of 64, a ballot marks the matches, and the wave subtracts F_s w_s(v) in item
+// order: the sums of the bucket sort of slaves by chosen facet.  Then the node's own slave
+// force is added; fc[node] is written once.
+constexpr int kReactWaves = kBlock / 64;
+
+__device__ __forceinline__ void react_chunk(bool match, int s, int v, const double *__restrict__ force,
+                                            const double *__restrict__ weight, double acc[3]) {
+  unsigned long long mask = __ballot(match);
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0, w = 0.0;
+  if (match) {
+    f0 = force[3 * (int64_t)s + 0];
+    f1 = force[3 * (int64_t)s + 1];
+    f2 = force[3 * (int64_t)s + 2];
+    w = weight[3 * (int64_t)s + v];
+  }
+  while (mask) {
+    const int b = __ffsll((long long)mask) - 1;
+    mask &= mask - 1;
+    const double wb = __shfl(w, b, 64);
+    acc[0] -= __shfl(f0, b, 64) * wb;
+    acc[1] -= __shfl(f1, b, 64) * wb;
+    acc[2] -= __shfl(f2, b, 64) * wb;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_contact_reactions_v(
+    int nc, int ns, const int32_t *__restrict__ cnode, const int32_t *__restrict__ cn_slave,
+    const int32_t *__restrict__ cn_master, const int32_t *__restrict__ nf_ptr, const int32_t *__restrict__ nf_facet,
+    const int32_t *__restrict__ nf_vert, const int32_t *__restrict__ m_ptr, const int32_t *__restrict__ m_items,
+    const int32_t *__restrict__ chosen, const unsigned long long *__restrict__ dmax, double half_skin2,
+    const int32_t *__restrict__ sup_max, int kmax, const double *__restrict__ force,
+    const double *__restrict__ weight, double *__restrict__ fc) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * kReactWaves + (threadIdx.x >> 6);  // uniform over the wave
+  if (i >= nc) return;
+  const bool valid = lists_valid(dmax, half_skin2, sup_max, kmax);
+  double r[3] = {0.0, 0.0, 0.0};
+  const int m = cn_master[i];
+  if (m >= 0) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (valid) {
+      const int k0 = m_ptr[m], k1 = m_ptr[m + 1];
+      for (int c = k0; c < k1; c += 64) {
+        const int k = c + lane;
+        int s = 0, fv = -1;
+        if (k < k1) {
+          s = m_items[2 * (int64_t)k];
+          fv = m_items[2 * (int64_t)k + 1];
+        }
+        const bool match = fv >= 0 && chosen[s] == (fv >> 2);
+        react_chunk(match, s, fv & 3, force, weight, acc);
+      }
+    } else {
+      for (int t = nf_ptr[m]; t < nf_ptr[m + 1]; ++t) {
+        const int f = nf_facet[t], v = nf_vert[t];
+        for (int c = 0; c < ns; c += 64) {
+          const int s = c + lane;
+          const bool match = s < ns && chosen[s] == f;
+          react_chunk(match, s, v, force, weight, acc);
+        }
+      }
+    }
+    r[0] += acc[0];
+    r[1] += acc[1];
+    r[2] += acc[2];
+  }
+  const int sl = cn_slave[i];
+  if (lane == 0) {
+    if (sl >= 0) {
+      r[0] += force[3 * (int64_t)sl + 0];
+      r[1] += force[3 * (int64_t)sl + 1];
+      r[2] += force[3 * (int64_t)sl + 2];
+    }
+    const int64_t n = cnode[i];
+    fc[3 * n + 0] = r[0];
+    fc[3 * n + 1] = r[1];
+    fc[3 * n + 2] = r[2];
+  }
+}
+
+// per master node: the number of stored (slave, facet) pairs over its facets
+__global__ __launch_bounds__(kBlock) void k_mlist_count(int nm, const int32_t *__restrict__ nf_ptr,
+                                                        const int32_t *__restrict__ nf_facet,
+                                                        const int32_t *__restrict__ inv_start,
+                                                        int32_t *__restrict__ m_ptr) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  int c = 0;
+  for (int t = nf_ptr[m]; t < nf_ptr[m + 1]; ++t) {
+    const int f = nf_facet[t];
+    c += inv_start[f + 1] - inv_start[f];
+  }
+  m_ptr[m] = c;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_inplace(int n, int32_t *__restrict__ a) { scan_1024(n, a, a); }
+
+// per master node: its items (slave, facet * 4 + local vertex) in facet order, slaves ascending
+__global__ __launch_bounds__(kBlock) void k_mlist_fill(int nm, const int32_t *__restrict__ nf_ptr,
+                                                       const int32_t *__restrict__ nf_facet,
+                                                       const int32_t *__restrict__ nf_vert,
+                                                       const int32_t *__restrict__ inv_start,
+                                                       const int32_t *__restrict__ inv_items,
+                                                       const int32_t *__restrict__ m_ptr,
+                                                       int32_t *__restrict__ m_items) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  int64_t pos = m_ptr[m];
+  for (int t = nf_ptr[m]; t < nf_ptr[m + 1]; ++t) {
+    const int f = nf_facet[t], v = nf_vert[t];
+    for (int it = inv_start[f]; it < inv_start[f + 1]; ++it) {
+      m_items[2 * pos] = inv_items[it];
+      m_items[2 * pos + 1] = f * 4 + v;
+      ++pos;
+    }
+  }
+}
+
+// facet -> slave pair counts of the stored candidate lists
+__global__ __launch_bounds__(kBlock) void k_inv_count(int ns, const int32_t *__restrict__ cand,
+                                                      const int32_t *__restrict__ count, int kmax,
+                                                      int32_t *__restrict__ cnt) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const int n = min(count[s], kmax);
+  for (int i = 0; i < n; ++i) atomicAdd(&cnt[cand[(int64_t)s * kmax + i]], 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_inv_fill(int ns, const int32_t *__restrict__ cand,
+                                                     const int32_t *__restrict__ count, int kmax,
+                                                     const int32_t *__restrict__ start, int32_t *__restrict__ cursor,
+                                                     int32_t *__restrict__ items) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const int n = min(count[s], kmax);
+  for (int i = 0; i < n; ++i) {
+    const int f = cand[(int64_t)s * kmax + i];
+    items[start[f] + atomicAdd(&cursor[f], 1)] = s;
+  }
 }
 
 static int bucket_sort(int n, const int32_t *keys, int nb, int32_t *start, int32_t *items, int32_t *work,
@@ -441,5 +755,76 @@ extern "C" int fea_contact_forces(int n_slave, const int32_t *slave, const doubl
                                                               sb_start, sb_items, force, weight, fc);
   k_contact_slave<<<blocks_for(n_slave), kBlock, 0, st>>>(n_slave, slave, force, fc);
   FEA_CHECK_LAUNCH("fea_contact_forces");
+  return FEA_OK;
+}
+
+extern "C" int fea_contact_set_size(void) { return (int)sizeof(fea_contact_set); }
+
+static int check_set(const fea_contact_set *cs, const char *name) {
+  FEA_CHECK_ARG(cs != nullptr, "%s: null set", name);
+  const fea_contact_set &c = *cs;
+  FEA_CHECK_ARG(c.n_slave > 0 && c.n_facets > 0 && c.n_cnode > 0 && c.n_master > 0 && c.kmax >= 1 &&
+                    c.kmax <= kMaxCand && c.n_facets < (1 << 29) &&
+                    c.table_bits >= 1 && c.table_bits <= 28 && c.radius > 0.0 && c.skin >= 0.0 &&
+                    c.radius + c.skin <= c.cell,
+                "%s: bad sizes (radius + skin must not exceed the cell size, kmax <= 64)", name);
+  FEA_CHECK_ARG(c.slave && c.conn && c.cnode && c.cn_slave && c.cn_master && c.nf_ptr && c.nf_facet && c.nf_vert &&
+                    c.centroid && c.force && c.weight && c.key && c.bucket_start && c.bucket_items && c.work &&
+                    c.cand && c.count && c.chosen && c.x0 && c.dmax && c.sup_max && c.inv_start && c.inv_items &&
+                    c.m_ptr && c.m_items,
+                "%s: null buffer", name);
+  return FEA_OK;
+}
+
+extern "C" int fea_contact_rebuild(const fea_contact_set *cs, const double *X, const double *U, void *stream) {
+  const int rc0 = check_set(cs, "fea_contact_rebuild");
+  if (rc0 != FEA_OK) return rc0;
+  FEA_CHECK_ARG(X && U, "fea_contact_rebuild: null X or U");
+  const fea_contact_set &c = *cs;
+  hipStream_t st = as_stream(stream);
+  const int nb = 1 << c.table_bits;
+  const double inv_h = 1.0 / c.cell;
+  const double rs = c.radius + c.skin;
+  k_facet_keys<<<blocks_for(c.n_facets), kBlock, 0, st>>>(c.n_facets, c.conn, X, U, c.origin[0], c.origin[1],
+                                                          c.origin[2], inv_h, (unsigned int)nb - 1u, c.centroid, c.key);
+  int rc = bucket_sort(c.n_facets, c.key, nb, c.bucket_start, c.bucket_items, c.work, st);
+  if (rc != FEA_OK) return rc;
+  k_fill_int<<<1, kBlock, 0, st>>>(1, c.sup_max, 0);
+  k_contact_search<<<(c.n_slave + kSearchWaves - 1) / kSearchWaves, kBlock, 0, st>>>(
+      c.n_slave, c.slave, X, U, c.centroid, c.bucket_start, c.bucket_items, c.origin[0], c.origin[1], c.origin[2],
+      inv_h, (unsigned int)nb - 1u, rs * rs, c.kmax, c.cand, c.count, c.sup_max);
+  // facet -> slave lists of the stored pairs, slaves ascending in each
+  k_fill_int<<<blocks_for(c.n_facets), kBlock, 0, st>>>(c.n_facets, c.work, 0);
+  k_inv_count<<<blocks_for(c.n_slave), kBlock, 0, st>>>(c.n_slave, c.cand, c.count, c.kmax, c.work);
+  k_bucket_scan<<<1, 1024, 0, st>>>(c.n_facets, c.work, c.inv_start);
+  k_inv_fill<<<blocks_for(c.n_slave), kBlock, 0, st>>>(c.n_slave, c.cand, c.count, c.kmax, c.inv_start, c.work,
+                                                       c.inv_items);
+  k_bucket_sort<<<blocks_for(c.n_facets), kBlock, 0, st>>>(c.n_facets, c.inv_start, c.inv_items);
+  // per master node lists (items in facet order, slaves ascending)
+  k_mlist_count<<<blocks_for(c.n_master), kBlock, 0, st>>>(c.n_master, c.nf_ptr, c.nf_facet, c.inv_start, c.m_ptr);
+  k_scan_inplace<<<1, 1024, 0, st>>>(c.n_master, c.m_ptr);
+  k_mlist_fill<<<blocks_for(c.n_master), kBlock, 0, st>>>(c.n_master, c.nf_ptr, c.nf_facet, c.nf_vert, c.inv_start,
+                                                          c.inv_items, c.m_ptr, c.m_items);
+  k_contact_snapshot<<<blocks_for(c.n_cnode), kBlock, 0, st>>>(c.n_cnode, c.cnode, X, U, c.x0, (unsigned long long *)c.dmax);
+  FEA_CHECK_LAUNCH("fea_contact_rebuild");
+  return FEA_OK;
+}
+
+extern "C" int fea_contact_eval(const fea_contact_set *cs, const double *X, const double *U, double *fc,
+                                double *energy_partial, void *stream) {
+  const int rc0 = check_set(cs, "fea_contact_eval");
+  if (rc0 != FEA_OK) return rc0;
+  FEA_CHECK_ARG(X && U && fc, "fea_contact_eval: null X, U or fc");
+  const fea_contact_set &c = *cs;
+  hipStream_t st = as_stream(stream);
+  const double h2 = 0.25 * c.skin * c.skin;
+  k_contact_track<<<blocks_for(c.n_cnode), kBlock, 0, st>>>(c.n_cnode, c.cnode, X, U, c.x0, (unsigned long long *)c.dmax);
+  k_contact_penalty_v<<<fea_contact_blocks(c.n_slave), kBlock, 0, st>>>(
+      c.n_slave, c.slave, X, U, c.conn, c.n_facets, c.cand, c.count, c.kmax, c.radius * c.radius, (const unsigned long long *)c.dmax, h2,
+      c.sup_max, c.max_count, c.penalty, c.slave_area, c.depth_max, c.chosen, c.force, c.weight, energy_partial);
+  k_contact_reactions_v<<<(c.n_cnode + kReactWaves - 1) / kReactWaves, kBlock, 0, st>>>(
+      c.n_cnode, c.n_slave, c.cnode, c.cn_slave, c.cn_master, c.nf_ptr, c.nf_facet, c.nf_vert, c.m_ptr,
+      c.m_items, c.chosen, (const unsigned long long *)c.dmax, h2, c.sup_max, c.kmax, c.force, c.weight, fc);
+  FEA_CHECK_LAUNCH("fea_contact_eval");
   return FEA_OK;
 }

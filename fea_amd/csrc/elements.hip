@@ -219,20 +219,16 @@ __global__ __launch_bounds__(kRkThreads) void k_elem_rk(
   }
 }
 
-// One thread per element: nodal forces R_a = sum_g w P dN_a with P = F S.
+// Nodal forces R_a = sum_g w P dN_a (P = F S) of element e, in T.
+// In float, coordinates and displacements are taken relative to the element's first
+// node, formed in double: the Jacobian and grad u only see differences (the shape
+// function derivatives sum to zero), and a rigid translation of the element no longer
+// costs float digits of the strain.  In double the absolute values are used.
 template <typename T, int ET>
-__global__ __launch_bounds__(kBlock) void k_elem_r(int n_elem, const int32_t *__restrict__ conn,
-                                                   const double *__restrict__ X,
-                                                   const double *__restrict__ U, MatParams mp,
-                                                   const int32_t *__restrict__ dst_r,
-                                                   double *__restrict__ rstage) {
+__device__ __forceinline__ void elem_force(int e, const int32_t *__restrict__ conn, const double *__restrict__ X,
+                                           const double *__restrict__ U, const MatParams &mp,
+                                           T R[Elem<ET>::nen][3]) {
   constexpr int NEN = Elem<ET>::nen, NGP = Elem<ET>::ngp;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_elem) return;
-  // In float, coordinates and displacements are taken relative to the element's first
-  // node, formed in double: the Jacobian and grad u only see differences (the shape
-  // function derivatives sum to zero), and a rigid translation of the element no longer
-  // costs float digits of the strain.  In double the absolute values are used.
   constexpr bool kRel = sizeof(T) < sizeof(double);
   const int64_t n0 = conn[(int64_t)e * NEN];
   double x0[3] = {0.0, 0.0, 0.0}, u0[3] = {0.0, 0.0, 0.0};
@@ -253,7 +249,6 @@ __global__ __launch_bounds__(kBlock) void k_elem_r(int n_elem, const int32_t *__
       u[c][i] = (T)(U[3 * n + i] - u0[i]);
     }
   }
-  T R[NEN][3];
 #pragma unroll
   for (int c = 0; c < NEN; ++c) R[c][0] = R[c][1] = R[c][2] = T(0);
   for (int g = 0; g < NGP; ++g) {
@@ -278,6 +273,20 @@ __global__ __launch_bounds__(kBlock) void k_elem_r(int n_elem, const int32_t *__
       for (int j = 0; j < 3; ++j)
         R[c][j] += P[j * 3 + 0] * dN[c][0] + P[j * 3 + 1] * dN[c][1] + P[j * 3 + 2] * dN[c][2];
   }
+}
+
+// One thread per element: R_a (float64) to rstage[dst_r[e*nen + a]*3 ..] (-1 skips).
+template <typename T, int ET>
+__global__ __launch_bounds__(kBlock) void k_elem_r(int n_elem, const int32_t *__restrict__ conn,
+                                                   const double *__restrict__ X,
+                                                   const double *__restrict__ U, MatParams mp,
+                                                   const int32_t *__restrict__ dst_r,
+                                                   double *__restrict__ rstage) {
+  constexpr int NEN = Elem<ET>::nen;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_elem) return;
+  T R[NEN][3];
+  elem_force<T, ET>(e, conn, X, U, mp, R);
 #pragma unroll
   for (int c = 0; c < NEN; ++c) {
     const int d = dst_r[(int64_t)e * NEN + c];
@@ -286,6 +295,65 @@ __global__ __launch_bounds__(kBlock) void k_elem_r(int n_elem, const int32_t *__
       rstage[(int64_t)d * 3 + 1] = (double)R[c][1];
       rstage[(int64_t)d * 3 + 2] = (double)R[c][2];
     }
+  }
+}
+
+// Element forces reduced per tile in LDS (the explicit loop's residual).  Workgroup t owns
+// elements [t*TE, (t+1)*TE): their nodal forces (in T) go to LDS slot e_local*nen + a, then
+// thread per local node p (partials tile_lbase[t] .. tile_lbase[t+1]) sums its slots
+// pslot[pslot_ptr[p] .. pslot_ptr[p+1]) in that order in float64 into partial[3p ..].  The
+// update sums a node's partials; no per-element staging reaches HBM.
+template <int ET>
+struct TileCfg;
+template <>
+struct TileCfg<4> {
+  static constexpr int TE = 512;
+};
+template <>
+struct TileCfg<8> {
+  static constexpr int TE = 256;
+};
+template <>
+struct TileCfg<10> {
+  static constexpr int TE = 256;
+};
+
+template <typename T, int ET>
+__global__ __launch_bounds__(kBlock) void k_elem_r_tiles(int n_elem, const int32_t *__restrict__ conn,
+                                                         const double *__restrict__ X,
+                                                         const double *__restrict__ U, MatParams mp,
+                                                         const int32_t *__restrict__ tile_lbase,
+                                                         const int32_t *__restrict__ pslot_ptr,
+                                                         const uint16_t *__restrict__ pslot,
+                                                         double *__restrict__ partial) {
+  constexpr int NEN = Elem<ET>::nen, TE = TileCfg<ET>::TE;
+  __shared__ T lds[TE * NEN * 3];
+  const int t = blockIdx.x;
+  const int e0 = t * TE;
+  for (int el = threadIdx.x; el < TE; el += kBlock) {
+    const int e = e0 + el;
+    if (e >= n_elem) break;
+    T R[NEN][3];
+    elem_force<T, ET>(e, conn, X, U, mp, R);
+#pragma unroll
+    for (int c = 0; c < NEN; ++c)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) lds[(el * NEN + c) * 3 + j] = R[c][j];
+  }
+  __syncthreads();
+  const int p1 = tile_lbase[t + 1];
+  for (int p = tile_lbase[t] + threadIdx.x; p < p1; p += kBlock) {
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    const int q1 = pslot_ptr[p + 1];
+    for (int q = pslot_ptr[p]; q < q1; ++q) {
+      const int sl = pslot[q];
+      f0 += (double)lds[sl * 3 + 0];
+      f1 += (double)lds[sl * 3 + 1];
+      f2 += (double)lds[sl * 3 + 2];
+    }
+    partial[3 * (int64_t)p + 0] = f0;
+    partial[3 * (int64_t)p + 1] = f1;
+    partial[3 * (int64_t)p + 2] = f2;
   }
 }
 
@@ -501,6 +569,28 @@ static int launch_r(int et, int n, const int32_t *conn, const double *X, const d
   return FEA_OK;
 }
 
+template <typename T>
+static int launch_r_tiles(int et, int n, const int32_t *conn, const double *X, const double *U, MatParams mp,
+                          const int32_t *tile_lbase, const int32_t *pslot_ptr, const uint16_t *pslot,
+                          double *partial, hipStream_t st) {
+  switch (et) {
+    case 4:
+      k_elem_r_tiles<T, 4><<<(n + TileCfg<4>::TE - 1) / TileCfg<4>::TE, kBlock, 0, st>>>(
+          n, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial);
+      break;
+    case 8:
+      k_elem_r_tiles<T, 8><<<(n + TileCfg<8>::TE - 1) / TileCfg<8>::TE, kBlock, 0, st>>>(
+          n, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial);
+      break;
+    case 10:
+      k_elem_r_tiles<T, 10><<<(n + TileCfg<10>::TE - 1) / TileCfg<10>::TE, kBlock, 0, st>>>(
+          n, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial);
+      break;
+  }
+  FEA_CHECK_LAUNCH("fea_elem_residual_tiles");
+  return FEA_OK;
+}
+
 }  // namespace fea
 
 using namespace fea;
@@ -567,6 +657,32 @@ extern "C" int fea_elem_residual(int elem_type, int dtype, int n_elem, const int
   if (dtype == FEA_F64)
     return launch_r<double>(elem_type, n_elem, conn, X, U, mp, dst_r, rstage, as_stream(stream));
   return launch_r<float>(elem_type, n_elem, conn, X, U, mp, dst_r, rstage, as_stream(stream));
+}
+
+extern "C" int fea_elem_tile_size(int elem_type) {
+  switch (elem_type) {
+    case 4: return TileCfg<4>::TE;
+    case 8: return TileCfg<8>::TE;
+    case 10: return TileCfg<10>::TE;
+  }
+  return 0;
+}
+
+extern "C" int fea_elem_residual_tiles(int elem_type, int dtype, int n_elem, const int32_t *conn, const double *X,
+                                       const double *U, const double *params_host, const int32_t *tile_lbase,
+                                       const int32_t *pslot_ptr, const uint16_t *pslot, double *partial,
+                                       void *stream) {
+  FEA_CHECK_ARG(valid_elem(elem_type), "fea_elem_residual_tiles: bad element type %d", elem_type);
+  FEA_CHECK_ARG(dtype == FEA_F32 || dtype == FEA_F64, "fea_elem_residual_tiles: bad dtype %d", dtype);
+  FEA_CHECK_ARG(n_elem >= 0 && params_host != nullptr, "fea_elem_residual_tiles: bad arguments");
+  if (n_elem == 0) return FEA_OK;
+  FEA_CHECK_ARG(conn && X && U && tile_lbase && pslot_ptr && pslot && partial, "fea_elem_residual_tiles: null input");
+  const MatParams mp = mat_from(params_host);
+  if (dtype == FEA_F64)
+    return launch_r_tiles<double>(elem_type, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial,
+                                  as_stream(stream));
+  return launch_r_tiles<float>(elem_type, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial,
+                               as_stream(stream));
 }
 
 extern "C" int fea_elem_energy_blocks(int n_elem) { return n_elem > 0 ? blocks_for(n_elem) : 0; }

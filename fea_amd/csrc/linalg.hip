@@ -147,7 +147,12 @@ __global__ __launch_bounds__(kBlock) void k_gather_nodal(int n_nodes, const int3
   out[3 * (int64_t)A + 2] = f[2];
 }
 
-__global__ __launch_bounds__(kBlock) void k_explicit_update(int n_nodes, const int32_t *__restrict__ rseg,
+// P = false: node A's forces are rstage[s*3 ..] for s in seg[A] .. seg[A+1] (node-major
+// staging); P = true: they are partial[list[s]*3 ..] (per-tile partial sums).  Summed in
+// that order in float64.
+template <bool P>
+__global__ __launch_bounds__(kBlock) void k_explicit_update(int n_nodes, const int32_t *__restrict__ seg,
+                                                            const int32_t *__restrict__ list,
                                                             const int32_t *__restrict__ dof_map,
                                                             const double *__restrict__ rstage,
                                                             const double *__restrict__ minv,
@@ -159,10 +164,11 @@ __global__ __launch_bounds__(kBlock) void k_explicit_update(int n_nodes, const i
   const int A = blockIdx.x * blockDim.x + threadIdx.x;
   if (A >= n_nodes) return;
   double f[3] = {0.0, 0.0, 0.0};
-  for (int s = rseg[A]; s < rseg[A + 1]; ++s) {
-    f[0] += rstage[(int64_t)s * 3 + 0];
-    f[1] += rstage[(int64_t)s * 3 + 1];
-    f[2] += rstage[(int64_t)s * 3 + 2];
+  for (int s = seg[A]; s < seg[A + 1]; ++s) {
+    const int64_t q = P ? (int64_t)list[s] * 3 : (int64_t)s * 3;
+    f[0] += rstage[q + 0];
+    f[1] += rstage[q + 1];
+    f[2] += rstage[q + 2];
   }
   if (fc != nullptr) {
     f[0] -= fc[3 * (int64_t)A + 0];
@@ -322,9 +328,22 @@ extern "C" int fea_explicit_update(int n_nodes, const int32_t *rseg, const int32
   FEA_CHECK_ARG(n_nodes >= 0 && rseg && dof_map && rstage && minv && a && vh && U,
                 "fea_explicit_update: bad arguments");
   if (n_nodes == 0) return FEA_OK;
-  k_explicit_update<<<blocks_for(n_nodes), kBlock, 0, as_stream(stream)>>>(n_nodes, rseg, dof_map, rstage,
-                                                                            minv, fc, dt_v, dt_u, a, vh, U);
+  k_explicit_update<false><<<blocks_for(n_nodes), kBlock, 0, as_stream(stream)>>>(
+      n_nodes, rseg, nullptr, dof_map, rstage, minv, fc, dt_v, dt_u, a, vh, U);
   FEA_CHECK_LAUNCH("fea_explicit_update");
+  return FEA_OK;
+}
+
+extern "C" int fea_explicit_update_tiles(int n_nodes, const int32_t *node_pptr, const int32_t *node_plist,
+                                         const int32_t *dof_map, const double *partial, const double *minv,
+                                         const double *fc, double dt_v, double dt_u, double *a, double *vh,
+                                         double *U, void *stream) {
+  FEA_CHECK_ARG(n_nodes >= 0 && node_pptr && node_plist && dof_map && partial && minv && a && vh && U,
+                "fea_explicit_update_tiles: bad arguments");
+  if (n_nodes == 0) return FEA_OK;
+  k_explicit_update<true><<<blocks_for(n_nodes), kBlock, 0, as_stream(stream)>>>(
+      n_nodes, node_pptr, node_plist, dof_map, partial, minv, fc, dt_v, dt_u, a, vh, U);
+  FEA_CHECK_LAUNCH("fea_explicit_update_tiles");
   return FEA_OK;
 }
 

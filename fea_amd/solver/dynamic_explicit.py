@@ -62,9 +62,56 @@ class DynamicExplicitSolver(BaseSolver):
         m = p.lumped_mass()
         m = torch.where(m.abs() < 1e-12, torch.ones_like(m), m)     # dynamic_explicit.py:165
         self._minv = 1.0 / m
+        self._setup_tiles()
         self._deltaT = self.estimate_critical_timestep()
         if self._deltaT <= 0:
             raise ValueError("Critical timestep must be positive.")
+
+    def _setup_tiles(self):
+        """Per-tile force reduction for the step loop when every group of the plan is an
+        element group (fea_elem_residual_tiles + fea_explicit_update_tiles): each workgroup
+        sums its Morton-ordered tile's element forces per node in LDS and writes one float64
+        partial per (tile, node); the update adds a node's partials.  Plans with surface-load
+        groups keep the node-major staging.  Launch arguments are bound once (the buffers
+        never move)."""
+        from ._explicit_tiles import group_tiles, node_partials
+        self._tiles = None
+        p = self.assembly.plan
+        groups = [g for g in p.groups if g.n > 0]
+        if not groups or any(g.kind != 'elem' for g in groups):
+            return
+        X = p.X.reshape(-1, 3).cpu().numpy()
+        free = (p.dof_map.reshape(-1, 3).cpu().numpy() >= 0).any(axis=1)
+        L = nat.lib()
+        tabs, base, qoff = [], 0, 0
+        for g in groups:
+            t = group_tiles(g.conn_np, X, free, L.fea_elem_tile_size(g.ntype), base)
+            t['pslot_ptr'] = t['pslot_ptr'] + qoff
+            base += t['pnode'].shape[0]
+            qoff += t['pslot'].shape[0]
+            tabs.append(t)
+        if qoff >= 2 ** 31 or 3 * base >= 2 ** 31:
+            return
+        dev = p.device
+        i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=dev)
+        pslot_ptr = i32(np.concatenate([t['pslot_ptr'][:-1] for t in tabs] + [[qoff]]))
+        pslot = torch.as_tensor(np.concatenate([t['pslot'] for t in tabs]).view(np.int16), device=dev)
+        pptr, plist = node_partials(np.concatenate([t['pnode'] for t in tabs]), p.n_nodes)
+        keep = {'pslot_ptr': pslot_ptr, 'pslot': pslot, 'node_pptr': i32(pptr), 'node_plist': i32(plist),
+                'partial': torch.zeros(3 * max(1, base), dtype=torch.float64, device=dev), 'conn': [], 'lbase': []}
+        code = F32 if self.precision == 'fp32' else F64
+        elem_args = []
+        for g, t in zip(groups, tabs):
+            conn = i32(g.conn_np[t['order']])
+            lb = i32(t['tile_lbase'])
+            keep['conn'].append(conn)
+            keep['lbase'].append(lb)
+            elem_args.append((g.ntype, code, g.n, conn.data_ptr(), p.X.data_ptr(), g.params.ctypes.data,
+                              lb.data_ptr(), pslot_ptr.data_ptr(), pslot.data_ptr(), keep['partial'].data_ptr()))
+        keep['params'] = [g.params for g in groups]
+        keep['elem_args'] = elem_args
+        keep['n_partials'] = base
+        self._tiles = keep
 
     def estimate_critical_timestep(self, safety_factor=0.8) -> float:
         if self._dt_fixed is not None:
@@ -170,12 +217,18 @@ class DynamicExplicitSolver(BaseSolver):
 
     def _contact_force(self, U):
         p = self.assembly.plan
+
+        def one(c):
+            st = c.state(p)
+            if c.search_interval == 1:
+                return st.step_forces(U)         # search + penalty every step, three launches
+            return st.forces(U, check=False)
+
         if len(self._contacts) == 1:
-            return self._contacts[0].state(p).forces(U, check=False)
+            return one(self._contacts[0])
         self._fc_sum.zero_()
         for c in self._contacts:
-            call('fea_axpy', self._fc_sum.numel(), 1.0, F64, c.state(p).forces(U, check=False), self._fc_sum,
-                 nat.stream_ptr())
+            call('fea_axpy', self._fc_sum.numel(), 1.0, F64, one(c), self._fc_sum, nat.stream_ptr())
         return self._fc_sum
 
     def _check_contacts(self):
@@ -186,9 +239,20 @@ class DynamicExplicitSolver(BaseSolver):
         """One central-difference step at the current U (in place)."""
         p = self.assembly.plan
         s = nat.stream_ptr()
-        p.residual_stage(U, load_scale, F32 if self.precision == 'fp32' else F64)
+        tl = self._tiles
+        if tl is None:
+            p.residual_stage(U, load_scale, F32 if self.precision == 'fp32' else F64)
+        else:
+            fn, u = nat.raw('fea_elem_residual_tiles'), U.data_ptr()
+            for a in tl['elem_args']:
+                nat.check_rc(fn(a[0], a[1], a[2], a[3], a[4], u, a[5], a[6], a[7], a[8], a[9], s),
+                             'fea_elem_residual_tiles')
         fc = None
         if self._contacts:
             fc = self._contact_force(U)
-        call('fea_explicit_update', p.n_nodes, p.rseg, p.dof_map, p.rstage, self._minv, fc,
-             0.5 * dt if first else dt, dt, acc, vh, U, s)
+        if tl is None:
+            call('fea_explicit_update', p.n_nodes, p.rseg, p.dof_map, p.rstage, self._minv, fc,
+                 0.5 * dt if first else dt, dt, acc, vh, U, s)
+        else:
+            call('fea_explicit_update_tiles', p.n_nodes, tl['node_pptr'], tl['node_plist'], p.dof_map,
+                 tl['partial'], self._minv, fc, 0.5 * dt if first else dt, dt, acc, vh, U, s)

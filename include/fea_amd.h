@@ -143,6 +143,19 @@ int fea_elem_residual(int elem_type, int dtype, int n_elem, const int32_t *conn,
                       const double *X, const double *U, const double *params_host,
                       const int32_t *dst_r, double *rstage, void *stream);
 
+/* The explicit loop's residual with a per-tile reduction (the force branch of
+ * structural_Force, C3base.py:428-431, and its scatter-add, assembly.py:347-380):
+ * workgroup t takes elements [t*TE, (t+1)*TE) (TE = fea_elem_tile_size(elem_type)),
+ * keeps their nodal forces (arithmetic in dtype) in LDS, and writes for every partial
+ * p in tile_lbase[t] .. tile_lbase[t+1] the float64 sum, in order, of the local slots
+ * pslot[pslot_ptr[p] .. pslot_ptr[p+1]) (slot = e_local*nen + a) to partial[3p ..].
+ * One partial per (tile, node): fea_explicit_update_tiles adds a node's partials. */
+int fea_elem_tile_size(int elem_type);
+int fea_elem_residual_tiles(int elem_type, int dtype, int n_elem, const int32_t *conn,
+                            const double *X, const double *U, const double *params_host,
+                            const int32_t *tile_lbase, const int32_t *pslot_ptr, const uint16_t *pslot,
+                            double *partial, void *stream);
+
 /* Strain energy per element block (Element_3D.potential_Energy,
  * C3base.py:387-411): writes one float64 partial sum per 256-element block to
  * partial[0 .. ceil(n_elem/256)). */
@@ -444,6 +457,14 @@ int fea_explicit_update(int n_nodes, const int32_t *rseg, const int32_t *dof_map
                         const double *rstage, const double *minv, const double *fc, double dt_v,
                         double dt_u, double *a, double *vh, double *U, void *stream);
 
+/* fea_explicit_update with the forces of fea_elem_residual_tiles: node A sums
+ * partial[node_plist[s]*3 ..] for s in node_pptr[A] .. node_pptr[A+1] in float64, in
+ * that order.  Replaces the same loop body of dynamic_explicit.py:100-114. */
+int fea_explicit_update_tiles(int n_nodes, const int32_t *node_pptr, const int32_t *node_plist,
+                              const int32_t *dof_map, const double *partial, const double *minv,
+                              const double *fc, double dt_v, double dt_u, double *a, double *vh,
+                              double *U, void *stream);
+
 /* ---------------------------------------------------------------- contact --
  * Node-to-surface penalty contact on T3 master facets with a uniform-grid broad
  * phase (new law; the reference's is the Gauss-point-pair barrier with a
@@ -486,6 +507,44 @@ int fea_contact_forces(int n_slave, const int32_t *slave, const double *X, const
                        const int32_t *nf_vert, int32_t *chosen, double *force, double *weight,
                        int32_t *sb_start, int32_t *sb_items, int32_t *work, int64_t n_dof, double *fc,
                        double *energy_partial, void *stream);
+
+/* Node-to-surface contact for explicit steps with reused candidate lists (it stands
+ * for the contact force the reference's explicit loop would take from its broken
+ * force-only barrier branch, contact.py:658-659).  fea_contact_rebuild searches with
+ * the radius radius + skin (grid build and search as above, cell >= radius + skin),
+ * stores each slave's candidates (cand/count), facet -> slave lists of those pairs
+ * (inv_start[F + 1], inv_items[S*kmax], slaves ascending), per master node the
+ * pairs over its facets (m_ptr[n_master + 1], m_items[2 * 3*S*kmax]: slave, facet*4 +
+ * local vertex; facet order, slaves ascending), the contact node positions x0 and
+ * clears dmax.  fea_contact_eval, per step: dmax = the largest squared
+ * displacement of a contact node since the rebuild; then, while dmax <= (skin/2)^2 and
+ * no stored list overflowed kmax (sup_max), every exact candidate (the predicate of
+ * fea_contact_search with `radius`) is in the stored lists, which the penalty and the
+ * master reactions use; otherwise they scan every facet / slave.  Either way the
+ * result is that of fea_contact_search + fea_contact_forces at U (choice ties go to
+ * the smaller facet id, as in the sorted candidate list), with fc written only at the
+ * contact nodes (fc[node] = master reaction + slave force; the caller zeroes fc once)
+ * and max_count raised by exact candidate counts above kmax.  The host rebuilds when
+ * dmax approaches (skin/2)^2; a late rebuild costs time, not results.
+ *   cnode[n_cnode]: every slave and master node once; cn_slave[i] / cn_master[i]:
+ *   its slave index / master index (into nf_ptr) or -1. */
+typedef struct fea_contact_set {
+  int32_t n_slave, n_facets, n_cnode, kmax, table_bits, n_master;
+  double cell, radius, skin, penalty, depth_max;
+  double origin[3];
+  const int32_t *slave, *conn, *cnode, *cn_slave, *cn_master, *nf_ptr, *nf_facet, *nf_vert;
+  const double *slave_area;                  /* may be NULL (area 1) */
+  double *centroid, *force, *weight, *x0;    /* [3F], [3S], [3S], [3*n_cnode] */
+  uint64_t *dmax;                            /* [1] */
+  int32_t *key, *bucket_start, *bucket_items, *work;   /* [F], [2^bits + 1], [F], [max(2^bits, F)] */
+  int32_t *cand, *count, *max_count, *chosen;          /* [S*kmax], [S], [1] (may be NULL), [S] */
+  int32_t *sup_max, *inv_start, *inv_items;            /* [1], [F + 1], [S*kmax] */
+  int32_t *m_ptr, *m_items;                            /* [n_master + 1], [6*S*kmax] */
+} fea_contact_set;
+int fea_contact_set_size(void);
+int fea_contact_rebuild(const fea_contact_set *set_host, const double *X, const double *U, void *stream);
+int fea_contact_eval(const fea_contact_set *set_host, const double *X, const double *U, double *fc,
+                     double *energy_partial, void *stream);
 
 /* ---------------------------------------------------------------- adjoint --
  * Vector-Jacobian products for sensitivities.  The reference differentiates by

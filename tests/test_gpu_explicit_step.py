@@ -1,0 +1,117 @@
+"""GPU: contact forces from the reused candidate lists (fea_contact_rebuild +
+fea_contact_eval) equal those of the exact multi-launch search bit for bit, whether the
+lists are fresh, older but within skin / 2, or stale (full scans), and the explicit step
+with the per-tile force
+reduction (fea_elem_residual_tiles + fea_explicit_update_tiles) stays within round-off
+of the node-major staged step and repeats bit for bit; the oracle checks of the same
+loop are in test_gpu_contact.py."""
+import numpy as np
+import pytest
+import torch
+
+from fea_amd import solver, workloads
+from tests.test_gpu_contact import pressed
+
+pytestmark = pytest.mark.gpu
+
+
+def _contact_nodes_shift(st, U, shift):
+    """U with every contact node moved by ``shift`` (a 3-vector) from U."""
+    V = U.clone().view(-1, 3)
+    V[st.cnode.long()] += torch.as_tensor(shift, dtype=torch.float64, device=U.device)
+    return V.view(-1)
+
+
+@pytest.mark.parametrize('n,seed,move', [(4, 0, 0.0), (7, 1, 0.0), (7, 1, 0.3), (7, 1, 0.7), (9, 2, 0.45)])
+def test_reused_lists_equal_exact_search(gpu, n, seed, move):
+    """fea_contact_rebuild at a state whose contact nodes sit move * skin away from U, then
+    fea_contact_eval at U: the stored lists (move <= 1/2) or the full scans (move > 1/2)
+    give the exact search's choice, forces, energy and contact-node forces bit for bit."""
+    fe, info, U = pressed(n, 0.3, seed)
+    print('reused contact lists n=%d seed=%d move=%.2f' % (n, seed, move), flush=True)
+    load = fe.assembly.get_load('contact')
+    st = load.state(fe.assembly.plan)
+    fc_ref = st.forces(U, search=True).clone()
+    e_ref = st.energy()
+    chosen_ref, f_ref, w_ref = st.chosen.clone(), st.force.clone(), st.weight.clone()
+    assert int((chosen_ref >= 0).sum()) > 0.3 * st.S
+    for t in (st.fc, st.force, st.weight, st.chosen):
+        t.fill_(-7)
+    st.fc.zero_()
+    d = move * load.skin / np.sqrt(3.0)
+    st.rebuild(_contact_nodes_shift(st, U, (d, -d, d)))
+    assert int(st.sup_max.item()) <= load.kmax
+    fc = st.step_forces(U)
+    torch.cuda.synchronize()
+    dmax = float(st.dmax.cpu().numpy().view(np.float64)[0])
+    assert abs(np.sqrt(dmax) - move * load.skin) < 1e-9
+    assert torch.equal(st.chosen, chosen_ref)
+    assert torch.equal(st.force, f_ref) and torch.equal(st.weight, w_ref)
+    assert torch.equal(fc, fc_ref)                      # every entry: contact nodes and zeros
+    assert st.energy() == e_ref
+    # the stored facet -> slave lists are the inverse of the stored candidate lists
+    S, kmax = st.S, load.kmax
+    cnt = np.minimum(st.vcount.cpu().numpy(), kmax)
+    cand = st.vcand.view(S, kmax).cpu().numpy()
+    pairs = sorted((int(f), s) for s in range(S) for f in cand[s, :cnt[s]])
+    istart, items = st.inv_start.cpu().numpy(), st.inv_items.cpu().numpy()
+    inv = sorted((f, int(x)) for f in range(st.F) for x in items[istart[f]:istart[f + 1]])
+    assert inv == pairs
+    for f in range(st.F):
+        assert np.all(np.diff(items[istart[f]:istart[f + 1]]) > 0)
+
+
+def test_reused_lists_rebuild_on_motion(gpu):
+    """The host rebuilds once the polled displacement passes skin / 4, and the forces
+    stay those of the exact search at every step."""
+    fe, info, U = pressed(5, 0.3, 3)
+    load = fe.assembly.get_load('contact')
+    st = load.state(fe.assembly.plan)
+    step = 0.1 * load.skin
+    for k in range(12):
+        Uk = _contact_nodes_shift(st, U, (0.0, 0.0, k * step))
+        fc = st.step_forces(Uk, poll_every=1).clone()
+        torch.cuda.synchronize()
+        fc_ref = st.forces(Uk, search=True).clone()
+        assert torch.equal(fc, fc_ref), k
+        st.fc.zero_()
+    assert st.rebuilds >= 2
+
+
+def _run_steps(precision, tiles, nsteps=60):
+    fe, info = workloads.two_blocks_contact(4, velocity=1.0, gap=0.01, jitter=0.1, seed=5)
+    fe.solver = solver.DynamicExplicitSolver(time_end=1.0, time_per_storage=1.0, dt=3e-3, store=False,
+                                             precision=precision)
+    fe.initialize()
+    sol = fe.solver
+    assert sol._tiles is not None
+    p = fe.assembly.plan
+    st = fe.assembly.get_load('contact').state(p)
+    if not tiles:
+        sol._tiles = None
+        sol._contact_force = lambda u: st.forces(u, search=True, check=False)     # the multi-launch contact
+    U = p.gc_to_u(torch.zeros_like(fe.assembly.GC))
+    vh = workloads.initial_velocity(fe, info).clone()
+    acc = torch.zeros_like(vh)
+    active = 0
+    for k in range(nsteps):
+        sol.step(U, vh, acc, sol._deltaT, first=(k == 0))
+        if k % 10 == 9:
+            active = max(active, int((st.chosen >= 0).sum()))
+    torch.cuda.synchronize()
+    return U.clone(), vh.clone(), acc.clone(), active
+
+
+@pytest.mark.parametrize('precision,tol', [('fp32', 1e-9), ('fp64', 1e-12)])
+def test_tile_step_matches_node_major_and_repeats(gpu, precision, tol):
+    """Per-tile partial sums change only the association of each node's float64 force sum:
+    60 contact steps stay within tol of the node-major staged step, and two runs agree bit
+    for bit."""
+    print('explicit tiles vs node-major, %s' % precision, flush=True)
+    U1, v1, a1, n1 = _run_steps(precision, True)
+    U0, v0, a0, n0 = _run_steps(precision, False)
+    U2, v2, a2, n2 = _run_steps(precision, True)
+    assert n1 > 0 and n1 == n0                          # contact was active
+    assert torch.equal(U1, U2) and torch.equal(v1, v2) and torch.equal(a1, a2)
+    rel = lambda x, y: float((x - y).abs().max() / y.abs().max())
+    assert rel(U1, U0) < tol and rel(v1, v0) < tol and rel(a1, a0) < 100 * tol
