@@ -61,7 +61,9 @@ class NodeToSurfaceContact(BaseLoad):
         if self.radius is None:
             self.radius = float(self.depth_max) + 1.25 * rho
         if self.skin is None:
-            self.skin = 0.25 * float(self.radius)
+            # a margin of 0.35 radius keeps the radius + skin candidate lists within kmax on
+            # structured tet surfaces (at most 47 candidates of 64, against 26 within radius)
+            self.skin = 0.35 * float(self.radius)
         self.origin = tri.reshape(-1, 3).min(0) - self.radius
         self._state = None
 
@@ -192,6 +194,8 @@ class ContactState:
         self._need_rebuild = True
         self._pin = torch.zeros(1, dtype=torch.int64).pin_memory() if torch.cuda.is_available() else None
         self._poll = None
+        self._reading = None
+        self._rebuilt_at = 0
         self.rebuilds = 0
 
     def search(self, U, check=True):
@@ -267,35 +271,62 @@ class ContactState:
         call('fea_contact_rebuild', ctypes.addressof(self.contact_set()), self.plan.X, U, nat.stream_ptr())
         self._need_rebuild = False
         self._poll = None
+        self._rebuilt_at = self.steps
+        self._reading = None
         self.rebuilds += 1
 
-    def step_forces(self, U, poll_every=8):
-        """Penalty forces at U from the reused candidate lists (fea_contact_eval): the forces
-        of ``forces(U, search=True)``, written only at the contact nodes of ``fc`` (its other
-        entries stay zero).  Every ``poll_every`` steps the largest contact-node displacement
-        since the last rebuild is copied to pinned memory; the next poll waits for that copy
-        (so the host runs at most ``poll_every`` steps ahead of what it has read) and, once the
-        displacement exceeds skin/4 (half the budget), the step rebuilds the lists.  The kmax
-        overflow check is left to ``check()``."""
-        s = nat.stream_ptr()
-        if self.steps % poll_every == 0:
-            if self._poll is not None:
-                self._poll.synchronize()
-                d2 = float(self._pin.numpy().view(np.float64)[0])
-                self._poll = None
-                if d2 > (0.25 * self.load.skin) ** 2:
-                    self._need_rebuild = True
-            if self._need_rebuild:
-                self.rebuild(U)
-        elif self._need_rebuild:
-            self.rebuild(U)
-        call('fea_contact_eval', ctypes.addressof(self.contact_set()), self.plan.X, U, self.fc, self.epart, s)
+    def eval(self, U):
+        """Penalty forces at U from the stored lists (fea_contact_eval, three launches): the
+        forces of ``forces(U, search=True)``, written only at the contact nodes of ``fc`` (its
+        other entries stay zero).  No host logic: capturable in a HIP graph."""
+        call('fea_contact_eval', ctypes.addressof(self.contact_set()), self.plan.X, U, self.fc, self.epart,
+             nat.stream_ptr())
         self.searched = True
+        return self.fc
+
+    def poll_issue(self):
+        """Copy dmax (the largest squared contact-node displacement since the rebuild) to
+        pinned memory behind the work queued so far."""
+        self._pin.copy_(self.dmax, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        self._poll = (ev, self.steps)
+
+    def poll_read(self):
+        """Wait for the pending poll (if any) and keep its displacement reading."""
+        if self._poll is None:
+            return
+        ev, at = self._poll
+        ev.synchronize()
+        self._poll = None
+        self._reading = (at, float(np.sqrt(max(0.0, self._pin.numpy().view(np.float64)[0]))))
+
+    def rebuild_due(self, ahead):
+        """Whether the lists should be rebuilt now: the displacement of the last reading,
+        extrapolated at 1.5 times its mean rate since the rebuild to ``ahead`` steps from now
+        (the next chance to rebuild), would pass 0.45 skin (the lists hold while it is at
+        most skin / 2)."""
+        if self._need_rebuild:
+            return True
+        if self._reading is None:
+            return False
+        at, d = self._reading
+        rate = d / max(1, at - self._rebuilt_at)
+        return d + 1.5 * rate * (self.steps - at + ahead) > 0.45 * self.load.skin
+
+    def step_forces(self, U, poll_every=8):
+        """``eval`` with the host's rebuild policy for step-by-step use: every ``poll_every``
+        steps the previous poll is read (so the host runs at most ``poll_every`` steps ahead
+        of its reading), the lists are rebuilt when ``rebuild_due``, and a new poll is issued.
+        The kmax overflow check is left to ``check()``."""
+        if self._need_rebuild or self.steps % poll_every == 0:
+            self.poll_read()
+            if self.rebuild_due(poll_every):
+                self.rebuild(U)
+        self.eval(U)
         self.steps += 1
         if self.steps % poll_every == 0:
-            self._pin.copy_(self.dmax, non_blocking=True)
-            self._poll = torch.cuda.Event()
-            self._poll.record()
+            self.poll_issue()
         return self.fc
 
     def energy(self):

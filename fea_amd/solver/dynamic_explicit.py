@@ -1,16 +1,27 @@
 """Central-difference explicit dynamics (reference: solver/dynamic_explicit.py:10-166).
 
-Each step is one residual pass (element kernel + surface loads) followed by the
-fused gather + update kernel (a = -M^-1 R, v_{n+1/2} = v_{n-1/2} + dt a,
-u_{n+1} = u_n + dt v_{n+1/2}); the lumped mass is the reference's row-sum of
-the consistent mass restricted to free DOFs (dynamic_explicit.py:147-166).
+Each step is one residual pass followed by the fused gather + update kernel
+(a = -M^-1 R, v_{n+1/2} = v_{n-1/2} + dt a, u_{n+1} = u_n + dt v_{n+1/2}); the lumped
+mass is the reference's row-sum of the consistent mass restricted to free DOFs
+(dynamic_explicit.py:147-166).  When every group is an element group the residual
+is reduced per tile of Morton-ordered elements in LDS (one float64 partial per tile
+and node, about two per node) instead of staged per element node; node-to-surface
+contact forces come from candidate lists searched with radius + skin and reused
+while no contact node has moved skin / 2 (the host polls the displacement every few
+steps and rebuilds; the kernels fall back to full scans when a rebuild comes late),
+in three launches per step.
 
 Additions: ``dt`` (fixed step) or ``cfl_factor`` (dt = factor * 2 / omega_max,
 the central difference's stability limit for the discrete system, omega_max^2
 the largest eigenvalue of M^-1 (K + K_contact) by power iteration on the
 device); the reference uses the constant 0.8 * 5e-6 (dynamic_explicit.py:54-64).
 ``precision='fp32'`` runs the element force arithmetic in FP32 (the nodal
-state, the update and the energy balance stay FP64).
+state, the update and the energy balance stay FP64).  ``overlap`` forms the
+contact forces on a side stream during the element pass and ``use_graph`` replays
+chunks of ``chunk`` steps from a captured HIP graph; both give the same results
+bit for bit and, on the two-block benchmark, no speed-up (the element pass fills
+the GPU, and the host enqueues a step faster than the GPU runs it), so both are off
+by default.
 """
 from __future__ import annotations
 
@@ -28,8 +39,12 @@ class DynamicExplicitSolver(BaseSolver):
 
     def __init__(self, time_end: float = 1.0, time_per_storage: float = 1e-4, *, dt: float | None = None,
                  cfl_factor: float | None = None, verbose: bool = False, store: bool = True,
-                 energy_interval: int | None = None, precision: str = 'fp64') -> None:
+                 energy_interval: int | None = None, precision: str = 'fp64', overlap: bool = False,
+                 use_graph: bool = False, chunk: int = 8) -> None:
         super().__init__()
+        self.overlap = bool(overlap)
+        self.use_graph = bool(use_graph)
+        self.chunk = int(chunk)
         if precision not in ('fp64', 'fp32'):
             raise ValueError("precision must be 'fp64' or 'fp32'")
         self.precision = precision
@@ -171,14 +186,27 @@ class DynamicExplicitSolver(BaseSolver):
         self.energy_history = []
         if self.energy_interval:
             self.energy_history.append((0.0, self.energy_balance(U, vh, None, dt)))
-        for step in range(nsteps):
-            self.step(U, vh, acc, dt, first=(step == 0))
-            if self._contacts and (step % 256 == 255 or step == nsteps - 1):
+        step = 0
+        while step < nsteps:
+            # steps up to the next event: a contact check (every 256 steps), an energy
+            # record, a stored state (t accumulated as below) or the end
+            m, tt = 0, t
+            while True:
+                m += 1
+                tt += dt
+                k = step + m
+                if (k == nsteps or (self._contacts and k % 256 == 0)
+                        or (self.energy_interval and k % self.energy_interval == 0)
+                        or (self.store and tt >= self._next_storage_time)):
+                    break
+            self.run_steps(U, vh, acc, dt, m, first=(step == 0))
+            step += m
+            t = tt
+            if self._contacts and (step % 256 == 0 or step == nsteps):
                 self._check_contacts()          # candidate-list overflow, one sync per 256 steps
-            t += dt
-            if self.energy_interval and ((step + 1) % self.energy_interval == 0 or step == nsteps - 1):
+            if self.energy_interval and (step % self.energy_interval == 0 or step == nsteps):
                 self.energy_history.append((t, self.energy_balance(U, vh, acc, dt)))
-            if self.store and (t >= self._next_storage_time or step == nsteps - 1):
+            if self.store and (t >= self._next_storage_time or step == nsteps):
                 gv = vh - 0.5 * dt * acc
                 self._GC_list.append(p.u_to_gc(U))
                 self._GV_list.append(gv)
@@ -215,13 +243,15 @@ class DynamicExplicitSolver(BaseSolver):
         pe = p.energy(U)
         return {'kinetic': ke, 'potential': pe, 'contact': ce, 'total': ke + pe + ce}
 
-    def _contact_force(self, U):
+    def _contact_force(self, U, host_logic=True):
         p = self.assembly.plan
 
         def one(c):
             st = c.state(p)
             if c.search_interval == 1:
-                return st.step_forces(U)         # search + penalty every step, three launches
+                # exact forces from reused candidate lists; the host polls and rebuilds them
+                # step by step, or between graph replays (host_logic=False)
+                return st.step_forces(U) if host_logic else st.eval(U)
             return st.forces(U, check=False)
 
         if len(self._contacts) == 1:
@@ -236,10 +266,25 @@ class DynamicExplicitSolver(BaseSolver):
             c.state(self.assembly.plan).check()
 
     def step(self, U, vh, acc, dt, first=False, load_scale=1.0):
-        """One central-difference step at the current U (in place)."""
+        """One central-difference step at the current U (in place), contact lists managed
+        step by step."""
+        self._step_kernels(U, vh, acc, dt, first, load_scale, host_logic=True)
+
+    def _step_kernels(self, U, vh, acc, dt, first=False, load_scale=1.0, host_logic=False):
+        """The launches of one step.  With ``overlap`` the contact forces are formed on a side
+        stream while the element pass runs (both only read U); the update waits for both.
+        host_logic=False: the contact forces come from the stored lists with no host reads
+        or rebuilds (the form captured in a HIP graph)."""
         p = self.assembly.plan
         s = nat.stream_ptr()
         tl = self._tiles
+        fc = None
+        side = self._side_stream() if (self._contacts and self.overlap) else None
+        if side is not None:
+            main = torch.cuda.current_stream()
+            side.wait_stream(main)                      # U of this step is written
+            with torch.cuda.stream(side):
+                fc = self._contact_force(U, host_logic)
         if tl is None:
             p.residual_stage(U, load_scale, F32 if self.precision == 'fp32' else F64)
         else:
@@ -247,12 +292,74 @@ class DynamicExplicitSolver(BaseSolver):
             for a in tl['elem_args']:
                 nat.check_rc(fn(a[0], a[1], a[2], a[3], a[4], u, a[5], a[6], a[7], a[8], a[9], s),
                              'fea_elem_residual_tiles')
-        fc = None
-        if self._contacts:
-            fc = self._contact_force(U)
+        if side is not None:
+            main.wait_stream(side)                      # fc is formed
+        elif self._contacts:
+            fc = self._contact_force(U, host_logic)
         if tl is None:
             call('fea_explicit_update', p.n_nodes, p.rseg, p.dof_map, p.rstage, self._minv, fc,
                  0.5 * dt if first else dt, dt, acc, vh, U, s)
         else:
             call('fea_explicit_update_tiles', p.n_nodes, tl['node_pptr'], tl['node_plist'], p.dof_map,
                  tl['partial'], self._minv, fc, 0.5 * dt if first else dt, dt, acc, vh, U, s)
+
+    def _side_stream(self):
+        st = getattr(self, '_side', None)
+        if st is None or st.device != torch.device('cuda', torch.cuda.current_device()):
+            st = self._side = torch.cuda.Stream()       # default priority
+        return st
+
+    def run_steps(self, U, vh, acc, dt, n, first=False):
+        """``n`` steps (the first one with the half-step velocity when ``first``).  With
+        ``use_graph`` (and reusable contact lists or no contact) steps run in chunks of
+        ``chunk`` replayed from a HIP graph captured once per (U, vh, acc, dt); between
+        replays the host reads the contact displacement poll of the chunk before last (the
+        GPU keeps the queued chunk), rebuilds the lists when due and issues the next poll.
+        The graph replays exactly the launches of ``step``: same results bit for bit."""
+        n = int(n)
+        if first and n > 0:
+            self.step(U, vh, acc, dt, first=True)
+            n -= 1
+        usable = self._tiles is not None and all(c.search_interval == 1 for c in self._contacts)
+        if not (self.use_graph and usable) or n < 2 * self.chunk:
+            for _ in range(n):
+                self.step(U, vh, acc, dt)
+            return
+        K = self.chunk
+        key = (U.data_ptr(), vh.data_ptr(), acc.data_ptr(), float(dt), K)
+        if getattr(self, '_graph_key', None) != key:
+            self._graph = None
+            states = [c.state(self.assembly.plan) for c in self._contacts]
+            for st in states:                           # lists valid for the capture's replays
+                if st.rebuild_due(2 * K):
+                    st.rebuild(U)
+            if self._contacts and self.overlap:
+                self._side_stream()
+            g = torch.cuda.CUDAGraph()
+            torch.cuda.synchronize()
+            with torch.cuda.graph(g):
+                for _ in range(K):
+                    self._step_kernels(U, vh, acc, dt)
+            torch.cuda.synchronize()
+            self._graph, self._graph_key = g, key
+        states = [c.state(self.assembly.plan) for c in self._contacts]
+        pending = [[] for _ in states]                  # per contact set: polls of the last two chunks
+        while n >= K:
+            for st, pend in zip(states, pending):
+                if len(pend) >= 2:                      # the chunk before last has finished
+                    st._poll = pend.pop(0)
+                    st.poll_read()
+                if st.rebuild_due(2 * K):
+                    st.rebuild(U)
+                    pend.clear()
+            self._graph.replay()
+            for st, pend in zip(states, pending):
+                st.steps += K
+                st.poll_issue()
+                pend.append(st._poll)
+                st._poll = None
+            n -= K
+        for st in states:
+            st._poll = None
+        for _ in range(n):
+            self.step(U, vh, acc, dt)

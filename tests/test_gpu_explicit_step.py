@@ -89,7 +89,7 @@ def _run_steps(precision, tiles, nsteps=60):
     st = fe.assembly.get_load('contact').state(p)
     if not tiles:
         sol._tiles = None
-        sol._contact_force = lambda u: st.forces(u, search=True, check=False)     # the multi-launch contact
+        sol._contact_force = lambda u, *_: st.forces(u, search=True, check=False)     # the multi-launch contact
     U = p.gc_to_u(torch.zeros_like(fe.assembly.GC))
     vh = workloads.initial_velocity(fe, info).clone()
     acc = torch.zeros_like(vh)
@@ -115,3 +115,34 @@ def test_tile_step_matches_node_major_and_repeats(gpu, precision, tol):
     assert torch.equal(U1, U2) and torch.equal(v1, v2) and torch.equal(a1, a2)
     rel = lambda x, y: float((x - y).abs().max() / y.abs().max())
     assert rel(U1, U0) < tol and rel(v1, v0) < tol and rel(a1, a0) < 100 * tol
+
+
+@pytest.mark.parametrize('overlap', [True, False])
+def test_graph_chunks_equal_eager_steps(gpu, overlap):
+    """run_steps replays captured 8-step chunks (contact on a side stream with overlap), with
+    the host's polls and list rebuilds between replays: the same trajectory as stepping one
+    call at a time, bit for bit, over 150 steps with contact active and lists rebuilt."""
+    print('explicit graph chunks vs eager, overlap=%s' % overlap, flush=True)
+    out = []
+    for graph in (True, False):
+        fe, info = workloads.two_blocks_contact(4, velocity=1.0, gap=0.01, jitter=0.1, seed=5)
+        fe.solver = solver.DynamicExplicitSolver(time_end=1.0, time_per_storage=1.0, dt=1.5e-3, store=False,
+                                                 precision='fp32', overlap=overlap, use_graph=graph)
+        fe.initialize()
+        sol = fe.solver
+        p = fe.assembly.plan
+        st = fe.assembly.get_load('contact').state(p)
+        U = p.gc_to_u(torch.zeros_like(fe.assembly.GC))
+        vh = workloads.initial_velocity(fe, info).clone()
+        acc = torch.zeros_like(vh)
+        active = 0
+        sol.run_steps(U, vh, acc, sol._deltaT, 30, first=True)
+        for _ in range(6):
+            sol.run_steps(U, vh, acc, sol._deltaT, 20)
+            torch.cuda.synchronize()
+            active = max(active, int((st.chosen >= 0).sum()))
+        sol._check_contacts()
+        out.append((U.clone(), vh.clone(), acc.clone(), active, st.rebuilds))
+    (U1, v1, a1, n1, r1), (U0, v0, a0, n0, r0) = out
+    assert n1 > 0 and r1 >= 2 and r0 >= 2
+    assert torch.equal(U1, U0) and torch.equal(v1, v0) and torch.equal(a1, a0)
