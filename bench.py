@@ -70,6 +70,8 @@ def parse():
     ap.add_argument('--serving-qps', type=float, default=2.0)
     ap.add_argument('--serving-emax', type=float, default=None)
     ap.add_argument('--serving-emin', type=float, default=2e3)
+    ap.add_argument('--serving-eager', action='store_true',
+                    help='serving: launch each PCG chunk eagerly instead of replaying the captured HIP graph')
     ap.add_argument('--cells', type=int, default=40, help='explicit: cells per block edge')
     ap.add_argument('--fd-directions', type=int, default=64, help='adjoint: central-difference check directions')
     ap.add_argument('--couple-pressure', type=float, default=0.06, help='couple: pressure on surface_1_All')
@@ -445,6 +447,9 @@ def bench_beam(args, torch):
 
 
 # ============================================================================ serving (C5)
+_SERVING_EAGER = False      # --serving-eager: the tick path launches the PCG chunk without the graph
+
+
 def _serving_run(torch, reqs):
     """The stream through the server twice -- saturated (every request queued at
     t = 0: batching throughput) and at its Poisson arrival times (latency) --
@@ -462,7 +467,7 @@ def _serving_run(torch, reqs):
             if s.rounds % 500 == 0:
                 log('serving (%s) tick %d t=%.2f in flight %d' % ('poisson' if realtime else 'saturated', s.rounds,
                                                                   time.perf_counter() - t0, len(s.active)))
-        srv = serving.SolveServer(precision='mixed', max_batch=64)
+        srv = serving.SolveServer(precision='mixed', max_batch=64, use_graph=not _SERVING_EAGER)
         srv.run(rs, realtime=realtime, on_round=on_tick)
         return srv, time.perf_counter() - t0
 
@@ -1095,6 +1100,8 @@ def main():
         sys.exit(2)
     import torch
     torch.set_num_threads(len(os.sched_getaffinity(0)))
+    global _SERVING_EAGER
+    _SERVING_EAGER = bool(args.serving_eager)
     fn = {'beam': bench_beam, 'cube': bench_cube, 'explicit': bench_explicit, 'adjoint': bench_adjoint,
           'serving': bench_serving, 'kernels': bench_kernels, 'couple': bench_couple,
           'contact': bench_contact}[args.workload]

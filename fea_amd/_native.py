@@ -116,6 +116,7 @@ _SIGS = {
     'fea_pcg_batch_start': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_INT, C_INT, C_P]),
     'fea_pcg_batch_spmv': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_INT, C_P]),
     'fea_pcg_batch_iterate': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_INT, C_INT, C_INT, C_P, C_P]),
+    'fea_pcg_batch_iterate_dev': (C_INT, [C_INT, C_INT, C_P, C_P, C_P, C_P, C_INT, C_INT, C_INT, C_P, C_P]),
     'fea_contact_forces': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_INT, C_P, C_P, C_INT, C_DBL, C_P, C_DBL, C_INT, C_P,
                                    C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_I64, C_P, C_P, C_P]),
 }

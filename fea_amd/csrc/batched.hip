@@ -10,7 +10,9 @@
 // produces bit-identical iterates alone and in any batch.
 //
 // Converged systems (ws[W_DONE] set) return at once, so a batch keeps running
-// until its slowest member finishes without disturbing the others.
+// until its slowest member finishes without disturbing the others.  A grid may be
+// larger than the last range (the caller pads launch sizes so that one captured graph
+// serves several batch layouts): its extra workgroups return at once.
 #include <climits>
 
 #include "pcg_device.h"
@@ -19,6 +21,16 @@ namespace fea {
 
 constexpr int kBatchSW = 16;  // sub-wave per node group in batched SpMV (fixed: batch-independent)
 constexpr int kSteps = 4;     // grid-stride steps per workgroup a system's grid is sized for
+
+// Launch dimensions of a batched stage: the system count and the stage's total of virtual
+// workgroups, by value, or (hdr != NULL) from a device header {n_sys, spmv_total,
+// vec_total} so that a captured launch serves any batch layout.  A physical workgroup
+// takes virtual workgroups blockIdx.x, blockIdx.x + gridDim.x, ...; each does the work a
+// workgroup of the exact grid does, so results do not depend on the grid size.
+struct BatchDims {
+  int n_sys, total;
+  const int32_t *hdr;
+};
 
 // index k of the system whose workgroup range [start[k], start[k+1]) holds b: the number of
 // systems whose range ends at or before b.  Every lane loads its (up to 16) table entries
@@ -49,11 +61,12 @@ __device__ __forceinline__ int sys_group_spmv(const fea_pcg_system &S, int g, in
 
 // r = b - A x and dinv = 1/|diag A| (1 where the diagonal is 0) for every system
 template <typename TM, typename TV>
-__global__ __launch_bounds__(kBlock) void k_batch_residual(int n_sys, const fea_pcg_system *__restrict__ sys,
-                                                           const int32_t *__restrict__ spmv_start) {
-  const int k = find_system(spmv_start, n_sys, blockIdx.x);
+__device__ __forceinline__ void batch_residual_vb(int n_sys, const fea_pcg_system *__restrict__ sys,
+                                                           const int32_t *__restrict__ spmv_start, int vb) {
+  const int k = find_system(spmv_start, n_sys, vb);
+  if (k >= n_sys) return;  // a workgroup of a padded grid, past the last range
   const fea_pcg_system S = sys[k];
-  const int lb = blockIdx.x - spmv_start[k];
+  const int lb = vb - spmv_start[k];
   const int ngroups = S.n_groups < 0 ? -S.n_groups : S.n_groups;
   constexpr int GPB = kBlock / kBatchSW;
   const int lane = threadIdx.x % kBatchSW;
@@ -97,13 +110,22 @@ __global__ __launch_bounds__(kBlock) void k_batch_residual(int n_sys, const fea_
   }
 }
 
+template <typename TM, typename TV>
+__global__ __launch_bounds__(kBlock) void k_batch_residual(BatchDims d, const fea_pcg_system *__restrict__ sys,
+                                                           const int32_t *__restrict__ spmv_start) {
+  const int n_sys = d.hdr != nullptr ? d.hdr[0] : d.n_sys;
+  const int total = d.hdr != nullptr ? d.hdr[1] : d.total;
+  for (int b = blockIdx.x; b < total; b += gridDim.x) batch_residual_vb<TM, TV>(n_sys, sys, spmv_start, b);
+}
+
 // z = dinv r, p = z; r.z and b.(dinv b) partials; iteration counter reset
 template <typename TV>
-__global__ __launch_bounds__(kBlock) void k_batch_init(int n_sys, const fea_pcg_system *__restrict__ sys,
-                                                       const int32_t *__restrict__ vec_start) {
-  const int k = find_system(vec_start, n_sys, blockIdx.x);
+__device__ __forceinline__ void batch_init_vb(int n_sys, const fea_pcg_system *__restrict__ sys,
+                                                       const int32_t *__restrict__ vec_start, int vb) {
+  const int k = find_system(vec_start, n_sys, vb);
+  if (k >= n_sys) return;  // a workgroup of a padded grid, past the last range
   const fea_pcg_system S = sys[k];
-  const int lb = blockIdx.x - vec_start[k];
+  const int lb = vb - vec_start[k];
   const TV *dinv = (const TV *)S.dinv;
   const TV *b = (const TV *)S.b;
   const TV *r = (const TV *)S.r;
@@ -152,15 +174,24 @@ __global__ __launch_bounds__(kBlock) void k_batch_init(int n_sys, const fea_pcg_
   }
 }
 
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_batch_init(BatchDims d, const fea_pcg_system *__restrict__ sys,
+                                                       const int32_t *__restrict__ vec_start) {
+  const int n_sys = d.hdr != nullptr ? d.hdr[0] : d.n_sys;
+  const int total = d.hdr != nullptr ? d.hdr[2] : d.total;
+  for (int b = blockIdx.x; b < total; b += gridDim.x) batch_init_vb<TV>(n_sys, sys, vec_start, b);
+}
+
 // q = A p and p.q partials; the system's first workgroup advances its iteration count
 template <typename TM, typename TV>
-__global__ __launch_bounds__(kBlock) void k_batch_spmv(int n_sys, const fea_pcg_system *__restrict__ sys,
-                                                       const int32_t *__restrict__ spmv_start) {
-  const int k = find_system(spmv_start, n_sys, blockIdx.x);
+__device__ __forceinline__ void batch_spmv_vb(int n_sys, const fea_pcg_system *__restrict__ sys,
+                                                       const int32_t *__restrict__ spmv_start, int vb) {
+  const int k = find_system(spmv_start, n_sys, vb);
+  if (k >= n_sys) return;  // a workgroup of a padded grid, past the last range
   const fea_pcg_system S = sys[k];
   double *ws = S.ws;
   if (ws[W_DONE] != 0.0) return;
-  const int lb = blockIdx.x - spmv_start[k];
+  const int lb = vb - spmv_start[k];
   const int ngroups = S.n_groups < 0 ? -S.n_groups : S.n_groups;
   constexpr int GPB = kBlock / kBatchSW;
   const int lane = threadIdx.x % kBatchSW;
@@ -185,15 +216,24 @@ __global__ __launch_bounds__(kBlock) void k_batch_spmv(int n_sys, const fea_pcg_
   if (lb == 0 && threadIdx.x == 0) ws[W_ITER] += 1.0;
 }
 
+template <typename TM, typename TV>
+__global__ __launch_bounds__(kBlock) void k_batch_spmv(BatchDims d, const fea_pcg_system *__restrict__ sys,
+                                                       const int32_t *__restrict__ spmv_start) {
+  const int n_sys = d.hdr != nullptr ? d.hdr[0] : d.n_sys;
+  const int total = d.hdr != nullptr ? d.hdr[1] : d.total;
+  for (int b = blockIdx.x; b < total; b += gridDim.x) batch_spmv_vb<TM, TV>(n_sys, sys, spmv_start, b);
+}
+
 // alpha = rz_{i-1} / pq; x += alpha p; r -= alpha q; z = dinv r; r.z partials
 template <typename TV>
-__global__ __launch_bounds__(kBlock) void k_batch_update(int n_sys, const fea_pcg_system *__restrict__ sys,
-                                                         const int32_t *__restrict__ vec_start) {
-  const int k = find_system(vec_start, n_sys, blockIdx.x);
+__device__ __forceinline__ void batch_update_vb(int n_sys, const fea_pcg_system *__restrict__ sys,
+                                                         const int32_t *__restrict__ vec_start, int vb) {
+  const int k = find_system(vec_start, n_sys, vb);
+  if (k >= n_sys) return;  // a workgroup of a padded grid, past the last range
   const fea_pcg_system S = sys[k];
   double *ws = S.ws;
   if (ws[W_DONE] != 0.0) return;
-  const int lb = blockIdx.x - vec_start[k];
+  const int lb = vb - vec_start[k];
   const int it = (int)ws[W_ITER];
   // r.z of the previous iteration: stored by its direction stage (a re-sum of the same
   // partials in the same order gives the same bits); summed here only in iteration 1
@@ -243,15 +283,24 @@ __global__ __launch_bounds__(kBlock) void k_batch_update(int n_sys, const fea_pc
   if (threadIdx.x == 0) ws[W_RZP + (it & 1) * PCG_RING + lb] = s;
 }
 
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_batch_update(BatchDims d, const fea_pcg_system *__restrict__ sys,
+                                                         const int32_t *__restrict__ vec_start) {
+  const int n_sys = d.hdr != nullptr ? d.hdr[0] : d.n_sys;
+  const int total = d.hdr != nullptr ? d.hdr[2] : d.total;
+  for (int b = blockIdx.x; b < total; b += gridDim.x) batch_update_vb<TV>(n_sys, sys, vec_start, b);
+}
+
 // convergence test (r.z / b.(dinv b) < tol or max_iter) and p = z + beta p
 template <typename TV>
-__global__ __launch_bounds__(kBlock) void k_batch_dir(int n_sys, const fea_pcg_system *__restrict__ sys,
-                                                      const int32_t *__restrict__ vec_start) {
-  const int k = find_system(vec_start, n_sys, blockIdx.x);
+__device__ __forceinline__ void batch_dir_vb(int n_sys, const fea_pcg_system *__restrict__ sys,
+                                                      const int32_t *__restrict__ vec_start, int vb) {
+  const int k = find_system(vec_start, n_sys, vb);
+  if (k >= n_sys) return;  // a workgroup of a padded grid, past the last range
   const fea_pcg_system S = sys[k];
   double *ws = S.ws;
   if (ws[W_DONE] != 0.0) return;
-  const int lb = blockIdx.x - vec_start[k];
+  const int lb = vb - vec_start[k];
   const int it = (int)ws[W_ITER];
   const double rz_new = sum_partials(ws + W_RZP + (it & 1) * PCG_RING, S.vec_blocks);
   const double rz_old = it > 1 ? ws[W_RZH + ((it - 1) & 1)]
@@ -275,9 +324,19 @@ __global__ __launch_bounds__(kBlock) void k_batch_dir(int n_sys, const fea_pcg_s
     p[i] = z[i] + beta * p[i];
 }
 
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_batch_dir(BatchDims d, const fea_pcg_system *__restrict__ sys,
+                                                      const int32_t *__restrict__ vec_start) {
+  const int n_sys = d.hdr != nullptr ? d.hdr[0] : d.n_sys;
+  const int total = d.hdr != nullptr ? d.hdr[2] : d.total;
+  for (int b = blockIdx.x; b < total; b += gridDim.x) batch_dir_vb<TV>(n_sys, sys, vec_start, b);
+}
+
 // status[2k] = iterations, status[2k+1] = done flag of system k
-__global__ __launch_bounds__(kBlock) void k_batch_status(int n_sys, const fea_pcg_system *__restrict__ sys,
+__global__ __launch_bounds__(kBlock) void k_batch_status(int n_arg, const int32_t *__restrict__ hdr,
+                                                         const fea_pcg_system *__restrict__ sys,
                                                          double *__restrict__ status) {
+  const int n_sys = hdr != nullptr ? hdr[0] : n_arg;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_sys) return;
   const double *ws = sys[k].ws;
@@ -292,17 +351,20 @@ static bool batch_combo(int m, int v) {
 template <typename TM, typename TV>
 static void batch_start_t(int n_sys, const fea_pcg_system *sys, const int32_t *ss, const int32_t *vs, int st_tot,
                           int vt_tot, hipStream_t st) {
-  k_batch_residual<TM, TV><<<st_tot, kBlock, 0, st>>>(n_sys, sys, ss);
-  k_batch_init<TV><<<vt_tot, kBlock, 0, st>>>(n_sys, sys, vs);
+  k_batch_residual<TM, TV><<<st_tot, kBlock, 0, st>>>(BatchDims{n_sys, st_tot, nullptr}, sys, ss);
+  k_batch_init<TV><<<vt_tot, kBlock, 0, st>>>(BatchDims{n_sys, vt_tot, nullptr}, sys, vs);
 }
 
+// grids of st_grid / vt_grid workgroups; with hdr the counts come from the device header
 template <typename TM, typename TV>
-static void batch_iterate_t(int n_sys, const fea_pcg_system *sys, const int32_t *ss, const int32_t *vs,
-                            int st_tot, int vt_tot, int iters, hipStream_t st) {
+static void batch_iterate_t(int n_sys, const int32_t *hdr, const fea_pcg_system *sys, const int32_t *ss,
+                            const int32_t *vs, int st_tot, int vt_tot, int st_grid, int vt_grid, int iters,
+                            hipStream_t st) {
+  const BatchDims ds{n_sys, st_tot, hdr}, dv{n_sys, vt_tot, hdr};
   for (int i = 0; i < iters; ++i) {
-    k_batch_spmv<TM, TV><<<st_tot, kBlock, 0, st>>>(n_sys, sys, ss);
-    k_batch_update<TV><<<vt_tot, kBlock, 0, st>>>(n_sys, sys, vs);
-    k_batch_dir<TV><<<vt_tot, kBlock, 0, st>>>(n_sys, sys, vs);
+    k_batch_spmv<TM, TV><<<st_grid, kBlock, 0, st>>>(ds, sys, ss);
+    k_batch_update<TV><<<vt_grid, kBlock, 0, st>>>(dv, sys, vs);
+    k_batch_dir<TV><<<vt_grid, kBlock, 0, st>>>(dv, sys, vs);
   }
 }
 
@@ -369,11 +431,14 @@ extern "C" int fea_pcg_batch_spmv(int mat_dtype, int vec_dtype, int n_sys, const
   FEA_CHECK_ARG(sys && spmv_start, "fea_pcg_batch_spmv: null argument");
   hipStream_t st = as_stream(stream);
   if (mat_dtype == FEA_F64)
-    k_batch_spmv<double, double><<<spmv_total, kBlock, 0, st>>>(n_sys, sys, spmv_start);
+    k_batch_spmv<double, double><<<spmv_total, kBlock, 0, st>>>(BatchDims{n_sys, spmv_total, nullptr}, sys,
+                                                                spmv_start);
   else if (vec_dtype == FEA_F64)
-    k_batch_spmv<float, double><<<spmv_total, kBlock, 0, st>>>(n_sys, sys, spmv_start);
+    k_batch_spmv<float, double><<<spmv_total, kBlock, 0, st>>>(BatchDims{n_sys, spmv_total, nullptr}, sys,
+                                                               spmv_start);
   else
-    k_batch_spmv<float, float><<<spmv_total, kBlock, 0, st>>>(n_sys, sys, spmv_start);
+    k_batch_spmv<float, float><<<spmv_total, kBlock, 0, st>>>(BatchDims{n_sys, spmv_total, nullptr}, sys,
+                                                              spmv_start);
   FEA_CHECK_LAUNCH("fea_pcg_batch_spmv");
   return FEA_OK;
 }
@@ -386,13 +451,35 @@ extern "C" int fea_pcg_batch_iterate(int mat_dtype, int vec_dtype, int n_sys, co
                 "fea_pcg_batch_iterate: bad sizes");
   FEA_CHECK_ARG(sys && spmv_start && vec_start, "fea_pcg_batch_iterate: null argument");
   hipStream_t st = as_stream(stream);
+  const int S = spmv_total, V = vec_total;
   if (mat_dtype == FEA_F64)
-    batch_iterate_t<double, double>(n_sys, sys, spmv_start, vec_start, spmv_total, vec_total, iters, st);
+    batch_iterate_t<double, double>(n_sys, nullptr, sys, spmv_start, vec_start, S, V, S, V, iters, st);
   else if (vec_dtype == FEA_F64)
-    batch_iterate_t<float, double>(n_sys, sys, spmv_start, vec_start, spmv_total, vec_total, iters, st);
+    batch_iterate_t<float, double>(n_sys, nullptr, sys, spmv_start, vec_start, S, V, S, V, iters, st);
   else
-    batch_iterate_t<float, float>(n_sys, sys, spmv_start, vec_start, spmv_total, vec_total, iters, st);
-  if (status != nullptr) k_batch_status<<<blocks_for(n_sys), kBlock, 0, st>>>(n_sys, sys, status);
+    batch_iterate_t<float, float>(n_sys, nullptr, sys, spmv_start, vec_start, S, V, S, V, iters, st);
+  if (status != nullptr) k_batch_status<<<blocks_for(n_sys), kBlock, 0, st>>>(n_sys, nullptr, sys, status);
   FEA_CHECK_LAUNCH("fea_pcg_batch_iterate");
+  return FEA_OK;
+}
+
+extern "C" int fea_pcg_batch_iterate_dev(int mat_dtype, int vec_dtype, const int32_t *hdr,
+                                         const fea_pcg_system *sys, const int32_t *spmv_start,
+                                         const int32_t *vec_start, int spmv_grid, int vec_grid, int iters,
+                                         double *status, void *stream) {
+  FEA_CHECK_ARG(batch_combo(mat_dtype, vec_dtype), "fea_pcg_batch_iterate_dev: unsupported precision pair");
+  FEA_CHECK_ARG(spmv_grid > 0 && vec_grid > 0 && iters >= 0, "fea_pcg_batch_iterate_dev: bad sizes");
+  FEA_CHECK_ARG(hdr && sys && spmv_start && vec_start, "fea_pcg_batch_iterate_dev: null argument");
+  hipStream_t st = as_stream(stream);
+  const int G = spmv_grid, H = vec_grid;
+  if (mat_dtype == FEA_F64)
+    batch_iterate_t<double, double>(0, hdr, sys, spmv_start, vec_start, 0, 0, G, H, iters, st);
+  else if (vec_dtype == FEA_F64)
+    batch_iterate_t<float, double>(0, hdr, sys, spmv_start, vec_start, 0, 0, G, H, iters, st);
+  else
+    batch_iterate_t<float, float>(0, hdr, sys, spmv_start, vec_start, 0, 0, G, H, iters, st);
+  if (status != nullptr)
+    k_batch_status<<<blocks_for(FEA_PCG_BATCH_MAX), kBlock, 0, st>>>(0, hdr, sys, status);
+  FEA_CHECK_LAUNCH("fea_pcg_batch_iterate_dev");
   return FEA_OK;
 }

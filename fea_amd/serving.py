@@ -330,7 +330,7 @@ class SolveServer:
     """
 
     def __init__(self, precision='mixed', hbm_budget=None, max_batch=64, rel_tol=1e-10, step_tol=1e-12,
-                 cg_tol=1e-8, cg_max_iter=20000, max_newton=200, chunk=32, use_graph=False,
+                 cg_tol=1e-8, cg_max_iter=20000, max_newton=200, chunk=32, use_graph=True,
                  armijo_slack=1e-12, verbose=False, predictor=True, preconditioner='block', admit_budget=0.004):
         nat.require_gpu()
         if precision not in ('fp64', 'mixed'):

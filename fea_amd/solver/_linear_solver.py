@@ -191,8 +191,9 @@ class BatchedPCG:
     criterion, _linear_solver.py:37) or max_iter; on return each ``x`` holds
     the solution.  The host reads one status array per chunk of ``chunk``
     iterations for the whole batch.  With ``use_graph`` the chunk (3 launches
-    per iteration plus the status kernel) is captured once per batch layout in
-    a HIP graph and replayed.  ``precond='block'``: node-block Jacobi for every
+    per iteration plus the status kernel, fixed grids that loop over the batch's
+    virtual workgroups, counts read from a device header) is captured once in a HIP
+    graph and replayed for every batch layout.  ``precond='block'``: node-block Jacobi for every
     system whose vectors were made with its row-group count (``PcgVectors(...,
     n_groups=)``), point Jacobi for the others.
     """
@@ -215,9 +216,12 @@ class BatchedPCG:
         self._ss_dev = torch.zeros(self.max_sys + 1, dtype=torch.int32, device=self.device)
         self._vs_dev = torch.zeros(self.max_sys + 1, dtype=torch.int32, device=self.device)
         self._status_dev = torch.zeros(2 * self.max_sys, dtype=torch.float64, device=self.device)
+        self._hdr_dev = torch.zeros(4, dtype=torch.int32, device=self.device)   # n, spmv / vec totals
         self._table = None             # the bytes of the table on the device
         self._graph = None
         self._graph_key = None
+        # the graph path launches fixed grids that loop over a batch's virtual workgroups
+        self.spmv_grid, self.vec_grid = 2048, 1024
         self.iterations = []
         self.chunks = 0
 
@@ -273,6 +277,8 @@ class BatchedPCG:
             self._sys_dev[:raw.size].copy_(torch.from_numpy(raw.copy()).pin_memory(), non_blocking=True)
             self._ss_dev[:n + 1].copy_(torch.from_numpy(ss).pin_memory(), non_blocking=True)
             self._vs_dev[:n + 1].copy_(torch.from_numpy(vs).pin_memory(), non_blocking=True)
+            hdr = np.array([n, ss[-1], vs[-1], 0], dtype=np.int32)
+            self._hdr_dev.copy_(torch.from_numpy(hdr).pin_memory(), non_blocking=True)
             self._table = blob
         return n, int(ss[-1]), int(vs[-1])
 
@@ -288,12 +294,34 @@ class BatchedPCG:
         call('fea_pcg_batch_start', self.mat_dtype, self.vec_dtype, n, self._sys_dev, self._ss_dev, self._vs_dev,
              st, vt, nat.stream_ptr())
 
+    def _iterate_dev(self, stream):
+        call('fea_pcg_batch_iterate_dev', self.mat_dtype, self.vec_dtype, self._hdr_dev, self._sys_dev, self._ss_dev,
+             self._vs_dev, self.spmv_grid, self.vec_grid, self.chunk, self._status_dev, stream)
+
+    def _chunk_graph(self):
+        """The chunk captured once (fixed grids, counts read from the device header): one
+        graph serves every batch layout; replays read the table uploaded before them."""
+        key = ('dev', self.chunk, self.spmv_grid, self.vec_grid)
+        if self._graph_key != key:
+            self._graph = None
+            g = torch.cuda.CUDAGraph()
+            torch.cuda.synchronize()
+            with torch.cuda.graph(g):
+                self._iterate_dev(nat.stream_ptr())
+            torch.cuda.synchronize()
+            self._graph, self._graph_key = g, key
+        return self._graph
+
     def iterate(self, entries):
         """Enqueue one chunk of iterations of the listed (started) systems; returns the
         device status tensor [n, 2] = (iterations, done) written after the chunk (the
-        next chunk overwrites it; systems already done are skipped by the chunk)."""
+        next chunk overwrites it; systems already done are skipped by the chunk).  With
+        ``use_graph`` the chunk is a replay of the one captured graph."""
         n, st, vt = self._layout(entries)
-        self._iterate(n, st, vt, nat.stream_ptr())
+        if self.use_graph:
+            self._chunk_graph().replay()
+        else:
+            self._iterate(n, st, vt, nat.stream_ptr())
         self.chunks += 1
         return self._status_dev[:2 * n].view(n, 2)
 
@@ -310,17 +338,7 @@ class BatchedPCG:
         call('fea_pcg_batch_start', self.mat_dtype, self.vec_dtype, n, self._sys_dev, self._ss_dev, self._vs_dev,
              st, vt, s)
         max_iter = max(int(e[4]) for e in entries)
-        graph = None
-        if self.use_graph:
-            key = (n, st, vt, self.chunk)
-            if self._graph_key != key:
-                self._graph = None
-                g = torch.cuda.CUDAGraph()
-                torch.cuda.synchronize()
-                with torch.cuda.graph(g):
-                    self._iterate(n, st, vt, nat.stream_ptr())
-                self._graph, self._graph_key = g, key
-            graph = self._graph
+        graph = self._chunk_graph() if self.use_graph else None
         done_iters = 0
         status = None
         while done_iters < max_iter:

@@ -417,6 +417,16 @@ int fea_pcg_batch_start(int mat_dtype, int vec_dtype, int n_sys, const fea_pcg_s
 int fea_pcg_batch_iterate(int mat_dtype, int vec_dtype, int n_sys, const fea_pcg_system *sys,
                           const int32_t *spmv_start, const int32_t *vec_start, int spmv_total,
                           int vec_total, int iters, double *status, void *stream);
+/* fea_pcg_batch_iterate with the batch's counts read on the device: hdr[3] =
+ * {n_sys, spmv_start[n_sys], vec_start[n_sys]} (device int32, written with the
+ * tables), and fixed grids of spmv_grid / vec_grid workgroups, each looping over the
+ * stage's virtual workgroups (same results as the exact grids).  Its launches do not
+ * depend on the batch layout, so one captured HIP graph of a chunk serves every batch
+ * the tables describe; status receives 2*n_sys doubles. */
+int fea_pcg_batch_iterate_dev(int mat_dtype, int vec_dtype, const int32_t *hdr,
+                              const fea_pcg_system *sys, const int32_t *spmv_start,
+                              const int32_t *vec_start, int spmv_grid, int vec_grid, int iters,
+                              double *status, void *stream);
 /* The SpMV stage of one batched iteration alone (q = A p and the p.q partials of
  * every unconverged system), for timing. */
 int fea_pcg_batch_spmv(int mat_dtype, int vec_dtype, int n_sys, const fea_pcg_system *sys,

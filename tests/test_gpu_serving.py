@@ -120,6 +120,24 @@ def test_batched_pcg_graph_replays_match_eager(gpu):
     assert bg._graph is not None
 
 
+def test_batched_pcg_one_graph_serves_changing_batches(gpu):
+    """The captured chunk reads the batch's counts on the device: one graph, replayed for
+    batches of different members and sizes (with eager work between replays), gives each
+    system's eager iterates bit for bit."""
+    systems = _systems(5, F32)
+    bg = BatchedPCG(F32, F64, 'cuda', chunk=8, use_graph=True)
+    for subset in ([0, 1, 2, 3, 4], [3], [4, 1], [0, 2, 3], [1, 2, 3, 4], [2]):
+        sel = [systems[i] for i in subset]
+        eager, it_e = _solve(BatchedPCG(F32, F64, 'cuda', chunk=8), sel, tol=1e-18)
+        torch.randn(1000, device='cuda').sum().item()
+        xg, it_g = _solve(bg, sel, tol=1e-18)
+        torch.cuda.synchronize()
+        assert it_g == it_e, subset
+        for a, b in zip(xg, eager):
+            assert torch.equal(a, b), subset
+    assert bg._graph is not None and bg._graph_key[0] == 'dev'
+
+
 def _small_requests(n=6, seed=3, e_range=(300, 2500), qps=0.0):
     return serving.random_requests(n, qps, seed=seed, e_range=e_range, aspect_range=(1.0, 2.5))
 
@@ -196,3 +214,20 @@ def test_requests_arriving_during_work_join_the_batch(gpu):
     assert len(set(srv.batch_sizes)) > 1
     for r in reqs:
         assert r.t_admit >= r.arrival - 1e-9 and r.latency >= 0
+
+
+def test_server_graph_chunks_identical_to_eager(gpu):
+    """The tick path replaying the captured PCG chunk (use_graph) gives every request's
+    solution bit for bit as the eager launches, with requests arriving while others run
+    (the batch layout changes between replays)."""
+    out = []
+    for graph in (True, False):
+        reqs = _small_requests(n=8, seed=11, qps=40.0)
+        srv = serving.SolveServer(precision='mixed', max_batch=16, use_graph=graph)
+        srv.run(reqs, realtime=True)
+        assert all(r.status == 'done' for r in reqs), [(r.rid, r.status, r.error) for r in reqs]
+        out.append({r.rid: r.GC.clone() for r in reqs})
+        if graph:
+            assert srv.pcg._graph is not None and srv.pcg.chunks > 3
+    for rid in out[0]:
+        assert torch.equal(out[0][rid], out[1][rid]), rid
