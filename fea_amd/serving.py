@@ -42,7 +42,9 @@ over load increments); ``precision`` is 'fp64' (FP64 tangent and PCG) or
 """
 from __future__ import annotations
 
+import concurrent.futures
 import math
+import os
 import time
 from dataclasses import dataclass, field
 
@@ -145,12 +147,14 @@ def request_model(spec):
     return _controller_from_mesh(mesh, mat, fixed_set=spec['fixed'], load=load), mesh
 
 
-def estimate_footprint(spec, precision='mixed'):
+def estimate_footprint(spec, precision='mixed', amg_min_dof=None):
     """Device bytes of a request before it is built (mesh sizes only).
 
     Counts the staging of every element's and loaded facet's node-pair blocks
     (the largest term), the CSR (neighbour count per node: 27 for bricks, 15
-    for Kuhn tets), the per-element maps and the nodal / free-DOF vectors."""
+    for Kuhn tets), the per-element maps and the nodal / free-DOF vectors, and,
+    when its 3N DOF reach amg_min_dof, the multigrid hierarchy and its PCG
+    vectors (about 600 bytes per DOF: 550 measured on the 418k-DOF beam)."""
     nx, ny, nz = spec['cells']
     N = (nx + 1) * (ny + 1) * (nz + 1)
     cells = {'x': ny * nz, 'y': nx * nz, 'z': nx * ny}
@@ -164,7 +168,8 @@ def estimate_footprint(spec, precision='mixed'):
     maps = E * nen * nen * 4 + 2 * E * nen * 4 + S * snen * snen * 4 + 2 * S * snen * 4
     csr = nnz * (4 + (8 if precision == 'fp64' else 4)) + 4 * N * 3 + 3 * 4 * nb * N
     vectors = 3 * N * (8 * 12 + 4)
-    return int(kstage + maps + csr + vectors + 64 * 1024)
+    amg = 600 * 3 * N if (amg_min_dof is not None and 3 * N >= amg_min_dof) else 0
+    return int(kstage + maps + csr + vectors + amg + 64 * 1024)
 
 
 def random_requests(n, qps, seed=0, e_range=(2e3, 2e5), aspect_range=(1.0, 4.0), load_steps=2):
@@ -281,6 +286,9 @@ class _Slot:
     state: str = 'tangent'
     max_iter: int = 0
     rd: object = None
+    amg: object = None          # multigrid hierarchy (requests of at least amg_min_dof free DOF)
+    apcg: object = None         # its PCG workspace (float64 vectors)
+    amg_job: object = None      # (future of its host set-up, submit time) until taken over
 
     @property
     def scale(self):
@@ -331,7 +339,8 @@ class SolveServer:
 
     def __init__(self, precision='mixed', hbm_budget=None, max_batch=64, rel_tol=1e-10, step_tol=1e-12,
                  cg_tol=1e-8, cg_max_iter=20000, max_newton=200, chunk=32, use_graph=True,
-                 armijo_slack=1e-12, verbose=False, predictor=True, preconditioner='block', admit_budget=0.004):
+                 armijo_slack=1e-12, verbose=False, predictor=True, preconditioner='block', admit_budget=0.004,
+                 amg_min_dof=30000, amg_chunk=8):
         nat.require_gpu()
         if precision not in ('fp64', 'mixed'):
             raise ValueError("precision must be 'fp64' or 'mixed'")
@@ -350,6 +359,13 @@ class SolveServer:
         self.verbose = verbose
         # node-block Jacobi by default (point Jacobi, the reference's scaling: 'jacobi')
         self.pcg = BatchedPCG(self.mat_dtype, F64, 'cuda', chunk=chunk, use_graph=use_graph, precond=preconditioner)
+        # requests of at least amg_min_dof free DOF solve their linear systems with the
+        # multigrid-preconditioned PCG (amg_chunk iterations per tick) instead of joining the
+        # batch; None: every request joins the batch
+        self.amg_min_dof = None if amg_min_dof is None else int(amg_min_dof)
+        self.amg_chunk = int(amg_chunk)
+        self.amg_setup_s = 0.0
+        self._pool = None
         self.active: list[_Slot] = []
         self._side = torch.cuda.Stream()     # admission uploads (default priority)
         self._retired = []
@@ -377,8 +393,55 @@ class SolveServer:
         _stats_launch(plan, plan.residual(U, 1.0, F64), rd)
         rss, _ = _stats_value(rd.fetch()[0])          # waits for this stream's work only
         slot.r_ref = math.sqrt(rss)
+        if self._uses_amg(plan.n_free) and plan.grp is not None and plan.n_groups != 0:
+            # the multigrid hierarchy from the float64 tangent at u = 0 under the full load (as
+            # the static solver builds it): its host set-up runs on the builder thread while
+            # the request starts with the batched PCG; the tick loop takes it over once ready
+            from .solver.amg import HostBuild
+            _, K = plan.tangent(U, 1.0, F64)
+            hb = HostBuild(plan, K, threads=max(1, len(os.sched_getaffinity(0)) - 2))
+            del K
+            slot.amg_job = (self._builder().submit(self._timed_build, hb), time.perf_counter())
         req.footprint = self._actual_footprint(slot)
         return slot
+
+    @staticmethod
+    def _timed_build(hb):
+        t0 = time.perf_counter()
+        h = hb.run()
+        return h, time.perf_counter() - t0
+
+    def _builder(self):
+        if self._pool is None:
+            self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+        return self._pool
+
+    def _take_amg(self, s):
+        """Once its host set-up has finished, move the request's hierarchy to the device and
+        give it a multigrid PCG workspace (float64 vectors)."""
+        job = s.amg_job
+        if job is None or not job[0].done():
+            return
+        from .solver.amg import AMG
+        from .solver._linear_solver import PCG
+        s.amg_job = None
+        h, secs = job[0].result()
+        p = s.plan
+        s.amg = AMG(p, None, handle=h)
+        self.amg_setup_s += secs + s.amg.setup_time
+        s.apcg = PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, self.mat_dtype, p.device, vec_dtype=F64,
+                     grp=p.grp, n_groups=p.spmv_groups, chunk=self.amg_chunk, precond='amg', spmv_col=p.spmv_col,
+                     amg=s.amg)
+
+    def _drop_amg(self, s):
+        """A retired request's hierarchy still on the builder: free its handle when done."""
+        job, s.amg_job = s.amg_job, None
+        if job is not None:
+            from .solver.amg import destroy_handle
+            job[0].add_done_callback(lambda f: destroy_handle(f.result()[0]))
+
+    def _uses_amg(self, n_free):
+        return self.amg_min_dof is not None and n_free >= self.amg_min_dof
 
     def _actual_footprint(self, s):
         p = s.plan
@@ -390,6 +453,10 @@ class SolveServer:
         # F64 staging (the non-element groups' only, under direct assembly), CSR values
         stage = p.n_kstage_rest if p.direct_assembly else p.n_kstage
         nb += 9 * stage * 8 + p.nnz * (8 if self.precision == 'fp64' else 4)
+        if s.amg is not None:
+            a = s.apcg
+            nb += s.amg.nbytes() + sum(t.numel() * t.element_size() for t in (a.x, a.r, a.z, a.p, a.q, a.b, a.dinv,
+                                                                               a.ws))
         return int(nb + s.vec.nbytes())
 
     def _retire(self, slot, status, now, reason=''):
@@ -404,6 +471,7 @@ class SolveServer:
             req.GC = torch.empty(gc.shape, dtype=gc.dtype, pin_memory=True)
             req.GC.copy_(gc, non_blocking=True)
         self.used -= req.footprint_est
+        self._drop_amg(slot)
         self.active.remove(slot)
         # its buffers came from the admission stream; they are released after the next
         # read, when no queued main-stream work can still touch them
@@ -464,25 +532,40 @@ class SolveServer:
         prev, prev_rd = self._chunk, None
         if prev:
             prev_rd = reads.add(self.pcg.status(len(prev)))
+        prev_amg = [(s, reads.add(s.apcg.ws[:2].clone())) for s in self._chunk_amg]
         new = by.get('pcg_new', [])
-        self.pcg.start([self._pcg_entry(s) for s in new])
         for s in new:
+            self._take_amg(s)
+        self.pcg.start([self._pcg_entry(s) for s in new if s.amg is None])
+        for s in new:
+            if s.amg is not None:
+                # multigrid PCG: node-block inverses and lambda_max of the new tangent, then
+                # r, z, p from b = -R and the warm start
+                a = s.apcg
+                a.set_matrix(s.K)
+                s.amg.pcg_start(self.mat_dtype, s.K, a.dinv, s.vec.b, s.vec.x, a.r, a.z, a.p, a.ws)
             s.state = 'pcg'
-        return by, prev, prev_rd
+        return by, prev, prev_rd, prev_amg
 
     def _launch_chunk(self):
         """One batched chunk of PCG iterations for every request in its PCG stage.  It is
         queued after the tick's read, so the device runs it while the host decides; its
         status is read at the next tick.  A request whose PCG finished in the previous
         chunk is still listed here (its result is not read yet): the chunk skips it."""
-        in_pcg = [s for s in self.active if s.state == 'pcg']
+        in_pcg = [s for s in self.active if s.state == 'pcg' and s.amg is None]
         if in_pcg:
             self.pcg.iterate([self._pcg_entry(s) for s in in_pcg])
         self._chunk = in_pcg
+        in_amg = [s for s in self.active if s.state == 'pcg' and s.amg is not None]
+        for s in in_amg:
+            a = s.apcg
+            s.amg.pcg_iterate(self.mat_dtype, s.K, a.dinv, s.vec.x, a.r, a.z, a.p, a.q, a.ws, self.cg_tol,
+                              s.max_iter, self.amg_chunk, None)
+        self._chunk_amg = in_amg
 
     def _decide(self, tick, host, now):
         """Advance every request's state machine from the tick's results."""
-        by, prev, prev_rd = tick
+        by, prev, prev_rd, prev_amg = tick
         for s in by.get('tangent', []):
             rss, _ = _stats_value(host[s.rd])
             if s.it_total >= self.max_newton or not math.isfinite(rss):
@@ -510,6 +593,13 @@ class SolveServer:
                     s.req.cg_iters += int(its)
                     s.it_total += 1
                     s.state = 'ls'
+        for s, i in prev_amg:
+            its, done = host[i][0], host[i][1]
+            if s.state == 'pcg' and done != 0.0:
+                s.dGC.copy_(s.vec.x)
+                s.req.cg_iters += int(its)
+                s.it_total += 1
+                s.state = 'ls'
         for s in by.get('ls', []):
             p = s.plan
             i_dd, i_e0, i_e1, i_d, i_u = s.rd
@@ -583,6 +673,7 @@ class SolveServer:
         qi = 0
         main = torch.cuda.current_stream()
         self._chunk = []
+        self._chunk_amg = []
         while qi < len(queue) or self.active:
             # one tick: the stage work of every request, one asynchronous read of its
             # results, then the next PCG chunk, which the device runs while the host admits
@@ -605,7 +696,7 @@ class SolveServer:
                        and len(self.active) + len(admitted) < self.max_batch
                        and (not admitted or now() - t < self.admit_budget)):
                     r = queue[qi]
-                    est = estimate_footprint(r.spec, self.precision)
+                    est = estimate_footprint(r.spec, self.precision, self.amg_min_dof)
                     if (self.active or admitted) and self.used + est > self.hbm_budget:
                         break
                     qi += 1
@@ -631,6 +722,9 @@ class SolveServer:
                     on_round(self)
             self.active.extend(admitted)
         torch.cuda.synchronize()
+        if self._pool is not None:
+            self._pool.shutdown(wait=True)
+            self._pool = None
         return requests
 
 

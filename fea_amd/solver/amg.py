@@ -57,7 +57,7 @@ class AMG:
     (dense inverse).
     """
 
-    def __init__(self, plan, K, degree=2, max_coarse=2000, max_levels=8, threads=None):
+    def __init__(self, plan, K, degree=2, max_coarse=2000, max_levels=8, threads=None, handle=None):
         import time
         t0 = time.perf_counter()
         L = _lib()
@@ -71,11 +71,10 @@ class AMG:
         self.device = plan.device
         n = plan.n_free
         self.n = n
-        # at least two levels: a small system keeps a coarsest level of about a tenth of its size
-        max_coarse = int(min(max_coarse, max(60, n // 10)))
-        self.max_coarse = max_coarse
-        h = _build(plan.rowptr_np, plan.col_np, K.detach().to(torch.float64).cpu().numpy(), plan.grp.cpu().numpy(),
-                   *dof_geometry(plan.dof_map_np, plan.X.cpu().numpy()), max_coarse, max_levels, threads)
+        if handle is None:
+            h = HostBuild(plan, K, max_coarse, max_levels, threads).run()
+        else:
+            h = handle          # built on the host already (HostBuild.run, e.g. on another thread)
         try:
             self._export(L, h)
         finally:
@@ -184,6 +183,31 @@ class AMG:
                                          _ptr(ws), float(tol), int(max_iter),
                                          int(iters), _np_ptr(status), nat.stream_ptr()),
               'fea_pcg_iterate_amg')
+
+
+class HostBuild:
+    """The host part of the set-up (fea_amg_build) separated from the device export: the
+    inputs are gathered on the calling thread (the tangent's values copied to the host),
+    and ``run()`` -- one native call, which releases the interpreter lock -- may then run
+    on another thread; its handle goes to ``AMG(plan, None, handle=...)`` on the thread
+    that owns the device work."""
+
+    def __init__(self, plan, K, max_coarse=2000, max_levels=8, threads=None):
+        n = plan.n_free
+        # at least two levels: a small system keeps a coarsest level of about a tenth of its size
+        self.max_coarse = int(min(max_coarse, max(60, n // 10)))
+        self.max_levels = int(max_levels)
+        self.threads = threads
+        self.args = (plan.rowptr_np, plan.col_np, K.detach().to(torch.float64).cpu().numpy(), plan.grp.cpu().numpy(),
+                     *dof_geometry(plan.dof_map_np, plan.X.cpu().numpy()))
+
+    def run(self):
+        return _build(*self.args, self.max_coarse, self.max_levels, self.threads)
+
+
+def destroy_handle(h):
+    """Free a host hierarchy that will not be exported (HostBuild.run's handle)."""
+    _lib().fea_amg_destroy(h)
 
 
 def dof_geometry(dof_map, X):

@@ -231,3 +231,22 @@ def test_server_graph_chunks_identical_to_eager(gpu):
             assert srv.pcg._graph is not None and srv.pcg.chunks > 3
     for rid in out[0]:
         assert torch.equal(out[0][rid], out[1][rid]), rid
+
+
+def test_server_multigrid_requests_match_one_at_a_time(gpu):
+    """Requests above amg_min_dof free DOF solve their linear systems with the
+    multigrid-preconditioned PCG, chunk by chunk, while the others share the batched PCG;
+    every result matches the one-at-a-time FP64 solve (1e-8), and the multigrid path ran."""
+    reqs = _small_requests(n=6, seed=21, e_range=(800, 4000), qps=30.0)
+    srv = serving.SolveServer(precision='mixed', max_batch=16, amg_min_dof=2500, amg_chunk=4)
+    srv.run(reqs, realtime=True)
+    assert all(r.status == 'done' for r in reqs), [(r.rid, r.status, r.error) for r in reqs]
+    assert srv.amg_setup_s > 0.0
+    ref, _, _ = serving.solve_one_at_a_time(reqs, precision='fp64')
+    n_amg = 0
+    for r in reqs:
+        fe, _ = serving.request_model(r.spec)
+        fe.assembly.initialize()
+        n_amg += fe.assembly.plan.n_free >= 2500
+        assert rel_err(r.GC.numpy(), ref[r.rid].numpy()) < 1e-8, r.rid
+    assert 0 < n_amg < len(reqs)
