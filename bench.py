@@ -77,7 +77,9 @@ def parse():
     ap.add_argument('--couple-pressure', type=float, default=0.06, help='couple: pressure on surface_1_All')
     ap.add_argument('--couple-increments', type=int, default=4, help='couple: load increments')
     ap.add_argument('--couple-predictor', default='auto', help='cube / couple / contact: load-increment predictor')
-    ap.add_argument('--couple-preconditioner', default='block', choices=['block', 'jacobi'],
+    ap.add_argument('--cube-preconditioner', default='amg', choices=['block', 'jacobi', 'amg'],
+                    help='cube: PCG preconditioner')
+    ap.add_argument('--couple-preconditioner', default='block', choices=['block', 'jacobi', 'amg'],
                     help='cube / couple / contact: PCG preconditioner')
     return ap.parse_args()
 
@@ -619,7 +621,7 @@ def bench_cube(args, torch):
     fe.solver = solver.StaticImplicitSolver(precision='fp64', rel_tol=1e-10, load_steps=5, cg_tol=1e-12,
                                             cg_max_iter=20000, armijo_slack=1e-12,
                                             predictor=_pred(args, args.couple_predictor),
-                                            preconditioner=args.couple_preconditioner)
+                                            preconditioner=args.cube_preconditioner)
     fe.initialize()
     p = fe.assembly.plan
     for _ in range(warmup):
@@ -648,7 +650,7 @@ def bench_cube(args, torch):
                      {'workload': 'unit cube 10x10x10 Kuhn-split C3D4 (6000 elements, 3630 free DOF), NeoHookean '
                                   'mu=1 kappa=100, z=0 fixed, follower pressure 0.2 on z=1, 5 increments, Newton rel '
                                   'tol 1e-10, 5% interior jitter', 'precision': 'fp64',
-                      'predictor': args.couple_predictor, 'preconditioner': args.couple_preconditioner})
+                      'predictor': args.couple_predictor, 'preconditioner': args.cube_preconditioner})
     line.update({'latency_p50_s': float(np.percentile(lat, 50)), 'latency_p99_s': float(np.percentile(lat, 99)),
                  'newton_iters_per_solve': stats['newton_iters'], 'cg_iters_per_solve': stats['cg_iters'],
                  'predictor_orders': stats.get('predictor_orders'),
