@@ -79,6 +79,8 @@ def parse():
     ap.add_argument('--couple-predictor', default='auto', help='cube / couple / contact: load-increment predictor')
     ap.add_argument('--cube-preconditioner', default='amg', choices=['block', 'jacobi', 'amg'],
                     help='cube: PCG preconditioner')
+    ap.add_argument('--adjoint-preconditioner', default='amg', choices=['block', 'jacobi', 'amg'],
+                    help='adjoint: preconditioner of the forward and the adjoint PCG')
     ap.add_argument('--couple-preconditioner', default='block', choices=['block', 'jacobi', 'amg'],
                     help='cube / couple / contact: PCG preconditioner')
     return ap.parse_args()
@@ -984,7 +986,7 @@ def bench_adjoint(args, torch):
     fe, mesh = workloads.void_block_pressure()
     fe.solver = solver.StaticImplicitSolver(precision='fp64', rel_tol=1e-11, load_steps=2, cg_tol=1e-22,
                                             cg_max_iter=20000, armijo_slack=1e-12,
-                                            preconditioner=args.couple_preconditioner)
+                                            preconditioner=args.adjoint_preconditioner)
     fe.initialize()
     p = fe.assembly.plan
     E = sum(g.n for g in p.groups if g.kind == 'elem')
@@ -1028,7 +1030,8 @@ def bench_adjoint(args, torch):
                      'gradients/s', steps, warmup, elapsed, 'f64',
                      {'workload': '2x1x1 block with a perturbed spherical void (r=0.3, Fourier amplitude 0.02), '
                                   '%d C3D10, NeoHookean mu=1 kappa=100, pressure 0.05 in the void, x=0 fixed, '
-                                  'gradient w.r.t. all %d nodal coordinates' % (E, 3 * p.n_nodes)})
+                                  'gradient w.r.t. all %d nodal coordinates' % (E, 3 * p.n_nodes),
+                      'preconditioner': args.adjoint_preconditioner})
     line.update({'compliance': J0, 'fd_check_rel_err_max': max(errs), 'fd_check_rel_err_median': float(np.median(errs)),
                  'fd_check_err_over_grad_norm_max': max(nerrs), 'fd_directions': len(errs), 'fd_h': h,
                  'roofline': roofline('k_elem_adjoint<10> (fea_elem_adjoint + nodal gather)', flops / t_vjp / 1e12,

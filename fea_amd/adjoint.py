@@ -13,7 +13,8 @@ quantities from (X, U, lambda):
 * ``PressureForce``     the follower-pressure external force -p dV/dx, nodal;
                         backward: -p Hess(V) g for both X and U (fea_surface_pressure_hvp).
 * ``StaticEquilibrium`` U*(X, theta) from the static Newton solver; backward:
-                        the adjoint solve K(U*) lambda = g (node-block Jacobi PCG, FP64)
+                        the adjoint solve K(U*) lambda = g (FP64 PCG: the forward solver's
+                        multigrid hierarchy when it has one, node-block Jacobi otherwise)
                         then -lambda . dR/dX and -lambda . dR/dtheta with
                         R = f_int - f_ext (pressure).
 * ``compliance_gradient``  J = f_ext(U*, X) . U* and dJ/dX, dJ/dtheta.
@@ -213,8 +214,12 @@ class StaticEquilibrium(torch.autograd.Function):
         plan.set_coordinates(X)
         _set_theta(plan, theta)
         _, K = plan.tangent(U, 1.0, F64)
+        # the forward solver's multigrid hierarchy preconditions the adjoint solve too when it
+        # has one (its finest level smooths with this K); node-block Jacobi otherwise
+        amg = getattr(fe.solver, 'amg', None) if getattr(fe.solver, 'preconditioner', None) == 'amg' else None
         pcg = PCG(plan.n_free, plan.nnz, plan.rowptr, plan.col, plan.diag_slot, F64, plan.device,
-                  grp=plan.grp, n_groups=plan.spmv_groups, spmv_col=plan.spmv_col, precond='block')
+                  grp=plan.grp, n_groups=plan.spmv_groups, spmv_col=plan.spmv_col,
+                  precond='amg' if amg is not None else 'block', amg=amg)
         pcg.set_matrix(K)
         rhs = plan.u_to_gc(_vec(g, plan)).contiguous()
         lam_free = pcg.solve(rhs, None, tol=ctx.tol, max_iter=ctx.max_iter)

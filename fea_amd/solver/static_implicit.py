@@ -129,6 +129,11 @@ class StaticImplicitSolver(BaseSolver):
             raise NotImplementedError('node-to-surface contact has no tangent; use DynamicExplicitSolver')
         self._pcg = {}
         if self.preconditioner == 'amg':
+            if getattr(assembly.plan, 'n_rp', 0):
+                # the hierarchy aggregates node DOF triples with their rigid-body modes; the
+                # reduced system of a Couple model also has 6 generalized DOFs per point
+                raise NotImplementedError("preconditioner='amg' is not available for models with reference "
+                                          "points (Couple constraints): use 'block' or 'jacobi'")
             self._build_amg()
 
     def _build_amg(self):
