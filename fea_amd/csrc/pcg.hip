@@ -161,7 +161,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const TV *__restri
 // also ends a solve with b = 0 where the ratio is 0/0) and p = z + beta p
 template <typename TV>
 __global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, const TV *__restrict__ z, TV *__restrict__ p,
-                                                    double *__restrict__ ws, double tol, int max_iter, int gu) {
+                                                    double *__restrict__ ws, double tol, int max_iter, int gu,
+                                                    int gu_start) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   TV vz[kPre], vp[kPre];
@@ -175,13 +176,14 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(int n, const TV *__restrict_
   }
   if (ws[W_DONE] != 0.0) return;
   const int it = (int)ws[W_ITER];
-  // gu: workgroups of the init / update stages, i.e. the number of their partials
+  // gu: the number of this iteration's r.z partials (workgroups of the stage that wrote them);
+  // gu_start: that of the start stage's r.z and b.z partials (read in iteration 1)
   const double rz_new = sum_partials(ws + W_RZP + (it & 1) * PCG_RING, gu);
   const double rz_old = it > 1 ? ws[W_RZH + ((it - 1) & 1)]
-                               : sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_RING, gu);
+                               : sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_RING, gu_start);
   // b.z is rewritten with the same value by every direction stage: from iteration 2 on
   // it is read back (workgroup 0 of this launch may be storing the identical bits)
-  const double bz = it > 1 ? ws[W_BZ] : sum_partials(ws + W_BZP, gu);
+  const double bz = it > 1 ? ws[W_BZ] : sum_partials(ws + W_BZP, gu_start);
   const bool done = (rz_new == 0.0) || (rz_new / bz < tol) || (it >= max_iter) || !(rz_new == rz_new);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ws[W_RZ] = rz_new;
@@ -546,7 +548,7 @@ static int pcg_iterate_t(const PcgArgs &a) {
     else
       k_pcg_update<TV><<<gu, kBlock, 0, a.st>>>(a.n, (const TV *)a.dinv, (TV *)a.x, (TV *)a.r, (TV *)a.z,
                                                (const TV *)a.p, (const TV *)a.q, a.ws);
-    k_pcg_dir<TV><<<PCG_G, kBlock, 0, a.st>>>(a.n, (const TV *)a.z, (TV *)a.p, a.ws, a.tol, a.max_iter, gu);
+    k_pcg_dir<TV><<<PCG_G, kBlock, 0, a.st>>>(a.n, (const TV *)a.z, (TV *)a.p, a.ws, a.tol, a.max_iter, gu, gu);
   }
   FEA_CHECK_LAUNCH("fea_pcg_iterate");
   if (a.status_host != nullptr) {
@@ -793,18 +795,21 @@ __device__ __forceinline__ void group6_spmv(int g, int lane, const int32_t *__re
 __device__ __forceinline__ bool amg_done(const double *done) { return done != nullptr && *done != 0.0; }
 
 // one Chebyshev step on level 0: r = b - A x (FIRST: x = 0, no product), d = c1 d + c2 M^-1 r,
-// xo = x + d with M the node-block diagonal (6 packed floats per group)
-template <typename TM, int SW, int MODE, int FIRST>
+// xo = x + d with M the node-block diagonal (6 packed floats per group).  RZ (the last
+// post-smoothing step of a PCG iteration's V-cycle, xo = z, b = the PCG residual): also the
+// workgroup's partial of b.xo = r.z into ring slot (iteration & 1) of the PCG workspace rz_ws.
+template <typename TM, int SW, int MODE, int FIRST, int RZ = 0>
 __global__ __launch_bounds__(kBlock) void k_amg_cheb0(int ngroups, const int32_t *__restrict__ grp,
                                                       const int32_t *__restrict__ rowptr,
                                                       const int32_t *__restrict__ col, const TM *__restrict__ val,
                                                       const float *__restrict__ binv, const double *__restrict__ b,
                                                       const double *__restrict__ x, double *__restrict__ xo,
                                                       double *__restrict__ d, double c1, double c2,
-                                                      const double *__restrict__ done) {
+                                                      const double *__restrict__ done, double *__restrict__ rz_ws = nullptr) {
   if (amg_done(done)) return;
   constexpr int GPB = kBlock / SW;
   const int lane = threadIdx.x % SW;
+  double rz = 0.0;
   for (int64_t g0 = (int64_t)blockIdx.x * GPB; g0 < ngroups; g0 += (int64_t)gridDim.x * GPB) {
     const int g = (int)(g0 + threadIdx.x / SW);
     if (g >= ngroups) continue;
@@ -819,8 +824,14 @@ __global__ __launch_bounds__(kBlock) void k_amg_cheb0(int ngroups, const int32_t
       const int a = lane;
       const double dn = (c1 != 0.0 ? c1 * d[r0 + a] : 0.0) + c2 * w[a];
       d[r0 + a] = dn;
-      xo[r0 + a] = (FIRST ? 0.0 : x[r0 + a]) + dn;
+      const double xn = (FIRST ? 0.0 : x[r0 + a]) + dn;
+      xo[r0 + a] = xn;
+      if (RZ) rz += b[r0 + a] * xn;
     }
+  }
+  if (RZ) {
+    const int it = (int)rz_ws[W_ITER];
+    write_partial(rz_ws + W_RZP + (it & 1) * PCG_RING, rz);
   }
 }
 
@@ -946,7 +957,8 @@ __global__ __launch_bounds__(kBlock) void k_amg_prolong(int n, const int32_t *__
   }
 }
 
-// x = inv b on the coarsest level: one wave per row
+// x = inv b on the coarsest level: one wave per row.  Rows of a multiple of 4 entries are read
+// as float4 (a lane takes 4 consecutive entries per step) with two accumulators.
 __global__ __launch_bounds__(kBlock) void k_amg_dense(int n, const float *__restrict__ inv,
                                                       const double *__restrict__ b, double *__restrict__ x,
                                                       const double *__restrict__ done) {
@@ -954,8 +966,20 @@ __global__ __launch_bounds__(kBlock) void k_amg_dense(int n, const float *__rest
   const int lane = threadIdx.x & 63;
   const int64_t row = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   if (row >= n) return;
-  double s = 0.0;
-  for (int j = lane; j < n; j += 64) s += (double)inv[row * n + j] * b[j];
+  double s = 0.0, t = 0.0;
+  if ((n & 3) == 0) {
+    const float4 *r4 = reinterpret_cast<const float4 *>(inv + row * n);
+    const int n4 = n >> 2;
+    for (int j = lane; j < n4; j += 64) {
+      const float4 v = r4[j];
+      const double *bj = b + 4 * j;
+      s += (double)v.x * bj[0] + (double)v.y * bj[1];
+      t += (double)v.z * bj[2] + (double)v.w * bj[3];
+    }
+  } else {
+    for (int j = lane; j < n; j += 64) s += (double)inv[row * n + j] * b[j];
+  }
+  s += t;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if (lane == 0) x[row] = s;
@@ -1008,13 +1032,55 @@ __global__ __launch_bounds__(kBlock) void k_amg_cheb0_first(int ngroups, const i
   }
 }
 
+// The PCG update fused with the V-cycle's first level-0 step: alpha = r.z / p.q (re-summed
+// partials, as the stages of the plain PCG do), x += alpha p, r -= alpha q, then the first Chebyshev step
+// from 0 on the new r: d = xo = c2 M^-1 r.  One thread per row group.  gu_rz: the number of
+// r.z partials in ring slot 0 (the start stage's), read in iteration 1 only.
+__global__ __launch_bounds__(kBlock) void k_amg_pcg_update_first(int ngroups, const int32_t *__restrict__ grp,
+                                                                 const float *__restrict__ binv,
+                                                                 double *__restrict__ x, double *__restrict__ r,
+                                                                 const double *__restrict__ p,
+                                                                 const double *__restrict__ q,
+                                                                 double *__restrict__ xo, double *__restrict__ d,
+                                                                 double c2, double *__restrict__ ws, int gu_rz) {
+  if (ws[W_DONE] != 0.0) return;
+  const int it = (int)ws[W_ITER];
+  const int g1 = (int)ws[W_G1];
+  const double rz_old = it > 1 ? ws[W_RZH + ((it - 1) & 1)]
+                               : sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_RING, gu_rz);
+  const double pq = sum_partials(ws + W_PQ, g1);
+  const double alpha = pq != 0.0 ? rz_old / pq : 0.0;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += (int64_t)gridDim.x * blockDim.x) {
+    const int r0 = grp[g], nr = grp[g + 1] - r0;
+    double rr[3] = {0.0, 0.0, 0.0}, w[3];
+    for (int a = 0; a < nr; ++a) {
+      const int i = r0 + a;
+      x[i] += alpha * p[i];
+      const double ri = r[i] - alpha * q[i];
+      r[i] = ri;
+      rr[a] = ri;
+    }
+    bj_apply(binv + 6 * g, rr, w);
+    for (int a = 0; a < nr; ++a) {
+      const double dn = c2 * w[a];
+      d[r0 + a] = dn;
+      xo[r0 + a] = dn;
+    }
+  }
+}
+
 template <typename TM>
 static void cheb0(const AmgFine<TM> &f, bool first, const double *b, const double *x, double *xo, double *d,
-                  double c1, double c2, const double *done, hipStream_t st) {
+                  double c1, double c2, const double *done, hipStream_t st, double *rz_ws = nullptr) {
   const int grid = spmv_grid(f.s);
   const auto &s = f.s;
   if (first) {
     k_amg_cheb0_first<<<amg_grid(s.ngroups, kBlock), kBlock, 0, st>>>(s.ngroups, f.grp, f.binv, b, xo, d, c2, done);
+  } else if (rz_ws != nullptr) {
+#define FEA_AC2(SW, MD) \
+  k_amg_cheb0<TM, SW, MD, 0, 1><<<grid, kBlock, 0, st>>>(s.ngroups, f.grp, f.rowptr, f.col, f.val, f.binv, b, x, xo, d, c1, c2, done, rz_ws)
+    FEA_SW_SWITCH(FEA_AC2);
+#undef FEA_AC2
   } else {
 #define FEA_AC1(SW, MD) \
   k_amg_cheb0<TM, SW, MD, 0><<<grid, kBlock, 0, st>>>(s.ngroups, f.grp, f.rowptr, f.col, f.val, f.binv, b, x, xo, d, c1, c2, done)
@@ -1044,10 +1110,20 @@ static void cheb_c(const fea_amg_level &L, bool first, const double *b, const do
     k_amg_cheb_c<0><<<grid, kBlock, 0, st>>>(ng, L.rowptr, L.col, L.val, L.dinv, b, x, xo, L.d, c1, c2, done);
 }
 
+// The PCG work a V-cycle takes over inside a PCG iteration (amg_pcg_iterate_t): the update
+// x += alpha p, r -= alpha q in its first level-0 step (b0 is r), and the r.z partials in its
+// last (gu_rz: the number of partials of ring slot 0, written by the start stage).
+struct PcgFuse {
+  double *x, *r;
+  const double *p, *q;
+  double *ws;
+  int gu_rz;
+};
+
 // z = V(b) on the hierarchy (b, z: level-0 vectors; z must not alias b)
 template <typename TM>
 static void amg_vcycle(const AmgFine<TM> &f, const fea_amg_level *lv, int nl, int deg, const double *b0, double *z0,
-                       const double *done, hipStream_t st) {
+                       const double *done, hipStream_t st, const PcgFuse *pf = nullptr) {
   double c1[AMG_MAX_DEG], c2[AMG_MAX_DEG];
   double *cur[AMG_MAX_LEVELS], *oth[AMG_MAX_LEVELS];
   for (int l = 0; l + 1 < nl; ++l) {  // down: pre-smooth, residual, restriction
@@ -1055,7 +1131,10 @@ static void amg_vcycle(const AmgFine<TM> &f, const fea_amg_level *lv, int nl, in
     const double *bl = l == 0 ? b0 : L.b;
     cheb_coeffs(l == 0 ? f.rho : L.rho, deg, c1, c2);
     double *xa = L.x, *xb = L.x2;
-    if (l == 0)
+    if (l == 0 && pf != nullptr)
+      k_amg_pcg_update_first<<<amg_grid(f.s.ngroups, kBlock), kBlock, 0, st>>>(
+          f.s.ngroups, f.grp, f.binv, pf->x, pf->r, pf->p, pf->q, xa, L.d, c2[0], pf->ws, pf->gu_rz);
+    else if (l == 0)
       cheb0(f, true, bl, nullptr, xa, L.d, c1[0], c2[0], done, st);
     else
       cheb_c(L, true, bl, nullptr, xa, c1[0], c2[0], done, st);
@@ -1088,9 +1167,10 @@ static void amg_vcycle(const AmgFine<TM> &f, const fea_amg_level *lv, int nl, in
     cheb_coeffs(l == 0 ? f.rho : L.rho, deg, c1, c2);
     double *x = cur[l], *o = oth[l];
     for (int k = 0; k < deg; ++k) {
-      double *out = (l == 0 && k == deg - 1) ? z0 : o;
+      const bool last = l == 0 && k == deg - 1;
+      double *out = last ? z0 : o;
       if (l == 0)
-        cheb0(f, false, bl, x, out, L.d, c1[k], c2[k], done, st);
+        cheb0(f, false, bl, x, out, L.d, c1[k], c2[k], done, st, (last && pf != nullptr) ? pf->ws : nullptr);
       else
         cheb_c(L, false, bl, x, out, c1[k], c2[k], done, st);
       o = x;
@@ -1125,34 +1205,6 @@ __global__ __launch_bounds__(kBlock) void k_amg_pcg_init(int n, const double *__
     ws[W_ITER] = 0.0;
     ws[W_DONE] = 0.0;
   }
-}
-
-// alpha = r.z / p.q; x += alpha p; r -= alpha q
-__global__ __launch_bounds__(kBlock) void k_amg_pcg_update(int n, double *__restrict__ x, double *__restrict__ r,
-                                                           const double *__restrict__ p,
-                                                           const double *__restrict__ q, double *__restrict__ ws) {
-  if (ws[W_DONE] != 0.0) return;
-  const int it = (int)ws[W_ITER];
-  const int g1 = (int)ws[W_G1];
-  const double rz_old = it > 1 ? ws[W_RZH + ((it - 1) & 1)]
-                               : sum_partials(ws + W_RZP + ((it - 1) & 1) * PCG_RING, gridDim.x);
-  const double pq = sum_partials(ws + W_PQ, g1);
-  const double alpha = pq != 0.0 ? rz_old / pq : 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    x[i] += alpha * p[i];
-    r[i] -= alpha * q[i];
-  }
-}
-
-// partials of r.z of this iteration (ring slot it & 1), z = M^-1 r from the V-cycle
-__global__ __launch_bounds__(kBlock) void k_amg_pcg_rz(int n, const double *__restrict__ r,
-                                                       const double *__restrict__ z, double *__restrict__ ws) {
-  if (ws[W_DONE] != 0.0) return;
-  const int it = (int)ws[W_ITER];
-  double rz = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    rz += r[i] * z[i];
-  write_partial(ws + W_RZP + (it & 1) * PCG_RING, rz);
 }
 
 // y = s D^-1 A x (float) and partials of y.y: one step of the power iteration for lambda_max
@@ -1236,12 +1288,14 @@ static int amg_pcg_iterate_t(const AmgPcgArgs &a) {
   const AmgFine<TM> f = amg_fine<TM>(a.n, a.nnz, a.n_groups, a.grp, a.rowptr, a.col, a.val, a.binv, a.rho0);
   const int gu = vec_grid();
   const double *done = a.ws + W_DONE;
+  // the PCG update rides in the V-cycle's first level-0 step and the r.z partials in its last
+  // (one partial per workgroup of that step's grid)
+  const PcgFuse pf{a.x, a.r, a.p, a.q, a.ws, gu};
+  const int g_rz = spmv_grid(f.s);
   for (int it = 0; it < a.iters; ++it) {
     launch_pcg_spmv<TM, double>(f.s, a.grp, a.rowptr, a.col, f.val, a.p, a.q, a.ws, a.st);
-    k_amg_pcg_update<<<gu, kBlock, 0, a.st>>>(a.n, a.x, a.r, a.p, a.q, a.ws);
-    amg_vcycle<TM>(f, a.lv, a.nl, a.deg, a.r, a.z, done, a.st);
-    k_amg_pcg_rz<<<gu, kBlock, 0, a.st>>>(a.n, a.r, a.z, a.ws);
-    k_pcg_dir<double><<<PCG_G, kBlock, 0, a.st>>>(a.n, a.z, a.p, a.ws, a.tol, a.max_iter, gu);
+    amg_vcycle<TM>(f, a.lv, a.nl, a.deg, a.r, a.z, done, a.st, &pf);
+    k_pcg_dir<double><<<PCG_G, kBlock, 0, a.st>>>(a.n, a.z, a.p, a.ws, a.tol, a.max_iter, g_rz, gu);
   }
   FEA_CHECK_LAUNCH("fea_pcg_iterate_amg");
   if (a.status_host != nullptr) {
