@@ -140,7 +140,7 @@ def spmv_bytes(plan, val_bytes):
 # whose workload or template differs from the measured one reports null.
 MEASURED_TRAFFIC = {
     ('beam 128x32x32 C3D8', 'k_pcg_spmv<float,double,16,2>'): 156.13e6 + 3.59e6,
-    ('beam 128x32x32 C3D8', 'k_amg_cheb0<float,16,2,0>'): 166.38e6 + 6.74e6,
+    ('beam 128x32x32 C3D8', 'k_amg_cheb0<float,16,2,0,0>'): 166.38e6 + 6.74e6,
 }
 
 
@@ -403,11 +403,11 @@ def bench_beam(args, torch):
                                       'k_pcg_spmv<%s,double,%d,2>' % (kname_t, _triple_sw(p))))
     if args.preconditioner == 'amg':
         t_sm, alg_sm = smooth_roofline(fe, torch, args.spmv_reps, mat)
-        dominant = roofline('k_amg_cheb0<%s,%d,2,0> (fea_amg_smooth0: one level-0 Chebyshev step of the '
+        dominant = roofline('k_amg_cheb0<%s,%d,2,0,0> (fea_amg_smooth0: one level-0 Chebyshev step of the '
                             'multigrid V-cycle)' % (kname_t, _triple_sw(p)), alg_sm / t_sm / 1e9, HBM_PEAK_GBS,
                             'GB/s', 'hbm', alg_bytes_per_launch=alg_sm, avg_launch_us=t_sm * 1e6,
                             traffic_key=('beam %dx%dx%d C3D8' % (args.nx, args.ny, args.nz),
-                                         'k_amg_cheb0<%s,%d,2,0>' % (kname_t, _triple_sw(p))))
+                                         'k_amg_cheb0<%s,%d,2,0,0>' % (kname_t, _triple_sw(p))))
     else:
         dominant = spmv_line
     elem_rate, t_tan = element_rate(fe, torch)
