@@ -931,8 +931,8 @@ def bench_explicit(args, torch):
     def residual():
         s = nat.stream_ptr()
         for a in tl['elem_args']:
-            nat.check_rc(fn(a[0], a[1], a[2], a[3], a[4], U.data_ptr(), a[5], a[6], a[7], a[8], a[9], s),
-                         'fea_elem_residual_tiles')
+            nat.check_rc(fn(a[0], a[1], a[2], a[3], a[4], U.data_ptr(), a[5], a[6], a[7], a[8], a[9], a[10],
+                            a[11], a[12], a[13], s), 'fea_elem_residual_tiles')
     t_res = event_time(torch, residual, 200)
     alg = E * 4 * 4 + p.n_nodes * 3 * 8 * 3                 # conn + X + U read, nodal force written
     cpu = None

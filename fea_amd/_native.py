@@ -88,7 +88,8 @@ _SIGS = {
     'fea_explicit_update': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_DBL, C_DBL, C_P, C_P, C_P, C_P]),
     'fea_explicit_update_tiles': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_DBL, C_DBL, C_P, C_P, C_P, C_P]),
     'fea_elem_tile_size': (C_INT, [C_INT]),
-    'fea_elem_residual_tiles': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_elem_residual_tiles': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P,
+                                        C_INT, C_P]),
     'fea_contact_set_size': (C_INT, []),
     'fea_contact_rebuild': (C_INT, [C_P, C_P, C_P, C_P]),
     'fea_contact_eval': (C_INT, [C_P, C_P, C_P, C_P, C_P, C_P]),

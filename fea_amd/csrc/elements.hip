@@ -224,13 +224,14 @@ __global__ __launch_bounds__(kRkThreads) void k_elem_rk(
 // node, formed in double: the Jacobian and grad u only see differences (the shape
 // function derivatives sum to zero), and a rigid translation of the element no longer
 // costs float digits of the strain.  In double the absolute values are used.
-template <typename T, int ET>
-__device__ __forceinline__ void elem_force(int e, const int32_t *__restrict__ conn, const double *__restrict__ X,
-                                           const double *__restrict__ U, const MatParams &mp,
-                                           T R[Elem<ET>::nen][3]) {
+// idx: the element's nen node indices into X and U (3 doubles per node: the global arrays, or
+// a tile's nodes staged in LDS with tile-local indices).
+template <typename T, int ET, typename I>
+__device__ __forceinline__ void elem_force_at(const I *idx, const double *X, const double *U, const MatParams &mp,
+                                              T R[Elem<ET>::nen][3]) {
   constexpr int NEN = Elem<ET>::nen, NGP = Elem<ET>::ngp;
   constexpr bool kRel = sizeof(T) < sizeof(double);
-  const int64_t n0 = conn[(int64_t)e * NEN];
+  const int64_t n0 = idx[0];
   double x0[3] = {0.0, 0.0, 0.0}, u0[3] = {0.0, 0.0, 0.0};
   if (kRel) {
 #pragma unroll
@@ -242,7 +243,7 @@ __device__ __forceinline__ void elem_force(int e, const int32_t *__restrict__ co
   T x[NEN][3], u[NEN][3];
 #pragma unroll
   for (int c = 0; c < NEN; ++c) {
-    const int64_t n = conn[(int64_t)e * NEN + c];
+    const int64_t n = idx[c];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       x[c][i] = (T)(X[3 * n + i] - x0[i]);
@@ -273,6 +274,13 @@ __device__ __forceinline__ void elem_force(int e, const int32_t *__restrict__ co
       for (int j = 0; j < 3; ++j)
         R[c][j] += P[j * 3 + 0] * dN[c][0] + P[j * 3 + 1] * dN[c][1] + P[j * 3 + 2] * dN[c][2];
   }
+}
+
+template <typename T, int ET>
+__device__ __forceinline__ void elem_force(int e, const int32_t *__restrict__ conn, const double *__restrict__ X,
+                                           const double *__restrict__ U, const MatParams &mp,
+                                           T R[Elem<ET>::nen][3]) {
+  elem_force_at<T, ET, int32_t>(conn + (int64_t)e * Elem<ET>::nen, X, U, mp, R);
 }
 
 // One thread per element: R_a (float64) to rstage[dst_r[e*nen + a]*3 ..] (-1 skips).
@@ -318,23 +326,43 @@ struct TileCfg<10> {
   static constexpr int TE = 256;
 };
 
-template <typename T, int ET>
+// STAGE: the tile's nodes (tnode[tnode_ptr[t] .. tnode_ptr[t+1]), at most max_nodes) have
+// their X and U copied to LDS first (coalesced), and lconn gives each element's nodes as
+// tile-local indices; the element arithmetic is the same, on the same values.
+template <typename T, int ET, bool STAGE>
 __global__ __launch_bounds__(kBlock) void k_elem_r_tiles(int n_elem, const int32_t *__restrict__ conn,
                                                          const double *__restrict__ X,
                                                          const double *__restrict__ U, MatParams mp,
                                                          const int32_t *__restrict__ tile_lbase,
                                                          const int32_t *__restrict__ pslot_ptr,
                                                          const uint16_t *__restrict__ pslot,
-                                                         double *__restrict__ partial) {
+                                                         double *__restrict__ partial,
+                                                         const int32_t *__restrict__ tnode_ptr,
+                                                         const int32_t *__restrict__ tnode,
+                                                         const uint16_t *__restrict__ lconn, int max_nodes) {
   constexpr int NEN = Elem<ET>::nen, TE = TileCfg<ET>::TE;
   __shared__ T lds[TE * NEN * 3];
+  extern __shared__ double node_lds[];  // STAGE: X then U of the tile's nodes, 3 each
   const int t = blockIdx.x;
   const int e0 = t * TE;
+  if (STAGE) {
+    const int n0 = tnode_ptr[t], nn = tnode_ptr[t + 1] - n0;
+    double *xs = node_lds, *us = node_lds + 3 * max_nodes;
+    for (int k = threadIdx.x; k < 3 * nn; k += kBlock) {
+      const int64_t g = 3 * (int64_t)tnode[n0 + k / 3] + k % 3;
+      xs[k] = X[g];
+      us[k] = U[g];
+    }
+    __syncthreads();
+  }
   for (int el = threadIdx.x; el < TE; el += kBlock) {
     const int e = e0 + el;
     if (e >= n_elem) break;
     T R[NEN][3];
-    elem_force<T, ET>(e, conn, X, U, mp, R);
+    if (STAGE)
+      elem_force_at<T, ET, uint16_t>(lconn + (int64_t)e * NEN, node_lds, node_lds + 3 * max_nodes, mp, R);
+    else
+      elem_force<T, ET>(e, conn, X, U, mp, R);
 #pragma unroll
     for (int c = 0; c < NEN; ++c)
 #pragma unroll
@@ -569,23 +597,29 @@ static int launch_r(int et, int n, const int32_t *conn, const double *X, const d
   return FEA_OK;
 }
 
+template <typename T, int ET>
+static void launch_r_tiles_t(int n, const int32_t *conn, const double *X, const double *U, MatParams mp,
+                             const int32_t *lb, const int32_t *pp, const uint16_t *ps, double *partial,
+                             const int32_t *tnode_ptr, const int32_t *tnode, const uint16_t *lconn, int max_nodes,
+                             hipStream_t st) {
+  const int grid = (n + TileCfg<ET>::TE - 1) / TileCfg<ET>::TE;
+  if (tnode_ptr != nullptr)
+    k_elem_r_tiles<T, ET, true><<<grid, kBlock, (size_t)6 * max_nodes * sizeof(double), st>>>(
+        n, conn, X, U, mp, lb, pp, ps, partial, tnode_ptr, tnode, lconn, max_nodes);
+  else
+    k_elem_r_tiles<T, ET, false><<<grid, kBlock, 0, st>>>(n, conn, X, U, mp, lb, pp, ps, partial, nullptr, nullptr,
+                                                          nullptr, 0);
+}
+
 template <typename T>
 static int launch_r_tiles(int et, int n, const int32_t *conn, const double *X, const double *U, MatParams mp,
                           const int32_t *tile_lbase, const int32_t *pslot_ptr, const uint16_t *pslot,
-                          double *partial, hipStream_t st) {
+                          double *partial, const int32_t *tnode_ptr, const int32_t *tnode, const uint16_t *lconn,
+                          int max_nodes, hipStream_t st) {
   switch (et) {
-    case 4:
-      k_elem_r_tiles<T, 4><<<(n + TileCfg<4>::TE - 1) / TileCfg<4>::TE, kBlock, 0, st>>>(
-          n, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial);
-      break;
-    case 8:
-      k_elem_r_tiles<T, 8><<<(n + TileCfg<8>::TE - 1) / TileCfg<8>::TE, kBlock, 0, st>>>(
-          n, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial);
-      break;
-    case 10:
-      k_elem_r_tiles<T, 10><<<(n + TileCfg<10>::TE - 1) / TileCfg<10>::TE, kBlock, 0, st>>>(
-          n, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial);
-      break;
+    case 4: launch_r_tiles_t<T, 4>(n, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial, tnode_ptr, tnode, lconn, max_nodes, st); break;
+    case 8: launch_r_tiles_t<T, 8>(n, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial, tnode_ptr, tnode, lconn, max_nodes, st); break;
+    case 10: launch_r_tiles_t<T, 10>(n, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial, tnode_ptr, tnode, lconn, max_nodes, st); break;
   }
   FEA_CHECK_LAUNCH("fea_elem_residual_tiles");
   return FEA_OK;
@@ -671,18 +705,21 @@ extern "C" int fea_elem_tile_size(int elem_type) {
 extern "C" int fea_elem_residual_tiles(int elem_type, int dtype, int n_elem, const int32_t *conn, const double *X,
                                        const double *U, const double *params_host, const int32_t *tile_lbase,
                                        const int32_t *pslot_ptr, const uint16_t *pslot, double *partial,
-                                       void *stream) {
+                                       const int32_t *tnode_ptr, const int32_t *tnode, const uint16_t *lconn,
+                                       int max_tile_nodes, void *stream) {
   FEA_CHECK_ARG(valid_elem(elem_type), "fea_elem_residual_tiles: bad element type %d", elem_type);
   FEA_CHECK_ARG(dtype == FEA_F32 || dtype == FEA_F64, "fea_elem_residual_tiles: bad dtype %d", dtype);
   FEA_CHECK_ARG(n_elem >= 0 && params_host != nullptr, "fea_elem_residual_tiles: bad arguments");
   if (n_elem == 0) return FEA_OK;
   FEA_CHECK_ARG(conn && X && U && tile_lbase && pslot_ptr && pslot && partial, "fea_elem_residual_tiles: null input");
+  FEA_CHECK_ARG(tnode_ptr == nullptr || (tnode && lconn && max_tile_nodes > 0 && max_tile_nodes <= 2048),
+                "fea_elem_residual_tiles: tile node staging needs tnode, lconn and 1 <= max_tile_nodes <= 2048");
   const MatParams mp = mat_from(params_host);
   if (dtype == FEA_F64)
-    return launch_r_tiles<double>(elem_type, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial,
-                                  as_stream(stream));
-  return launch_r_tiles<float>(elem_type, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial,
-                               as_stream(stream));
+    return launch_r_tiles<double>(elem_type, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial, tnode_ptr,
+                                  tnode, lconn, max_tile_nodes, as_stream(stream));
+  return launch_r_tiles<float>(elem_type, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial, tnode_ptr,
+                               tnode, lconn, max_tile_nodes, as_stream(stream));
 }
 
 extern "C" int fea_elem_energy_blocks(int n_elem) { return n_elem > 0 ? blocks_for(n_elem) : 0; }

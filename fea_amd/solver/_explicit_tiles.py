@@ -37,7 +37,10 @@ def group_tiles(conn, X, free_node, tile_size, partial_base=0):
       tile_lbase  [T + 1]    first partial of each tile (global index, from partial_base),
       pslot_ptr   [P + 1]    each partial's range in pslot (relative to this group),
       pslot       [Q] uint16 local slots e_local * nen + a, element order within a partial,
-      pnode       [P]        the node of each partial.
+      pnode       [P]        the node of each partial,
+      tnode_ptr   [T + 1], tnode: the nodes each tile's elements touch (sorted),
+      lconn       [E * nen] uint16: element i's nodes as indices into its tile's list,
+      max_tile_nodes: the largest tile's node count.
     """
     conn = np.asarray(conn, dtype=np.int64)
     E, nen = conn.shape
@@ -61,8 +64,16 @@ def group_tiles(conn, X, free_node, tile_size, partial_base=0):
     o = np.argsort(inv, kind='stable')                  # keeps element order inside a partial
     pslot = slot[keep][o].astype(np.uint16)
     pslot_ptr = np.concatenate([[0], np.cumsum(np.bincount(inv, minlength=P))])
+    # every node a tile's elements touch (free or not), sorted, and each element's nodes as
+    # indices into its tile's list (the kernel stages these nodes' X and U in LDS)
+    tkey = np.unique(e // TE * N + node)
+    tnode = tkey % N
+    tnode_ptr = np.searchsorted(tkey // N, np.arange(T + 1))
+    lconn = (np.searchsorted(tkey, e // TE * N + node) - tnode_ptr[e // TE]).astype(np.uint16)
     return {'order': order, 'te': TE, 'tile_lbase': lbase.astype(np.int64),
-            'pslot_ptr': pslot_ptr.astype(np.int64), 'pslot': pslot, 'pnode': (uniq % N).astype(np.int64)}
+            'pslot_ptr': pslot_ptr.astype(np.int64), 'pslot': pslot, 'pnode': (uniq % N).astype(np.int64),
+            'tnode_ptr': tnode_ptr.astype(np.int64), 'tnode': tnode.astype(np.int64), 'lconn': lconn,
+            'max_tile_nodes': int(np.diff(tnode_ptr).max()) if T else 0}
 
 
 def node_partials(pnodes, n_nodes):

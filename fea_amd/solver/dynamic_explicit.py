@@ -44,6 +44,9 @@ class DynamicExplicitSolver(BaseSolver):
         super().__init__()
         self.overlap = bool(overlap)
         self.use_graph = bool(use_graph)
+        # element tiles read their nodes from LDS: bit-identical, measured slower on the
+        # two-block benchmark (15.8 vs 14.7 us per block), so off by default
+        self.stage_nodes = False
         self.chunk = int(chunk)
         if precision not in ('fp64', 'fp32'):
             raise ValueError("precision must be 'fp64' or 'fp32'")
@@ -121,8 +124,19 @@ class DynamicExplicitSolver(BaseSolver):
             lb = i32(t['tile_lbase'])
             keep['conn'].append(conn)
             keep['lbase'].append(lb)
+            # the tile's nodes staged in LDS when, with the forces' LDS, a workgroup stays within
+            # 64 KB; otherwise each element gathers its nodes
+            tsz = L.fea_elem_tile_size(g.ntype)
+            lds = tsz * g.ntype * 3 * (4 if code == F32 else 8) + 48 * t['max_tile_nodes']
+            stage = None
+            if self.stage_nodes and 0 < t['max_tile_nodes'] <= 2048 and lds <= 65536:
+                stage = (i32(t['tnode_ptr']), i32(t['tnode']),
+                         torch.as_tensor(t['lconn'].view(np.int16), device=dev))
+                keep.setdefault('stage', []).append(stage)
             elem_args.append((g.ntype, code, g.n, conn.data_ptr(), p.X.data_ptr(), g.params.ctypes.data,
-                              lb.data_ptr(), pslot_ptr.data_ptr(), pslot.data_ptr(), keep['partial'].data_ptr()))
+                              lb.data_ptr(), pslot_ptr.data_ptr(), pslot.data_ptr(), keep['partial'].data_ptr(),
+                              stage[0].data_ptr() if stage else None, stage[1].data_ptr() if stage else None,
+                              stage[2].data_ptr() if stage else None, t['max_tile_nodes'] if stage else 0))
         keep['params'] = [g.params for g in groups]
         keep['elem_args'] = elem_args
         keep['n_partials'] = base
@@ -290,8 +304,8 @@ class DynamicExplicitSolver(BaseSolver):
         else:
             fn, u = nat.raw('fea_elem_residual_tiles'), U.data_ptr()
             for a in tl['elem_args']:
-                nat.check_rc(fn(a[0], a[1], a[2], a[3], a[4], u, a[5], a[6], a[7], a[8], a[9], s),
-                             'fea_elem_residual_tiles')
+                nat.check_rc(fn(a[0], a[1], a[2], a[3], a[4], u, a[5], a[6], a[7], a[8], a[9], a[10], a[11],
+                                a[12], a[13], s), 'fea_elem_residual_tiles')
         if side is not None:
             main.wait_stream(side)                      # fc is formed
         elif self._contacts:

@@ -149,12 +149,17 @@ int fea_elem_residual(int elem_type, int dtype, int n_elem, const int32_t *conn,
  * keeps their nodal forces (arithmetic in dtype) in LDS, and writes for every partial
  * p in tile_lbase[t] .. tile_lbase[t+1] the float64 sum, in order, of the local slots
  * pslot[pslot_ptr[p] .. pslot_ptr[p+1]) (slot = e_local*nen + a) to partial[3p ..].
- * One partial per (tile, node): fea_explicit_update_tiles adds a node's partials. */
+ * One partial per (tile, node): fea_explicit_update_tiles adds a node's partials.
+ * Optional (tnode_ptr non-NULL): tile t's nodes tnode[tnode_ptr[t] .. tnode_ptr[t+1])
+ * (at most max_tile_nodes <= 2048) have their X and U staged in LDS first, and
+ * lconn[e*nen + a] is element e's node a as an index into its tile's list: the same
+ * arithmetic on the same values (bit-identical partials), without per-element gathers. */
 int fea_elem_tile_size(int elem_type);
 int fea_elem_residual_tiles(int elem_type, int dtype, int n_elem, const int32_t *conn,
                             const double *X, const double *U, const double *params_host,
                             const int32_t *tile_lbase, const int32_t *pslot_ptr, const uint16_t *pslot,
-                            double *partial, void *stream);
+                            double *partial, const int32_t *tnode_ptr, const int32_t *tnode,
+                            const uint16_t *lconn, int max_tile_nodes, void *stream);
 
 /* Strain energy per element block (Element_3D.potential_Energy,
  * C3base.py:387-411): writes one float64 partial sum per 256-element block to

@@ -68,3 +68,16 @@ def test_morton_order_is_a_compact_permutation():
         c = m['conn'][order]
         return sum(np.unique(c[i:i + 512]).shape[0] for i in range(0, c.shape[0], 512))
     assert touched(o) < 0.9 * touched(np.arange(cent.shape[0]))
+
+
+def test_tile_node_lists_and_local_connectivity():
+    m = _mesh('tet')
+    conn, X = m['conn'], m['nodes']
+    t = group_tiles(conn, X, np.ones(X.shape[0], dtype=bool), 64)
+    c = conn[t['order']]
+    for i in range(c.shape[0]):
+        tt = i // 64
+        nodes = t['tnode'][t['tnode_ptr'][tt]:t['tnode_ptr'][tt + 1]]
+        assert np.all(np.diff(nodes) > 0)
+        assert np.array_equal(nodes[t['lconn'][4 * i:4 * i + 4]], c[i])
+    assert t['max_tile_nodes'] == int(np.diff(t['tnode_ptr']).max())

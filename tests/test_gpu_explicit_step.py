@@ -146,3 +146,27 @@ def test_graph_chunks_equal_eager_steps(gpu, overlap):
     (U1, v1, a1, n1, r1), (U0, v0, a0, n0, r0) = out
     assert n1 > 0 and r1 >= 2 and r0 >= 2
     assert torch.equal(U1, U0) and torch.equal(v1, v0) and torch.equal(a1, a0)
+
+
+@pytest.mark.parametrize('precision', ['fp32', 'fp64'])
+def test_staged_tile_nodes_bit_identical(gpu, precision):
+    """Element tiles reading their nodes from LDS (tile node lists, local connectivity) give
+    the gathering kernel's partials, hence the same 40 steps, bit for bit."""
+    print('explicit staged tile nodes, %s' % precision, flush=True)
+    out = []
+    for stage in (True, False):
+        fe, info = workloads.two_blocks_contact(4, velocity=1.0, gap=0.01, jitter=0.1, seed=5)
+        fe.solver = solver.DynamicExplicitSolver(time_end=1.0, time_per_storage=1.0, dt=3e-3, store=False,
+                                                 precision=precision)
+        fe.solver.stage_nodes = stage
+        fe.initialize()
+        assert ('stage' in fe.solver._tiles) == stage
+        p = fe.assembly.plan
+        U = p.gc_to_u(torch.zeros_like(fe.assembly.GC))
+        vh = workloads.initial_velocity(fe, info).clone()
+        acc = torch.zeros_like(vh)
+        fe.solver.run_steps(U, vh, acc, fe.solver._deltaT, 40, first=True)
+        torch.cuda.synchronize()
+        out.append((U.clone(), vh.clone(), acc.clone()))
+    for a, b in zip(out[0], out[1]):
+        assert torch.equal(a, b)
