@@ -46,9 +46,16 @@ inline int blocks_for(int64_t n, int per_block = kBlock) {
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 // ---------------------------------------------------------------- device math
+// The element-force math (these helpers, the kinematics, stress and shape-gradient helpers and
+// the per-element force routine) fuses a * b + c into an fma only inside one expression
+// (`#pragma clang fp contract(on)` in each body): the same source then rounds alike in every
+// kernel that inlines it.  Under the default cross-statement contraction the backend picks
+// which product of a * b + c * d to fuse by the surrounding kernel's code, and two kernels
+// computing one element's forces could differ in the last float bit.
 
 template <typename T>
 __device__ __forceinline__ T det3(const T a[9]) {
+  #pragma clang fp contract(on)
   return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
          a[2] * (a[3] * a[7] - a[4] * a[6]);
 }
@@ -56,6 +63,7 @@ __device__ __forceinline__ T det3(const T a[9]) {
 // inverse of a row-major 3x3 given its determinant
 template <typename T>
 __device__ __forceinline__ void inv3(const T a[9], T d, T out[9]) {
+  #pragma clang fp contract(on)
   T id = T(1) / d;
   out[0] = (a[4] * a[8] - a[5] * a[7]) * id;
   out[1] = (a[2] * a[7] - a[1] * a[8]) * id;
@@ -71,6 +79,7 @@ __device__ __forceinline__ void inv3(const T a[9], T d, T out[9]) {
 // det(I + H) - 1 without cancellation: tr H + (sum of principal 2-minors) + det H
 template <typename T>
 __device__ __forceinline__ T detm1_of_grad(const T h[9]) {
+  #pragma clang fp contract(on)
   T tr = h[0] + h[4] + h[8];
   T m2 = (h[0] * h[4] - h[1] * h[3]) + (h[4] * h[8] - h[5] * h[7]) + (h[0] * h[8] - h[2] * h[6]);
   return tr + m2 + det3(h);

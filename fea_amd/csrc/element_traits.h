@@ -21,6 +21,8 @@ struct Elem<4> {
   __device__ static void shape(int, double N[4]) {
     N[0] = 0.25; N[1] = 0.25; N[2] = 0.25; N[3] = 0.25;
   }
+  // structural zeros of dN[m][a] (skipped in the sums below: x * 0 is not folded away)
+  __device__ static constexpr bool dzero(int m, int a) { return a != 0 && a != m + 1; }
   // dN[m][a] = dN_a / dxi_m
   __device__ static void dshape(int, double dN[3][4]) {
     dN[0][0] = -1.0; dN[0][1] = 1.0; dN[0][2] = 0.0; dN[0][3] = 0.0;
@@ -42,6 +44,7 @@ struct Elem<8> {
   static constexpr int nen = 8, ngp = 8;
   static constexpr double q = 0.57735026918962576451;  // 1/sqrt(3)
   __device__ static constexpr double weight(int) { return 1.0; }
+  __device__ static constexpr bool dzero(int, int) { return false; }
   __device__ static void shape(int g, double N[8]) {
     const double x = q * hex_sign(g, 0), y = q * hex_sign(g, 1), z = q * hex_sign(g, 2);
 #pragma unroll
@@ -65,6 +68,10 @@ struct Elem<10> {
   static constexpr int nen = 10, ngp = 4;
   static constexpr double alpha = 0.58541020, beta = 0.13819660;  // tetrahedral.py:112-113
   __device__ static constexpr double weight(int) { return 1.0 / 24.0; }
+  __device__ static constexpr bool dzero(int m, int a) {
+    return (a == 1 && m != 0) || (a == 2 && m != 1) || (a == 3 && m != 2) || (a == 5 && m == 2) ||
+           (a == 8 && m == 1) || (a == 9 && m == 0);
+  }
   __device__ static void point(int g, double &x, double &y, double &z) {
     x = (g == 1) ? alpha : beta;
     y = (g == 2) ? alpha : beta;
@@ -129,6 +136,7 @@ struct GPState {
 
 template <typename T>
 __device__ __forceinline__ void kinematics(const T H[9], GPState<T> &s) {
+  #pragma clang fp contract(on)
 #pragma unroll
   for (int i = 0; i < 9; ++i) s.F[i] = H[i];
   s.F[0] += T(1); s.F[4] += T(1); s.F[8] += T(1);
@@ -157,10 +165,34 @@ __device__ __forceinline__ void kinematics(const T H[9], GPState<T> &s) {
   s.Ci[5] = (c01 * c12 - c02 * c11) * idet;
 }
 
+// J^(-2/3) for the stress of element code in T.  In double: pow.  In float: J = m 8^k with
+// m in [1/2, 4) (frexp, exact), a float estimate of y = m^(-1/3) from the hardware log2 / exp2,
+// refined by two float64 Newton steps on y^-3 = m, y <- y (4 - m y^3) / 3 (division-free; the
+// relative error goes ~1e-7 -> ~1e-13 -> round-off), and J^(-2/3) = y^2 4^-k: the volumetric
+// terms keep float64 accuracy over the whole double range for a fraction of a float64 pow's
+// work and registers.  J <= 0 gives NaN (pow: NaN, or inf at 0).
+template <typename T>
+__device__ __forceinline__ double jm23(double J) {
+  #pragma clang fp contract(on)
+  if constexpr (sizeof(T) < sizeof(double)) {
+    int e;
+    const double mant = frexp(J, &e);
+    const int k = e >= 0 ? e / 3 : -((2 - e) / 3);  // floor(e / 3)
+    const double m = ldexp(mant, e - 3 * k);
+    double y = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)m) * (-1.0f / 3.0f));
+#pragma unroll
+    for (int i = 0; i < 2; ++i) y = y * (4.0 - m * (y * y * y)) * (1.0 / 3.0);
+    return ldexp(y * y, -2 * k);
+  } else {
+    return pow(J, -2.0 / 3.0);
+  }
+}
+
 // Second Piola-Kirchhoff stress S = 2 dW/dC = s1 1 + sc C + si C^-1.
 template <typename T>
 __device__ __forceinline__ void stress(const GPState<T> &s, const MatParams &m, T S[6]) {
-  const double Jm23 = pow(s.J, -2.0 / 3.0);
+  #pragma clang fp contract(on)
+  const double Jm23 = jm23<T>(s.J);
   const double Jm43 = Jm23 * Jm23;
   const double I1 = (double)s.I1, I2 = (double)s.I2;
   const double up = m.kv * s.Jm1;  // U'(J)
@@ -168,7 +200,7 @@ __device__ __forceinline__ void stress(const GPState<T> &s, const MatParams &m, 
   const T sc = T(-2.0 * m.w2 * Jm43);
   const T si = T(-(2.0 / 3.0) * m.w1 * Jm23 * I1 - (4.0 / 3.0) * m.w2 * Jm43 * I2 + s.J * up);
 #pragma unroll
-  for (int I = 0; I < 6; ++I) S[I] = sc * s.C[I] + si * s.Ci[I] + (I < 3 ? s1 : T(0));
+  for (int I = 0; I < 6; ++I) S[I] = I < 3 ? sc * s.C[I] + si * s.Ci[I] + s1 : sc * s.C[I] + si * s.Ci[I];
 }
 
 // Material tangent CC = 2 dS/dC of W(I1bar, I2bar, J):
@@ -302,8 +334,13 @@ __device__ __forceinline__ void bmat(const T F[9], const T dN[3], T B[6][3]) {
 }
 
 // Gauss-point geometry: dN/dX [nen][3] and w*detJ from element coordinates.
-template <typename T, int ET>
+// Sums start from -0.0 (x + -0.0 == x, so the compiler drops the first add) and skip the
+// structural zeros of the reference derivatives.  C0 = 1: node 0's coordinates (x) or
+// displacements (u) are exactly zero (element-relative values) and are skipped too.  For
+// finite inputs the results equal the full sums (up to the sign of an exact zero).
+template <typename T, int ET, int C0 = 0>
 __device__ __forceinline__ T gp_geometry(int g, const T x[][3], T dNdX[][3]) {
+  #pragma clang fp contract(on)
   constexpr int NEN = Elem<ET>::nen;
   double dr[3][NEN];
   Elem<ET>::dshape(g, dr);
@@ -312,9 +349,10 @@ __device__ __forceinline__ T gp_geometry(int g, const T x[][3], T dNdX[][3]) {
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-      T acc = T(0);
+      T acc = T(-0.0);
 #pragma unroll
-      for (int c = 0; c < NEN; ++c) acc += x[c][i] * T(dr[m][c]);
+      for (int c = C0; c < NEN; ++c)
+        if (!Elem<ET>::dzero(m, c)) acc += x[c][i] * T(dr[m][c]);
       Jm[i * 3 + m] = acc;   // J_im = dx_i/dxi_m  (C3base.py:199-202)
     }
   const T dj = det3(Jm);
@@ -323,20 +361,26 @@ __device__ __forceinline__ T gp_geometry(int g, const T x[][3], T dNdX[][3]) {
 #pragma unroll
   for (int c = 0; c < NEN; ++c)
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-      dNdX[c][i] = iJ[0 * 3 + i] * T(dr[0][c]) + iJ[1 * 3 + i] * T(dr[1][c]) + iJ[2 * 3 + i] * T(dr[2][c]);
+    for (int i = 0; i < 3; ++i) {
+      T v = T(-0.0);
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+        if (!Elem<ET>::dzero(m, c)) v += iJ[m * 3 + i] * T(dr[m][c]);
+      dNdX[c][i] = v;
+    }
   return T(Elem<ET>::weight(g)) * dj;  // signed, C3base.py:214
 }
 
-template <typename T, int NEN>
+template <typename T, int NEN, int C0 = 0>
 __device__ __forceinline__ void disp_grad(const T u[][3], const T dNdX[][3], T H[9]) {
+  #pragma clang fp contract(on)
 #pragma unroll
   for (int I = 0; I < 3; ++I)
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      T acc = T(0);
+      T acc = T(-0.0);
 #pragma unroll
-      for (int c = 0; c < NEN; ++c) acc += u[c][I] * dNdX[c][i];
+      for (int c = C0; c < NEN; ++c) acc += u[c][I] * dNdX[c][i];
       H[I * 3 + i] = acc;
     }
 }

@@ -229,6 +229,7 @@ __global__ __launch_bounds__(kRkThreads) void k_elem_rk(
 template <typename T, int ET, typename I>
 __device__ __forceinline__ void elem_force_at(const I *idx, const double *X, const double *U, const MatParams &mp,
                                               T R[Elem<ET>::nen][3]) {
+  #pragma clang fp contract(on)
   constexpr int NEN = Elem<ET>::nen, NGP = Elem<ET>::ngp;
   constexpr bool kRel = sizeof(T) < sizeof(double);
   const int64_t n0 = idx[0];
@@ -250,13 +251,14 @@ __device__ __forceinline__ void elem_force_at(const I *idx, const double *X, con
       u[c][i] = (T)(U[3 * n + i] - u0[i]);
     }
   }
+  constexpr int C0 = kRel ? 1 : 0;  // node 0's relative x and u are exactly zero
 #pragma unroll
-  for (int c = 0; c < NEN; ++c) R[c][0] = R[c][1] = R[c][2] = T(0);
+  for (int c = 0; c < NEN; ++c) R[c][0] = R[c][1] = R[c][2] = T(-0.0);
   for (int g = 0; g < NGP; ++g) {
     T dN[NEN][3];
-    const T w = gp_geometry<T, ET>(g, x, dN);
+    const T w = gp_geometry<T, ET, C0>(g, x, dN);
     T H[9];
-    disp_grad<T, NEN>(u, dN, H);
+    disp_grad<T, NEN, C0>(u, dN, H);
     GPState<T> s;
     kinematics(H, s);
     T S[6];
@@ -306,8 +308,8 @@ __global__ __launch_bounds__(kBlock) void k_elem_r(int n_elem, const int32_t *__
   }
 }
 
-// Element forces reduced per tile in LDS (the explicit loop's residual).  Workgroup t owns
-// elements [t*TE, (t+1)*TE): their nodal forces (in T) go to LDS slot e_local*nen + a, then
+// Element forces reduced per tile in LDS (the explicit loop's residual).  Workgroup t (TB
+// threads: one element per thread for TE = TB) owns elements [t*TE, (t+1)*TE): their nodal forces (in T) go to LDS slot e_local*nen + a, then
 // thread per local node p (partials tile_lbase[t] .. tile_lbase[t+1]) sums its slots
 // pslot[pslot_ptr[p] .. pslot_ptr[p+1]) in that order in float64 into partial[3p ..].  The
 // update sums a node's partials; no per-element staging reaches HBM.
@@ -315,22 +317,22 @@ template <int ET>
 struct TileCfg;
 template <>
 struct TileCfg<4> {
-  static constexpr int TE = 512;
+  static constexpr int TE = 512, TB = 512;
 };
 template <>
 struct TileCfg<8> {
-  static constexpr int TE = 256;
+  static constexpr int TE = 256, TB = 256;
 };
 template <>
 struct TileCfg<10> {
-  static constexpr int TE = 256;
+  static constexpr int TE = 256, TB = 256;
 };
 
 // STAGE: the tile's nodes (tnode[tnode_ptr[t] .. tnode_ptr[t+1]), at most max_nodes) have
 // their X and U copied to LDS first (coalesced), and lconn gives each element's nodes as
 // tile-local indices; the element arithmetic is the same, on the same values.
 template <typename T, int ET, bool STAGE>
-__global__ __launch_bounds__(kBlock) void k_elem_r_tiles(int n_elem, const int32_t *__restrict__ conn,
+__global__ __launch_bounds__(TileCfg<ET>::TB) void k_elem_r_tiles(int n_elem, const int32_t *__restrict__ conn,
                                                          const double *__restrict__ X,
                                                          const double *__restrict__ U, MatParams mp,
                                                          const int32_t *__restrict__ tile_lbase,
@@ -340,22 +342,37 @@ __global__ __launch_bounds__(kBlock) void k_elem_r_tiles(int n_elem, const int32
                                                          const int32_t *__restrict__ tnode_ptr,
                                                          const int32_t *__restrict__ tnode,
                                                          const uint16_t *__restrict__ lconn, int max_nodes) {
-  constexpr int NEN = Elem<ET>::nen, TE = TileCfg<ET>::TE;
+  constexpr int NEN = Elem<ET>::nen, TE = TileCfg<ET>::TE, TB = TileCfg<ET>::TB;
+  constexpr int QT = TE * NEN, QPT = (QT + TB - 1) / TB;
   __shared__ T lds[TE * NEN * 3];
+  __shared__ uint16_t sps[QT];          // the tile's pslot entries
   extern __shared__ double node_lds[];  // STAGE: X then U of the tile's nodes, 3 each
   const int t = blockIdx.x;
   const int e0 = t * TE;
+  // the reduction's tables are loaded before the element work, so that their latency hides
+  // behind it: the tile's pslot entries (at most TE * nen, one per free (element, node) slot)
+  // and each thread's first partial range
+  const int pb = tile_lbase[t], p1 = tile_lbase[t + 1];
+  const int qb = pslot_ptr[pb], qn = min(pslot_ptr[p1] - qb, QT);
+  uint16_t qv[QPT];
+#pragma unroll
+  for (int i = 0; i < QPT; ++i) {
+    const int q = threadIdx.x + i * TB;
+    qv[i] = q < qn ? pslot[qb + q] : uint16_t(0);
+  }
+  const int pf = pb + threadIdx.x;
+  const int qf0 = pf < p1 ? pslot_ptr[pf] : 0, qf1 = pf < p1 ? pslot_ptr[pf + 1] : 0;
   if (STAGE) {
     const int n0 = tnode_ptr[t], nn = tnode_ptr[t + 1] - n0;
     double *xs = node_lds, *us = node_lds + 3 * max_nodes;
-    for (int k = threadIdx.x; k < 3 * nn; k += kBlock) {
+    for (int k = threadIdx.x; k < 3 * nn; k += TB) {
       const int64_t g = 3 * (int64_t)tnode[n0 + k / 3] + k % 3;
       xs[k] = X[g];
       us[k] = U[g];
     }
     __syncthreads();
   }
-  for (int el = threadIdx.x; el < TE; el += kBlock) {
+  for (int el = threadIdx.x; el < TE; el += TB) {
     const int e = e0 + el;
     if (e >= n_elem) break;
     T R[NEN][3];
@@ -368,13 +385,17 @@ __global__ __launch_bounds__(kBlock) void k_elem_r_tiles(int n_elem, const int32
 #pragma unroll
       for (int j = 0; j < 3; ++j) lds[(el * NEN + c) * 3 + j] = R[c][j];
   }
+#pragma unroll
+  for (int i = 0; i < QPT; ++i) {
+    const int q = threadIdx.x + i * TB;
+    if (q < qn) sps[q] = qv[i];
+  }
   __syncthreads();
-  const int p1 = tile_lbase[t + 1];
-  for (int p = tile_lbase[t] + threadIdx.x; p < p1; p += kBlock) {
+  for (int p = pf; p < p1; p += TB) {
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    const int q1 = pslot_ptr[p + 1];
-    for (int q = pslot_ptr[p]; q < q1; ++q) {
-      const int sl = pslot[q];
+    const int q0 = (p == pf ? qf0 : pslot_ptr[p]) - qb, q1 = min((p == pf ? qf1 : pslot_ptr[p + 1]) - qb, qn);
+    for (int q = max(q0, 0); q < q1; ++q) {
+      const int sl = sps[q];
       f0 += (double)lds[sl * 3 + 0];
       f1 += (double)lds[sl * 3 + 1];
       f2 += (double)lds[sl * 3 + 2];
@@ -604,10 +625,10 @@ static void launch_r_tiles_t(int n, const int32_t *conn, const double *X, const 
                              hipStream_t st) {
   const int grid = (n + TileCfg<ET>::TE - 1) / TileCfg<ET>::TE;
   if (tnode_ptr != nullptr)
-    k_elem_r_tiles<T, ET, true><<<grid, kBlock, (size_t)6 * max_nodes * sizeof(double), st>>>(
+    k_elem_r_tiles<T, ET, true><<<grid, TileCfg<ET>::TB, (size_t)6 * max_nodes * sizeof(double), st>>>(
         n, conn, X, U, mp, lb, pp, ps, partial, tnode_ptr, tnode, lconn, max_nodes);
   else
-    k_elem_r_tiles<T, ET, false><<<grid, kBlock, 0, st>>>(n, conn, X, U, mp, lb, pp, ps, partial, nullptr, nullptr,
+    k_elem_r_tiles<T, ET, false><<<grid, TileCfg<ET>::TB, 0, st>>>(n, conn, X, U, mp, lb, pp, ps, partial, nullptr, nullptr,
                                                           nullptr, 0);
 }
 

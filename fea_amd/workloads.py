@@ -59,11 +59,17 @@ def beam_traction(nx=128, ny=32, nz=32, *, C10=0.5, C01=0.1, D1=0.01, traction=(
 
 
 def two_blocks_contact(n=40, *, mu=1.0, kappa=50.0, rho=1.0, penalty=1e3, velocity=0.1, gap=None,
-                       jitter=0.0, seed=0, upper_width=1.0):
+                       jitter=0.0, seed=0, upper_width=1.0, offset=(0.0, 0.0)):
     """Two blocks of n^3 Kuhn-split C3D4 cells, a unit cube and above it, by
-    ``gap`` (default a quarter cell), a centred upper_width x upper_width x 1
-    block moving down at ``velocity``; the lower block's base z=0 is fixed.  Node-to-surface penalty contact: slave nodes the
-    upper block's bottom face, master facets the lower block's top face.
+    ``gap`` (default a quarter cell), an upper_width x upper_width x 1 block, centred and
+    then shifted by ``offset`` in x and y, moving down at ``velocity``; the lower block's
+    base z=0 is fixed.  Node-to-surface penalty contact: slave nodes the upper block's
+    bottom face, master facets the lower block's top face.
+
+    With the default offset and width the slave nodes sit exactly above master vertices and
+    edges, and stay there (to round-off) while the blocks first meet: which facet takes such
+    a node is then decided by round-off.  A trajectory compared with another implementation
+    step by step needs an offset that keeps the slave nodes off the master edges.
 
     Returns (controller, info) with info = {'meshes': (lower, upper),
     'GV0': initial velocity in free-DOF order (after fe.initialize()), ...}."""
@@ -71,7 +77,8 @@ def two_blocks_contact(n=40, *, mu=1.0, kappa=50.0, rho=1.0, penalty=1e3, veloci
     gap = 0.25 * h if gap is None else gap
     lower = meshgen.kuhn_tet_block(n, n, n, jitter=jitter, seed=seed)
     w = float(upper_width)
-    upper = meshgen.kuhn_tet_block(n, n, n, size=(w, w, 1.0), origin=(0.5 - 0.5 * w, 0.5 - 0.5 * w, 1.0 + gap),
+    upper = meshgen.kuhn_tet_block(n, n, n, size=(w, w, 1.0),
+                                   origin=(0.5 - 0.5 * w + offset[0], 0.5 - 0.5 * w + offset[1], 1.0 + gap),
                                    jitter=jitter, seed=seed + 1)
     mat = materials.NeoHookean(mu, kappa)
     asm = Assembly()

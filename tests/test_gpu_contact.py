@@ -81,7 +81,8 @@ def test_candidate_overflow_is_an_error(gpu):
 
 
 def test_explicit_with_contact_matches_oracle(gpu):
-    fe, info = workloads.two_blocks_contact(3, velocity=1.0, gap=0.01, jitter=0.1, seed=5)
+    fe, info = workloads.two_blocks_contact(3, velocity=1.0, gap=0.01, jitter=0.1, seed=5, upper_width=0.9,
+                                            offset=(0.037, 0.021))
     dt, nsteps = 4e-3, 40
     fe.solver = solver.DynamicExplicitSolver(time_end=dt * nsteps, time_per_storage=1.0, dt=dt)
     fe.initialize()

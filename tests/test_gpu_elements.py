@@ -91,6 +91,28 @@ def test_fp32_kernels(gpu, etype):
     assert rel_err(R32.cpu().numpy(), R64.cpu().numpy()) < 2e-5
 
 
+@pytest.mark.parametrize('etype', ['C3D4', 'C3D8', 'C3D10'])
+@pytest.mark.parametrize('scale', [0.01, 0.03])
+def test_residual_kernel_fp64(gpu, etype, scale):
+    """The residual-only kernel (fea_elem_residual in FP64, the explicit loop's element pass)
+    against the oracle's assembled internal force, at moderate and large strains (no element
+    inverted: the oracle's forces are finite)."""
+    from fea_amd.device import DevicePlan, Group
+    from fea_amd._native import F64
+    m = mk(etype, 6)
+    N = m['nodes'].shape[0]
+    plan = DevicePlan(m['nodes'], np.ones(3 * N, bool),
+                      [Group('elem', m['conn'].shape[1], m['conn'], params=materials.NeoHookean(1.0, 100.0).kernel_params())])
+    U = scale * np.random.default_rng(11).standard_normal((N, 3))
+    R = plan.residual(torch.as_tensor(U.reshape(-1), device='cuda'), 1.0, F64).cpu().numpy()
+    dNdX, w, _ = oel.pre_load_gaussian(m['etype'], m['nodes'], m['conn'])
+    R_o, _ = oel.structural_force(dNdX, w, m['conn'], U, nh(1.0, 100.0))
+    Ra = np.zeros(3 * N)
+    np.add.at(Ra, oel.force_indices(m['conn']), R_o)
+    assert np.isfinite(Ra).all()
+    assert rel_err(R, Ra) < 1e-10
+
+
 def test_rigid_rotation_is_stress_free(gpu):
     # u = (Q - I) X for a proper rotation Q: zero residual (objectivity of the kernels)
     m = mk('C3D8', 1)

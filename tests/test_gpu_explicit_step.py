@@ -78,8 +78,9 @@ def test_reused_lists_rebuild_on_motion(gpu):
     assert st.rebuilds >= 2
 
 
-def _run_steps(precision, tiles, nsteps=60):
-    fe, info = workloads.two_blocks_contact(4, velocity=1.0, gap=0.01, jitter=0.1, seed=5)
+def _run_steps(precision, tiles, nsteps=60, n=4):
+    fe, info = workloads.two_blocks_contact(n, velocity=1.0, gap=0.01, jitter=0.1, seed=5, upper_width=0.9,
+                                            offset=(0.037, 0.021))
     fe.solver = solver.DynamicExplicitSolver(time_end=1.0, time_per_storage=1.0, dt=3e-3, store=False,
                                              precision=precision)
     fe.initialize()
@@ -102,15 +103,17 @@ def _run_steps(precision, tiles, nsteps=60):
     return U.clone(), vh.clone(), acc.clone(), active
 
 
-@pytest.mark.parametrize('precision,tol', [('fp32', 1e-9), ('fp64', 1e-12)])
-def test_tile_step_matches_node_major_and_repeats(gpu, precision, tol):
+@pytest.mark.parametrize('n,precision,tol', [(4, 'fp32', 1e-9), (4, 'fp64', 1e-12), (9, 'fp32', 1e-9),
+                                             (9, 'fp64', 1e-12)])
+def test_tile_step_matches_node_major_and_repeats(gpu, n, precision, tol):
     """Per-tile partial sums change only the association of each node's float64 force sum:
     60 contact steps stay within tol of the node-major staged step, and two runs agree bit
-    for bit."""
-    print('explicit tiles vs node-major, %s' % precision, flush=True)
-    U1, v1, a1, n1 = _run_steps(precision, True)
-    U0, v0, a0, n0 = _run_steps(precision, False)
-    U2, v2, a2, n2 = _run_steps(precision, True)
+    for bit (n = 4: one partial tile per block; n = 9: 4374 tets per block, 8 full tiles and
+    a partial one)."""
+    print('explicit tiles vs node-major, n=%d %s' % (n, precision), flush=True)
+    U1, v1, a1, n1 = _run_steps(precision, True, n=n)
+    U0, v0, a0, n0 = _run_steps(precision, False, n=n)
+    U2, v2, a2, n2 = _run_steps(precision, True, n=n)
     assert n1 > 0 and n1 == n0                          # contact was active
     assert torch.equal(U1, U2) and torch.equal(v1, v2) and torch.equal(a1, a2)
     rel = lambda x, y: float((x - y).abs().max() / y.abs().max())

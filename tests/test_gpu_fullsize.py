@@ -114,7 +114,11 @@ def test_void_c3d10_gradient_at_size(gpu):
 
 @pytest.mark.parametrize('precision,tol', [('fp64', 1e-10), ('fp32', 1e-5)])
 def test_explicit_contact_ten_cubed_blocks(gpu, precision, tol):
-    fe, info = workloads.two_blocks_contact(10, velocity=1.0, gap=0.01, jitter=0.1, seed=5)
+    # a narrower, shifted upper block keeps the slave nodes off the master edges: the trajectory
+    # is then a smooth function of round-off (aligned blocks leave facet choices to it when
+    # they first meet)
+    fe, info = workloads.two_blocks_contact(10, velocity=1.0, gap=0.01, jitter=0.1, seed=5, upper_width=0.9,
+                                            offset=(0.037, 0.021))
     fe.solver = solver.DynamicExplicitSolver(time_end=1.0, time_per_storage=1.0, cfl_factor=0.9,
                                              precision=precision)
     fe.initialize()

@@ -94,7 +94,7 @@ def test_pcg_ops_match_ctypes(gpu, precond, mat):
     pcg = solver.PCG(p.n_free, p.nnz, p.rowptr, p.col, p.diag_slot, mat, p.device, vec_dtype=F64, grp=p.grp,
                      n_groups=p.spmv_groups, chunk=8, precond=precond, spmv_col=p.spmv_col)
     pcg.set_matrix(K)
-    x_op = pcg.solve(b, None, tol=1e-20, max_iter=4000).clone()
+    x_op = pcg.solve(b, None, tol=1e-18, max_iter=4000).clone()
     it_op = pcg.iterations
     # ctypes path on the same preconditioner and fresh vectors
     s = nat.stream_ptr()
@@ -108,7 +108,7 @@ def test_pcg_ops_match_ctypes(gpu, precond, mat):
     done = 0
     while True:
         call('fea_pcg_iterate', mat, F64, n, p.nnz, p.spmv_groups, p.grp, p.rowptr, p.spmv_col, K, pc, pcg.dinv, x, r,
-             z, pp, q, ws, 1e-20, 4000, 8, st, s)
+             z, pp, q, ws, 1e-18, 4000, 8, st, s)
         done += 8
         if st[1] != 0.0 or done >= 4000:
             break
