@@ -73,6 +73,10 @@ def parse():
     ap.add_argument('--serving-eager', action='store_true',
                     help='serving: launch each PCG chunk eagerly instead of replaying the captured HIP graph')
     ap.add_argument('--cells', type=int, default=40, help='explicit: cells per block edge')
+    ap.add_argument('--explicit-overlap', action='store_true',
+                    help='explicit: contact forces on a side stream, concurrent with the element pass')
+    ap.add_argument('--explicit-graph', action='store_true',
+                    help='explicit: steps replayed in 8-step chunks from a captured HIP graph')
     ap.add_argument('--fd-directions', type=int, default=64, help='adjoint: central-difference check directions')
     ap.add_argument('--couple-pressure', type=float, default=0.06, help='couple: pressure on surface_1_All')
     ap.add_argument('--couple-increments', type=int, default=4, help='couple: load increments')
@@ -895,7 +899,8 @@ def bench_explicit(args, torch):
     warmup = 200 if args.warmup is None else args.warmup
     fe, info = workloads.two_blocks_contact(args.cells)
     fe.solver = solver.DynamicExplicitSolver(time_end=1.0, time_per_storage=1e30, cfl_factor=0.9, store=False,
-                                             precision='fp32')
+                                             precision='fp32', overlap=args.explicit_overlap,
+                                             use_graph=args.explicit_graph)
     fe.initialize()
     sol = fe.solver
     p = fe.assembly.plan
@@ -942,7 +947,9 @@ def bench_explicit(args, torch):
                      steps, warmup, elapsed, 'f32+f64',
                      {'workload': 'two %d^3-cell Kuhn-tet blocks (%d C3D4), NeoHookean mu=1 kappa=50 rho=1, upper '
                                   'block at 0.1 onto the lower (base fixed), node-to-surface penalty 1e3, dt = 0.9 CFL '
-                                  '(%.3e)' % (args.cells, E, dt), 'precision': 'fp32 element forces, fp64 state'})
+                                  '(%.3e)' % (args.cells, E, dt), 'precision': 'fp32 element forces, fp64 state',
+                      'contact_stream': 'side' if args.explicit_overlap else 'same',
+                      'graph_chunks': bool(args.explicit_graph)})
     line.update({'dt': dt, 'simulated_time': dt * (steps + warmup),
                  'energy': {'initial': hist[0], 'final': hist[-1], 'drift_rel_to_initial_kinetic': drift},
                  'roofline': roofline('k_elem_r_tiles<float,4> (fea_elem_residual_tiles, one step\'s element pass: '

@@ -93,6 +93,11 @@ _SIGS = {
     'fea_contact_set_size': (C_INT, []),
     'fea_contact_rebuild': (C_INT, [C_P, C_P, C_P, C_P]),
     'fea_contact_eval': (C_INT, [C_P, C_P, C_P, C_P, C_P, C_P]),
+    'fea_contact_track': (C_INT, [C_P, C_P, C_P, C_P]),
+    'fea_elem_residual_tiles_contact': (C_INT, [C_INT, C_INT, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P,
+                                                C_P]),
+    'fea_explicit_update_tiles_contact': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_DBL, C_DBL,
+                                                  C_P, C_P, C_P, C_P]),
     'fea_contact_grid_build': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_DBL, C_INT, C_P, C_P, C_P, C_P, C_P, C_P]),
     'fea_contact_search': (C_INT, [C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_DBL, C_INT, C_DBL, C_INT, C_P,
                                    C_P, C_P, C_P]),

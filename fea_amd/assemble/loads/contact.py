@@ -118,6 +118,14 @@ def _tributary_areas(instance, surface):
     return nodes, area
 
 
+DMAX_SLOTS, DMAX_STRIDE = 64, 16      # FEA_CONTACT_DMAX_SLOTS / _STRIDE of include/fea_amd.h
+
+
+def slots_max(dmax_words):
+    """The tracked squared displacement from the dmax words (int64 bits of doubles >= 0)."""
+    return float(np.asarray(dmax_words).view(np.float64)[::DMAX_STRIDE][:DMAX_SLOTS].max())
+
+
 class ContactSet(ctypes.Structure):
     """fea_contact_set of include/fea_amd.h (sizes, constants and device buffers of
     one contact evaluation)."""
@@ -129,7 +137,7 @@ class ContactSet(ctypes.Structure):
                                                'nf_facet', 'nf_vert', 'slave_area', 'centroid', 'force', 'weight',
                                                'x0', 'dmax', 'key', 'bucket_start', 'bucket_items', 'work', 'cand',
                                                'count', 'max_count', 'chosen', 'sup_max', 'inv_start', 'inv_items', 'm_ptr',
-                                               'm_items')]
+                                               'm_items', 'lists_ok')]
 
 
 class ContactState:
@@ -184,15 +192,17 @@ class ContactState:
         self.x0 = z(3 * cnode.shape[0], torch.float64)
         self.vcand = z(S * load.kmax, torch.int32)
         self.vcount = z(S, torch.int32)
-        self.dmax = z(1, torch.int64)
+        self.dmax = z(DMAX_SLOTS * DMAX_STRIDE, torch.int64)     # slot k at [k * DMAX_STRIDE]
         self.sup_max = z(1, torch.int32)
         self.inv_start = z(F + 1, torch.int32)
         self.inv_items = z(S * load.kmax, torch.int32)
         self.m_ptr = z(mnodes.shape[0] + 1, torch.int32)
         self.m_items = z(6 * S * load.kmax, torch.int32)
+        self.lists_ok = z(1, torch.int32)
         self._set = None
         self._need_rebuild = True
-        self._pin = torch.zeros(1, dtype=torch.int64).pin_memory() if torch.cuda.is_available() else None
+        self._pin = torch.zeros(DMAX_SLOTS * DMAX_STRIDE, dtype=torch.int64).pin_memory() \
+            if torch.cuda.is_available() else None
         self._poll = None
         self._reading = None
         self._rebuilt_at = 0
@@ -261,7 +271,8 @@ class ContactState:
                             ('bucket_items', self.bitems), ('work', self.work), ('cand', self.vcand),
                             ('count', self.vcount), ('max_count', self.max_count), ('chosen', self.chosen),
                             ('sup_max', self.sup_max), ('inv_start', self.inv_start),
-                            ('inv_items', self.inv_items), ('m_ptr', self.m_ptr), ('m_items', self.m_items)):
+                            ('inv_items', self.inv_items), ('m_ptr', self.m_ptr), ('m_items', self.m_items),
+                            ('lists_ok', self.lists_ok)):
                 setattr(c, name, t.data_ptr())
             self._set = c
         return self._set
@@ -299,7 +310,7 @@ class ContactState:
         ev, at = self._poll
         ev.synchronize()
         self._poll = None
-        self._reading = (at, float(np.sqrt(max(0.0, self._pin.numpy().view(np.float64)[0]))))
+        self._reading = (at, float(np.sqrt(max(0.0, slots_max(self._pin.numpy())))))
 
     def rebuild_due(self, ahead):
         """Whether the lists should be rebuilt now: the displacement of the last reading,
@@ -319,15 +330,39 @@ class ContactState:
         steps the previous poll is read (so the host runs at most ``poll_every`` steps ahead
         of its reading), the lists are rebuilt when ``rebuild_due``, and a new poll is issued.
         The kmax overflow check is left to ``check()``."""
+        self.step_begin(U, poll_every)
+        self.eval(U)
+        self.step_end(poll_every)
+        return self.fc
+
+    def step_begin(self, U, poll_every=8):
+        """The host's part of ``step_forces`` before the step's launches (poll read, rebuild)."""
         if self._need_rebuild or self.steps % poll_every == 0:
             self.poll_read()
             if self.rebuild_due(poll_every):
                 self.rebuild(U)
-        self.eval(U)
+
+    def step_end(self, poll_every=8):
+        """The host's part of ``step_forces`` after the step's launches (count, poll issue)."""
         self.steps += 1
         if self.steps % poll_every == 0:
             self.poll_issue()
-        return self.fc
+
+    def track(self, U):
+        """Raise dmax by the contact nodes' displacement at U (fea_contact_track): what the
+        explicit loop's fused launches need at a U they did not produce."""
+        call('fea_contact_track', ctypes.addressof(self.contact_set()), self.plan.X, U, nat.stream_ptr())
+
+    def node_mask(self):
+        """uint32 bit mask over the plan's nodes marking the contact nodes (bit A % 32 of word
+        A / 32), for fea_explicit_update_tiles_contact."""
+        if getattr(self, '_cmask', None) is None:
+            n = self.plan.n_nodes
+            bits = np.zeros(32 * ((n + 31) // 32), dtype=bool)
+            bits[self.cnode.cpu().numpy()] = True
+            words = np.packbits(bits.reshape(-1, 32)[:, ::-1], axis=1, bitorder='big').view('>u4').astype(np.uint32)
+            self._cmask = torch.as_tensor(words.reshape(-1).view(np.int32), device=self.plan.device)
+        return self._cmask
 
     def energy(self):
         nb = nat.lib().fea_contact_blocks(self.S)

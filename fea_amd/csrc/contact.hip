@@ -8,20 +8,9 @@
 //   pair (s, f) is a candidate iff d2 <= r2.
 // Deterministic throughout: bucket sorts order items by id inside each bucket
 // and every force is summed by one thread in a fixed order (no float atomics).
-#include <climits>
-
-#include "common.h"
+#include "contact_device.h"
 
 namespace fea {
-
-__device__ __forceinline__ double dadd(double a, double b) { return __dadd_rn(a, b); }
-__device__ __forceinline__ double dmul(double a, double b) { return __dmul_rn(a, b); }
-
-__device__ __forceinline__ void node_pos(const double *X, const double *U, int64_t n, double p[3]) {
-  p[0] = dadd(X[3 * n + 0], U[3 * n + 0]);
-  p[1] = dadd(X[3 * n + 1], U[3 * n + 1]);
-  p[2] = dadd(X[3 * n + 2], U[3 * n + 2]);
-}
 
 __device__ __forceinline__ unsigned int cell_key(int64_t ix, int64_t iy, int64_t iz, unsigned int mask) {
   const uint64_t h = ((uint64_t)ix * 73856093ull) ^ ((uint64_t)iy * 19349663ull) ^ ((uint64_t)iz * 83492791ull);
@@ -199,123 +188,6 @@ __global__ __launch_bounds__(kBlock) void k_contact_search(
   store_candidates(s, lane, v, total, kmax, cand, count, max_count);
 }
 
-// ---------------------------------------------------------------- penalty
-// Facet choice per slave, over the candidates whose plane it penetrates
-// (-depth_max < g < 0), with v = the smallest barycentric coordinate of the
-// slave's projection on the facet plane:
-//   tier 0  v >= -kInsideTol (inside; edges and vertices count, whatever the
-//           rounding, since conforming meshes put slaves right over master
-//           nodes): the largest g wins (least penetration);
-//   tier 1  -kExtendTol <= v < -kInsideTol, used only when no facet is in
-//           tier 0: the largest v wins.  This closes the dead zone under a
-//           concave crease (a valley, such as the dent a slave presses into the
-//           master surface), where a penetrating slave's projections on both
-//           adjacent planes fall just outside their facets and the slave would
-//           otherwise drop out of contact and re-enter deeper (an energy jump).
-// Ties keep the smaller facet id.  Force F = -k A_s g n on the slave (A_s its
-// tributary area, 1 when area is null).
-constexpr double kInsideTol = 1e-10;
-constexpr double kExtendTol = 0.1;
-
-// The penalty terms of slave position p against facet f: false when the slave does not
-// penetrate the facet's plane within depth_max or projects outside the extended facet.
-struct PenCand {
-  int tier;
-  double key, g, n[3], w[3];
-};
-
-__device__ __forceinline__ bool pen_eval(const double p[3], const double *__restrict__ X,
-                                         const double *__restrict__ U, const int32_t *__restrict__ conn, int f,
-                                         double depth_max, PenCand &out) {
-  double a[3], b[3], c[3];
-  node_pos(X, U, conn[3 * (int64_t)f + 0], a);
-  node_pos(X, U, conn[3 * (int64_t)f + 1], b);
-  node_pos(X, U, conn[3 * (int64_t)f + 2], c);
-  double e1[3], e2[3], ap[3];
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    e1[j] = b[j] - a[j];
-    e2[j] = c[j] - a[j];
-    ap[j] = p[j] - a[j];
-  }
-  double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-  const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-  if (!(nn > 0.0)) return false;
-  n[0] /= nn;
-  n[1] /= nn;
-  n[2] /= nn;
-  const double g = ap[0] * n[0] + ap[1] * n[1] + ap[2] * n[2];
-  if (!(g < 0.0) || !(g > -depth_max)) return false;
-  double q[3];
-#pragma unroll
-  for (int j = 0; j < 3; ++j) q[j] = ap[j] - g * n[j];
-  const double d00 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
-  const double d01 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
-  const double d11 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
-  const double d20 = q[0] * e1[0] + q[1] * e1[1] + q[2] * e1[2];
-  const double d21 = q[0] * e2[0] + q[1] * e2[1] + q[2] * e2[2];
-  const double den = d00 * d11 - d01 * d01;
-  const double xi = (d11 * d20 - d01 * d21) / den;
-  const double eta = (d00 * d21 - d01 * d20) / den;
-  const double w0 = 1.0 - xi - eta;
-  const double v = fmin(w0, fmin(xi, eta));
-  if (!(v >= -kExtendTol)) return false;
-  out.tier = (v >= -kInsideTol) ? 0 : 1;
-  out.key = (out.tier == 0) ? g : v;
-  out.g = g;
-  out.n[0] = n[0];
-  out.n[1] = n[1];
-  out.n[2] = n[2];
-  out.w[0] = w0;
-  out.w[1] = xi;
-  out.w[2] = eta;
-  return true;
-}
-
-// The wave's best lane by (tier ascending, key descending, index ascending); lane 0 writes
-// the slave's choice, force and weights and returns its penalty energy (0 on other lanes).
-__device__ __forceinline__ double pen_finish(int s, int lane, int bf, int bi, int btier, double bkey, double bg,
-                                             double bn[3], double bw[3], double k_pen,
-                                             const double *__restrict__ area, int32_t *__restrict__ chosen,
-                                             double *__restrict__ force, double *__restrict__ weight) {
-  int wl = lane;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const int ot = __shfl_xor(btier, off, 64);
-    const double ok = __shfl_xor(bkey, off, 64);
-    const int oi = __shfl_xor(bi, off, 64);
-    const int ol = __shfl_xor(wl, off, 64);
-    if (ot < btier || (ot == btier && (ok > bkey || (ok == bkey && oi < bi)))) {
-      btier = ot;
-      bkey = ok;
-      bi = oi;
-      wl = ol;
-    }
-  }
-  bf = __shfl(bf, wl, 64);
-  bg = __shfl(bg, wl, 64);
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    bn[j] = __shfl(bn[j], wl, 64);
-    bw[j] = __shfl(bw[j], wl, 64);
-  }
-  double e = 0.0;
-  if (lane == 0) {
-    const int best = btier < 2 ? bf : -1;
-    chosen[s] = best;
-    const double ks = (area != nullptr) ? k_pen * area[s] : k_pen;
-    const double fm = (best >= 0) ? -ks * bg : 0.0;
-    force[3 * (int64_t)s + 0] = fm * bn[0];
-    force[3 * (int64_t)s + 1] = fm * bn[1];
-    force[3 * (int64_t)s + 2] = fm * bn[2];
-    weight[3 * (int64_t)s + 0] = best >= 0 ? bw[0] : 0.0;
-    weight[3 * (int64_t)s + 1] = best >= 0 ? bw[1] : 0.0;
-    weight[3 * (int64_t)s + 2] = best >= 0 ? bw[2] : 0.0;
-    if (best >= 0) e = 0.5 * ks * bg * bg;
-  }
-  return e;
-}
-
 // One wave per slave node: lane i evaluates candidates i, i + 64, ... of the slave's sorted
 // candidate list and keeps its best by (tier ascending, key descending, index ascending); a
 // butterfly over the wave picks the best of the lanes by the same order, which is the
@@ -406,31 +278,6 @@ __global__ __launch_bounds__(kBlock) void k_fill_dbl(int64_t n, double *__restri
   if (i < n) a[i] = 0.0;
 }
 
-// ---------------------------------------------------------------- reused candidate lists
-// Between rebuilds each slave keeps the candidates of a search with the enlarged radius
-// r + skin, and facet -> slave lists of those pairs.  While no contact node has moved more
-// than skin/2 since the rebuild, every pair within r now was within r + skin then, so the
-// stored lists hold every exact candidate; each step tests the exact predicate on them.
-// Otherwise (the host rebuilds late) the kernels search every facet / slave instead.
-// dmax holds the largest squared displacement of a contact node since the rebuild, as the
-// bits of a non-negative double (ordered like the values).
-
-__device__ __forceinline__ void facet_centroid(const double *__restrict__ X, const double *__restrict__ U,
-                                               const int32_t *__restrict__ conn, int f, double m[3]) {
-  double a[3], b[3], c[3];
-  node_pos(X, U, conn[3 * (int64_t)f + 0], a);
-  node_pos(X, U, conn[3 * (int64_t)f + 1], b);
-  node_pos(X, U, conn[3 * (int64_t)f + 2], c);
-#pragma unroll
-  for (int i = 0; i < 3; ++i) m[i] = dadd(dadd(a[i], b[i]), c[i]) / 3.0;
-}
-
-// the exact candidate predicate of k_contact_search at the current centroid
-__device__ __forceinline__ bool exact_candidate(const double p[3], const double m[3], double r2) {
-  const double ddx = p[0] - m[0], ddy = p[1] - m[1], ddz = p[2] - m[2];
-  return dadd(dadd(dmul(ddx, ddx), dmul(ddy, ddy)), dmul(ddz, ddz)) <= r2;
-}
-
 // dmax = max over contact nodes of |x - x0|^2
 __global__ __launch_bounds__(kBlock) void k_contact_track(int nc, const int32_t *__restrict__ cnode,
                                                           const double *__restrict__ X,
@@ -442,11 +289,10 @@ __global__ __launch_bounds__(kBlock) void k_contact_track(int nc, const int32_t 
   if (i < nc) {
     double p[3];
     node_pos(X, U, cnode[i], p);
-    const double dx = p[0] - x0[3 * (int64_t)i], dy = p[1] - x0[3 * (int64_t)i + 1], dz = p[2] - x0[3 * (int64_t)i + 2];
-    d2 = dx * dx + dy * dy + dz * dz;
+    d2 = track_d2(p, x0, i);
   }
   for (int off = 32; off > 0; off >>= 1) d2 = fmax(d2, __shfl_down(d2, off, 64));
-  if ((threadIdx.x & 63) == 0 && d2 > 0.0) atomicMax(dmax, (unsigned long long)__double_as_longlong(d2));
+  if ((threadIdx.x & 63) == 0) dmax_raise(dmax, (int)(i >> 6), d2);
 }
 
 // x0 = current positions of the contact nodes, dmax = 0 (after a rebuild's search)
@@ -455,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_contact_snapshot(int nc, const int32
                                                              const double *__restrict__ U, double *__restrict__ x0,
                                                              unsigned long long *__restrict__ dmax) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *dmax = 0ull;
+  if (i < kDmaxSlots) dmax[(int64_t)i * kDmaxStride] = 0ull;  // the launch has >= 256 threads
   if (i >= nc) return;
   double p[3];
   node_pos(X, U, cnode[i], p);
@@ -464,57 +310,18 @@ __global__ __launch_bounds__(kBlock) void k_contact_snapshot(int nc, const int32
   x0[3 * (int64_t)i + 2] = p[2];
 }
 
-__device__ __forceinline__ bool lists_valid(const unsigned long long *dmax, double half_skin2, const int32_t *sup_max,
-                                            int kmax) {
-  return __longlong_as_double((long long)*dmax) <= half_skin2 && *sup_max <= kmax;
-}
-
-// One wave per slave: candidates from the slave's stored list (or every facet), the exact
-// predicate, then the penalty choice with ties broken by facet id (the order of the exact
-// search's sorted list).  count_exact[s] = the exact candidate count; counts above kmax raise
-// max_count as the search does.
-__global__ __launch_bounds__(kBlock) void k_contact_penalty_v(
-    int ns, const int32_t *__restrict__ slave, const double *__restrict__ X, const double *__restrict__ U,
-    const int32_t *__restrict__ conn, int nf, const int32_t *__restrict__ cand, const int32_t *__restrict__ count,
-    int kmax, double r2, const unsigned long long *__restrict__ dmax, double half_skin2,
-    const int32_t *__restrict__ sup_max, int32_t *__restrict__ max_count, double k_pen,
-    const double *__restrict__ area, double depth_max, int32_t *__restrict__ chosen, double *__restrict__ force,
-    double *__restrict__ weight, double *__restrict__ energy_partial) {
+// One wave per slave (penalty_slave).  lists_ok (when set) gets the validity of the stored
+// lists at this launch (the reactions of a later launch read it).
+__global__ __launch_bounds__(kBlock) void k_contact_penalty_v(ContactDev c, const double *__restrict__ X,
+                                                              const double *__restrict__ U,
+                                                              double *__restrict__ energy_partial) {
   const int lane = threadIdx.x & 63;
   const int s = blockIdx.x * kPenWaves + (threadIdx.x >> 6);
   double e = 0.0;
-  if (s < ns) {
-    const bool valid = lists_valid(dmax, half_skin2, sup_max, kmax);
-    double p[3];
-    node_pos(X, U, slave[s], p);
-    int bf = -1, bi = INT_MAX, btier = 2, hits = 0;
-    double bg = 0.0, bkey = 0.0, bn[3] = {0, 0, 0}, bw[3] = {0, 0, 0};
-    const int n = valid ? min(count[s], kmax) : nf;
-    for (int i = lane; i < n; i += 64) {
-      const int f = valid ? cand[(int64_t)s * kmax + i] : i;
-      double m[3];
-      facet_centroid(X, U, conn, f, m);
-      if (!exact_candidate(p, m, r2)) continue;
-      ++hits;
-      PenCand c;
-      if (!pen_eval(p, X, U, conn, f, depth_max, c)) continue;
-      if (c.tier < btier || (c.tier == btier && c.key > bkey)) {
-        bf = f;
-        bi = f;
-        btier = c.tier;
-        bkey = c.key;
-        bg = c.g;
-        bn[0] = c.n[0];
-        bn[1] = c.n[1];
-        bn[2] = c.n[2];
-        bw[0] = c.w[0];
-        bw[1] = c.w[1];
-        bw[2] = c.w[2];
-      }
-    }
-    for (int off = 32; off > 0; off >>= 1) hits += __shfl_xor(hits, off, 64);
-    if (lane == 0 && max_count != nullptr && hits > kmax) atomicMax(max_count, hits);
-    e = pen_finish(s, lane, bf, bi, btier, bkey, bg, bn, bw, k_pen, area, chosen, force, weight);
+  if (s < c.n_slave) {
+    const bool valid = lists_valid(c.dmax, c.half_skin2, c.sup_max, c.kmax, lane);
+    e = penalty_slave(c, X, U, s, lane, valid);
+    if (s == 0 && lane == 0 && c.lists_ok != nullptr) *c.lists_ok = valid ? 1 : 0;
   }
   if (energy_partial != nullptr) {
     const double t = block_sum(e);
@@ -522,84 +329,18 @@ __global__ __launch_bounds__(kBlock) void k_contact_penalty_v(
   }
 }
 
-// Contact node forces, one wave per contact node.  Its master items (slave s, facet f,
-// local vertex v), in facet order and ascending slaves within a facet, come from the stored
-// per-master-node lists (valid lists: every slave that chose one of the node's facets is
-// there) or, otherwise, from a scan of every slave per facet.  Lanes test items chosen[s] == f
-// in chunks of 64, a ballot marks the matches, and the wave subtracts F_s w_s(v) in item
-// order: the sums of the bucket sort of slaves by chosen facet.  Then the node's own slave
-// force is added; fc[node] is written once.
+// Contact node forces, one wave per contact node (reaction_node); fc[node] is written once.
 constexpr int kReactWaves = kBlock / 64;
 
-__device__ __forceinline__ void react_chunk(bool match, int s, int v, const double *__restrict__ force,
-                                            const double *__restrict__ weight, double acc[3]) {
-  unsigned long long mask = __ballot(match);
-  double f0 = 0.0, f1 = 0.0, f2 = 0.0, w = 0.0;
-  if (match) {
-    f0 = force[3 * (int64_t)s + 0];
-    f1 = force[3 * (int64_t)s + 1];
-    f2 = force[3 * (int64_t)s + 2];
-    w = weight[3 * (int64_t)s + v];
-  }
-  while (mask) {
-    const int b = __ffsll((long long)mask) - 1;
-    mask &= mask - 1;
-    const double wb = __shfl(w, b, 64);
-    acc[0] -= __shfl(f0, b, 64) * wb;
-    acc[1] -= __shfl(f1, b, 64) * wb;
-    acc[2] -= __shfl(f2, b, 64) * wb;
-  }
-}
-
-__global__ __launch_bounds__(kBlock) void k_contact_reactions_v(
-    int nc, int ns, const int32_t *__restrict__ cnode, const int32_t *__restrict__ cn_slave,
-    const int32_t *__restrict__ cn_master, const int32_t *__restrict__ nf_ptr, const int32_t *__restrict__ nf_facet,
-    const int32_t *__restrict__ nf_vert, const int32_t *__restrict__ m_ptr, const int32_t *__restrict__ m_items,
-    const int32_t *__restrict__ chosen, const unsigned long long *__restrict__ dmax, double half_skin2,
-    const int32_t *__restrict__ sup_max, int kmax, const double *__restrict__ force,
-    const double *__restrict__ weight, double *__restrict__ fc) {
+__global__ __launch_bounds__(kBlock) void k_contact_reactions_v(ContactDev c, double *__restrict__ fc) {
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * kReactWaves + (threadIdx.x >> 6);  // uniform over the wave
-  if (i >= nc) return;
-  const bool valid = lists_valid(dmax, half_skin2, sup_max, kmax);
-  double r[3] = {0.0, 0.0, 0.0};
-  const int m = cn_master[i];
-  if (m >= 0) {
-    double acc[3] = {0.0, 0.0, 0.0};
-    if (valid) {
-      const int k0 = m_ptr[m], k1 = m_ptr[m + 1];
-      for (int c = k0; c < k1; c += 64) {
-        const int k = c + lane;
-        int s = 0, fv = -1;
-        if (k < k1) {
-          s = m_items[2 * (int64_t)k];
-          fv = m_items[2 * (int64_t)k + 1];
-        }
-        const bool match = fv >= 0 && chosen[s] == (fv >> 2);
-        react_chunk(match, s, fv & 3, force, weight, acc);
-      }
-    } else {
-      for (int t = nf_ptr[m]; t < nf_ptr[m + 1]; ++t) {
-        const int f = nf_facet[t], v = nf_vert[t];
-        for (int c = 0; c < ns; c += 64) {
-          const int s = c + lane;
-          const bool match = s < ns && chosen[s] == f;
-          react_chunk(match, s, v, force, weight, acc);
-        }
-      }
-    }
-    r[0] += acc[0];
-    r[1] += acc[1];
-    r[2] += acc[2];
-  }
-  const int sl = cn_slave[i];
+  if (i >= c.n_cnode) return;
+  const bool valid = lists_valid(c.dmax, c.half_skin2, c.sup_max, c.kmax, lane);
+  double r[3];
+  reaction_node(c, i, lane, valid, r);
   if (lane == 0) {
-    if (sl >= 0) {
-      r[0] += force[3 * (int64_t)sl + 0];
-      r[1] += force[3 * (int64_t)sl + 1];
-      r[2] += force[3 * (int64_t)sl + 2];
-    }
-    const int64_t n = cnode[i];
+    const int64_t n = c.cnode[i];
     fc[3 * n + 0] = r[0];
     fc[3 * n + 1] = r[1];
     fc[3 * n + 2] = r[2];
@@ -760,7 +501,8 @@ extern "C" int fea_contact_forces(int n_slave, const int32_t *slave, const doubl
 
 extern "C" int fea_contact_set_size(void) { return (int)sizeof(fea_contact_set); }
 
-static int check_set(const fea_contact_set *cs, const char *name) {
+namespace fea {
+int contact_check_set(const fea_contact_set *cs, const char *name) {
   FEA_CHECK_ARG(cs != nullptr, "%s: null set", name);
   const fea_contact_set &c = *cs;
   FEA_CHECK_ARG(c.n_slave > 0 && c.n_facets > 0 && c.n_cnode > 0 && c.n_master > 0 && c.kmax >= 1 &&
@@ -775,9 +517,10 @@ static int check_set(const fea_contact_set *cs, const char *name) {
                 "%s: null buffer", name);
   return FEA_OK;
 }
+}  // namespace fea
 
 extern "C" int fea_contact_rebuild(const fea_contact_set *cs, const double *X, const double *U, void *stream) {
-  const int rc0 = check_set(cs, "fea_contact_rebuild");
+  const int rc0 = contact_check_set(cs, "fea_contact_rebuild");
   if (rc0 != FEA_OK) return rc0;
   FEA_CHECK_ARG(X && U, "fea_contact_rebuild: null X or U");
   const fea_contact_set &c = *cs;
@@ -810,21 +553,28 @@ extern "C" int fea_contact_rebuild(const fea_contact_set *cs, const double *X, c
   return FEA_OK;
 }
 
+extern "C" int fea_contact_track(const fea_contact_set *cs, const double *X, const double *U, void *stream) {
+  const int rc0 = contact_check_set(cs, "fea_contact_track");
+  if (rc0 != FEA_OK) return rc0;
+  FEA_CHECK_ARG(X && U, "fea_contact_track: null X or U");
+  const fea_contact_set &c = *cs;
+  k_contact_track<<<blocks_for(c.n_cnode), kBlock, 0, as_stream(stream)>>>(c.n_cnode, c.cnode, X, U, c.x0,
+                                                                            (unsigned long long *)c.dmax);
+  FEA_CHECK_LAUNCH("fea_contact_track");
+  return FEA_OK;
+}
+
 extern "C" int fea_contact_eval(const fea_contact_set *cs, const double *X, const double *U, double *fc,
                                 double *energy_partial, void *stream) {
-  const int rc0 = check_set(cs, "fea_contact_eval");
+  const int rc0 = contact_check_set(cs, "fea_contact_eval");
   if (rc0 != FEA_OK) return rc0;
   FEA_CHECK_ARG(X && U && fc, "fea_contact_eval: null X, U or fc");
   const fea_contact_set &c = *cs;
   hipStream_t st = as_stream(stream);
-  const double h2 = 0.25 * c.skin * c.skin;
-  k_contact_track<<<blocks_for(c.n_cnode), kBlock, 0, st>>>(c.n_cnode, c.cnode, X, U, c.x0, (unsigned long long *)c.dmax);
-  k_contact_penalty_v<<<fea_contact_blocks(c.n_slave), kBlock, 0, st>>>(
-      c.n_slave, c.slave, X, U, c.conn, c.n_facets, c.cand, c.count, c.kmax, c.radius * c.radius, (const unsigned long long *)c.dmax, h2,
-      c.sup_max, c.max_count, c.penalty, c.slave_area, c.depth_max, c.chosen, c.force, c.weight, energy_partial);
-  k_contact_reactions_v<<<(c.n_cnode + kReactWaves - 1) / kReactWaves, kBlock, 0, st>>>(
-      c.n_cnode, c.n_slave, c.cnode, c.cn_slave, c.cn_master, c.nf_ptr, c.nf_facet, c.nf_vert, c.m_ptr,
-      c.m_items, c.chosen, (const unsigned long long *)c.dmax, h2, c.sup_max, c.kmax, c.force, c.weight, fc);
+  const ContactDev d = contact_dev(c);
+  k_contact_track<<<blocks_for(c.n_cnode), kBlock, 0, st>>>(c.n_cnode, c.cnode, X, U, c.x0, d.dmax);
+  k_contact_penalty_v<<<fea_contact_blocks(c.n_slave), kBlock, 0, st>>>(d, X, U, energy_partial);
+  k_contact_reactions_v<<<(c.n_cnode + kReactWaves - 1) / kReactWaves, kBlock, 0, st>>>(d, fc);
   FEA_CHECK_LAUNCH("fea_contact_eval");
   return FEA_OK;
 }

@@ -1,5 +1,6 @@
 // Element kernels: residual + consistent tangent, residual only, strain energy,
 // Gauss-point geometry export and lumped mass for C3D4 / C3D8 / C3D10.
+#include "contact_device.h"
 #include "element_traits.h"
 
 namespace fea {
@@ -332,7 +333,7 @@ struct TileCfg<10> {
 // their X and U copied to LDS first (coalesced), and lconn gives each element's nodes as
 // tile-local indices; the element arithmetic is the same, on the same values.
 template <typename T, int ET, bool STAGE>
-__global__ __launch_bounds__(TileCfg<ET>::TB) void k_elem_r_tiles(int n_elem, const int32_t *__restrict__ conn,
+__device__ __forceinline__ void tile_pass(int t, int n_elem, const int32_t *__restrict__ conn,
                                                          const double *__restrict__ X,
                                                          const double *__restrict__ U, MatParams mp,
                                                          const int32_t *__restrict__ tile_lbase,
@@ -347,7 +348,6 @@ __global__ __launch_bounds__(TileCfg<ET>::TB) void k_elem_r_tiles(int n_elem, co
   __shared__ T lds[TE * NEN * 3];
   __shared__ uint16_t sps[QT];          // the tile's pslot entries
   extern __shared__ double node_lds[];  // STAGE: X then U of the tile's nodes, 3 each
-  const int t = blockIdx.x;
   const int e0 = t * TE;
   // the reduction's tables are loaded before the element work, so that their latency hides
   // behind it: the tile's pslot entries (at most TE * nen, one per free (element, node) slot)
@@ -405,6 +405,69 @@ __global__ __launch_bounds__(TileCfg<ET>::TB) void k_elem_r_tiles(int n_elem, co
     partial[3 * (int64_t)p + 2] = f2;
   }
 }
+
+template <typename T, int ET, bool STAGE>
+__global__ __launch_bounds__(TileCfg<ET>::TB) void k_elem_r_tiles(int n_elem, const int32_t *__restrict__ conn,
+                                                                  const double *__restrict__ X,
+                                                                  const double *__restrict__ U, MatParams mp,
+                                                                  const int32_t *__restrict__ tile_lbase,
+                                                                  const int32_t *__restrict__ pslot_ptr,
+                                                                  const uint16_t *__restrict__ pslot,
+                                                                  double *__restrict__ partial,
+                                                                  const int32_t *__restrict__ tnode_ptr,
+                                                                  const int32_t *__restrict__ tnode,
+                                                                  const uint16_t *__restrict__ lconn, int max_nodes) {
+  tile_pass<T, ET, STAGE>(blockIdx.x, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial, tnode_ptr, tnode,
+                          lconn, max_nodes);
+}
+
+// The explicit loop's fused first launch: workgroups [0, n_tiles) run the tile pass, the
+// ones after them the contact penalty of one slave per wave (penalty_slave on the stored
+// lists, or every facet when the lists are stale; it reads only X and U).  (Penalty
+// workgroups first measured 1.4 % slower per step.)  The validity of the lists goes to
+// lists_ok for the second launch's contact-node forces.
+template <typename T, int ET>
+__device__ __forceinline__ void tiles_pen_body(int n_tiles, int n_elem, const int32_t *__restrict__ conn,
+                                               const double *__restrict__ X, const double *__restrict__ U,
+                                               const MatParams &mp, const int32_t *__restrict__ tile_lbase,
+                                               const int32_t *__restrict__ pslot_ptr,
+                                               const uint16_t *__restrict__ pslot, double *__restrict__ partial,
+                                               const ContactDev &cd) {
+  if ((int)blockIdx.x < n_tiles) {
+    tile_pass<T, ET, false>((int)blockIdx.x, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial,
+                            nullptr, nullptr, nullptr, 0);
+    return;
+  }
+  constexpr int W = TileCfg<ET>::TB / 64;
+  const int lane = threadIdx.x & 63;
+  const int s = ((int)blockIdx.x - n_tiles) * W + (threadIdx.x >> 6);  // uniform over the wave
+  if (s >= cd.n_slave) return;
+  const bool valid = lists_valid(cd.dmax, cd.half_skin2, cd.sup_max, cd.kmax, lane);
+  penalty_slave(cd, X, U, s, lane, valid);
+  if (s == 0 && lane == 0) *cd.lists_ok = valid ? 1 : 0;
+}
+
+template <typename T, int ET>
+__global__ __launch_bounds__(TileCfg<ET>::TB) void k_elem_r_tiles_pen(
+    int n_tiles, int n_elem, const int32_t *__restrict__ conn, const double *__restrict__ X,
+    const double *__restrict__ U, MatParams mp, const int32_t *__restrict__ tile_lbase,
+    const int32_t *__restrict__ pslot_ptr, const uint16_t *__restrict__ pslot, double *__restrict__ partial,
+    ContactDev cd) {
+  tiles_pen_body<T, ET>(n_tiles, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial, cd);
+}
+
+// The float C3D4 form: the penalty code alone needs ~110 VGPRs, which would cut the tile
+// pass (74) from 6 to 4 waves per SIMD; the register budget of 6 waves (80) is imposed and
+// the penalty spills a few values instead (measured on the two-block step: 24.3k -> 27.1k
+// steps/s).
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k_elem_r_tiles_pen_f4(
+    int n_tiles, int n_elem, const int32_t *__restrict__ conn, const double *__restrict__ X,
+    const double *__restrict__ U, MatParams mp, const int32_t *__restrict__ tile_lbase,
+    const int32_t *__restrict__ pslot_ptr, const uint16_t *__restrict__ pslot, double *__restrict__ partial,
+    ContactDev cd) {
+  tiles_pen_body<float, 4>(n_tiles, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial, cd);
+}
+static_assert(TileCfg<4>::TB == 512, "k_elem_r_tiles_pen_f4 launch bounds");
 
 template <int ET>
 __global__ __launch_bounds__(kBlock) void k_elem_energy(int n_elem, const int32_t *__restrict__ conn,
@@ -741,6 +804,56 @@ extern "C" int fea_elem_residual_tiles(int elem_type, int dtype, int n_elem, con
                                   tnode, lconn, max_tile_nodes, as_stream(stream));
   return launch_r_tiles<float>(elem_type, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial, tnode_ptr,
                                tnode, lconn, max_tile_nodes, as_stream(stream));
+}
+
+template <typename T, int ET>
+static void launch_r_tiles_pen_t(int n, const int32_t *conn, const double *X, const double *U, MatParams mp,
+                                 const int32_t *lb, const int32_t *pp, const uint16_t *ps, double *partial,
+                                 const ContactDev &cd, hipStream_t st) {
+  constexpr int TE = TileCfg<ET>::TE, W = TileCfg<ET>::TB / 64;
+  const int n_tiles = (n + TE - 1) / TE, n_pen = (cd.n_slave + W - 1) / W;
+  if constexpr (ET == 4 && sizeof(T) == sizeof(float))
+    k_elem_r_tiles_pen_f4<<<n_tiles + n_pen, TileCfg<ET>::TB, 0, st>>>(n_tiles, n, conn, X, U, mp, lb, pp, ps,
+                                                                      partial, cd);
+  else
+    k_elem_r_tiles_pen<T, ET><<<n_tiles + n_pen, TileCfg<ET>::TB, 0, st>>>(n_tiles, n, conn, X, U, mp, lb, pp, ps,
+                                                                            partial, cd);
+}
+
+template <typename T>
+static void launch_r_tiles_pen(int et, int n, const int32_t *conn, const double *X, const double *U, MatParams mp,
+                               const int32_t *lb, const int32_t *pp, const uint16_t *ps, double *partial,
+                               const ContactDev &cd, hipStream_t st) {
+  switch (et) {
+    case 4: launch_r_tiles_pen_t<T, 4>(n, conn, X, U, mp, lb, pp, ps, partial, cd, st); break;
+    case 8: launch_r_tiles_pen_t<T, 8>(n, conn, X, U, mp, lb, pp, ps, partial, cd, st); break;
+    case 10: launch_r_tiles_pen_t<T, 10>(n, conn, X, U, mp, lb, pp, ps, partial, cd, st); break;
+  }
+}
+
+extern "C" int fea_elem_residual_tiles_contact(int elem_type, int dtype, int n_elem, const int32_t *conn,
+                                               const double *X, const double *U, const double *params_host,
+                                               const int32_t *tile_lbase, const int32_t *pslot_ptr,
+                                               const uint16_t *pslot, double *partial, const fea_contact_set *cs,
+                                               void *stream) {
+  FEA_CHECK_ARG(valid_elem(elem_type), "fea_elem_residual_tiles_contact: bad element type %d", elem_type);
+  FEA_CHECK_ARG(dtype == FEA_F32 || dtype == FEA_F64, "fea_elem_residual_tiles_contact: bad dtype %d", dtype);
+  FEA_CHECK_ARG(n_elem >= 0 && params_host != nullptr && X && U, "fea_elem_residual_tiles_contact: bad arguments");
+  FEA_CHECK_ARG(n_elem == 0 || (conn && tile_lbase && pslot_ptr && pslot && partial),
+                "fea_elem_residual_tiles_contact: null input");
+  const int rc = contact_check_set(cs, "fea_elem_residual_tiles_contact");
+  if (rc != FEA_OK) return rc;
+  FEA_CHECK_ARG(cs->lists_ok != nullptr, "fea_elem_residual_tiles_contact: null lists_ok");
+  const MatParams mp = mat_from(params_host);
+  const ContactDev cd = contact_dev(*cs);
+  if (dtype == FEA_F64)
+    launch_r_tiles_pen<double>(elem_type, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial, cd,
+                               as_stream(stream));
+  else
+    launch_r_tiles_pen<float>(elem_type, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial, cd,
+                              as_stream(stream));
+  FEA_CHECK_LAUNCH("fea_elem_residual_tiles_contact");
+  return FEA_OK;
 }
 
 extern "C" int fea_elem_energy_blocks(int n_elem) { return n_elem > 0 ? blocks_for(n_elem) : 0; }

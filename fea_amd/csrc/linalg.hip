@@ -1,6 +1,6 @@
 // Assembly gathers, reductions, CSR SpMV, Jacobi-PCG with device-resident
 // scalars, dof-mapped vector updates and the fused explicit update.
-#include "common.h"
+#include "contact_device.h"
 
 namespace fea {
 
@@ -151,6 +151,41 @@ __global__ __launch_bounds__(kBlock) void k_gather_nodal(int n_nodes, const int3
 // staging); P = true: they are partial[list[s]*3 ..] (per-tile partial sums).  Summed in
 // that order in float64.
 template <bool P>
+__device__ __forceinline__ void node_force(int A, const int32_t *__restrict__ seg, const int32_t *__restrict__ list,
+                                           const double *__restrict__ rstage, double f[3]) {
+  f[0] = f[1] = f[2] = 0.0;
+  for (int s = seg[A]; s < seg[A + 1]; ++s) {
+    const int64_t q = P ? (int64_t)list[s] * 3 : (int64_t)s * 3;
+    f[0] += rstage[q + 0];
+    f[1] += rstage[q + 1];
+    f[2] += rstage[q + 2];
+  }
+}
+
+// The central-difference update of node A from its force f: per free DOF r of the node,
+// a = -minv[r] f_j, vh[r] += dt_v a, U[3A+j] += dt_u vh[r]; u gets the node's new U.
+__device__ __forceinline__ void node_cd_update(int A, const double f[3], const int32_t *__restrict__ dof_map,
+                                               const double *__restrict__ minv, double dt_v, double dt_u,
+                                               double *__restrict__ acc_out, double *__restrict__ vh,
+                                               double *__restrict__ U, double u[3]) {
+  #pragma clang fp contract(on)
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int r = dof_map[3 * (int64_t)A + j];
+    if (r < 0) {
+      u[j] = U[3 * (int64_t)A + j];
+      continue;
+    }
+    const double a = -minv[r] * f[j];
+    acc_out[r] = a;
+    const double v = vh[r] + dt_v * a;
+    vh[r] = v;
+    u[j] = U[3 * (int64_t)A + j] + dt_u * v;
+    U[3 * (int64_t)A + j] = u[j];
+  }
+}
+
+template <bool P>
 __global__ __launch_bounds__(kBlock) void k_explicit_update(int n_nodes, const int32_t *__restrict__ seg,
                                                             const int32_t *__restrict__ list,
                                                             const int32_t *__restrict__ dof_map,
@@ -163,28 +198,56 @@ __global__ __launch_bounds__(kBlock) void k_explicit_update(int n_nodes, const i
                                                             double *__restrict__ U) {
   const int A = blockIdx.x * blockDim.x + threadIdx.x;
   if (A >= n_nodes) return;
-  double f[3] = {0.0, 0.0, 0.0};
-  for (int s = seg[A]; s < seg[A + 1]; ++s) {
-    const int64_t q = P ? (int64_t)list[s] * 3 : (int64_t)s * 3;
-    f[0] += rstage[q + 0];
-    f[1] += rstage[q + 1];
-    f[2] += rstage[q + 2];
-  }
+  double f[3], u[3];
+  node_force<P>(A, seg, list, rstage, f);
   if (fc != nullptr) {
     f[0] -= fc[3 * (int64_t)A + 0];
     f[1] -= fc[3 * (int64_t)A + 1];
     f[2] -= fc[3 * (int64_t)A + 2];
   }
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    const int r = dof_map[3 * (int64_t)A + j];
-    if (r < 0) continue;
-    const double a = -minv[r] * f[j];
-    acc_out[r] = a;
-    const double v = vh[r] + dt_v * a;
-    vh[r] = v;
-    U[3 * (int64_t)A + j] += dt_u * v;
+  node_cd_update(A, f, dof_map, minv, dt_v, dt_u, acc_out, vh, U, u);
+}
+
+// The explicit loop's fused second launch.  Workgroups [0, node_blocks): one thread per node
+// not marked in cmask (bit A % 32 of word A / 32), its partials summed and the update made
+// (a node without contact force: the same bits as subtracting fc's zero).  The workgroups
+// after them: one wave per contact node i, its contact force (reaction_node, with the list
+// validity the first launch stored) written to fc, then on lane 0 the node's update with
+// f - fc (the order of k_explicit_update) and the tracking of its displacement since the
+// rebuild into dmax, which the next step's penalty reads.
+__global__ __launch_bounds__(kBlock) void k_explicit_update_contact(
+    int n_nodes, int node_blocks, const int32_t *__restrict__ seg, const int32_t *__restrict__ list,
+    const int32_t *__restrict__ dof_map, const double *__restrict__ partial, const double *__restrict__ minv,
+    const uint32_t *__restrict__ cmask, const double *__restrict__ X, double *__restrict__ fc, double dt_v,
+    double dt_u, double *__restrict__ acc_out, double *__restrict__ vh, double *__restrict__ U, ContactDev cd) {
+  double f[3], u[3];
+  if ((int)blockIdx.x < node_blocks) {
+    const int A = blockIdx.x * kBlock + threadIdx.x;
+    if (A >= n_nodes || ((cmask[A >> 5] >> (A & 31)) & 1u)) return;
+    node_force<true>(A, seg, list, partial, f);
+    node_cd_update(A, f, dof_map, minv, dt_v, dt_u, acc_out, vh, U, u);
+    return;
   }
+  const int lane = threadIdx.x & 63;
+  const int i = ((int)blockIdx.x - node_blocks) * (kBlock / 64) + (threadIdx.x >> 6);  // uniform over the wave
+  if (i >= cd.n_cnode) return;
+  double r[3];
+  reaction_node(cd, i, lane, *cd.lists_ok != 0, r);
+  if (lane != 0) return;
+  const int A = cd.cnode[i];
+  fc[3 * (int64_t)A + 0] = r[0];
+  fc[3 * (int64_t)A + 1] = r[1];
+  fc[3 * (int64_t)A + 2] = r[2];
+  node_force<true>(A, seg, list, partial, f);
+  f[0] -= r[0];
+  f[1] -= r[1];
+  f[2] -= r[2];
+  node_cd_update(A, f, dof_map, minv, dt_v, dt_u, acc_out, vh, U, u);
+  double p[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) p[j] = dadd(X[3 * (int64_t)A + j], u[j]);
+  const double d2 = track_d2(p, cd.x0, i);
+  dmax_raise(cd.dmax, i, d2);
 }
 
 // y(float64) += alpha * x(T)
@@ -344,6 +407,26 @@ extern "C" int fea_explicit_update_tiles(int n_nodes, const int32_t *node_pptr, 
   k_explicit_update<true><<<blocks_for(n_nodes), kBlock, 0, as_stream(stream)>>>(
       n_nodes, node_pptr, node_plist, dof_map, partial, minv, fc, dt_v, dt_u, a, vh, U);
   FEA_CHECK_LAUNCH("fea_explicit_update_tiles");
+  return FEA_OK;
+}
+
+extern "C" int fea_explicit_update_tiles_contact(int n_nodes, const int32_t *node_pptr, const int32_t *node_plist,
+                                                 const int32_t *dof_map, const double *partial, const double *minv,
+                                                 const uint32_t *cmask, const fea_contact_set *cs, const double *X,
+                                                 double *fc, double dt_v, double dt_u, double *a, double *vh,
+                                                 double *U, void *stream) {
+  FEA_CHECK_ARG(n_nodes > 0 && node_pptr && node_plist && dof_map && partial && minv && cmask && X && fc && a && vh &&
+                    U,
+                "fea_explicit_update_tiles_contact: bad arguments");
+  const int rc = contact_check_set(cs, "fea_explicit_update_tiles_contact");
+  if (rc != FEA_OK) return rc;
+  FEA_CHECK_ARG(cs->lists_ok != nullptr, "fea_explicit_update_tiles_contact: null lists_ok");
+  const int node_blocks = blocks_for(n_nodes);
+  const int grid = node_blocks + (cs->n_cnode + kBlock / 64 - 1) / (kBlock / 64);
+  k_explicit_update_contact<<<grid, kBlock, 0, as_stream(stream)>>>(n_nodes, node_blocks, node_pptr, node_plist,
+                                                                    dof_map, partial, minv, cmask, X, fc, dt_v, dt_u,
+                                                                    a, vh, U, contact_dev(*cs));
+  FEA_CHECK_LAUNCH("fea_explicit_update_tiles_contact");
   return FEA_OK;
 }
 

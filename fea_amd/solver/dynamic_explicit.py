@@ -25,7 +25,9 @@ by default.
 """
 from __future__ import annotations
 
+import ctypes
 import time
+from types import SimpleNamespace
 
 import numpy as np
 import torch
@@ -40,10 +42,14 @@ class DynamicExplicitSolver(BaseSolver):
     def __init__(self, time_end: float = 1.0, time_per_storage: float = 1e-4, *, dt: float | None = None,
                  cfl_factor: float | None = None, verbose: bool = False, store: bool = True,
                  energy_interval: int | None = None, precision: str = 'fp64', overlap: bool = False,
-                 use_graph: bool = False, chunk: int = 8) -> None:
+                 use_graph: bool = False, chunk: int = 8, fuse_contact: bool = True) -> None:
         super().__init__()
         self.overlap = bool(overlap)
         self.use_graph = bool(use_graph)
+        # with tiles and one contact set of reusable lists, a step is two launches (element
+        # pass + contact penalty; node update + contact-node forces + tracking): the same bits
+        # as the five launches of the separate path
+        self.fuse_contact = bool(fuse_contact)
         # element tiles read their nodes from LDS: bit-identical, measured slower on the
         # two-block benchmark (15.8 vs 14.7 us per block), so off by default
         self.stage_nodes = False
@@ -101,6 +107,17 @@ class DynamicExplicitSolver(BaseSolver):
         X = p.X.reshape(-1, 3).cpu().numpy()
         free = (p.dof_map.reshape(-1, 3).cpu().numpy() >= 0).any(axis=1)
         L = nat.lib()
+        # groups of one element type and material are tiled, and launched, as one: a launch of
+        # twice the workgroups overlaps one half's node loads with the other's arithmetic
+        merged = {}
+        for g in groups:
+            key = (g.ntype, np.asarray(g.params, dtype=np.float64).tobytes())
+            if key in merged:
+                m = merged[key]
+                m['conn_np'] = np.concatenate([m['conn_np'], g.conn_np])
+            else:
+                merged[key] = {'ntype': g.ntype, 'params': g.params, 'conn_np': g.conn_np}
+        groups = [SimpleNamespace(n=int(m['conn_np'].shape[0]), **m) for m in merged.values()]
         tabs, base, qoff = [], 0, 0
         for g in groups:
             t = group_tiles(g.conn_np, X, free, L.fea_elem_tile_size(g.ntype), base)
@@ -284,11 +301,53 @@ class DynamicExplicitSolver(BaseSolver):
         step by step."""
         self._step_kernels(U, vh, acc, dt, first, load_scale, host_logic=True)
 
+    def _fused_contact(self):
+        """The contact state of the fused two-launch step, or None (no tiles, not exactly one
+        contact set of reusable lists, a side stream for contact, or fuse_contact off)."""
+        if not (self.fuse_contact and self._tiles is not None and len(self._contacts) == 1 and not self.overlap
+                and self._contacts[0].search_interval == 1):
+            return None
+        return self._contacts[0].state(self.assembly.plan)
+
+    def _step_fused(self, st, U, vh, acc, dt, first, host_logic):
+        """One step in two launches: fea_elem_residual_tiles_contact (the first element launch
+        carries the contact penalty) and fea_explicit_update_tiles_contact.  dmax must hold the
+        displacement at U: tracked by the previous step's update, by a rebuild, or here on the
+        first step."""
+        p = self.assembly.plan
+        s = nat.stream_ptr()
+        tl = self._tiles
+        if host_logic:
+            st.step_begin(U)
+        cs = ctypes.addressof(st.contact_set())
+        if first:
+            nat.check_rc(nat.raw('fea_contact_track')(cs, p.X.data_ptr(), U.data_ptr(), s), 'fea_contact_track')
+        u = U.data_ptr()
+        for k, a in enumerate(tl['elem_args']):
+            if k == 0:
+                nat.check_rc(nat.raw('fea_elem_residual_tiles_contact')(a[0], a[1], a[2], a[3], a[4], u, a[5], a[6],
+                                                                        a[7], a[8], a[9], cs, s),
+                             'fea_elem_residual_tiles_contact')
+            else:
+                nat.check_rc(nat.raw('fea_elem_residual_tiles')(a[0], a[1], a[2], a[3], a[4], u, a[5], a[6], a[7],
+                                                                a[8], a[9], None, None, None, 0, s),
+                             'fea_elem_residual_tiles')
+        nat.check_rc(nat.raw('fea_explicit_update_tiles_contact')(
+            p.n_nodes, tl['node_pptr'].data_ptr(), tl['node_plist'].data_ptr(), p.dof_map.data_ptr(),
+            tl['partial'].data_ptr(), self._minv.data_ptr(), st.node_mask().data_ptr(), cs, p.X.data_ptr(),
+            st.fc.data_ptr(), 0.5 * dt if first else dt, dt, acc.data_ptr(), vh.data_ptr(), u, s),
+            'fea_explicit_update_tiles_contact')
+        if host_logic:
+            st.step_end()
+
     def _step_kernels(self, U, vh, acc, dt, first=False, load_scale=1.0, host_logic=False):
         """The launches of one step.  With ``overlap`` the contact forces are formed on a side
         stream while the element pass runs (both only read U); the update waits for both.
         host_logic=False: the contact forces come from the stored lists with no host reads
         or rebuilds (the form captured in a HIP graph)."""
+        st = self._fused_contact()
+        if st is not None:
+            return self._step_fused(st, U, vh, acc, dt, first, host_logic)
         p = self.assembly.plan
         s = nat.stream_ptr()
         tl = self._tiles
@@ -331,6 +390,9 @@ class DynamicExplicitSolver(BaseSolver):
         GPU keeps the queued chunk), rebuilds the lists when due and issues the next poll.
         The graph replays exactly the launches of ``step``: same results bit for bit."""
         n = int(n)
+        fst = self._fused_contact()
+        if fst is not None and not first and n > 0:
+            fst.track(U)                                # U may come from elsewhere than a step
         if first and n > 0:
             self.step(U, vh, acc, dt, first=True)
             n -= 1

@@ -543,6 +543,10 @@ int fea_contact_forces(int n_slave, const int32_t *slave, const double *X, const
  * dmax approaches (skin/2)^2; a late rebuild costs time, not results.
  *   cnode[n_cnode]: every slave and master node once; cn_slave[i] / cn_master[i]:
  *   its slave index / master index (into nf_ptr) or -1. */
+/* dmax is kept in 64 slots a cache line apart (contact node i raises slot i % 64), so that
+ * the per-step tracking's atomics do not all meet at one address. */
+#define FEA_CONTACT_DMAX_SLOTS 64
+#define FEA_CONTACT_DMAX_STRIDE 16
 typedef struct fea_contact_set {
   int32_t n_slave, n_facets, n_cnode, kmax, table_bits, n_master;
   double cell, radius, skin, penalty, depth_max;
@@ -550,16 +554,46 @@ typedef struct fea_contact_set {
   const int32_t *slave, *conn, *cnode, *cn_slave, *cn_master, *nf_ptr, *nf_facet, *nf_vert;
   const double *slave_area;                  /* may be NULL (area 1) */
   double *centroid, *force, *weight, *x0;    /* [3F], [3S], [3S], [3*n_cnode] */
-  uint64_t *dmax;                            /* [1] */
+  uint64_t *dmax;  /* [FEA_CONTACT_DMAX_SLOTS * FEA_CONTACT_DMAX_STRIDE]: slot k at
+                      dmax[k * FEA_CONTACT_DMAX_STRIDE]; the tracked value is the slots' max */
   int32_t *key, *bucket_start, *bucket_items, *work;   /* [F], [2^bits + 1], [F], [max(2^bits, F)] */
   int32_t *cand, *count, *max_count, *chosen;          /* [S*kmax], [S], [1] (may be NULL), [S] */
   int32_t *sup_max, *inv_start, *inv_items;            /* [1], [F + 1], [S*kmax] */
   int32_t *m_ptr, *m_items;                            /* [n_master + 1], [6*S*kmax] */
+  int32_t *lists_ok;  /* [1]: validity of the stored lists at the last penalty launch (may be
+                         NULL for fea_contact_eval; the explicit loop's fused launches need it) */
 } fea_contact_set;
 int fea_contact_set_size(void);
 int fea_contact_rebuild(const fea_contact_set *set_host, const double *X, const double *U, void *stream);
 int fea_contact_eval(const fea_contact_set *set_host, const double *X, const double *U, double *fc,
                      double *energy_partial, void *stream);
+/* dmax = max(dmax, the largest squared displacement of a contact node from x0 at U): the
+ * first launch of fea_contact_eval, alone (before the explicit loop's fused launches at a U
+ * they did not produce). */
+int fea_contact_track(const fea_contact_set *set_host, const double *X, const double *U, void *stream);
+
+/* The explicit loop's step in two launches (no counterpart in the reference; they replace
+ * fea_elem_residual_tiles + fea_contact_eval + fea_explicit_update_tiles of one step, with
+ * the same results bit for bit).
+ * fea_elem_residual_tiles_contact: fea_elem_residual_tiles (no node staging) and, in further
+ * workgroups of the same launch, the penalty of fea_contact_eval at U on the stored lists
+ * (or every facet when dmax > (skin/2)^2 or a list overflowed), which stores the lists'
+ * validity in lists_ok.  dmax must hold the displacement at U (fea_contact_rebuild or
+ * fea_contact_track at U, or the previous fea_explicit_update_tiles_contact).
+ * fea_explicit_update_tiles_contact: fea_explicit_update_tiles with the contact force of
+ * fea_contact_eval (the master reactions and slave force of each contact node, from the
+ * penalty launch's choice and lists_ok), written to fc at the contact nodes; the nodes with
+ * bit A % 32 of cmask[A / 32] set (exactly the set's contact nodes cnode) are updated by the
+ * contact workgroups, which then raise dmax by each contact node's displacement at the new U. */
+int fea_elem_residual_tiles_contact(int elem_type, int dtype, int n_elem, const int32_t *conn,
+                                    const double *X, const double *U, const double *params_host,
+                                    const int32_t *tile_lbase, const int32_t *pslot_ptr, const uint16_t *pslot,
+                                    double *partial, const fea_contact_set *set_host, void *stream);
+int fea_explicit_update_tiles_contact(int n_nodes, const int32_t *node_pptr, const int32_t *node_plist,
+                                      const int32_t *dof_map, const double *partial, const double *minv,
+                                      const uint32_t *cmask, const fea_contact_set *set_host, const double *X,
+                                      double *fc, double dt_v, double dt_u, double *a, double *vh, double *U,
+                                      void *stream);
 
 /* ---------------------------------------------------------------- adjoint --
  * Vector-Jacobian products for sensitivities.  The reference differentiates by

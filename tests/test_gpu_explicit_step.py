@@ -43,7 +43,8 @@ def test_reused_lists_equal_exact_search(gpu, n, seed, move):
     assert int(st.sup_max.item()) <= load.kmax
     fc = st.step_forces(U)
     torch.cuda.synchronize()
-    dmax = float(st.dmax.cpu().numpy().view(np.float64)[0])
+    from fea_amd.assemble.loads.contact import slots_max
+    dmax = slots_max(st.dmax.cpu().numpy())
     assert abs(np.sqrt(dmax) - move * load.skin) < 1e-9
     assert torch.equal(st.chosen, chosen_ref)
     assert torch.equal(st.force, f_ref) and torch.equal(st.weight, w_ref)
@@ -173,3 +174,37 @@ def test_staged_tile_nodes_bit_identical(gpu, precision):
         out.append((U.clone(), vh.clone(), acc.clone()))
     for a, b in zip(out[0], out[1]):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize('precision,graph', [('fp32', False), ('fp32', True), ('fp64', False)])
+def test_fused_contact_step_bit_identical(gpu, precision, graph):
+    """The two-launch step (element pass + contact penalty; node update + contact-node forces
+    + displacement tracking) against the separate launches (element pass, fea_contact_eval,
+    update): the same trajectory and contact forces, bit for bit, over 150 steps with contact
+    active and the lists rebuilt."""
+    print('explicit fused contact step, %s graph=%s' % (precision, graph), flush=True)
+    out = []
+    for fuse in (True, False):
+        fe, info = workloads.two_blocks_contact(4, velocity=1.0, gap=0.01, jitter=0.1, seed=5)
+        fe.solver = solver.DynamicExplicitSolver(time_end=1.0, time_per_storage=1.0, dt=1.5e-3, store=False,
+                                                 precision=precision, use_graph=graph, fuse_contact=fuse)
+        fe.initialize()
+        sol = fe.solver
+        p = fe.assembly.plan
+        st = fe.assembly.get_load('contact').state(p)
+        assert (sol._fused_contact() is not None) == fuse
+        U = p.gc_to_u(torch.zeros_like(fe.assembly.GC))
+        vh = workloads.initial_velocity(fe, info).clone()
+        acc = torch.zeros_like(vh)
+        active = 0
+        sol.run_steps(U, vh, acc, sol._deltaT, 30, first=True)
+        for _ in range(6):
+            sol.run_steps(U, vh, acc, sol._deltaT, 20)
+            torch.cuda.synchronize()
+            active = max(active, int((st.chosen >= 0).sum()))
+        sol._check_contacts()
+        out.append((U.clone(), vh.clone(), acc.clone(), st.fc.clone(), active, st.rebuilds))
+    (U1, v1, a1, f1, n1, r1), (U0, v0, a0, f0, n0, r0) = out
+    assert n1 > 0 and r1 >= 2 and r0 >= 2
+    assert torch.equal(U1, U0) and torch.equal(v1, v0) and torch.equal(a1, a0)
+    assert torch.equal(f1, f0)
