@@ -162,27 +162,57 @@ __device__ __forceinline__ void node_force(int A, const int32_t *__restrict__ se
   }
 }
 
+// A node's update inputs: per DOF j its free-DOF row r[j] (-1: fixed), minv, vh and U.
+struct NodeIn {
+  int r[3];
+  double mi[3], v[3], u[3];
+};
+
+// NODE_MINV: minv holds one value per node (the node's free DOFs share their lumped mass),
+// otherwise one per free DOF.
+template <bool NODE_MINV = false>
+__device__ __forceinline__ void node_load(int A, const int32_t *__restrict__ dof_map, const double *__restrict__ minv,
+                                          const double *__restrict__ vh, const double *__restrict__ U, NodeIn &in) {
+  const double mn = NODE_MINV ? minv[A] : 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int r = dof_map[3 * (int64_t)A + j];
+    in.r[j] = r;
+    in.mi[j] = NODE_MINV ? mn : (r >= 0 ? minv[r] : 0.0);
+    in.v[j] = r >= 0 ? vh[r] : 0.0;
+    in.u[j] = U[3 * (int64_t)A + j];
+  }
+}
+
 // The central-difference update of node A from its force f: per free DOF r of the node,
 // a = -minv[r] f_j, vh[r] += dt_v a, U[3A+j] += dt_u vh[r]; u gets the node's new U.
+__device__ __forceinline__ void node_step(int A, const double f[3], const NodeIn &in, double dt_v, double dt_u,
+                                          double *__restrict__ acc_out, double *__restrict__ vh,
+                                          double *__restrict__ U, double u[3]) {
+  #pragma clang fp contract(on)
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int r = in.r[j];
+    if (r < 0) {
+      u[j] = in.u[j];
+      continue;
+    }
+    const double a = -in.mi[j] * f[j];
+    if (acc_out != nullptr) acc_out[r] = a;
+    const double v = in.v[j] + dt_v * a;
+    vh[r] = v;
+    u[j] = in.u[j] + dt_u * v;
+    U[3 * (int64_t)A + j] = u[j];
+  }
+}
+
 __device__ __forceinline__ void node_cd_update(int A, const double f[3], const int32_t *__restrict__ dof_map,
                                                const double *__restrict__ minv, double dt_v, double dt_u,
                                                double *__restrict__ acc_out, double *__restrict__ vh,
                                                double *__restrict__ U, double u[3]) {
-  #pragma clang fp contract(on)
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    const int r = dof_map[3 * (int64_t)A + j];
-    if (r < 0) {
-      u[j] = U[3 * (int64_t)A + j];
-      continue;
-    }
-    const double a = -minv[r] * f[j];
-    acc_out[r] = a;
-    const double v = vh[r] + dt_v * a;
-    vh[r] = v;
-    u[j] = U[3 * (int64_t)A + j] + dt_u * v;
-    U[3 * (int64_t)A + j] = u[j];
-  }
+  NodeIn in;
+  node_load(A, dof_map, minv, vh, U, in);
+  node_step(A, f, in, dt_v, dt_u, acc_out, vh, U, u);
 }
 
 template <bool P>
@@ -208,7 +238,8 @@ __global__ __launch_bounds__(kBlock) void k_explicit_update(int n_nodes, const i
   node_cd_update(A, f, dof_map, minv, dt_v, dt_u, acc_out, vh, U, u);
 }
 
-// The explicit loop's fused second launch.  Workgroups [0, node_blocks): one thread per node
+// The explicit loop's fused second launch (minv per node; acc_out may be null: the
+// accelerations are then not stored).  Workgroups [0, node_blocks): one thread per node
 // not marked in cmask (bit A % 32 of word A / 32), its partials summed and the update made
 // (a node without contact force: the same bits as subtracting fc's zero).  The workgroups
 // after them: one wave per contact node i, its contact force (reaction_node, with the list
@@ -224,25 +255,31 @@ __global__ __launch_bounds__(kBlock) void k_explicit_update_contact(
   if ((int)blockIdx.x < node_blocks) {
     const int A = blockIdx.x * kBlock + threadIdx.x;
     if (A >= n_nodes || ((cmask[A >> 5] >> (A & 31)) & 1u)) return;
+    NodeIn in;
     node_force<true>(A, seg, list, partial, f);
-    node_cd_update(A, f, dof_map, minv, dt_v, dt_u, acc_out, vh, U, u);
+    node_load<true>(A, dof_map, minv, vh, U, in);
+    node_step(A, f, in, dt_v, dt_u, acc_out, vh, U, u);
     return;
   }
   const int lane = threadIdx.x & 63;
   const int i = ((int)blockIdx.x - node_blocks) * (kBlock / 64) + (threadIdx.x >> 6);  // uniform over the wave
   if (i >= cd.n_cnode) return;
+  // the node's own force and update inputs do not depend on its contact force: loaded first,
+  // their latency overlaps the reaction's
+  const int A = cd.cnode[i];
+  NodeIn in;
+  node_force<true>(A, seg, list, partial, f);
+  node_load<true>(A, dof_map, minv, vh, U, in);
   double r[3];
   reaction_node(cd, i, lane, *cd.lists_ok != 0, r);
   if (lane != 0) return;
-  const int A = cd.cnode[i];
   fc[3 * (int64_t)A + 0] = r[0];
   fc[3 * (int64_t)A + 1] = r[1];
   fc[3 * (int64_t)A + 2] = r[2];
-  node_force<true>(A, seg, list, partial, f);
   f[0] -= r[0];
   f[1] -= r[1];
   f[2] -= r[2];
-  node_cd_update(A, f, dof_map, minv, dt_v, dt_u, acc_out, vh, U, u);
+  node_step(A, f, in, dt_v, dt_u, acc_out, vh, U, u);
   double p[3];
 #pragma unroll
   for (int j = 0; j < 3; ++j) p[j] = dadd(X[3 * (int64_t)A + j], u[j]);
@@ -411,11 +448,11 @@ extern "C" int fea_explicit_update_tiles(int n_nodes, const int32_t *node_pptr, 
 }
 
 extern "C" int fea_explicit_update_tiles_contact(int n_nodes, const int32_t *node_pptr, const int32_t *node_plist,
-                                                 const int32_t *dof_map, const double *partial, const double *minv,
+                                                 const int32_t *dof_map, const double *partial, const double *minv_node,
                                                  const uint32_t *cmask, const fea_contact_set *cs, const double *X,
                                                  double *fc, double dt_v, double dt_u, double *a, double *vh,
                                                  double *U, void *stream) {
-  FEA_CHECK_ARG(n_nodes > 0 && node_pptr && node_plist && dof_map && partial && minv && cmask && X && fc && a && vh &&
+  FEA_CHECK_ARG(n_nodes > 0 && node_pptr && node_plist && dof_map && partial && minv_node && cmask && X && fc && vh &&
                     U,
                 "fea_explicit_update_tiles_contact: bad arguments");
   const int rc = contact_check_set(cs, "fea_explicit_update_tiles_contact");
@@ -424,7 +461,7 @@ extern "C" int fea_explicit_update_tiles_contact(int n_nodes, const int32_t *nod
   const int node_blocks = blocks_for(n_nodes);
   const int grid = node_blocks + (cs->n_cnode + kBlock / 64 - 1) / (kBlock / 64);
   k_explicit_update_contact<<<grid, kBlock, 0, as_stream(stream)>>>(n_nodes, node_blocks, node_pptr, node_plist,
-                                                                    dof_map, partial, minv, cmask, X, fc, dt_v, dt_u,
+                                                                    dof_map, partial, minv_node, cmask, X, fc, dt_v, dt_u,
                                                                     a, vh, U, contact_dev(*cs));
   FEA_CHECK_LAUNCH("fea_explicit_update_tiles_contact");
   return FEA_OK;

@@ -86,6 +86,7 @@ class DynamicExplicitSolver(BaseSolver):
         m = p.lumped_mass()
         m = torch.where(m.abs() < 1e-12, torch.ones_like(m), m)     # dynamic_explicit.py:165
         self._minv = 1.0 / m
+        self._minv_node = self._node_minv()
         self._setup_tiles()
         self._deltaT = self.estimate_critical_timestep()
         if self._deltaT <= 0:
@@ -296,20 +297,33 @@ class DynamicExplicitSolver(BaseSolver):
         for c in self._contacts:
             c.state(self.assembly.plan).check()
 
-    def step(self, U, vh, acc, dt, first=False, load_scale=1.0):
+    def step(self, U, vh, acc, dt, first=False, load_scale=1.0, store_acc=True):
         """One central-difference step at the current U (in place), contact lists managed
-        step by step."""
-        self._step_kernels(U, vh, acc, dt, first, load_scale, host_logic=True)
+        step by step.  store_acc=False: the fused step may leave acc unwritten (a step reads
+        no acceleration; run_steps stores it on its last step)."""
+        self._step_kernels(U, vh, acc, dt, first, load_scale, host_logic=True, store_acc=store_acc)
+
+    def _node_minv(self):
+        """1/m per node (the value of its free DOFs, which share the row-sum lumped mass; 0 for
+        a node without free DOFs), or None if some node's free DOFs differ."""
+        p = self.assembly.plan
+        dm = p.dof_map.reshape(-1, 3).long()
+        mi = torch.where(dm >= 0, self._minv[dm.clamp(min=0)], torch.full_like(dm, float('nan'), dtype=torch.float64))
+        first = torch.nan_to_num(mi, nan=0.0).amax(dim=1)          # lumped masses are positive
+        free = dm >= 0
+        if bool(((mi != first[:, None]) & free).any()):
+            return None
+        return first.contiguous()
 
     def _fused_contact(self):
         """The contact state of the fused two-launch step, or None (no tiles, not exactly one
         contact set of reusable lists, a side stream for contact, or fuse_contact off)."""
         if not (self.fuse_contact and self._tiles is not None and len(self._contacts) == 1 and not self.overlap
-                and self._contacts[0].search_interval == 1):
+                and self._contacts[0].search_interval == 1 and self._minv_node is not None):
             return None
         return self._contacts[0].state(self.assembly.plan)
 
-    def _step_fused(self, st, U, vh, acc, dt, first, host_logic):
+    def _step_fused(self, st, U, vh, acc, dt, first, host_logic, store_acc=True):
         """One step in two launches: fea_elem_residual_tiles_contact (the first element launch
         carries the contact penalty) and fea_explicit_update_tiles_contact.  dmax must hold the
         displacement at U: tracked by the previous step's update, by a rebuild, or here on the
@@ -334,20 +348,21 @@ class DynamicExplicitSolver(BaseSolver):
                              'fea_elem_residual_tiles')
         nat.check_rc(nat.raw('fea_explicit_update_tiles_contact')(
             p.n_nodes, tl['node_pptr'].data_ptr(), tl['node_plist'].data_ptr(), p.dof_map.data_ptr(),
-            tl['partial'].data_ptr(), self._minv.data_ptr(), st.node_mask().data_ptr(), cs, p.X.data_ptr(),
-            st.fc.data_ptr(), 0.5 * dt if first else dt, dt, acc.data_ptr(), vh.data_ptr(), u, s),
+            tl['partial'].data_ptr(), self._minv_node.data_ptr(), st.node_mask().data_ptr(), cs, p.X.data_ptr(),
+            st.fc.data_ptr(), 0.5 * dt if first else dt, dt, acc.data_ptr() if store_acc else None, vh.data_ptr(), u,
+            s),
             'fea_explicit_update_tiles_contact')
         if host_logic:
             st.step_end()
 
-    def _step_kernels(self, U, vh, acc, dt, first=False, load_scale=1.0, host_logic=False):
+    def _step_kernels(self, U, vh, acc, dt, first=False, load_scale=1.0, host_logic=False, store_acc=True):
         """The launches of one step.  With ``overlap`` the contact forces are formed on a side
         stream while the element pass runs (both only read U); the update waits for both.
         host_logic=False: the contact forces come from the stored lists with no host reads
         or rebuilds (the form captured in a HIP graph)."""
         st = self._fused_contact()
         if st is not None:
-            return self._step_fused(st, U, vh, acc, dt, first, host_logic)
+            return self._step_fused(st, U, vh, acc, dt, first, host_logic, store_acc)
         p = self.assembly.plan
         s = nat.stream_ptr()
         tl = self._tiles
@@ -394,12 +409,12 @@ class DynamicExplicitSolver(BaseSolver):
         if fst is not None and not first and n > 0:
             fst.track(U)                                # U may come from elsewhere than a step
         if first and n > 0:
-            self.step(U, vh, acc, dt, first=True)
+            self.step(U, vh, acc, dt, first=True, store_acc=(n == 1))
             n -= 1
         usable = self._tiles is not None and all(c.search_interval == 1 for c in self._contacts)
         if not (self.use_graph and usable) or n < 2 * self.chunk:
-            for _ in range(n):
-                self.step(U, vh, acc, dt)
+            for k in range(n):
+                self.step(U, vh, acc, dt, store_acc=(k == n - 1))
             return
         K = self.chunk
         key = (U.data_ptr(), vh.data_ptr(), acc.data_ptr(), float(dt), K)
@@ -414,8 +429,8 @@ class DynamicExplicitSolver(BaseSolver):
             g = torch.cuda.CUDAGraph()
             torch.cuda.synchronize()
             with torch.cuda.graph(g):
-                for _ in range(K):
-                    self._step_kernels(U, vh, acc, dt)
+                for k in range(K):
+                    self._step_kernels(U, vh, acc, dt, store_acc=(k == K - 1))
             torch.cuda.synchronize()
             self._graph, self._graph_key = g, key
         states = [c.state(self.assembly.plan) for c in self._contacts]
@@ -437,5 +452,5 @@ class DynamicExplicitSolver(BaseSolver):
             n -= K
         for st in states:
             st._poll = None
-        for _ in range(n):
-            self.step(U, vh, acc, dt)
+        for k in range(n):
+            self.step(U, vh, acc, dt, store_acc=(k == n - 1))

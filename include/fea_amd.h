@@ -584,13 +584,15 @@ int fea_contact_track(const fea_contact_set *set_host, const double *X, const do
  * fea_contact_eval (the master reactions and slave force of each contact node, from the
  * penalty launch's choice and lists_ok), written to fc at the contact nodes; the nodes with
  * bit A % 32 of cmask[A / 32] set (exactly the set's contact nodes cnode) are updated by the
- * contact workgroups, which then raise dmax by each contact node's displacement at the new U. */
+ * contact workgroups, which then raise dmax by each contact node's displacement at the new U.
+ * minv_node[A] is 1/m of node A (its free DOFs share the lumped mass); a may be NULL (the
+ * accelerations are not stored). */
 int fea_elem_residual_tiles_contact(int elem_type, int dtype, int n_elem, const int32_t *conn,
                                     const double *X, const double *U, const double *params_host,
                                     const int32_t *tile_lbase, const int32_t *pslot_ptr, const uint16_t *pslot,
                                     double *partial, const fea_contact_set *set_host, void *stream);
 int fea_explicit_update_tiles_contact(int n_nodes, const int32_t *node_pptr, const int32_t *node_plist,
-                                      const int32_t *dof_map, const double *partial, const double *minv,
+                                      const int32_t *dof_map, const double *partial, const double *minv_node,
                                       const uint32_t *cmask, const fea_contact_set *set_host, const double *X,
                                       double *fc, double dt_v, double dt_u, double *a, double *vh, double *U,
                                       void *stream);
