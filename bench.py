@@ -217,11 +217,12 @@ def spmv_roofline(fe, torch, reps, mat_dtype):
     return t, spmv_bytes(p, 4 if mat_dtype == F32 else 8)
 
 
-def smooth_roofline(fe, torch, reps, mat_dtype):
-    """The multigrid V-cycle's dominant kernel: one level-0 Chebyshev step (fea_amg_smooth0:
-    r = b - A x with the tangent, node-block Jacobi, d and x updates) timed with HIP events
-    over back-to-back launches (rocprof gives these dispatches the same duration; inside the
-    solve, between the V-cycle's other kernels, the same kernel runs about 15 % faster).
+def smooth_roofline(fe, torch, reps, mat_dtype, half=False):
+    """The multigrid V-cycle's dominant kernel: one level-0 Chebyshev step (r = b - A x with
+    the tangent, fea_amg_smooth0, as the solver's V-cycle runs it, or with its binary16 copy,
+    fea_amg_smooth0_half, the AMG option half_level0; node-block Jacobi, d and x updates) timed with
+    HIP events over back-to-back launches (rocprof gives these dispatches the same duration;
+    inside the solve, between the V-cycle's other kernels, the same kernel runs faster).
     Algorithmic bytes: the SpMV's (values, column stream, x read, group heads) plus b and d
     read, d and the new x written (float64) and the 6-float block inverse per group."""
     from fea_amd import _native as nat
@@ -234,9 +235,19 @@ def smooth_roofline(fe, torch, reps, mat_dtype):
     b, x, d = (torch.rand(n, dtype=torch.float64, device='cuda') for _ in range(3))
     xo = torch.empty_like(x)
     s = nat.stream_ptr()
-    t = event_time(torch, bound_call('fea_amg_smooth0', mat_dtype, n, p.nnz, p.spmv_groups, p.grp, p.rowptr,
-                                     p.spmv_col, val, binv, b, x, xo, d, 0.5, 0.25, s), reps)
-    alg = spmv_bytes(p, 4 if mat_dtype == F32 else 8) + 24 * n + 24 * p.n_groups
+    half = half and p.spmv_groups < 0
+    if half:
+        v16 = torch.empty(4 * p.nnz // 3, dtype=torch.int16, device='cuda')
+        sc = torch.zeros(2, dtype=torch.float64, device='cuda')
+        call('fea_amg_half_values', mat_dtype, p.nnz, val, v16, sc, s)
+        t = event_time(torch, bound_call('fea_amg_smooth0_half', n, p.nnz, p.spmv_groups, p.grp, p.rowptr,
+                                         p.spmv_col, v16, sc, binv, b, x, xo, d, 0.5, 0.25, s), reps)
+        vb = 2
+    else:
+        t = event_time(torch, bound_call('fea_amg_smooth0', mat_dtype, n, p.nnz, p.spmv_groups, p.grp, p.rowptr,
+                                         p.spmv_col, val, binv, b, x, xo, d, 0.5, 0.25, s), reps)
+        vb = 4 if mat_dtype == F32 else 8
+    alg = spmv_bytes(p, vb) + 24 * n + 24 * p.n_groups
     return t, alg
 
 
@@ -1095,6 +1106,12 @@ def bench_kernels(args, torch):
     for _ in range(20):
         call('fea_amg_smooth0', F32, p.n_free, p.nnz, p.spmv_groups, p.grp, p.rowptr, p.spmv_col, val, binv, x, xb, q,
              xd, 0.5, 0.25, s)
+    v16 = torch.empty(4 * p.nnz // 3, dtype=torch.int16, device='cuda')
+    sc = torch.zeros(2, dtype=torch.float64, device='cuda')
+    call('fea_amg_half_values', F32, p.nnz, val, v16, sc, s)
+    for _ in range(20):
+        call('fea_amg_smooth0_half', p.n_free, p.nnz, p.spmv_groups, p.grp, p.rowptr, p.spmv_col, v16, sc, binv, x,
+             xb, q, xd, 0.5, 0.25, s)
     a = torch.empty(256 << 20, dtype=torch.uint8, device='cuda').fill_(1)
     b = torch.empty_like(a)
     for _ in range(20):

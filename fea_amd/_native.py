@@ -69,6 +69,9 @@ _SIGS = {
                                       C_DBL, C_P, C_INT, C_INT, C_P, C_P, C_P]),
     'fea_amg_smooth0': (C_INT, [C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_DBL, C_DBL,
                                 C_P]),
+    'fea_amg_half_values': (C_INT, [C_INT, C_I64, C_P, C_P, C_P, C_P]),
+    'fea_amg_smooth0_half': (C_INT, [C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_P, C_DBL,
+                                     C_DBL, C_P]),
     'fea_amg_power': (C_INT, [C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P, C_P,
                                      C_P, C_P, C_INT, ctypes.POINTER(C_DBL), C_P]),
     'fea_pcg_start_amg': (C_INT, [C_INT, C_INT, C_I64, C_INT, C_P, C_P, C_P, C_P, C_P,

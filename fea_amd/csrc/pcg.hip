@@ -13,6 +13,7 @@
 // node, cutting index + gather traffic by up to 3x.  grp == NULL selects one
 // row per group (plain CSR).
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -792,12 +793,61 @@ __device__ __forceinline__ void group6_spmv(int g, int lane, const int32_t *__re
   for (int m = 0; m < 6; ++m) o[m] = a[m];
 }
 
+__device__ __forceinline__ double half_lo(uint32_t w) { return (double)(float)__builtin_bit_cast(half_t, (uint16_t)(w & 0xffffu)); }
+__device__ __forceinline__ double half_hi(uint32_t w) { return (double)(float)__builtin_bit_cast(half_t, (uint16_t)(w >> 16)); }
+
+// group_spmv (triple mode) on the quads of k_to_half: row r's triple j of group g is quad
+// rowptr[r0] / 3 + r * nb + j
+template <int SW>
+__device__ __forceinline__ void group_spmv_h4(int g, int lane, const int32_t *__restrict__ grp,
+                                              const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                              const uint2 *__restrict__ q4, const double *__restrict__ x,
+                                              double out[3]) {
+  const int r0 = grp[g];
+  const int s0 = rowptr[r0];
+  const int nb = (rowptr[r0 + 1] - s0) / 3;
+  const uint2 *base = q4 + s0 / 3;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int j = lane; j < nb; j += SW) {
+    const int c = col[s0 / 9 + j];
+    const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
+    const uint2 u = base[j], v = base[nb + j], w = base[2 * nb + j];
+    a0 += half_lo(u.x) * x0 + half_hi(u.x) * x1 + half_lo(u.y) * x2;
+    a1 += half_lo(v.x) * x0 + half_hi(v.x) * x1 + half_lo(v.y) * x2;
+    a2 += half_lo(w.x) * x0 + half_hi(w.x) * x1 + half_lo(w.y) * x2;
+  }
+#pragma unroll
+  for (int off = SW / 2; off > 0; off >>= 1) {
+    a0 += __shfl_xor(a0, off, SW);
+    a1 += __shfl_xor(a1, off, SW);
+    a2 += __shfl_xor(a2, off, SW);
+  }
+  out[0] = a0;
+  out[1] = a1;
+  out[2] = a2;
+}
+
+// the level-0 product of a group: the matrix's own values, or (TM = uint2) its binary16 quads
+template <typename TM, int SW, int MODE>
+__device__ __forceinline__ int level0_spmv(int g, int lane, const int32_t *__restrict__ grp,
+                                           const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                           const TM *__restrict__ val, const double *__restrict__ x, double o[3]) {
+  if constexpr (std::is_same<TM, uint2>::value) {
+    static_assert(MODE == 2, "binary16 quads exist for triple-structured matrices only");
+    group_spmv_h4<SW>(g, lane, grp, rowptr, col, val, x, o);
+    return grp[g];
+  } else {
+    return group_spmv<TM, double, SW, MODE>(g, lane, grp, rowptr, col, val, x, o);
+  }
+}
+
 __device__ __forceinline__ bool amg_done(const double *done) { return done != nullptr && *done != 0.0; }
 
 // one Chebyshev step on level 0: r = b - A x (FIRST: x = 0, no product), d = c1 d + c2 M^-1 r,
 // xo = x + d with M the node-block diagonal (6 packed floats per group).  RZ (the last
 // post-smoothing step of a PCG iteration's V-cycle, xo = z, b = the PCG residual): also the
 // workgroup's partial of b.xo = r.z into ring slot (iteration & 1) of the PCG workspace rz_ws.
+// vscale: for binary16 values, the factor that undoes their scaling (vscale[1]); else null.
 template <typename TM, int SW, int MODE, int FIRST, int RZ = 0>
 __global__ __launch_bounds__(kBlock) void k_amg_cheb0(int ngroups, const int32_t *__restrict__ grp,
                                                       const int32_t *__restrict__ rowptr,
@@ -805,16 +855,23 @@ __global__ __launch_bounds__(kBlock) void k_amg_cheb0(int ngroups, const int32_t
                                                       const float *__restrict__ binv, const double *__restrict__ b,
                                                       const double *__restrict__ x, double *__restrict__ xo,
                                                       double *__restrict__ d, double c1, double c2,
-                                                      const double *__restrict__ done, double *__restrict__ rz_ws = nullptr) {
+                                                      const double *__restrict__ done, double *__restrict__ rz_ws,
+                                                      const double *__restrict__ vscale) {
   if (amg_done(done)) return;
   constexpr int GPB = kBlock / SW;
   const int lane = threadIdx.x % SW;
+  const double vs = vscale != nullptr ? vscale[1] : 1.0;
   double rz = 0.0;
   for (int64_t g0 = (int64_t)blockIdx.x * GPB; g0 < ngroups; g0 += (int64_t)gridDim.x * GPB) {
     const int g = (int)(g0 + threadIdx.x / SW);
     if (g >= ngroups) continue;
     double o[3] = {0.0, 0.0, 0.0};
-    if (!FIRST) group_spmv<TM, double, SW, MODE>(g, lane, grp, rowptr, col, val, x, o);
+    if (!FIRST) {
+      level0_spmv<TM, SW, MODE>(g, lane, grp, rowptr, col, val, x, o);
+      o[0] *= vs;
+      o[1] *= vs;
+      o[2] *= vs;
+    }
     // every lane holds the group's products (butterfly sums): lane a updates row a
     const int r0 = grp[g], nr = grp[g + 1] - r0;
     if (lane < nr) {
@@ -841,16 +898,18 @@ __global__ __launch_bounds__(kBlock) void k_amg_resid0(int ngroups, const int32_
                                                        const int32_t *__restrict__ rowptr,
                                                        const int32_t *__restrict__ col, const TM *__restrict__ val,
                                                        const double *__restrict__ b, const double *__restrict__ x,
-                                                       double *__restrict__ r, const double *__restrict__ done) {
+                                                       double *__restrict__ r, const double *__restrict__ done,
+                                                       const double *__restrict__ vscale) {
   if (amg_done(done)) return;
   constexpr int GPB = kBlock / SW;
   const int lane = threadIdx.x % SW;
+  const double vs = vscale != nullptr ? vscale[1] : 1.0;
   for (int64_t g0 = (int64_t)blockIdx.x * GPB; g0 < ngroups; g0 += (int64_t)gridDim.x * GPB) {
     const int g = (int)(g0 + threadIdx.x / SW);
     if (g >= ngroups) continue;
     double o[3];
-    const int r0 = group_spmv<TM, double, SW, MODE>(g, lane, grp, rowptr, col, val, x, o);
-    if (lane < grp[g + 1] - r0) r[r0 + lane] = b[r0 + lane] - (lane == 0 ? o[0] : (lane == 1 ? o[1] : o[2]));
+    const int r0 = level0_spmv<TM, SW, MODE>(g, lane, grp, rowptr, col, val, x, o);
+    if (lane < grp[g + 1] - r0) r[r0 + lane] = b[r0 + lane] - vs * (lane == 0 ? o[0] : (lane == 1 ? o[1] : o[2]));
   }
 }
 
@@ -957,6 +1016,42 @@ __global__ __launch_bounds__(kBlock) void k_amg_prolong(int n, const int32_t *__
   }
 }
 
+// max |val| as the bits of a non-negative float (they order like the values) into *amax
+template <typename TM>
+__global__ __launch_bounds__(kBlock) void k_absmax(int64_t n, const TM *__restrict__ val, unsigned int *__restrict__ amax) {
+  float m = 0.f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf((float)val[i]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(amax, __float_as_uint(m));
+}
+
+__global__ void k_absmax_reset(unsigned int *amax) { *amax = 0u; }
+
+// val16 = binary16(2^k val), 2^k putting max |val| in [2^14, 2^15); scale[1] = 2^-k.  The
+// values come in triples (a row's entries of one neighbour node, triple-structured
+// matrices); each triple t is stored as the quad val16[4t .. 4t+3] (the fourth zero), so a
+// lane reads a row's triple with one 8-byte load.
+template <typename TM>
+__global__ __launch_bounds__(kBlock) void k_to_half(int64_t ntrip, const TM *__restrict__ val,
+                                                    const unsigned int *__restrict__ amax, uint2 *__restrict__ out,
+                                                    double *__restrict__ scale) {
+  const float m = __uint_as_float(*amax);
+  int e = 0;
+  if (m > 0.f) frexpf(m, &e);  // m = f 2^e, f in [1/2, 1)
+  const int k = m > 0.f ? 15 - e : 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) scale[1] = ldexp(1.0, -k);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ntrip; t += (int64_t)gridDim.x * blockDim.x) {
+    const half_t h0 = (half_t)ldexp((double)val[3 * t], k), h1 = (half_t)ldexp((double)val[3 * t + 1], k),
+                 h2 = (half_t)ldexp((double)val[3 * t + 2], k);
+    uint2 q;
+    q.x = (uint32_t)__builtin_bit_cast(uint16_t, h0) | ((uint32_t)__builtin_bit_cast(uint16_t, h1) << 16);
+    q.y = (uint32_t)__builtin_bit_cast(uint16_t, h2);
+    out[t] = q;
+  }
+}
+
 // x = inv b on the coarsest level: one wave per row.  Rows of a multiple of 4 entries are read
 // as float4 (a lane takes 4 consecutive entries per step) with two accumulators.
 __global__ __launch_bounds__(kBlock) void k_amg_dense(int n, const float *__restrict__ inv,
@@ -1010,6 +1105,8 @@ struct AmgFine {
   const TM *val;
   const float *binv;
   double rho;
+  const uint2 *val16;        // binary16 quads for the smoothing and residual (or null)
+  const double *val16_scale;
 };
 
 // the first Chebyshev step from x = 0 on level 0: d = xo = c2 M^-1 b, one thread per group
@@ -1069,24 +1166,43 @@ __global__ __launch_bounds__(kBlock) void k_amg_pcg_update_first(int ngroups, co
   }
 }
 
-template <typename TM>
-static void cheb0(const AmgFine<TM> &f, bool first, const double *b, const double *x, double *xo, double *d,
-                  double c1, double c2, const double *done, hipStream_t st, double *rz_ws = nullptr) {
-  const int grid = spmv_grid(f.s);
-  const auto &s = f.s;
-  if (first) {
-    k_amg_cheb0_first<<<amg_grid(s.ngroups, kBlock), kBlock, 0, st>>>(s.ngroups, f.grp, f.binv, b, xo, d, c2, done);
-  } else if (rz_ws != nullptr) {
+// the level-0 products of the smoothing steps and the residual read VT values (the matrix's
+// own, or its binary16 copy with the scale vscale)
+// triple-structured matrices only (the binary16 quads)
+#define FEA_SW_SWITCH_TRIPLE(MACRO)                                        \
+  do {                                                                     \
+    switch (s.sw) { case 8: MACRO(8, 2); break; case 16: MACRO(16, 2); break; \
+                    default: MACRO(32, 2); }                               \
+  } while (0)
+
+template <typename VT>
+static void cheb0_v(const SpmvShape &s, const int32_t *grp, const int32_t *rowptr, const int32_t *col,
+                    const VT *val, const double *vscale, const float *binv, const double *b, const double *x,
+                    double *xo, double *d, double c1, double c2, const double *done, hipStream_t st, double *rz_ws) {
+  const int grid = spmv_grid(s);
+  if (rz_ws != nullptr) {
 #define FEA_AC2(SW, MD) \
-  k_amg_cheb0<TM, SW, MD, 0, 1><<<grid, kBlock, 0, st>>>(s.ngroups, f.grp, f.rowptr, f.col, f.val, f.binv, b, x, xo, d, c1, c2, done, rz_ws)
-    FEA_SW_SWITCH(FEA_AC2);
+  k_amg_cheb0<VT, SW, MD, 0, 1><<<grid, kBlock, 0, st>>>(s.ngroups, grp, rowptr, col, val, binv, b, x, xo, d, c1, c2, done, rz_ws, vscale)
+    if constexpr (std::is_same<VT, uint2>::value) FEA_SW_SWITCH_TRIPLE(FEA_AC2); else FEA_SW_SWITCH(FEA_AC2);
 #undef FEA_AC2
   } else {
 #define FEA_AC1(SW, MD) \
-  k_amg_cheb0<TM, SW, MD, 0><<<grid, kBlock, 0, st>>>(s.ngroups, f.grp, f.rowptr, f.col, f.val, f.binv, b, x, xo, d, c1, c2, done)
-    FEA_SW_SWITCH(FEA_AC1);
+  k_amg_cheb0<VT, SW, MD, 0><<<grid, kBlock, 0, st>>>(s.ngroups, grp, rowptr, col, val, binv, b, x, xo, d, c1, c2, done, nullptr, vscale)
+    if constexpr (std::is_same<VT, uint2>::value) FEA_SW_SWITCH_TRIPLE(FEA_AC1); else FEA_SW_SWITCH(FEA_AC1);
 #undef FEA_AC1
   }
+}
+
+template <typename TM>
+static void cheb0(const AmgFine<TM> &f, bool first, const double *b, const double *x, double *xo, double *d,
+                  double c1, double c2, const double *done, hipStream_t st, double *rz_ws = nullptr) {
+  const auto &s = f.s;
+  if (first)
+    k_amg_cheb0_first<<<amg_grid(s.ngroups, kBlock), kBlock, 0, st>>>(s.ngroups, f.grp, f.binv, b, xo, d, c2, done);
+  else if (f.val16 != nullptr)
+    cheb0_v<uint2>(s, f.grp, f.rowptr, f.col, f.val16, f.val16_scale, f.binv, b, x, xo, d, c1, c2, done, st, rz_ws);
+  else
+    cheb0_v<TM>(s, f.grp, f.rowptr, f.col, f.val, nullptr, f.binv, b, x, xo, d, c1, c2, done, st, rz_ws);
 }
 
 template <typename TM>
@@ -1094,10 +1210,17 @@ static void resid0(const AmgFine<TM> &f, const double *b, const double *x, doubl
                    hipStream_t st) {
   const int grid = spmv_grid(f.s);
   const auto &s = f.s;
+  if (f.val16 != nullptr) {
 #define FEA_AR0(SW, MD) \
-  k_amg_resid0<TM, SW, MD><<<grid, kBlock, 0, st>>>(s.ngroups, f.grp, f.rowptr, f.col, f.val, b, x, r, done)
-  FEA_SW_SWITCH(FEA_AR0);
+  k_amg_resid0<uint2, SW, MD><<<grid, kBlock, 0, st>>>(s.ngroups, f.grp, f.rowptr, f.col, f.val16, b, x, r, done, f.val16_scale)
+    FEA_SW_SWITCH_TRIPLE(FEA_AR0);
 #undef FEA_AR0
+  } else {
+#define FEA_AR0(SW, MD) \
+  k_amg_resid0<TM, SW, MD><<<grid, kBlock, 0, st>>>(s.ngroups, f.grp, f.rowptr, f.col, f.val, b, x, r, done, nullptr)
+    FEA_SW_SWITCH(FEA_AR0);
+#undef FEA_AR0
+  }
 }
 
 static void cheb_c(const fea_amg_level &L, bool first, const double *b, const double *x, double *xo, double c1,
@@ -1239,7 +1362,8 @@ __global__ __launch_bounds__(kBlock) void k_amg_power0(int ngroups, const int32_
 
 template <typename TM>
 static AmgFine<TM> amg_fine(int n, int64_t nnz, int n_groups, const int32_t *grp, const int32_t *rowptr,
-                            const int32_t *col, const void *val, const float *binv, double rho) {
+                            const int32_t *col, const void *val, const float *binv, double rho,
+                            const fea_amg_level *lv0 = nullptr) {
   AmgFine<TM> f;
   f.s = shape_of(n, nnz, n_groups, grp);
   f.grp = grp;
@@ -1248,6 +1372,8 @@ static AmgFine<TM> amg_fine(int n, int64_t nnz, int n_groups, const int32_t *grp
   f.val = (const TM *)val;
   f.binv = binv;
   f.rho = rho;
+  f.val16 = lv0 != nullptr ? (const uint2 *)lv0->val16 : nullptr;
+  f.val16_scale = lv0 != nullptr ? lv0->val16_scale : nullptr;
   return f;
 }
 
@@ -1272,7 +1398,7 @@ struct AmgPcgArgs {
 
 template <typename TM>
 static int amg_pcg_start_t(const AmgPcgArgs &a) {
-  const AmgFine<TM> f = amg_fine<TM>(a.n, a.nnz, a.n_groups, a.grp, a.rowptr, a.col, a.val, a.binv, a.rho0);
+  const AmgFine<TM> f = amg_fine<TM>(a.n, a.nnz, a.n_groups, a.grp, a.rowptr, a.col, a.val, a.binv, a.rho0, a.lv);
   const int gu = vec_grid();
   launch_spmv<TM, double>(f.s, a.grp, a.rowptr, a.col, f.val, a.x, a.b, a.r, a.st);  // r = b - A x
   amg_vcycle<TM>(f, a.lv, a.nl, a.deg, a.b, a.z, nullptr, a.st);                    // z = M^-1 b
@@ -1285,7 +1411,7 @@ static int amg_pcg_start_t(const AmgPcgArgs &a) {
 
 template <typename TM>
 static int amg_pcg_iterate_t(const AmgPcgArgs &a) {
-  const AmgFine<TM> f = amg_fine<TM>(a.n, a.nnz, a.n_groups, a.grp, a.rowptr, a.col, a.val, a.binv, a.rho0);
+  const AmgFine<TM> f = amg_fine<TM>(a.n, a.nnz, a.n_groups, a.grp, a.rowptr, a.col, a.val, a.binv, a.rho0, a.lv);
   const int gu = vec_grid();
   const double *done = a.ws + W_DONE;
   // the PCG update rides in the V-cycle's first level-0 step and the r.z partials in its last
@@ -1312,6 +1438,8 @@ static int check_amg_levels(const fea_amg_level *lv, int nl, int deg, int n, con
                 AMG_MAX_LEVELS);
   FEA_CHECK_ARG(deg >= 1 && deg <= AMG_MAX_DEG, "%s: smoother degree 1 .. %d", who, AMG_MAX_DEG);
   FEA_CHECK_ARG(lv[0].n == n, "%s: level 0 has %d rows, the matrix %d", who, lv[0].n, n);
+  FEA_CHECK_ARG(lv[0].val16 == nullptr || lv[0].val16_scale != nullptr, "%s: level-0 binary16 values need their scale",
+                who);
   for (int l = 0; l < nl; ++l) {
     const fea_amg_level &L = lv[l];
     FEA_CHECK_ARG(L.x && L.x2 && L.d && L.r && (l == 0 || L.b), "%s: level %d work vectors missing", who, l);
@@ -1331,6 +1459,26 @@ static int check_amg_levels(const fea_amg_level *lv, int nl, int deg, int n, con
 
 extern "C" int fea_amg_level_size(void) { return (int)sizeof(fea_amg_level); }
 
+extern "C" int fea_amg_half_values(int mat_dtype, int64_t nnz, const void *val, uint16_t *val16, double *scale,
+                                   void *stream) {
+  FEA_CHECK_ARG(mat_dtype == FEA_F64 || mat_dtype == FEA_F32, "fea_amg_half_values: bad matrix dtype");
+  FEA_CHECK_ARG(nnz > 0 && nnz % 3 == 0 && val && val16 && scale,
+                "fea_amg_half_values: bad arguments (values of a triple-structured matrix expected)");
+  hipStream_t st = as_stream(stream);
+  unsigned int *amax = reinterpret_cast<unsigned int *>(scale);
+  const int grid = (int)std::min<int64_t>(2048, (nnz + kBlock - 1) / kBlock);
+  k_absmax_reset<<<1, 1, 0, st>>>(amax);
+  if (mat_dtype == FEA_F64) {
+    k_absmax<double><<<grid, kBlock, 0, st>>>(nnz, (const double *)val, amax);
+    k_to_half<double><<<grid, kBlock, 0, st>>>(nnz / 3, (const double *)val, amax, (uint2 *)val16, scale);
+  } else {
+    k_absmax<float><<<grid, kBlock, 0, st>>>(nnz, (const float *)val, amax);
+    k_to_half<float><<<grid, kBlock, 0, st>>>(nnz / 3, (const float *)val, amax, (uint2 *)val16, scale);
+  }
+  FEA_CHECK_LAUNCH("fea_amg_half_values");
+  return FEA_OK;
+}
+
 extern "C" int fea_amg_vcycle(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp,
                               const int32_t *rowptr, const int32_t *col, const void *val, const float *binv,
                               double rho0, const fea_amg_level *levels, int n_levels, int degree, const double *b,
@@ -1340,13 +1488,15 @@ extern "C" int fea_amg_vcycle(int mat_dtype, int n, int64_t nnz, int n_groups, c
                 "fea_amg_vcycle: bad arguments (node groups and block-Jacobi inverses are required)");
   const int rc = check_amg_levels(levels, n_levels, degree, n, "fea_amg_vcycle");
   if (rc != FEA_OK) return rc;
+  FEA_CHECK_ARG(levels[0].val16 == nullptr || n_groups < 0,
+                "fea_amg_vcycle: level-0 binary16 values need a triple-structured matrix (n_groups < 0)");
   hipStream_t st = as_stream(stream);
   if (mat_dtype == FEA_F64)
-    amg_vcycle<double>(amg_fine<double>(n, nnz, n_groups, grp, rowptr, col, val, binv, rho0), levels, n_levels,
-                       degree, b, z, nullptr, st);
+    amg_vcycle<double>(amg_fine<double>(n, nnz, n_groups, grp, rowptr, col, val, binv, rho0, levels), levels,
+                       n_levels, degree, b, z, nullptr, st);
   else
-    amg_vcycle<float>(amg_fine<float>(n, nnz, n_groups, grp, rowptr, col, val, binv, rho0), levels, n_levels,
-                      degree, b, z, nullptr, st);
+    amg_vcycle<float>(amg_fine<float>(n, nnz, n_groups, grp, rowptr, col, val, binv, rho0, levels), levels,
+                      n_levels, degree, b, z, nullptr, st);
   FEA_CHECK_LAUNCH("fea_amg_vcycle");
   return FEA_OK;
 }
@@ -1366,6 +1516,20 @@ extern "C" int fea_amg_smooth0(int mat_dtype, int n, int64_t nnz, int n_groups, 
   else
     cheb0(ff, x == nullptr, b, x, xo, d, c1, c2, nullptr, st);
   FEA_CHECK_LAUNCH("fea_amg_smooth0");
+  return FEA_OK;
+}
+
+extern "C" int fea_amg_smooth0_half(int n, int64_t nnz, int n_groups, const int32_t *grp, const int32_t *rowptr,
+                                    const int32_t *col, const uint16_t *val16, const double *scale,
+                                    const float *binv, const double *b, const double *x, double *xo, double *d,
+                                    double c1, double c2, void *stream) {
+  FEA_CHECK_ARG(n > 0 && grp && n_groups < 0 && rowptr && col && val16 && scale && binv && b && xo && d && x != xo,
+                "fea_amg_smooth0_half: bad arguments (a triple-structured matrix, n_groups < 0, is required)");
+  AmgFine<float> f = amg_fine<float>(n, nnz, n_groups, grp, rowptr, col, nullptr, binv, 1.0);
+  f.val16 = (const uint2 *)val16;
+  f.val16_scale = scale;
+  cheb0(f, x == nullptr, b, x, xo, d, c1, c2, nullptr, as_stream(stream));
+  FEA_CHECK_LAUNCH("fea_amg_smooth0_half");
   return FEA_OK;
 }
 
@@ -1421,6 +1585,8 @@ extern "C" int fea_pcg_start_amg(int mat_dtype, int n, int64_t nnz, int n_groups
                 "fea_pcg_start_amg: bad arguments");
   const int rc = check_amg_levels(levels, n_levels, degree, n, "fea_pcg_start_amg");
   if (rc != FEA_OK) return rc;
+  FEA_CHECK_ARG(levels[0].val16 == nullptr || n_groups < 0,
+                "fea_pcg_start_amg: level-0 binary16 values need a triple-structured matrix (n_groups < 0)");
   AmgPcgArgs a{n, nnz, n_groups, grp, rowptr, col, val, binv, rho0, levels, n_levels, degree, b, x, r, z, p,
                nullptr, ws, 0.0, 0, 0, nullptr, as_stream(stream)};
   return mat_dtype == FEA_F64 ? amg_pcg_start_t<double>(a) : amg_pcg_start_t<float>(a);
@@ -1437,6 +1603,8 @@ extern "C" int fea_pcg_iterate_amg(int mat_dtype, int n, int64_t nnz, int n_grou
                 "fea_pcg_iterate_amg: bad arguments");
   const int rc = check_amg_levels(levels, n_levels, degree, n, "fea_pcg_iterate_amg");
   if (rc != FEA_OK) return rc;
+  FEA_CHECK_ARG(levels[0].val16 == nullptr || n_groups < 0,
+                "fea_pcg_iterate_amg: level-0 binary16 values need a triple-structured matrix (n_groups < 0)");
   AmgPcgArgs a{n, nnz, n_groups, grp, rowptr, col, val, binv, rho0, levels, n_levels, degree, nullptr, x, r, z, p,
                q, ws, tol, max_iter, iters, status_host, as_stream(stream)};
   return mat_dtype == FEA_F64 ? amg_pcg_iterate_t<double>(a) : amg_pcg_iterate_t<float>(a);

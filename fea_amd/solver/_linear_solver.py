@@ -80,7 +80,8 @@ class PCG:
         else:
             nat.ops().jacobi_inv(self.mat_dtype, self.vec_dtype, self.diag_slot, val, self.dinv)
         if self.amg is not None:
-            # the V-cycle's finest level smooths with this matrix: lambda_max of D^-1 A for it
+            # the V-cycle's finest level smooths with this matrix (its binary16 copy):
+            # lambda_max of D^-1 A for it, and the copy refreshed
             self.amg.estimate_rho(self.mat_dtype, val, self.dinv)
 
     def solve(self, b, x0=None, tol=1e-5, max_iter=1500):

@@ -31,7 +31,7 @@ class AmgLevel(ctypes.Structure):
     _fields_ = [('n', ctypes.c_int32), ('nagg', ctypes.c_int32), ('rho', ctypes.c_double),
                 ('rowptr', _P), ('col', _P), ('val', _P), ('dinv', _P), ('prow', _P), ('pagg', _P), ('pval', _P),
                 ('pt_ptr', _P), ('pt_row', _P), ('ptval', _P), ('dense_inv', _P),
-                ('b', _P), ('x', _P), ('x2', _P), ('d', _P), ('r', _P)]
+                ('b', _P), ('x', _P), ('x2', _P), ('d', _P), ('r', _P), ('val16', _P), ('val16_scale', _P)]
 
 
 
@@ -57,7 +57,8 @@ class AMG:
     (dense inverse).
     """
 
-    def __init__(self, plan, K, degree=2, max_coarse=2000, max_levels=8, threads=None, handle=None):
+    def __init__(self, plan, K, degree=2, max_coarse=2000, max_levels=8, threads=None, handle=None,
+                 half_level0=False):
         import time
         t0 = time.perf_counter()
         L = _lib()
@@ -86,6 +87,18 @@ class AMG:
         self._px = torch.as_tensor(x0.astype(np.float32), device=dev)
         self._pwork = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(2)]
         self._ppart = torch.zeros(2048, dtype=torch.float64, device=dev)
+        # half_level0: the level-0 smoothing and residual read a binary16 copy of the current
+        # matrix (triple-structured matrices; two thirds of the float values' bytes, a third of
+        # their loads; the beam's PCG iteration counts do not change).  Off by default: on the
+        # beam those passes get 8-13 % faster but the PCG product, which keeps the float matrix,
+        # 26 % slower -- the two copies no longer both fit the 256 MB Infinity Cache -- and the
+        # solve time is unchanged
+        self.half_level0 = bool(half_level0) and plan.spmv_groups < 0 and plan.nnz % 3 == 0
+        if self.half_level0:
+            self._val16 = torch.empty(4 * plan.nnz // 3, dtype=torch.int16, device=dev)
+            self._val16_scale = torch.zeros(2, dtype=torch.float64, device=dev)
+            self._arr[0].val16 = _ptr(self._val16)
+            self._arr[0].val16_scale = _ptr(self._val16_scale)
         self.rho0 = None
         self.setup_time = time.perf_counter() - t0
 
@@ -147,8 +160,21 @@ class AMG:
     def nbytes(self):
         return sum(t.numel() * t.element_size() for d in self._keep for t in d.values())
 
+    def set_level0(self, mat_dtype, val):
+        """Refresh the binary16 copy of the level-0 matrix (fea_amg_half_values) after the
+        matrix values changed; without half_level0 nothing to do."""
+        if self.half_level0:
+            check(_lib().fea_amg_half_values(mat_dtype, self.plan.nnz, _ptr(val), _ptr(self._val16),
+                                             _ptr(self._val16_scale), nat.stream_ptr()), 'fea_amg_half_values')
+
+    def nbytes_level0(self):
+        return self._val16.numel() * 2 if self.half_level0 else 0
+
     def estimate_rho(self, mat_dtype, val, binv, iters=15):
-        """lambda_max of D^-1 A for the current level-0 matrix (power iteration on the device)."""
+        """lambda_max of D^-1 A for the current level-0 matrix (power iteration on the device),
+        and the level-0 binary16 copy refreshed (``set_level0``): called whenever the matrix
+        values change, before the V-cycle uses them."""
+        self.set_level0(mat_dtype, val)
         p = self.plan
         rho = ctypes.c_double(0.0)
         x, y = self._pwork

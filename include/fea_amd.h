@@ -332,6 +332,12 @@ typedef struct fea_amg_level {
   const uint16_t *ptval;
   const float *dense_inv;     /* coarsest level: n x n inverse */
   double *b, *x, *x2, *d, *r; /* work vectors of length n (b unused on level 0) */
+  /* level 0 only, may be NULL: the level-0 matrix values in IEEE binary16, times a power of
+   * two (fea_amg_half_values), with val16_scale[1] the power that undoes it.  When set, the
+   * V-cycle's level-0 smoothing and residual read these (half the value bytes of a float
+   * matrix); the PCG product keeps the matrix itself. */
+  const uint16_t *val16;
+  const double *val16_scale;
 } fea_amg_level;
 int fea_amg_level_size(void);
 /* z = V-cycle(b): level 0 is the matrix (node-grouped CSR as for fea_spmv, current values)
@@ -347,6 +353,20 @@ int fea_amg_vcycle(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_
 int fea_amg_smooth0(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp, const int32_t *rowptr,
                     const int32_t *col, const void *val, const float *binv, const double *b, const double *x,
                     double *xo, double *d, double c1, double c2, void *stream);
+/* val16 = binary16(2^k val) for the V-cycle's level 0, k chosen from max |val| so that the
+ * largest value lands in [2^14, 2^15); scale[1] = 2^-k, scale[0] is work.  The matrix must be
+ * triple-structured (n_groups < 0 in the SpMV calls; nnz a multiple of 3): each triple of
+ * values t becomes the four binary16 val16[4t .. 4t+3] (the fourth zero; val16 holds 4 nnz / 3
+ * entries), read with one 8-byte load.  Values below 2^-24 of the scaled range flush to zero
+ * (no counterpart in the reference: an operand of the multigrid preconditioner, whose
+ * result the outer PCG still tests against the matrix). */
+int fea_amg_half_values(int mat_dtype, int64_t nnz, const void *val, uint16_t *val16, double *scale,
+                        void *stream);
+/* fea_amg_smooth0 on the binary16 values of fea_amg_half_values (scale as written there). */
+int fea_amg_smooth0_half(int n, int64_t nnz, int n_groups, const int32_t *grp, const int32_t *rowptr,
+                         const int32_t *col, const uint16_t *val16, const double *scale, const float *binv,
+                         const double *b, const double *x, double *xo, double *d, double c1, double c2,
+                         void *stream);
 /* lambda_max(D^-1 A) of the level-0 matrix by `iters` power steps from x (unit 2-norm;
  * x, y float work vectors, partial: float64 scratch of the SpMV grid); synchronises. */
 int fea_amg_power(int mat_dtype, int n, int64_t nnz, int n_groups, const int32_t *grp, const int32_t *rowptr,

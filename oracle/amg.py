@@ -11,10 +11,24 @@ GPU's preconditioner and PCG can be checked against it:
   polynomial before and after the coarse correction;
 * residual restriction r_c = P^T r, prolongation x += P x_c, a dense solve on the
   coarsest level;
-* level 0 smooths with the current matrix and its node-block Jacobi inverses.
+* level 0 smooths with the current matrix and its node-block Jacobi inverses; the device
+  reads a binary16 copy of that matrix there (``half_values``), and so may the oracle.
 """
 import numpy as np
 import scipy.sparse as sp
+
+
+def half_values(v):
+    """The binary16 copy of matrix values the device's V-cycle uses on level 0
+    (fea_amg_half_values): with max |v| = f 2^e (f in [1/2, 1), taken in float32), each value
+    times 2^k, k = 15 - e, rounded to binary16, returned in float64 and divided by 2^k
+    again.  Pinned in tests/test_oracle_solvers.py by the binary16 range and rounding bound."""
+    v = np.asarray(v, dtype=np.float64)
+    m = float(np.abs(v.astype(np.float32)).max()) if v.size else 0.0
+    if m == 0.0:
+        return v.copy()
+    k = 15 - int(np.frexp(np.float32(m))[1])
+    return np.ldexp(np.ldexp(v, k).astype(np.float16).astype(np.float64), -k)
 
 
 def cheb_coeffs(rho, deg):

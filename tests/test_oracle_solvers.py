@@ -216,3 +216,23 @@ def test_block_jacobi_pcg():
         assert np.all(M[g, nr:, :] == 0) and np.all(M[g, :, nr:] == 0)
     x, _ = osol.pcg_block_jacobi(A, b, grp, tol=1e-26, max_iter=500)
     assert np.allclose(x, np.linalg.solve(A.toarray(), b), rtol=1e-9, atol=1e-12)
+
+
+def test_half_values_range_and_rounding():
+    """oracle.amg.half_values: the largest scaled value lands in [2^14, 2^15) (inside the
+    binary16 range), each value is within half a binary16 ulp (2^-11 relative) of the input
+    where it stays normal, signs and zeros are kept, and a power-of-two scaling of the input
+    scales the output exactly."""
+    from oracle import amg as oamg
+    rng = np.random.default_rng(4)
+    for scale in (1e-3, 1.0, 7.5e4):
+        v = scale * rng.standard_normal(5000)
+        v[::97] = 0.0
+        h = oamg.half_values(v)
+        m = np.abs(v.astype(np.float32)).max()
+        k = 15 - int(np.frexp(np.float32(m))[1])
+        assert 2 ** 14 <= m * 2.0 ** k < 2 ** 15
+        big = np.abs(v) * 2.0 ** k >= 2.0 ** -14                    # normal binary16
+        assert np.all(np.abs(h[big] - v[big]) <= 2.0 ** -11 * np.abs(v[big]))
+        assert np.all(h[v == 0.0] == 0.0) and np.all(np.sign(h[big]) == np.sign(v[big]))
+        assert np.array_equal(oamg.half_values(8.0 * v), 8.0 * h)
