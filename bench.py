@@ -145,6 +145,9 @@ def spmv_bytes(plan, val_bytes):
 MEASURED_TRAFFIC = {
     ('beam 128x32x32 C3D8', 'k_pcg_spmv<float,double,16,2>'): 156.13e6 + 3.59e6,
     ('beam 128x32x32 C3D8', 'k_amg_cheb0<float,16,2,0,0>'): 166.38e6 + 6.74e6,
+    # the same two counters on `bench.py --workload explicit --steps 100 --warmup 20 --no-cpu`
+    # (profiles/explicit_element_pass_counters.md)
+    ('two 40^3-cell blocks C3D4', 'k_elem_r_tiles<float,4>'): 46.18e6 + 7.23e6,
 }
 
 
@@ -966,7 +969,8 @@ def bench_explicit(args, torch):
                  'roofline': roofline('k_elem_r_tiles<float,4> (fea_elem_residual_tiles, one step\'s element pass: '
                                       'both blocks)',
                                       alg / t_res / 1e9, HBM_PEAK_GBS, 'GB/s', 'hbm', alg_bytes_per_launch=alg,
-                                      avg_launch_us=t_res * 1e6),
+                                      avg_launch_us=t_res * 1e6,
+                                      traffic_key=('two %d^3-cell blocks C3D4' % args.cells, 'k_elem_r_tiles<float,4>')),
                  'cpu_baseline': cpu})
     return line
 
