@@ -406,6 +406,14 @@ __device__ __forceinline__ void tile_pass(int t, int n_elem, const int32_t *__re
   }
 }
 
+// Workgroup b runs on XCD b % 8 (round-robin dispatch over the 8 XCDs); tile_of gives XCD x
+// the contiguous tiles [x q + min(x, r), ...) (q = n / 8, r = n % 8) instead of every eighth
+// tile, so that Morton-adjacent tiles, which share nodes, share an L2.
+__device__ __forceinline__ int tile_of(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 template <typename T, int ET, bool STAGE>
 __global__ __launch_bounds__(TileCfg<ET>::TB) void k_elem_r_tiles(int n_elem, const int32_t *__restrict__ conn,
                                                                   const double *__restrict__ X,
@@ -417,8 +425,8 @@ __global__ __launch_bounds__(TileCfg<ET>::TB) void k_elem_r_tiles(int n_elem, co
                                                                   const int32_t *__restrict__ tnode_ptr,
                                                                   const int32_t *__restrict__ tnode,
                                                                   const uint16_t *__restrict__ lconn, int max_nodes) {
-  tile_pass<T, ET, STAGE>(blockIdx.x, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial, tnode_ptr, tnode,
-                          lconn, max_nodes);
+  tile_pass<T, ET, STAGE>(tile_of(blockIdx.x, gridDim.x), n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot,
+                          partial, tnode_ptr, tnode, lconn, max_nodes);
 }
 
 // The explicit loop's fused first launch: workgroups [0, n_tiles) run the tile pass, the
@@ -434,8 +442,8 @@ __device__ __forceinline__ void tiles_pen_body(int n_tiles, int n_elem, const in
                                                const uint16_t *__restrict__ pslot, double *__restrict__ partial,
                                                const ContactDev &cd) {
   if ((int)blockIdx.x < n_tiles) {
-    tile_pass<T, ET, false>((int)blockIdx.x, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial,
-                            nullptr, nullptr, nullptr, 0);
+    tile_pass<T, ET, false>(tile_of(blockIdx.x, n_tiles), n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot,
+                            partial, nullptr, nullptr, nullptr, 0);
     return;
   }
   constexpr int W = TileCfg<ET>::TB / 64;
