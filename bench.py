@@ -147,7 +147,7 @@ MEASURED_TRAFFIC = {
     ('beam 128x32x32 C3D8', 'k_amg_cheb0<float,16,2,0,0>'): 166.38e6 + 6.74e6,
     # the same two counters on `bench.py --workload explicit --steps 100 --warmup 20 --no-cpu`
     # (profiles/explicit_element_pass_counters.md)
-    ('two 40^3-cell blocks C3D4', 'k_elem_r_tiles<float,4>'): 46.18e6 + 7.23e6,
+    ('two 40^3-cell blocks C3D4', 'k_elem_r_tiles<float,4>'): 28.51e6 + 7.19e6,
 }
 
 
