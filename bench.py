@@ -70,6 +70,8 @@ def parse():
     ap.add_argument('--serving-qps', type=float, default=2.0)
     ap.add_argument('--serving-emax', type=float, default=None)
     ap.add_argument('--serving-emin', type=float, default=2e3)
+    ap.add_argument('--serving-amg-builders', type=int, default=1,
+                    help='serving: host threads building multigrid hierarchies concurrently')
     ap.add_argument('--serving-eager', action='store_true',
                     help='serving: launch each PCG chunk eagerly instead of replaying the captured HIP graph')
     ap.add_argument('--cells', type=int, default=40, help='explicit: cells per block edge')
@@ -470,6 +472,7 @@ def bench_beam(args, torch):
 
 # ============================================================================ serving (C5)
 _SERVING_EAGER = False      # --serving-eager: the tick path launches the PCG chunk without the graph
+_SERVING_BUILDERS = 1       # --serving-amg-builders: concurrent multigrid host set-ups
 
 
 def _serving_run(torch, reqs):
@@ -489,7 +492,8 @@ def _serving_run(torch, reqs):
             if s.rounds % 500 == 0:
                 log('serving (%s) tick %d t=%.2f in flight %d' % ('poisson' if realtime else 'saturated', s.rounds,
                                                                   time.perf_counter() - t0, len(s.active)))
-        srv = serving.SolveServer(precision='mixed', max_batch=64, use_graph=not _SERVING_EAGER)
+        srv = serving.SolveServer(precision='mixed', max_batch=64, use_graph=not _SERVING_EAGER,
+                                  amg_builders=_SERVING_BUILDERS)
         srv.run(rs, realtime=realtime, on_round=on_tick)
         return srv, time.perf_counter() - t0
 
@@ -1133,8 +1137,9 @@ def main():
         sys.exit(2)
     import torch
     torch.set_num_threads(len(os.sched_getaffinity(0)))
-    global _SERVING_EAGER
+    global _SERVING_EAGER, _SERVING_BUILDERS
     _SERVING_EAGER = bool(args.serving_eager)
+    _SERVING_BUILDERS = int(args.serving_amg_builders)
     fn = {'beam': bench_beam, 'cube': bench_cube, 'explicit': bench_explicit, 'adjoint': bench_adjoint,
           'serving': bench_serving, 'kernels': bench_kernels, 'couple': bench_couple,
           'contact': bench_contact}[args.workload]

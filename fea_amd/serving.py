@@ -340,7 +340,7 @@ class SolveServer:
     def __init__(self, precision='mixed', hbm_budget=None, max_batch=64, rel_tol=1e-10, step_tol=1e-12,
                  cg_tol=1e-8, cg_max_iter=20000, max_newton=200, chunk=32, use_graph=True,
                  armijo_slack=1e-12, verbose=False, predictor=True, preconditioner='block', admit_budget=0.004,
-                 amg_min_dof=30000, amg_chunk=8, amg_threads=None):
+                 amg_min_dof=30000, amg_chunk=8, amg_threads=None, amg_builders=1):
         nat.require_gpu()
         if precision not in ('fp64', 'mixed'):
             raise ValueError("precision must be 'fp64' or 'mixed'")
@@ -367,7 +367,8 @@ class SolveServer:
         # host threads of a hierarchy's set-up: by default half the cores less two, so the tick
         # loop keeps its core (configured 16-request stream on 16 cores: 6 threads 3.33 solves/s,
         # 14 threads 3.13)
-        self.amg_threads = amg_threads or max(1, len(os.sched_getaffinity(0)) // 2 - 2)
+        self.amg_builders = max(1, int(amg_builders))
+        self.amg_threads = amg_threads or max(1, len(os.sched_getaffinity(0)) // (2 * self.amg_builders) - 2)
         self.amg_setup_s = 0.0
         self._pool = None
         self.active: list[_Slot] = []
@@ -417,7 +418,7 @@ class SolveServer:
 
     def _builder(self):
         if self._pool is None:
-            self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+            self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=self.amg_builders)
         return self._pool
 
     def _take_amg(self, s):
