@@ -468,14 +468,13 @@ __global__ __launch_bounds__(TileCfg<ET>::TB) void k_elem_r_tiles_pen(
 // pass (74) from 6 to 4 waves per SIMD; the register budget of 6 waves (80) is imposed and
 // the penalty spills a few values instead (measured on the two-block step: 24.3k -> 27.1k
 // steps/s).
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6))) void k_elem_r_tiles_pen_f4(
+__global__ __launch_bounds__(TileCfg<4>::TB) __attribute__((amdgpu_waves_per_eu(6))) void k_elem_r_tiles_pen_f4(
     int n_tiles, int n_elem, const int32_t *__restrict__ conn, const double *__restrict__ X,
     const double *__restrict__ U, MatParams mp, const int32_t *__restrict__ tile_lbase,
     const int32_t *__restrict__ pslot_ptr, const uint16_t *__restrict__ pslot, double *__restrict__ partial,
     ContactDev cd) {
   tiles_pen_body<float, 4>(n_tiles, n_elem, conn, X, U, mp, tile_lbase, pslot_ptr, pslot, partial, cd);
 }
-static_assert(TileCfg<4>::TB == 512, "k_elem_r_tiles_pen_f4 launch bounds");
 
 template <int ET>
 __global__ __launch_bounds__(kBlock) void k_elem_energy(int n_elem, const int32_t *__restrict__ conn,
