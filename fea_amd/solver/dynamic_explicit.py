@@ -51,7 +51,8 @@ class DynamicExplicitSolver(BaseSolver):
         # as the five launches of the separate path
         self.fuse_contact = bool(fuse_contact)
         # element tiles read their nodes from LDS: bit-identical, measured slower on the
-        # two-block benchmark (15.8 vs 14.7 us per block), so off by default
+        # two-block benchmark (15.8 vs 14.7 us per block), so off by default; the fused
+        # contact step (fuse_contact) does not stage
         self.stage_nodes = False
         self.chunk = int(chunk)
         if precision not in ('fp64', 'fp32'):

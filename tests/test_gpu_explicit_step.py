@@ -160,8 +160,9 @@ def test_staged_tile_nodes_bit_identical(gpu, precision):
     out = []
     for stage in (True, False):
         fe, info = workloads.two_blocks_contact(4, velocity=1.0, gap=0.01, jitter=0.1, seed=5)
+        # the separate launches: the fused step's element launch has no node staging
         fe.solver = solver.DynamicExplicitSolver(time_end=1.0, time_per_storage=1.0, dt=3e-3, store=False,
-                                                 precision=precision)
+                                                 precision=precision, fuse_contact=False)
         fe.solver.stage_nodes = stage
         fe.initialize()
         assert ('stage' in fe.solver._tiles) == stage
